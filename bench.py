@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""SPH particle-steps/s on one MI355X: the 1,048,576-particle dam break.
+
+Workload: 128 x 128 x 64 lattice, spacing 0.01 m, +-1 % seeded jitter, in a
+3.2 x 1.6 x 1.6 m reflective box; cubic spline h = 1.3 dx, Tait rho0 = 1000,
+gamma = 7, c0 = 20, Monaghan alpha = 0.1 (beta = 0), g = -9.81 z, dt = 1e-4.
+A step is one velocity-Verlet tick of the reference schedule (main.rs:103-157)
+over all particles, state resident in HBM (no host copies inside the timed
+region).  The FP64 drop-in is `value`; the FP32-mixed form is reported beside it.
+
+Prints one JSON line (rank 0).  --gpus N > 1 is not supported: this build runs
+on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "phys-rs_amd"))
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed)
+
+# Algorithmic work per unit (f64 SoA, state touched once per kernel, neighbour
+# reads on-chip): bytes per particle per launch, FLOPs per directed neighbour
+# pair (add/sub/mul/div/sqrt = 1, fma = 2) from the reference formulas.
+ALG = {
+    "sort":      {"bytes": 224, "flops_pair": 0.0},   # hash 28 + counting sort 12 + reorder 184
+    "neighbour": {"bytes": 32, "flops_pair": 0.0},    # x, y, z, h read; candidates are not work
+    "density":   {"bytes": 56, "flops_pair": 26.0},   # x, m, h -> rho, P
+    "force":     {"bytes": 128, "flops_pair": 58.0},  # x, v, m, h, rho, P -> F_p, a*
+    "visc_only": {"bytes": 120, "flops_pair": 45.0},  # x, v, m, h, rho, F_p -> a_n
+    "integrate": {"bytes": 96, "flops_pair": 0.0},    # vv_position / vv_velocity, each 96
+    "mhd":       {"bytes": 200, "flops_pair": 0.0},
+}
+STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
+STEP_FLOPS_PER_PAIR = 26.0 + 58.0 + 45.0
+
+
+def sync_all(ctx) -> None:
+    ctx.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def run_gpu(precision: str, steps: int, warmup: int):
+    import sphx
+    from sphx import scenarios
+
+    st, p = scenarios.dam_break(nx=128, ny=128, nz=64, spacing=0.01, box=(3.2, 1.6, 1.6))
+    p.precision = sphx.PREC_FP64 if precision == "fp64" else sphx.PREC_FP32_MIXED
+    n = len(st["x"])
+    ctx = sphx.Context(n, p)
+    ctx.upload(st)
+    ctx.step(warmup)
+    sync_all(ctx)
+    ctx.enable_timing(True)
+    ctx.kernel_times(reset=True)
+    sync_all(ctx)
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    sync_all(ctx)
+    t1 = time.perf_counter()
+    times = ctx.kernel_times(reset=True)
+    e = ctx.energy()
+    ctx.close()
+    return n, t1 - t0, times, e
+
+
+def roofline(n: int, times: dict, pairs: int, steps: int):
+    kern = {}
+    for k, v in times.items():
+        if v["launches"] == 0:
+            continue
+        avg_s = v["ms"] / v["launches"] * 1e-3
+        b = ALG[k]["bytes"] * n
+        f = ALG[k]["flops_pair"] * pairs
+        kern[k] = {
+            "avg_ms": round(v["ms"] / v["launches"], 5),
+            "launches": v["launches"],
+            "ms_per_step": round(v["ms"] / steps, 5),
+            "alg_GBs": round(b / avg_s / 1e9, 1),
+            "alg_TFLOPs": round(f / avg_s / 1e12, 3),
+        }
+    dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
+    v = times[dom]
+    avg_s = v["ms"] / v["launches"] * 1e-3
+    b = ALG[dom]["bytes"] * n
+    f = ALG[dom]["flops_pair"] * pairs
+    ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+    if f > 0 and f / b > ridge:
+        # FP64 vector ALU bound; MI355X's FP64 vector and matrix peaks coincide
+        # and no MFMA instruction is used.
+        ach = f / avg_s / 1e12
+        rf = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+              "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+              "note": "FP64 vector ALU (no MFMA); peak = MI355X FP64 78.6 TFLOP/s",
+              "hbm_GBs": round(b / avg_s / 1e9, 1)}
+    else:
+        ach = b / avg_s / 1e9
+        rf = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+              "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+    return rf, kern
+
+
+def cpu_baseline(steps: int):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from sphx import scenarios
+
+    st, p = scenarios.dam_break(nx=128, ny=128, nz=64, spacing=0.01, box=(3.2, 1.6, 1.6))
+    s = O.State(st)
+    t0 = time.perf_counter()
+    O.step(s, p, steps)
+    dt = time.perf_counter() - t0
+    n = len(st["x"])
+    return {"value": round(n * steps / dt, 1), "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle (C restatement of the reference CPU path, pair list, 1 thread) on the "
+                      f"same 1,048,576-particle dam break, {steps} steps, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU domain decomposition is a design, not part of this build)", file=sys.stderr)
+        return 2
+    if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+
+    n, secs, times, e = run_gpu(args.precision, args.steps, args.warmup)
+    pairs = e["total_neighbours"]
+    value = n * args.steps / secs
+    ms_step = secs / args.steps * 1e3
+    rf, kern = roofline(n, times, pairs, args.steps)
+    step_s = secs / args.steps
+    out = {
+        "metric": "SPH particle-steps/s @1M (FP64 vs FP32-mixed, err<=1e-6); adjoint/fwd time",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-pair/f64-accumulate",
+        "data": "synthetic",
+        "config": {"workload": "dam_break_1M", "particles": n, "lattice": "128x128x64", "spacing_m": 0.01,
+                   "h_m": 0.013, "box_m": [3.2, 1.6, 1.6], "dt_s": 1e-4, "precision": args.precision,
+                   "directed_neighbour_pairs": pairs, "max_neighbours": e["max_neighbours"]},
+        "roofline": rf,
+        "step_roofline": {
+            "hbm_frac": round(n * STEP_BYTES_PER_PARTICLE / (HBM_PEAK_GBS * 1e9) / step_s, 4),
+            "fp64_frac": round(pairs * STEP_FLOPS_PER_PAIR / (FP64_PEAK_TFLOPS * 1e12) / step_s, 4),
+            "alg_bytes_per_particle": STEP_BYTES_PER_PARTICLE,
+            "alg_flops_per_pair": STEP_FLOPS_PER_PAIR,
+        },
+        "kernels": kern,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,221 @@
+/*
+ * sphx.h — C ABI of the MI355X (gfx950) SPH particle step.
+ *
+ * This is the drop-in boundary for the SPH step of phys-rs. Each pass
+ * replaces one of the engine's ECS systems (cited per entry point), and the
+ * lifecycle/data calls replace the wgpu object API of src/gpu/ (GpuContext,
+ * SphPipeline::create_buffers_and_bind_group, dispatch_all, read_densities /
+ * read_forces).  Everything is plain C: POD structs, host pointers and sizes;
+ * the library owns device memory, the caller keeps its host arrays.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (SPHX_OK == 0) and never aborts;
+ *    sphx_last_error() returns a message for the last failing call on a ctx.
+ *  - Particle arrays are SoA, f64 (the reference's DVec3 / f64 components),
+ *    in the caller's order ("upload order").  Results are returned in the
+ *    same order.  `id` is the ECS iteration order of the entity: it decides
+ *    pair orientation exactly like Entity ordering does for CandidatePairs
+ *    (src/physics/broadphase.rs:87) and is required for MHD induction,
+ *    whose per-pair sign depends on it (src/physics/mhd.rs:121-122).
+ *  - A ctx is not thread safe; all passes are enqueued on the ctx stream and
+ *    sphx_download / sphx_export_pairs / sphx_synchronize block on it.
+ *  - All fluid particles must share one Sector (src/core/coordinates.rs:51-65
+ *    then reduces to local_b - local_a exactly); positions are LocalPosition.
+ */
+#ifndef SPHX_H
+#define SPHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    SPHX_OK = 0,
+    SPHX_E_NODEVICE = 1,   /* no HIP device / device index out of range      */
+    SPHX_E_OOM = 2,        /* device allocation failed                        */
+    SPHX_E_CAPACITY = 3,   /* n > capacity, pair buffer too small, grid too big */
+    SPHX_E_INVALID = 4,    /* bad argument or parameter                        */
+    SPHX_E_MULTISECTOR = 5,/* reserved: particles in different sectors        */
+    SPHX_E_NONFINITE = 6,  /* NaN/Inf in inputs or state                       */
+    SPHX_E_HIP = 7,        /* a HIP runtime call failed                        */
+    SPHX_E_STATE = 8       /* call order violated (e.g. forces before density) */
+};
+
+/* ---- parameter enums --------------------------------------------------- */
+enum SphxEos {
+    SPHX_EOS_TAIT = 0,       /* P = B((rho/rho0)^gamma - 1), B = rho0 c^2/gamma, clamp >= 0 (sph.rs:157-167) */
+    SPHX_EOS_POLYTROPIC = 1, /* adiabatic ideal gas at fixed entropy: P = K rho^gamma                       */
+    SPHX_EOS_ISOTHERMAL = 2  /* P = rho * (k_B T / m) = rho * cs2 (thermodynamics.rs:232-238)              */
+};
+enum SphxViscConvention {
+    SPHX_VISC_REFERENCE = 0, /* active iff (v_i - v_j).(x_j - x_i) < 0, as sph.rs:292-295 evaluates it */
+    SPHX_VISC_MONAGHAN = 1   /* active iff (v_i - v_j).(x_i - x_j) < 0 (approaching pairs)              */
+};
+enum SphxBoundaryFlags {
+    SPHX_BND_PENALTY_CUBE = 1, /* sph_boundary_system on [-half_extent, half_extent]^3 (sph.rs:443-484) */
+    SPHX_BND_REFLECT = 2,      /* specular reflection at box_lo / box_hi after the position update   */
+    SPHX_BND_PERIODIC = 4      /* periodic box [box_lo, box_hi) with minimum-image pair vectors       */
+};
+enum SphxMhdFlags {
+    SPHX_MHD_INDUCTION = 1,      /* mhd_induction_system (mhd.rs:68-152)              */
+    SPHX_MHD_CLEANING = 2,       /* divergence_cleaning_system (mhd.rs:172-259)        */
+    SPHX_MHD_MAXWELL_STRESS = 4, /* SPMHD Lorentz force from M = BB - |B|^2 I / 2 (mu0 = 1) */
+    SPHX_MHD_LORENTZ_CHARGE = 8  /* per-particle q (E + v x B) (lorentz.rs:21-39)     */
+};
+enum SphxPrecision {
+    SPHX_PREC_FP64 = 0,      /* plain f64 drop-in                                                  */
+    SPHX_PREC_FP32_MIXED = 1 /* f32 pair math on cell-relative coordinates, f64 sums/integration   */
+};
+
+/* ---- parameters: the reference Resources, field for field -------------- */
+typedef struct SphxParams {
+    double dt;                 /* SimulationTime.dt (core/time.rs:18)                           */
+    double sector_size;        /* UniverseConfig.sector_size (core/config.rs); informational    */
+    /* TaitEquationConfig (sph.rs:125-132) */
+    double rest_density;
+    double speed_of_sound;     /* also the c of the viscosity term (sph.rs:280)                 */
+    double gamma;
+    /* other EOS kinds */
+    int32_t eos;               /* enum SphxEos                                                   */
+    int32_t visc_convention;   /* enum SphxViscConvention                                        */
+    double poly_k;             /* SPHX_EOS_POLYTROPIC: K                                         */
+    double iso_cs2;            /* SPHX_EOS_ISOTHERMAL: k_B T / m                                 */
+    /* SphViscosityConfig (sph.rs:234-239) */
+    double visc_alpha;
+    double visc_beta;
+    /* XsphConfig (sph.rs:328-331) */
+    double xsph_epsilon;
+    /* SphBoundaryConfig (sph.rs:415-422) */
+    double half_extent;
+    double activation_distance;
+    double stiffness;
+    int32_t boundary_flags;    /* enum SphxBoundaryFlags, OR-ed                                  */
+    int32_t mhd_flags;         /* enum SphxMhdFlags, OR-ed                                       */
+    double box_lo[3];          /* reflective / periodic box                                      */
+    double box_hi[3];
+    double gravity[3];         /* uniform body acceleration, added as F += m g                  */
+    double e_field[3];         /* uniform E for SPHX_MHD_LORENTZ_CHARGE                          */
+    /* MhdConfig (mhd.rs:26-39) */
+    double resistivity;
+    double cleaning_speed;
+    double cleaning_damping;
+    /* neighbour grid */
+    double cell_size;          /* <= 0: 2 * max(h) * (1 + 1e-9)                                  */
+    double grid_lo[3];         /* grid box; grid_hi <= grid_lo on any axis: taken from the      */
+    double grid_hi[3];         /* uploaded positions (padded), or from box_lo/hi when periodic   */
+    int32_t precision;         /* enum SphxPrecision                                             */
+    int32_t max_neighbours;    /* per-particle neighbour-list capacity; <= 0: 192                */
+} SphxParams;
+
+/* Fill *p with the reference defaults: TaitEquationConfig::default (sph.rs:134-142),
+ * SphViscosityConfig::default (241-248), XsphConfig::default (333-339),
+ * SphBoundaryConfig::default (424-432), MhdConfig::default (mhd.rs:41-49),
+ * dt = 1/60 (time.rs:37), sector_size = 1e12, no boundary flags, FP64. */
+void sphx_default_params(SphxParams* p);
+
+/* ---- host-side particle state (the ECS components, gathered to SoA) ----- */
+typedef struct SphxState {
+    uint32_t n;
+    double* x; double* y; double* z;      /* LocalPosition                 */
+    double* vx; double* vy; double* vz;   /* Velocity                      */
+    double* m;                            /* Mass                          */
+    double* h;                            /* SmoothingRadius               */
+    double* rho;                          /* SmoothedDensity  (output)     */
+    double* p;                            /* Pressure         (output)     */
+    double* ax; double* ay; double* az;   /* Acceleration = Force / Mass   */
+    double* bx; double* by; double* bz;   /* MagneticField  (optional)     */
+    double* psi;                          /* DivCleaningPsi (optional)     */
+    double* q;                            /* Charge         (optional)     */
+    uint32_t* id;                         /* ECS order; NULL -> 0..n-1     */
+} SphxState;
+
+/* fields for sphx_download */
+enum SphxFields {
+    SPHX_F_POS = 1, SPHX_F_VEL = 2, SPHX_F_RHO = 4, SPHX_F_P = 8, SPHX_F_ACC = 16,
+    SPHX_F_B = 32, SPHX_F_PSI = 64, SPHX_F_ALL = 127
+};
+
+typedef struct SphxCtx SphxCtx;
+
+/* ---- lifecycle: replaces initialize_gpu_context / try_init_gpu (gpu/mod.rs:69-142)
+ *      and SphPipeline::new (gpu/pipeline.rs:27) ------------------------- */
+int sphx_device_count(int* count);
+int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx** out);
+void sphx_destroy(SphxCtx* ctx);
+int sphx_set_params(SphxCtx* ctx, const SphxParams* params);
+const char* sphx_last_error(const SphxCtx* ctx);
+
+/* ---- data: replaces create_buffers_and_bind_group (gpu/pipeline.rs:168-206)
+ *      and read_densities / read_forces (gpu/buffers.rs:163-213) --------- */
+int sphx_upload(SphxCtx* ctx, const SphxState* host);
+int sphx_download(SphxCtx* ctx, SphxState* host, uint32_t fields);
+
+/* ---- passes (each replaces one ECS system; async on the ctx stream) ---- */
+/* build_broadphase_system + broadphase_query_system (broadphase.rs:33-96):
+ * cell hash, counting sort, reorder, exact neighbour lists r < 2*h_avg.   */
+int sphx_neighbours(SphxCtx* ctx);
+/* sph_density_system (sph.rs:72-116) + sph_eos_system (sph.rs:149-169)    */
+int sphx_density(SphxCtx* ctx);
+/* reset_forces (forces.rs:15) + sph_pressure_force_system (sph.rs:177-228)
+ * + sph_viscosity_system (sph.rs:260-322) + sph_boundary_system
+ * (sph.rs:443-484) + lorentz_force_system (lorentz.rs:21-39)
+ * + compute_acceleration_system (integrators.rs:110-118).
+ * flags = SPHX_FORCES_VISC_ONLY reuses the pressure sum stored by the last
+ * full call at the same positions (bit-identical to recomputing it).       */
+enum { SPHX_FORCES_FULL = 0, SPHX_FORCES_VISC_ONLY = 1 };
+int sphx_forces(SphxCtx* ctx, uint32_t flags);
+/* velocity_verlet_position_system (integrators.rs:63-83)                  */
+int sphx_vv_position(SphxCtx* ctx);
+/* velocity_verlet_velocity_system (integrators.rs:92-104)                 */
+int sphx_vv_velocity(SphxCtx* ctx);
+/* sph_xsph_system (sph.rs:349-405)                                         */
+int sphx_xsph(SphxCtx* ctx);
+/* mhd_induction_system (mhd.rs:68-152)                                     */
+int sphx_mhd_induction(SphxCtx* ctx);
+/* divergence_cleaning_system (mhd.rs:172-259)                              */
+int sphx_div_clean(SphxCtx* ctx);
+/* One velocity-Verlet tick of the schedule in main.rs:103-157, repeated
+ * nsteps times: forces(x_n, v_n) -> vv_position -> neighbours/density/
+ * forces(x_n+1, v_n) -> vv_velocity -> xsph -> induction -> cleaning.     */
+int sphx_step(SphxCtx* ctx, uint32_t nsteps);
+int sphx_synchronize(SphxCtx* ctx);
+
+/* ---- test and measurement -------------------------------------------- */
+/* Undirected neighbour pairs (id_min, id_max) with r < 2*h_avg at the current
+ * positions, sorted ascending; *count receives the total (even when > cap).
+ * This is the set CandidatePairs holds after the SPH systems' r < 2h filter. */
+int sphx_export_pairs(SphxCtx* ctx, uint32_t* pairs, uint64_t cap_pairs, uint64_t* count);
+
+typedef struct SphxTimes {       /* device milliseconds, accumulated since last reset */
+    double sort_ms;              /* hash + counting sort + reorder                  */
+    double neighbour_ms;         /* neighbour-list build (exact predicate)          */
+    double density_ms;           /* density + EOS                                   */
+    double force_ms;             /* full force pass                                 */
+    double visc_only_ms;         /* viscosity-only force pass                       */
+    double integrate_ms;         /* vv position + vv velocity                       */
+    double mhd_ms;               /* xsph + induction + cleaning                     */
+    uint64_t launches[7];        /* launches counted per slot above                 */
+} SphxTimes;
+/* enable != 0: bracket each kernel with HIP events on the ctx stream. */
+int sphx_enable_timing(SphxCtx* ctx, int enable);
+int sphx_kernel_times(SphxCtx* ctx, SphxTimes* out, int reset);
+
+typedef struct SphxEnergy {
+    double kinetic;              /* sum 1/2 m |v|^2                      */
+    double potential;            /* sum -m g.x                           */
+    double internal;             /* Tait / polytropic internal energy    */
+    double magnetic;             /* sum (m / rho) |B|^2 / 2  (mu0 = 1)   */
+    double momentum[3];          /* sum m v                              */
+    uint64_t max_neighbours;     /* largest neighbour count seen         */
+    uint64_t total_neighbours;   /* directed neighbour pairs             */
+} SphxEnergy;
+int sphx_energy(SphxCtx* ctx, SphxEnergy* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHX_H */

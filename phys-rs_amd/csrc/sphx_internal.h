@@ -1,0 +1,114 @@
+// sphx_internal.h — device buffers, grid description and kernel launchers
+// shared by the kernel TU (sphx_kernels.hip) and the runtime TU (sphx_runtime.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/sphx.h"
+
+namespace sphx {
+
+// Uniform cell grid: cell (cx,cy,cz) holds positions with
+// floor((x - lo) * inv_w) == cx after clamping to [0, dims).  Cell width is
+// >= 2*max(h) so every neighbour (r < 2*h_avg <= 2*max(h)) lies in the 27 cells
+// around a particle's own; clamping is monotone and 1-Lipschitz in cell
+// units, so particles outside the grid box stay correct (just slower).
+struct Grid {
+    double lo[3];
+    double w[3];      // cell width per axis
+    double inv_w[3];
+    int dims[3];
+    int periodic;     // 1: periodic box, cells wrap, minimum image
+    double L[3];      // periodic box length
+    double halfL[3];
+    uint32_t ncell;
+};
+
+// Persistent per-particle state, in sorted (cell) order.  Two copies: the
+// counting sort gathers from one into the other.
+struct Particles {
+    double *x, *y, *z;
+    double *vx, *vy, *vz;
+    double *apx, *apy, *apz;   // PreviousAcceleration
+    double *m, *h;
+    double *bx, *by, *bz, *psi;
+    double *q;
+    uint32_t *id, *orig;       // ECS order id; upload slot
+};
+
+// Per-particle quantities that are recomputed after every sort.
+struct Derived {
+    double *rho, *p, *pr2;     // density, pressure, P / rho^2
+    double *ax, *ay, *az;      // acceleration
+    double *spx, *spy, *spz;   // m g + sum of pressure terms (reused by VISC_ONLY)
+    double *tmp3[3];           // Jacobi scratch (xsph / induction / cleaning)
+    double *tmp1;
+};
+
+struct Sort {
+    uint32_t *key, *rank, *perm;
+    uint32_t* skey;            // key in sorted order (written by the reorder)
+    uint32_t *cell_count;      // ncell
+    uint32_t *cell_start;      // ncell + 1
+    uint32_t *block_sums;      // scan scratch
+    uint32_t cell_cap;
+};
+
+struct NbrList {
+    uint32_t* idx;             // column-major: idx[k * cap + i]
+    uint32_t* cnt;             // neighbours of i
+    uint32_t* overflow;        // [0]: max count seen, [1]: overflow flag
+    int kmax;
+};
+
+struct KernelParams {          // what the kernels need from SphxParams, by value
+    double dt, half_dt, half_dt_sq;
+    double rest_density, speed_of_sound, gamma, tait_B;
+    double poly_k, iso_cs2;
+    double alpha, beta;
+    double xsph_eps;
+    double half_extent, act_dist, stiffness;
+    double box_lo[3], box_hi[3];
+    double g[3], e[3];
+    double eta, ch, sigma_c;
+    int eos, visc_conv, bnd, mhd;
+};
+
+// ---- kernel launchers (sphx_kernels.hip) --------------------------------
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_iota_u32(uint32_t* p, uint32_t n, hipStream_t s);
+void launch_cell_hash(const Particles& P, uint32_t n, const Grid& g, const Sort& S, hipStream_t s);
+void launch_scan(const Sort& S, uint32_t ncell, hipStream_t s);
+void launch_scatter(const Sort& S, uint32_t n, hipStream_t s);
+void launch_cell_order(const Particles& P, const Sort& S, uint32_t ncell, hipStream_t s);
+void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n,
+                    bool mhd, bool charge, hipStream_t s);
+void launch_build_nbr(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
+                      const NbrList& L, hipStream_t s);
+void launch_density(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                    const NbrList& L, const KernelParams& kp, hipStream_t s);
+void launch_forces(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                   const NbrList& L, const KernelParams& kp, bool visc_only, bool mhd_stress,
+                   bool charge, hipStream_t s);
+void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+                        hipStream_t s);
+void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+                        hipStream_t s);
+void launch_xsph(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                 const NbrList& L, const KernelParams& kp, hipStream_t s);
+void launch_induction(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                      const NbrList& L, const KernelParams& kp, hipStream_t s);
+void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                      const NbrList& L, const KernelParams& kp, hipStream_t s);
+void launch_export_pairs(const Particles& P, uint32_t n, uint32_t cap, const NbrList& L,
+                         uint32_t* out, unsigned long long* counter, uint64_t out_cap, hipStream_t s);
+void launch_energy(const Particles& P, const Derived& D, uint32_t n, uint32_t cap,
+                   const NbrList& L, const KernelParams& kp, bool mhd, double* partial,
+                   uint32_t nblocks, hipStream_t s);
+uint32_t energy_blocks(uint32_t n);
+
+}  // namespace sphx

@@ -1,0 +1,816 @@
+// sphx_runtime.hip — context, device memory, pass sequencing and the C ABI.
+//
+// The context owns all device memory.  State stays resident in HBM between
+// calls; sphx_upload / sphx_download are the only host<->device copies.
+// No entry point throws or aborts: every failure becomes a status code and a
+// message for sphx_last_error().
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sphx_internal.h"
+
+using namespace sphx;
+
+namespace {
+
+enum Slot { kSort = 0, kNbr, kDensity, kForce, kViscOnly, kIntegrate, kMhd, kNumSlots };
+
+struct TimedRange {
+    hipEvent_t a, b;
+    int slot;
+};
+
+}  // namespace
+
+struct SphxCtx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cap = 0, n = 0;
+    SphxParams prm{};
+    KernelParams kp{};
+    Grid grid{};
+    bool grid_set = false;
+    Particles A[2]{};
+    int cur = 0;
+    Derived D{};
+    Sort S{};
+    NbrList L{};
+    double* energy_partial = nullptr;
+    uint32_t* pair_buf = nullptr;
+    unsigned long long* pair_counter = nullptr;
+    uint64_t pair_buf_cap = 0;
+    bool has_mhd = false, has_charge = false;
+    // validity of derived data for the current positions and sorted order
+    bool sorted = false, listed = false, dens = false, sp = false, acc = false;
+    // timing
+    bool timing = false;
+    std::vector<TimedRange> pending;
+    std::vector<hipEvent_t> free_events;
+    double ms[kNumSlots] = {0};
+    uint64_t launches[kNumSlots] = {0};
+    std::vector<void*> allocs;
+    std::string err;
+
+    Particles& P() { return A[cur]; }
+};
+
+namespace {
+
+int fail(SphxCtx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                  \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(ctx, SPHX_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+template <typename T>
+int dalloc(SphxCtx* c, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, SPHX_E_OOM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                    hipGetErrorString(e));
+    }
+    c->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return SPHX_OK;
+}
+
+void free_all(SphxCtx* c) {
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+}
+
+int check_params(SphxCtx* c, const SphxParams* p) {
+    if (!p) return fail(c, SPHX_E_INVALID, "params is NULL");
+    if (!(p->dt > 0.0) || !std::isfinite(p->dt)) return fail(c, SPHX_E_INVALID, "dt must be > 0");
+    if (p->eos < SPHX_EOS_TAIT || p->eos > SPHX_EOS_ISOTHERMAL)
+        return fail(c, SPHX_E_INVALID, "unknown eos %d", p->eos);
+    if (p->eos == SPHX_EOS_TAIT && (!(p->rest_density > 0.0) || !(p->gamma != 0.0)))
+        return fail(c, SPHX_E_INVALID, "Tait EOS needs rest_density > 0 and gamma != 0");
+    if (p->visc_convention != SPHX_VISC_REFERENCE && p->visc_convention != SPHX_VISC_MONAGHAN)
+        return fail(c, SPHX_E_INVALID, "unknown viscosity convention %d", p->visc_convention);
+    if ((p->boundary_flags & SPHX_BND_REFLECT) && (p->boundary_flags & SPHX_BND_PERIODIC))
+        return fail(c, SPHX_E_INVALID, "reflective and periodic boxes are exclusive");
+    if (p->boundary_flags & (SPHX_BND_REFLECT | SPHX_BND_PERIODIC))
+        for (int k = 0; k < 3; ++k)
+            if (!(p->box_hi[k] > p->box_lo[k]))
+                return fail(c, SPHX_E_INVALID, "box_hi must exceed box_lo on every axis");
+    if (p->precision != SPHX_PREC_FP64 && p->precision != SPHX_PREC_FP32_MIXED)
+        return fail(c, SPHX_E_INVALID, "unknown precision %d", p->precision);
+    return SPHX_OK;
+}
+
+void make_kernel_params(SphxCtx* c) {
+    const SphxParams& p = c->prm;
+    KernelParams& k = c->kp;
+    k.dt = p.dt;
+    k.half_dt = 0.5 * p.dt;                 // integrators.rs:97
+    k.half_dt_sq = 0.5 * p.dt * p.dt;       // integrators.rs:73
+    k.rest_density = p.rest_density;
+    k.speed_of_sound = p.speed_of_sound;
+    k.gamma = p.gamma;
+    k.tait_B = p.rest_density * (p.speed_of_sound * p.speed_of_sound) / p.gamma;  // sph.rs:157
+    k.poly_k = p.poly_k;
+    k.iso_cs2 = p.iso_cs2;
+    k.alpha = p.visc_alpha;
+    k.beta = p.visc_beta;
+    k.xsph_eps = p.xsph_epsilon;
+    k.half_extent = p.half_extent;
+    k.act_dist = p.activation_distance;
+    k.stiffness = p.stiffness;
+    for (int a = 0; a < 3; ++a) {
+        k.box_lo[a] = p.box_lo[a];
+        k.box_hi[a] = p.box_hi[a];
+        k.g[a] = p.gravity[a];
+        k.e[a] = p.e_field[a];
+    }
+    k.eta = p.resistivity;
+    k.ch = p.cleaning_speed;
+    k.sigma_c = p.cleaning_damping;
+    k.eos = p.eos;
+    k.visc_conv = p.visc_convention;
+    k.bnd = p.boundary_flags;
+    k.mhd = p.mhd_flags;
+}
+
+// Grid from the parameters and (for open boxes) the uploaded positions.
+int setup_grid(SphxCtx* c, const SphxState* host) {
+    const SphxParams& p = c->prm;
+    double hmax = 0.0;
+    for (uint32_t i = 0; i < host->n; ++i) hmax = std::max(hmax, host->h[i]);
+    if (host->n && !(hmax > 0.0)) return fail(c, SPHX_E_INVALID, "smoothing radius must be > 0");
+    double cs = 2.0 * hmax * (1.0 + 1e-9);
+    if (p.cell_size > cs) cs = p.cell_size;
+    if (!(cs > 0.0)) cs = 1.0;
+    Grid g{};
+    const bool periodic = (p.boundary_flags & SPHX_BND_PERIODIC) != 0;
+    double lo[3], hi[3];
+    bool explicit_grid = true;
+    for (int k = 0; k < 3; ++k)
+        if (!(p.grid_hi[k] > p.grid_lo[k])) explicit_grid = false;
+    if (periodic) {
+        for (int k = 0; k < 3; ++k) { lo[k] = p.box_lo[k]; hi[k] = p.box_hi[k]; }
+    } else if (explicit_grid) {
+        for (int k = 0; k < 3; ++k) { lo[k] = p.grid_lo[k]; hi[k] = p.grid_hi[k]; }
+    } else if (p.boundary_flags & SPHX_BND_REFLECT) {
+        for (int k = 0; k < 3; ++k) { lo[k] = p.box_lo[k]; hi[k] = p.box_hi[k]; }
+    } else {
+        // bounding box of the uploaded particles, padded by a quarter on each side
+        if (host->n == 0) { for (int k = 0; k < 3; ++k) { lo[k] = 0; hi[k] = 1; } }
+        else {
+            const double* xs[3] = {host->x, host->y, host->z};
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = hi[k] = xs[k][0];
+                for (uint32_t i = 1; i < host->n; ++i) {
+                    lo[k] = std::min(lo[k], xs[k][i]);
+                    hi[k] = std::max(hi[k], xs[k][i]);
+                }
+                double pad = 0.25 * (hi[k] - lo[k]) + cs;
+                lo[k] -= pad;
+                hi[k] += pad;
+            }
+        }
+    }
+    // Cells never get narrower than 2*max(h); a grid above kMaxCells cells is
+    // made coarser (wider cells stay correct, the walk just tests more).
+    const double kMaxCells = (double)(1u << 24);
+    double ext[3];
+    for (int k = 0; k < 3; ++k) {
+        ext[k] = hi[k] - lo[k];
+        if (!std::isfinite(ext[k]) || !(ext[k] > 0.0)) return fail(c, SPHX_E_NONFINITE, "bad grid extent on axis %d", k);
+    }
+    for (int iter = 0;; ++iter) {
+        double dd[3], prod = 1.0;
+        for (int k = 0; k < 3; ++k) {
+            dd[k] = periodic ? std::floor(ext[k] / cs) : std::ceil(ext[k] / cs);
+            dd[k] = std::max(dd[k], 1.0);
+            prod *= dd[k];
+        }
+        if (prod <= kMaxCells || iter > 64) {
+            for (int k = 0; k < 3; ++k) g.dims[k] = (int)std::min(dd[k], 65536.0);
+            break;
+        }
+        cs *= std::max(1.01, std::cbrt(prod / kMaxCells));
+    }
+    uint64_t ncell = 1;
+    for (int k = 0; k < 3; ++k) {
+        const int d = g.dims[k];
+        if (periodic) {
+            if (d < 3)
+                return fail(c, SPHX_E_INVALID,
+                            "periodic box axis %d holds %d cells of width 2*h_max; at least 3 are needed", k, d);
+            g.w[k] = ext[k] / d;
+            g.L[k] = ext[k];
+            g.halfL[k] = 0.5 * ext[k];
+        } else {
+            g.w[k] = cs;
+        }
+        g.lo[k] = lo[k];
+        g.inv_w[k] = 1.0 / g.w[k];
+        ncell *= (uint64_t)d;
+    }
+    if (ncell > (1ull << 27)) return fail(c, SPHX_E_CAPACITY, "grid of %llu cells exceeds 2^27", (unsigned long long)ncell);
+    g.periodic = periodic ? 1 : 0;
+    g.ncell = (uint32_t)ncell;
+    if (g.ncell > c->S.cell_cap) {
+        int rc;
+        if ((rc = dalloc(c, &c->S.cell_count, g.ncell))) return rc;
+        if ((rc = dalloc(c, &c->S.cell_start, (size_t)g.ncell + 1))) return rc;
+        size_t tiles = ((size_t)g.ncell + 1023) / 1024 + 1;
+        if ((rc = dalloc(c, &c->S.block_sums, tiles))) return rc;
+        c->S.cell_cap = g.ncell;
+    }
+    c->grid = g;
+    c->grid_set = true;
+    return SPHX_OK;
+}
+
+// ---- timing -----------------------------------------------------------
+hipEvent_t take_event(SphxCtx* c) {
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct Timer {
+    SphxCtx* c;
+    int slot;
+    hipEvent_t a = nullptr;
+    Timer(SphxCtx* ctx, int s) : c(ctx), slot(s) {
+        if (c->timing) {
+            a = take_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timer() {
+        if (!c->timing || !a) return;
+        hipEvent_t b = take_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back({a, b, slot});
+    }
+};
+
+int drain_timing(SphxCtx* c) {
+    for (auto& t : c->pending) {
+        HIP_TRY(c, hipEventSynchronize(t.b));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, t.a, t.b));
+        c->ms[t.slot] += ms;
+        c->launches[t.slot] += 1;
+        c->free_events.push_back(t.a);
+        c->free_events.push_back(t.b);
+    }
+    c->pending.clear();
+    return SPHX_OK;
+}
+
+int launch_check(SphxCtx* c) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, SPHX_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return SPHX_OK;
+}
+
+// ---- passes ------------------------------------------------------------
+int do_neighbours(SphxCtx* c) {
+    const uint32_t n = c->n;
+    {
+        Timer t(c, kSort);
+        Particles& src = c->A[c->cur];
+        Particles& dst = c->A[1 - c->cur];
+        launch_cell_hash(src, n, c->grid, c->S, c->stream);
+        launch_scan(c->S, c->grid.ncell, c->stream);
+        launch_scatter(c->S, n, c->stream);
+        launch_cell_order(src, c->S, c->grid.ncell, c->stream);
+        launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->stream);
+        c->cur = 1 - c->cur;
+    }
+    {
+        Timer t(c, kNbr);
+        launch_build_nbr(c->P(), n, c->cap, c->grid, c->S, c->L, c->stream);
+    }
+    c->sorted = c->listed = true;
+    c->dens = c->sp = c->acc = false;
+    return launch_check(c);
+}
+
+int ensure_list(SphxCtx* c) {
+    if (c->listed) return SPHX_OK;
+    return do_neighbours(c);
+}
+
+int do_density(SphxCtx* c) {
+    int rc;
+    if ((rc = ensure_list(c))) return rc;
+    {
+        Timer t(c, kDensity);
+        launch_density(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->stream);
+    }
+    c->dens = true;
+    return launch_check(c);
+}
+
+int do_forces(SphxCtx* c, bool visc_only) {
+    int rc;
+    if (!c->dens && (rc = do_density(c))) return rc;
+    if (visc_only && !c->sp)
+        return fail(c, SPHX_E_STATE, "SPHX_FORCES_VISC_ONLY needs a full force pass at the current positions");
+    const bool stress = c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_MAXWELL_STRESS);
+    const bool charge = c->has_charge && c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_LORENTZ_CHARGE);
+    {
+        Timer t(c, visc_only ? kViscOnly : kForce);
+        launch_forces(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, visc_only, stress, charge, c->stream);
+    }
+    c->sp = true;
+    c->acc = true;
+    return launch_check(c);
+}
+
+int do_vv_position(SphxCtx* c) {
+    if (!c->acc) return fail(c, SPHX_E_STATE, "vv_position needs accelerations (call sphx_forces)");
+    {
+        Timer t(c, kIntegrate);
+        launch_vv_position(c->P(), c->D, c->n, c->kp, c->stream);
+    }
+    c->sorted = c->listed = c->dens = c->sp = c->acc = false;
+    return launch_check(c);
+}
+
+int do_vv_velocity(SphxCtx* c) {
+    if (!c->acc) return fail(c, SPHX_E_STATE, "vv_velocity needs accelerations (call sphx_forces)");
+    {
+        Timer t(c, kIntegrate);
+        launch_vv_velocity(c->P(), c->D, c->n, c->kp, c->stream);
+    }
+    c->acc = false;  // a is a function of v, which just changed
+    return launch_check(c);
+}
+
+void swap3(double*& a, double*& b, double*& cc, double** t) {
+    std::swap(a, t[0]);
+    std::swap(b, t[1]);
+    std::swap(cc, t[2]);
+}
+
+// The Jacobi passes write into D.tmp3 / tmp1 and swap those with the state
+// arrays; the other copy of Particles keeps pointing at buffers the next
+// reorder overwrites, so both copies are kept consistent by swapping in place.
+int do_xsph(SphxCtx* c) {
+    int rc;
+    if (!c->dens && (rc = do_density(c))) return rc;
+    {
+        Timer t(c, kMhd);
+        launch_xsph(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->stream);
+    }
+    Particles& P = c->P();
+    swap3(P.vx, P.vy, P.vz, c->D.tmp3);
+    c->acc = false;
+    return launch_check(c);
+}
+
+int do_induction(SphxCtx* c) {
+    int rc;
+    if (!c->has_mhd) return fail(c, SPHX_E_STATE, "no magnetic field was uploaded");
+    if (!c->dens && (rc = do_density(c))) return rc;
+    {
+        Timer t(c, kMhd);
+        launch_induction(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->stream);
+    }
+    Particles& P = c->P();
+    swap3(P.bx, P.by, P.bz, c->D.tmp3);
+    c->acc = false;
+    return launch_check(c);
+}
+
+int do_div_clean(SphxCtx* c) {
+    int rc;
+    if (!c->has_mhd) return fail(c, SPHX_E_STATE, "no magnetic field was uploaded");
+    if (!c->dens && (rc = do_density(c))) return rc;
+    {
+        Timer t(c, kMhd);
+        launch_div_clean(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->stream);
+    }
+    Particles& P = c->P();
+    swap3(P.bx, P.by, P.bz, c->D.tmp3);
+    std::swap(P.psi, c->D.tmp1);
+    c->acc = false;
+    return launch_check(c);
+}
+
+int check_overflow(SphxCtx* c) {
+    uint32_t ovf[2] = {0, 0};
+    HIP_TRY(c, hipMemcpyAsync(ovf, c->L.overflow, sizeof ovf, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (ovf[1])
+        return fail(c, SPHX_E_CAPACITY,
+                    "a particle has %u neighbours, more than max_neighbours = %d; raise it and upload again",
+                    ovf[0], c->L.kmax);
+    return SPHX_OK;
+}
+
+}  // namespace
+
+// =========================================================================
+extern "C" {
+
+void sphx_default_params(SphxParams* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->dt = 1.0 / 60.0;
+    p->sector_size = 1e12;
+    p->rest_density = 1000.0;
+    p->speed_of_sound = 1500.0;
+    p->gamma = 7.0;
+    p->eos = SPHX_EOS_TAIT;
+    p->visc_convention = SPHX_VISC_REFERENCE;
+    p->poly_k = 1.0;
+    p->iso_cs2 = 1.0;
+    p->visc_alpha = 0.1;
+    p->visc_beta = 0.2;
+    p->xsph_epsilon = 0.5;
+    p->half_extent = 10.0;
+    p->activation_distance = 0.5;
+    p->stiffness = 1e6;
+    p->resistivity = 1e-4;
+    p->cleaning_speed = 100.0;
+    p->cleaning_damping = 0.5;
+    p->precision = SPHX_PREC_FP64;
+    p->max_neighbours = 0;
+}
+
+int sphx_device_count(int* count) {
+    if (!count) return SPHX_E_INVALID;
+    *count = 0;
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return SPHX_E_NODEVICE;
+    }
+    *count = c;
+    return SPHX_OK;
+}
+
+const char* sphx_last_error(const SphxCtx* ctx) {
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx** out) {
+    if (!out) return SPHX_E_INVALID;
+    *out = nullptr;
+    int rc = check_params(nullptr, params);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return SPHX_E_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) return SPHX_E_NODEVICE;
+    SphxCtx* c = new (std::nothrow) SphxCtx();
+    if (!c) return SPHX_E_OOM;
+    c->device = device;
+    c->cap = capacity ? capacity : 1;
+    c->prm = *params;
+    make_kernel_params(c);
+    c->L.kmax = params->max_neighbours > 0 ? params->max_neighbours : 192;
+    auto bail = [&](int code) {
+        free_all(c);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bail(SPHX_E_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(SPHX_E_HIP);
+    const size_t N = c->cap;
+    for (int b = 0; b < 2; ++b) {
+        Particles& P = c->A[b];
+        double** dd[] = {&P.x, &P.y, &P.z, &P.vx, &P.vy, &P.vz, &P.apx, &P.apy, &P.apz,
+                         &P.m, &P.h, &P.bx, &P.by, &P.bz, &P.psi, &P.q};
+        for (double** d : dd)
+            if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
+        if (dalloc(c, &P.id, N) || dalloc(c, &P.orig, N)) return bail(SPHX_E_OOM);
+    }
+    Derived& D = c->D;
+    double** dd[] = {&D.rho, &D.p, &D.pr2, &D.ax, &D.ay, &D.az, &D.spx, &D.spy, &D.spz,
+                     &D.tmp3[0], &D.tmp3[1], &D.tmp3[2], &D.tmp1};
+    for (double** d : dd)
+        if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
+    if (dalloc(c, &c->S.key, N) || dalloc(c, &c->S.rank, N) || dalloc(c, &c->S.perm, N) ||
+        dalloc(c, &c->S.skey, N))
+        return bail(SPHX_E_OOM);
+    if (dalloc(c, &c->L.idx, N * (size_t)c->L.kmax) || dalloc(c, &c->L.cnt, N) ||
+        dalloc(c, &c->L.overflow, 2))
+        return bail(SPHX_E_OOM);
+    if (dalloc(c, &c->energy_partial, (size_t)energy_blocks(c->cap) * 9)) return bail(SPHX_E_OOM);
+    if (dalloc(c, &c->pair_counter, 1)) return bail(SPHX_E_OOM);
+    launch_fill_u32(c->L.overflow, 0u, 2, c->stream);
+    launch_fill_u32(c->L.cnt, 0u, N, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(SPHX_E_HIP);
+    *out = c;
+    return SPHX_OK;
+}
+
+void sphx_destroy(SphxCtx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& t : c->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
+    free_all(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sphx_set_params(SphxCtx* c, const SphxParams* p) {
+    if (!c) return SPHX_E_INVALID;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    const SphxParams old = c->prm;
+    c->prm = *p;
+    make_kernel_params(c);
+    // a change of grid-relevant parameters needs a new upload
+    bool grid_change = old.boundary_flags != p->boundary_flags || old.cell_size != p->cell_size;
+    for (int k = 0; k < 3; ++k)
+        grid_change |= old.box_lo[k] != p->box_lo[k] || old.box_hi[k] != p->box_hi[k] ||
+                       old.grid_lo[k] != p->grid_lo[k] || old.grid_hi[k] != p->grid_hi[k];
+    if (grid_change && c->n) c->grid_set = false;
+    if (p->max_neighbours > 0 && p->max_neighbours != c->L.kmax)
+        return fail(c, SPHX_E_INVALID, "max_neighbours is fixed at sphx_create");
+    // derived quantities depend on the EOS, viscosity and walls
+    c->dens = c->sp = c->acc = false;
+    return SPHX_OK;
+}
+
+int sphx_upload(SphxCtx* c, const SphxState* h) {
+    if (!c || !h) return SPHX_E_INVALID;
+    if (h->n > c->cap) return fail(c, SPHX_E_CAPACITY, "n = %u exceeds capacity %u", h->n, c->cap);
+    if (h->n && (!h->x || !h->y || !h->z || !h->vx || !h->vy || !h->vz || !h->m || !h->h))
+        return fail(c, SPHX_E_INVALID, "x, y, z, vx, vy, vz, m and h are required");
+    for (uint32_t i = 0; i < h->n; ++i) {
+        const double vals[8] = {h->x[i], h->y[i], h->z[i], h->vx[i], h->vy[i], h->vz[i], h->m[i], h->h[i]};
+        for (double v : vals)
+            if (!std::isfinite(v)) return fail(c, SPHX_E_NONFINITE, "non-finite input at particle %u", i);
+        if (!(h->m[i] > 0.0)) return fail(c, SPHX_E_INVALID, "mass must be > 0 (particle %u)", i);
+        if (!(h->h[i] > 0.0)) return fail(c, SPHX_E_INVALID, "smoothing radius must be > 0 (particle %u)", i);
+    }
+    const bool mhd = h->bx && h->by && h->bz;
+    if ((h->bx || h->by || h->bz) && !mhd) return fail(c, SPHX_E_INVALID, "bx, by, bz go together");
+    if (h->q && !mhd) return fail(c, SPHX_E_INVALID, "charge needs a magnetic field");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->n = h->n;
+    int rc = setup_grid(c, h);
+    if (rc) return rc;
+    c->has_mhd = mhd;
+    c->has_charge = h->q != nullptr;
+    c->cur = 0;
+    Particles& P = c->A[0];
+    const size_t bytes = sizeof(double) * h->n;
+    if (h->n) {
+        const double* src[] = {h->x, h->y, h->z, h->vx, h->vy, h->vz, h->m, h->h};
+        double* dst[] = {P.x, P.y, P.z, P.vx, P.vy, P.vz, P.m, P.h};
+        for (int k = 0; k < 8; ++k) HIP_TRY(c, hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyHostToDevice, c->stream));
+        std::vector<double> zero(h->n, 0.0);
+        double* z[] = {P.apx, P.apy, P.apz};
+        for (double* d : z) HIP_TRY(c, hipMemcpyAsync(d, zero.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        if (mhd) {
+            HIP_TRY(c, hipMemcpyAsync(P.bx, h->bx, bytes, hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(P.by, h->by, bytes, hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(P.bz, h->bz, bytes, hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(P.psi, h->psi ? h->psi : zero.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        }
+        if (h->q) HIP_TRY(c, hipMemcpyAsync(P.q, h->q, bytes, hipMemcpyHostToDevice, c->stream));
+        if (h->id) HIP_TRY(c, hipMemcpyAsync(P.id, h->id, sizeof(uint32_t) * h->n, hipMemcpyHostToDevice, c->stream));
+        else launch_iota_u32(P.id, h->n, c->stream);
+        launch_iota_u32(P.orig, h->n, c->stream);
+        HIP_TRY(c, hipStreamSynchronize(c->stream));  // `zero` goes out of scope
+    }
+    launch_fill_u32(c->L.overflow, 0u, 2, c->stream);
+    c->sorted = c->listed = c->dens = c->sp = c->acc = false;
+    return launch_check(c);
+}
+
+int sphx_download(SphxCtx* c, SphxState* h, uint32_t fields) {
+    if (!c || !h) return SPHX_E_INVALID;
+    if (h->n != c->n) return fail(c, SPHX_E_INVALID, "state.n = %u but the context holds %u", h->n, c->n);
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((fields & (SPHX_F_RHO | SPHX_F_P)) && !c->dens)
+        return fail(c, SPHX_E_STATE, "density is not valid for the current positions (call sphx_density)");
+    if ((fields & SPHX_F_ACC) && !c->acc)
+        return fail(c, SPHX_E_STATE, "accelerations are not valid for the current state (call sphx_forces)");
+    if ((fields & (SPHX_F_B | SPHX_F_PSI)) && !c->has_mhd)
+        return fail(c, SPHX_E_STATE, "no magnetic field was uploaded");
+    const uint32_t n = c->n;
+    if (!n) return SPHX_OK;
+    Particles& P = c->P();
+    std::vector<uint32_t> orig(n);
+    HIP_TRY(c, hipMemcpyAsync(orig.data(), P.orig, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> buf(n);
+    auto fetch = [&](const double* dev, double* dst) -> int {
+        if (!dst) return SPHX_OK;
+        HIP_TRY(c, hipMemcpyAsync(buf.data(), dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (uint32_t k = 0; k < n; ++k) dst[orig[k]] = buf[k];
+        return SPHX_OK;
+    };
+    int rc = SPHX_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (fields & SPHX_F_POS) { if ((rc = fetch(P.x, h->x)) || (rc = fetch(P.y, h->y)) || (rc = fetch(P.z, h->z))) return rc; }
+    if (fields & SPHX_F_VEL) { if ((rc = fetch(P.vx, h->vx)) || (rc = fetch(P.vy, h->vy)) || (rc = fetch(P.vz, h->vz))) return rc; }
+    if (fields & SPHX_F_RHO) { if ((rc = fetch(c->D.rho, h->rho))) return rc; }
+    if (fields & SPHX_F_P) { if ((rc = fetch(c->D.p, h->p))) return rc; }
+    if (fields & SPHX_F_ACC) { if ((rc = fetch(c->D.ax, h->ax)) || (rc = fetch(c->D.ay, h->ay)) || (rc = fetch(c->D.az, h->az))) return rc; }
+    if (fields & SPHX_F_B) { if ((rc = fetch(P.bx, h->bx)) || (rc = fetch(P.by, h->by)) || (rc = fetch(P.bz, h->bz))) return rc; }
+    if (fields & SPHX_F_PSI) { if ((rc = fetch(P.psi, h->psi))) return rc; }
+    return SPHX_OK;
+}
+
+#define SPHX_ENTER(c)                                        \
+    if (!(c)) return SPHX_E_INVALID;                         \
+    if (!(c)->grid_set) return fail(c, SPHX_E_STATE, "upload particles first"); \
+    HIP_TRY(c, hipSetDevice((c)->device))
+
+int sphx_neighbours(SphxCtx* c) {
+    SPHX_ENTER(c);
+    return do_neighbours(c);
+}
+int sphx_density(SphxCtx* c) {
+    SPHX_ENTER(c);
+    return do_density(c);
+}
+int sphx_forces(SphxCtx* c, uint32_t flags) {
+    SPHX_ENTER(c);
+    if (flags & ~1u) return fail(c, SPHX_E_INVALID, "unknown force flags 0x%x", flags);
+    return do_forces(c, (flags & SPHX_FORCES_VISC_ONLY) != 0);
+}
+int sphx_vv_position(SphxCtx* c) {
+    SPHX_ENTER(c);
+    return do_vv_position(c);
+}
+int sphx_vv_velocity(SphxCtx* c) {
+    SPHX_ENTER(c);
+    return do_vv_velocity(c);
+}
+int sphx_xsph(SphxCtx* c) {
+    SPHX_ENTER(c);
+    return do_xsph(c);
+}
+int sphx_mhd_induction(SphxCtx* c) {
+    SPHX_ENTER(c);
+    return do_induction(c);
+}
+int sphx_div_clean(SphxCtx* c) {
+    SPHX_ENTER(c);
+    return do_div_clean(c);
+}
+
+int sphx_step(SphxCtx* c, uint32_t nsteps) {
+    SPHX_ENTER(c);
+    int rc;
+    for (uint32_t t = 0; t < nsteps; ++t) {
+        // a_n = A(x_n, v_n): the pressure sum at x_n is still valid from the
+        // previous tick's second evaluation unless positions were replaced.
+        if ((rc = do_forces(c, c->sp && c->dens))) return rc;
+        if ((rc = do_vv_position(c))) return rc;
+        if ((rc = do_neighbours(c))) return rc;
+        if ((rc = do_density(c))) return rc;
+        if ((rc = do_forces(c, false))) return rc;  // a* = A(x_{n+1}, v_n)
+        if ((rc = do_vv_velocity(c))) return rc;
+        if (c->prm.xsph_epsilon != 0.0 && (rc = do_xsph(c))) return rc;
+        if (c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_INDUCTION) && (rc = do_induction(c))) return rc;
+        if (c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_CLEANING) && (rc = do_div_clean(c))) return rc;
+    }
+    return check_overflow(c);
+}
+
+int sphx_synchronize(SphxCtx* c) {
+    if (!c) return SPHX_E_INVALID;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_overflow(c);
+}
+
+int sphx_export_pairs(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t* count) {
+    SPHX_ENTER(c);
+    if (!count) return fail(c, SPHX_E_INVALID, "count is NULL");
+    int rc;
+    if ((rc = ensure_list(c))) return rc;
+    if ((rc = check_overflow(c))) return rc;
+    // every directed neighbour appears once per side: total/2 undirected pairs
+    std::vector<uint32_t> cnt(c->n);
+    if (c->n) HIP_TRY(c, hipMemcpy(cnt.data(), c->L.cnt, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
+    uint64_t total = 0;
+    for (uint32_t v : cnt) total += v;
+    const uint64_t np = total / 2;
+    if (np > c->pair_buf_cap) {
+        if ((rc = dalloc(c, &c->pair_buf, 2 * np))) return rc;
+        c->pair_buf_cap = np;
+    }
+    launch_fill_u32(reinterpret_cast<uint32_t*>(c->pair_counter), 0u, 2, c->stream);
+    launch_export_pairs(c->P(), c->n, c->cap, c->L, c->pair_buf, c->pair_counter, c->pair_buf_cap, c->stream);
+    if ((rc = launch_check(c))) return rc;
+    unsigned long long got = 0;
+    HIP_TRY(c, hipMemcpyAsync(&got, c->pair_counter, sizeof got, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (got != np) return fail(c, SPHX_E_INVALID, "asymmetric neighbour lists: %llu pairs from i<j, %llu expected", got, (unsigned long long)np);
+    *count = np;
+    if (np > cap_pairs) return fail(c, SPHX_E_CAPACITY, "%llu pairs do not fit in %llu", (unsigned long long)np, (unsigned long long)cap_pairs);
+    if (!np) return SPHX_OK;
+    if (!pairs) return fail(c, SPHX_E_INVALID, "pairs is NULL");
+    std::vector<uint64_t> packed(np);
+    std::vector<uint32_t> raw(2 * np);
+    HIP_TRY(c, hipMemcpy(raw.data(), c->pair_buf, sizeof(uint32_t) * 2 * np, hipMemcpyDeviceToHost));
+    for (uint64_t k = 0; k < np; ++k) packed[k] = ((uint64_t)raw[2 * k] << 32) | raw[2 * k + 1];
+    std::sort(packed.begin(), packed.end());
+    for (uint64_t k = 0; k < np; ++k) {
+        pairs[2 * k] = (uint32_t)(packed[k] >> 32);
+        pairs[2 * k + 1] = (uint32_t)packed[k];
+    }
+    return SPHX_OK;
+}
+
+int sphx_enable_timing(SphxCtx* c, int enable) {
+    if (!c) return SPHX_E_INVALID;
+    c->timing = enable != 0;
+    return SPHX_OK;
+}
+
+int sphx_kernel_times(SphxCtx* c, SphxTimes* out, int reset) {
+    if (!c || !out) return SPHX_E_INVALID;
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc = drain_timing(c);
+    if (rc) return rc;
+    out->sort_ms = c->ms[kSort];
+    out->neighbour_ms = c->ms[kNbr];
+    out->density_ms = c->ms[kDensity];
+    out->force_ms = c->ms[kForce];
+    out->visc_only_ms = c->ms[kViscOnly];
+    out->integrate_ms = c->ms[kIntegrate];
+    out->mhd_ms = c->ms[kMhd];
+    for (int k = 0; k < kNumSlots; ++k) out->launches[k] = c->launches[k];
+    if (reset) {
+        for (int k = 0; k < kNumSlots; ++k) { c->ms[k] = 0; c->launches[k] = 0; }
+    }
+    return SPHX_OK;
+}
+
+int sphx_energy(SphxCtx* c, SphxEnergy* out) {
+    SPHX_ENTER(c);
+    if (!out) return fail(c, SPHX_E_INVALID, "out is NULL");
+    int rc;
+    if (!c->dens && (rc = do_density(c))) return rc;
+    const uint32_t nb = energy_blocks(c->n);
+    launch_energy(c->P(), c->D, c->n, c->cap, c->L, c->kp, c->has_mhd, c->energy_partial, nb, c->stream);
+    if ((rc = launch_check(c))) return rc;
+    std::vector<double> part((size_t)nb * 9);
+    HIP_TRY(c, hipMemcpyAsync(part.data(), c->energy_partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double s[9] = {0};
+    for (uint32_t b = 0; b < nb; ++b) {
+        for (int k = 0; k < 8; ++k) s[k] += part[(size_t)b * 9 + k];
+        s[8] = std::max(s[8], part[(size_t)b * 9 + 8]);
+    }
+    out->kinetic = s[0];
+    out->potential = s[1];
+    out->internal = s[2];
+    out->magnetic = s[3];
+    out->momentum[0] = s[4];
+    out->momentum[1] = s[5];
+    out->momentum[2] = s[6];
+    out->total_neighbours = (uint64_t)s[7];
+    out->max_neighbours = (uint64_t)s[8];
+    return SPHX_OK;
+}
+
+}  // extern "C"

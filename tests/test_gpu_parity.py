@@ -1,0 +1,312 @@
+"""Parity of the HIP path (libsphx.so on the GPU) against the CPU oracle.
+
+Bars:
+  * neighbour sets: bit-exact equality with the oracle's pair lists;
+  * one f64 evaluation: rho within 1e-13 relative; accelerations within
+    1e-11 of the largest |a| (gather vs. pair-scatter summation order);
+  * trajectories: stated per test (chaotic growth of summation-order noise);
+  * integrators given identical accelerations: bit-exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sphx
+from sphx import scenarios
+
+pytestmark = pytest.mark.gpu
+
+
+def make_ctx(state, params, cap=None):
+    ctx = sphx.Context(cap or max(len(state["x"]), 1), params)
+    ctx.upload(state)
+    return ctx
+
+
+def with_velocity(state, sigma, seed=7):
+    rng = scenarios.SplitMix64(seed)
+    n = len(state["x"])
+    s = dict(state)
+    s["vx"], s["vy"], s["vz"] = sigma * rng.normal(n), sigma * rng.normal(n), sigma * rng.normal(n)
+    return s
+
+
+def oracle_pairs_ids(state, params, brute=False):
+    s = O.State(state)
+    return O.id_pairs(s, O.pairs(s, params, brute=brute))
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ------------------------------------------------------------- neighbour sets
+def test_pairs_bit_exact_dam_break(require_gpu):
+    st, p = scenarios.dam_break()  # 16,384 particles
+    with make_ctx(st, p) as ctx:
+        got = ctx.export_pairs()
+    ref = oracle_pairs_ids(st, p)
+    assert len(ref) > 500_000
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_pairs_bit_exact_random_variable_h(require_gpu, periodic):
+    rng = scenarios.SplitMix64(21)
+    n = 3000
+    st = {k: rng.uniform(n) for k in ("x", "y", "z")}
+    st.update(vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n), m=np.ones(n),
+              h=0.02 + 0.03 * rng.uniform(n), id=np.argsort(rng.uniform(n)).astype(np.uint32))
+    kw = dict(boundary_flags=sphx.BND_PERIODIC, box_lo=(0, 0, 0), box_hi=(1, 1, 1)) if periodic else {}
+    p = sphx.make_params(1e-4, **kw)
+    with make_ctx(st, p) as ctx:
+        got = ctx.export_pairs()
+    ref = oracle_pairs_ids(st, p, brute=True)
+    assert len(ref) > 5000
+    assert np.array_equal(got, ref)
+
+
+def test_pairs_exactly_at_cutoff(require_gpu):
+    x, y, z = scenarios.lattice(6, 6, 6, 1.0)
+    n = x.size
+    base = dict(x=x, y=y, z=z, vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n), m=np.ones(n))
+    p = sphx.make_params(1e-3)
+    for h, expect in ((0.5, 0), (0.5 * (1 + 2 ** -52), 3 * 6 * 6 * 5)):
+        st = dict(base, h=np.full(n, h))
+        with make_ctx(st, p) as ctx:
+            got = ctx.export_pairs()
+        assert len(got) == expect
+        assert np.array_equal(got, oracle_pairs_ids(st, p, brute=True).reshape(-1, 2))
+
+
+# ------------------------------------------------------------- one evaluation
+def test_density_parity_dam_break(require_gpu):
+    st, p = scenarios.dam_break()
+    with make_ctx(st, p) as ctx:
+        ctx.density()
+        g = ctx.download(sphx.F_RHO | sphx.F_P)
+    s = O.State(st)
+    O.density(s, p, O.pairs(s, p))
+    assert rel_err(g["rho"], s["rho"]) < 1e-13
+    assert np.abs(g["p"] - s["p"]).max() <= 1e-11 * max(s["p"].max(), 1.0)
+
+
+@pytest.mark.parametrize("conv", [sphx.VISC_REFERENCE, sphx.VISC_MONAGHAN])
+def test_force_parity_dam_break(require_gpu, conv):
+    st, p = scenarios.dam_break(nx=24, ny=24, nz=24)
+    st = with_velocity(st, 0.3)
+    # compress slightly so Tait pressures are non-zero everywhere inside
+    st["m"] = st["m"] * 1.05
+    p.visc_alpha, p.visc_beta, p.visc_convention = 0.5, 0.3, conv
+    with make_ctx(st, p) as ctx:
+        ctx.forces()
+        g = ctx.download(sphx.F_ACC | sphx.F_RHO)
+    s = O.State(st)
+    O.evaluate(s, p)
+    assert rel_err(g["rho"], s["rho"]) < 1e-13
+    for k in ("ax", "ay", "az"):
+        scale = np.abs(s[k]).max()
+        assert scale > 1.0
+        assert np.abs(g[k] - s[k]).max() < 1e-11 * scale, k
+
+
+def test_visc_only_reuse_is_bit_identical(require_gpu):
+    st, p = scenarios.dam_break()
+    st = with_velocity(st, 0.2)
+    with make_ctx(st, p) as ctx:
+        ctx.forces()
+        a1 = ctx.download(sphx.F_ACC)
+        ctx.forces(visc_only=True)
+        a2 = ctx.download(sphx.F_ACC)
+    for k in a1:
+        assert np.array_equal(a1[k], a2[k])
+
+
+def test_penalty_walls_lorentz_parity(require_gpu):
+    st, p = scenarios.dam_break(nx=16, ny=16, nz=16, spacing=0.05)
+    n = len(st["x"])
+    st = with_velocity(st, 0.5)
+    rng = scenarios.SplitMix64(9)
+    st.update(bx=rng.normal(n), by=rng.normal(n), bz=rng.normal(n), q=rng.normal(n) * 1e-3)
+    p.boundary_flags = sphx.BND_PENALTY_CUBE
+    p.half_extent, p.activation_distance, p.stiffness = 0.45, 0.1, 50.0
+    p.mhd_flags = sphx.MHD_LORENTZ_CHARGE
+    p.e_field = sphx._abi.D3(0.1, -0.2, 0.3)
+    with make_ctx(st, p) as ctx:
+        ctx.forces()
+        g = ctx.download(sphx.F_ACC)
+    s = O.State(st)
+    O.evaluate(s, p)
+    for k in ("ax", "ay", "az"):
+        assert np.abs(g[k] - s[k]).max() < 1e-11 * np.abs(s[k]).max(), k
+
+
+# ------------------------------------------------------------- MHD
+def test_mhd_passes_parity(require_gpu):
+    st, p = scenarios.mhd_periodic(16)
+    p.resistivity = 1e-3  # exercise the resistive term too
+    with make_ctx(st, p) as ctx:
+        ctx.forces()
+        a = ctx.download(sphx.F_ACC | sphx.F_RHO)
+        ctx.mhd_induction()
+        b1 = ctx.download(sphx.F_B)
+        ctx.div_clean()
+        b2 = ctx.download(sphx.F_B | sphx.F_PSI)
+    s = O.State(st)
+    pr = O.pairs(s, p)
+    O.evaluate(s, p, pr)
+    assert rel_err(a["rho"], s["rho"]) < 1e-13
+    for k in ("ax", "ay", "az"):
+        assert np.abs(a[k] - s[k]).max() < 1e-11 * np.abs(s[k]).max(), k
+    b0 = {k: st[k].copy() for k in ("bx", "by", "bz")}
+    O.mhd_induction(s, p, pr)
+    for k in ("bx", "by", "bz"):
+        d_ref = s[k] - b0[k]
+        assert np.abs(b1[k] - s[k]).max() < 1e-12 * np.abs(d_ref).max() + 1e-15, k
+    psi0 = s["psi"].copy()
+    bb = {k: s[k].copy() for k in ("bx", "by", "bz")}
+    O.div_clean(s, p, pr)
+    for k in ("bx", "by", "bz"):
+        assert np.abs(b2[k] - s[k]).max() < 1e-12 * max(np.abs(s[k] - bb[k]).max(), 1e-300) + 1e-15, k
+    assert np.abs(b2["psi"] - s["psi"]).max() < 1e-12 * np.abs(s["psi"] - psi0).max()
+
+
+def test_xsph_parity(require_gpu):
+    st, p = scenarios.dam_break(nx=16, ny=16, nz=16)
+    st = with_velocity(st, 0.4)
+    p.xsph_epsilon = 0.5
+    with make_ctx(st, p) as ctx:
+        ctx.density()
+        ctx.xsph()
+        g = ctx.download(sphx.F_VEL)
+    s = O.State(st)
+    pr = O.pairs(s, p)
+    O.density(s, p, pr)
+    v0 = {k: s[k].copy() for k in ("vx", "vy", "vz")}
+    O.xsph(s, p, pr)
+    for k in ("vx", "vy", "vz"):
+        assert np.abs(g[k] - s[k]).max() < 1e-12 * np.abs(s[k] - v0[k]).max(), k
+
+
+# ------------------------------------------------------------- trajectories
+def test_step_parity_dam_break(require_gpu):
+    st, p = scenarios.dam_break()
+    nsteps = 20
+    with make_ctx(st, p) as ctx:
+        ctx.step(nsteps)
+        g = ctx.download(sphx.F_POS | sphx.F_VEL)
+    s = O.State(st)
+    O.step(s, p, nsteps)
+    for k in ("x", "y", "z"):
+        assert np.abs(g[k] - s[k]).max() < 1e-12, k
+    for k in ("vx", "vy", "vz"):
+        assert np.abs(g[k] - s[k]).max() < 1e-9 * max(np.abs(s[k]).max(), 1e-3), k
+
+
+def test_step_parity_mhd(require_gpu):
+    st, p = scenarios.mhd_periodic(16)
+    nsteps = 5
+    with make_ctx(st, p) as ctx:
+        ctx.step(nsteps)
+        g = ctx.download(sphx.F_POS | sphx.F_VEL | sphx.F_B | sphx.F_PSI)
+    s = O.State(st)
+    O.step(s, p, nsteps)
+    for k in ("x", "y", "z"):
+        d = np.abs(g[k] - s[k])
+        d = np.minimum(d, 1.0 - d)  # periodic wrap
+        assert d.max() < 1e-12, k
+    for k in ("vx", "vy", "vz", "bx", "by", "bz"):
+        assert np.abs(g[k] - s[k]).max() < 1e-10 * np.abs(s[k]).max(), k
+    assert np.abs(g["psi"] - s["psi"]).max() < 1e-8 * max(np.abs(s["psi"]).max(), 1e-12)
+
+
+def test_free_fall_matches_oracle_bitwise(require_gpu):
+    # isolated particles: no pair terms, so the integrator path is bit-exact
+    n = 64
+    rng = scenarios.SplitMix64(4)
+    st = dict(x=rng.uniform(n) * 10, y=rng.uniform(n) * 10, z=rng.uniform(n) * 10, vx=rng.normal(n),
+              vy=rng.normal(n), vz=rng.normal(n), m=np.ones(n), h=np.full(n, 1e-3))
+    p = sphx.make_params(1e-3, gravity=(0.1, -0.2, -9.81), boundary_flags=sphx.BND_REFLECT,
+                         box_lo=(0, 0, 0), box_hi=(10, 10, 10))
+    with make_ctx(st, p) as ctx:
+        ctx.step(500)
+        g = ctx.download(sphx.F_POS | sphx.F_VEL)
+    s = O.State(st)
+    O.step(s, p, 500)
+    for k in g:
+        assert np.array_equal(g[k], s[k]), k
+
+
+# ------------------------------------------------------------- edge cases
+def test_empty_and_single_particle(require_gpu):
+    p = sphx.make_params(1e-4, gravity=(0, 0, -9.81))
+    empty = {k: np.zeros(0) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    with make_ctx(empty, p, cap=8) as ctx:
+        ctx.step(3)
+        assert len(ctx.export_pairs()) == 0
+        assert ctx.download(sphx.F_POS)["x"].size == 0
+    one = {k: np.array([0.5]) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    with make_ctx(one, p) as ctx:
+        ctx.step(10)
+        g = ctx.download(sphx.F_POS)
+    s = O.State(one)
+    O.step(s, p, 10)
+    assert np.array_equal(g["z"], s["z"])
+
+
+def test_error_paths(require_gpu):
+    st, p = scenarios.dam_break(nx=8, ny=8, nz=8)
+    with sphx.Context(100, p) as ctx:
+        with pytest.raises(sphx.SphxError) as e:
+            ctx.upload(st)  # 512 > capacity 100
+        assert e.value.code == 3
+        bad = {k: v[:50].copy() for k, v in st.items()}
+        bad["x"][7] = np.nan
+        with pytest.raises(sphx.SphxError) as e:
+            ctx.upload(bad)
+        assert e.value.code == 6
+        with pytest.raises(sphx.SphxError) as e:
+            ctx.forces(visc_only=True)
+        assert e.value.code == 8
+    p.max_neighbours = 8  # lattice interior has 80
+    with sphx.Context(len(st["x"]), p) as ctx:
+        ctx.upload(st)
+        with pytest.raises(sphx.SphxError) as e:
+            ctx.step(1)
+        assert e.value.code == 3
+
+
+def test_determinism(require_gpu):
+    st, p = scenarios.dam_break()
+    out = []
+    for _ in range(2):
+        with make_ctx(st, p) as ctx:
+            ctx.step(10)
+            out.append(ctx.download(sphx.F_POS | sphx.F_VEL))
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k])
+
+
+# ------------------------------------------------------------- full size
+def test_full_size_lattice_properties(require_gpu):
+    # 1,048,576 particles without jitter: every interior particle has 80
+    # neighbours and rho = 0.99726183 m / dx^3; pair lists are symmetric.
+    dx = 0.01
+    st, p = scenarios.dam_break(nx=128, ny=128, nz=64, spacing=dx, box=(3.2, 1.6, 1.6), jitter=0.0)
+    with make_ctx(st, p) as ctx:
+        ctx.density()
+        g = ctx.download(sphx.F_RHO | sphx.F_POS)
+        e = ctx.energy()
+        pairs = ctx.export_pairs()
+    ix = np.rint(g["x"] / dx - 0.5).astype(int)
+    iy = np.rint(g["y"] / dx - 0.5).astype(int)
+    iz = np.rint(g["z"] / dx - 0.5).astype(int)
+    interior = (ix >= 2) & (ix < 126) & (iy >= 2) & (iy < 126) & (iz >= 2) & (iz < 62)
+    m = st["m"][0]
+    assert np.abs(g["rho"][interior] / (0.99726183 * m / dx ** 3) - 1).max() < 1e-7
+    deg = np.bincount(pairs.ravel(), minlength=len(st["x"]))
+    assert (deg[interior] == 80).all()
+    assert e["total_neighbours"] == 2 * len(pairs)
+    assert e["max_neighbours"] == 80
