@@ -58,11 +58,27 @@ struct Sort {
     uint32_t cell_cap;
 };
 
+// Neighbour list entry: bits 0..25 the neighbour's sorted index, bits 26..31
+// its cell offset from the owner's cell, 2 bits per axis (offset + 1).
+static constexpr uint32_t kIdxBits = 26;
+static constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+static constexpr uint32_t kMaxParticles = 1u << kIdxBits;
+
 struct NbrList {
     uint32_t* idx;             // column-major: idx[k * cap + i]
     uint32_t* cnt;             // neighbours of i
     uint32_t* overflow;        // [0]: max count seen, [1]: overflow flag
     int kmax;
+};
+
+// FP32-mixed form: pair-side copies in f32.  Positions are offsets from the
+// particle's own cell origin (computed in f64, rounded once), so a pair vector
+// is (xr_j - xr_i) + (cell offset) * w, never a difference of large numbers.
+struct Mixed {
+    float *xr, *yr, *zr;
+    float *vx, *vy, *vz;
+    float *m, *h;
+    float *rho, *pr2;
 };
 
 struct KernelParams {          // what the kernels need from SphxParams, by value
@@ -84,22 +100,22 @@ void launch_iota_u32(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_cell_hash(const Particles& P, uint32_t n, const Grid& g, const Sort& S, hipStream_t s);
 void launch_scan(const Sort& S, uint32_t ncell, hipStream_t s);
 void launch_scatter(const Sort& S, uint32_t n, hipStream_t s);
-void launch_cell_order(const Particles& P, const Sort& S, uint32_t ncell, hipStream_t s);
+void launch_cell_order(const Sort& S, uint32_t ncell, hipStream_t s);
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n,
-                    bool mhd, bool charge, hipStream_t s);
+                    bool mhd, bool charge, bool mixed, const Mixed& X, const Grid& g, hipStream_t s);
 void launch_build_nbr(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                      const NbrList& L, hipStream_t s);
+                      const NbrList& L, bool mixed, const Mixed& X, hipStream_t s);
 void launch_density(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                    const NbrList& L, const KernelParams& kp, hipStream_t s);
+                    const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
                    const NbrList& L, const KernelParams& kp, bool visc_only, bool mhd_stress,
-                   bool charge, hipStream_t s);
+                   bool charge, bool mixed, const Mixed& X, hipStream_t s);
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         hipStream_t s);
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
-                        hipStream_t s);
+                        bool mixed, const Mixed& X, hipStream_t s);
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                 const NbrList& L, const KernelParams& kp, hipStream_t s);
+                 const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
                       const NbrList& L, const KernelParams& kp, hipStream_t s);
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,

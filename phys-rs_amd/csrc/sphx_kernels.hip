@@ -222,30 +222,41 @@ __global__ void k_cell_order(const uint32_t* __restrict__ start, uint32_t ncell,
         perm[t] = v;
     }
 }
-void launch_cell_order(const Particles& P, const Sort& S, uint32_t ncell, hipStream_t s) {
-    (void)P;
+void launch_cell_order(const Sort& S, uint32_t ncell, hipStream_t s) {
     hipLaunchKernelGGL(k_cell_order, dim3(nblocks(ncell)), dim3(kBlock), 0, s, S.cell_start, ncell,
                        S.perm);
 }
 
+__device__ __forceinline__ void decode_cell(uint32_t key, const Grid& g, int& cx, int& cy, int& cz) {
+    const uint32_t nx = (uint32_t)g.dims[0], ny = (uint32_t)g.dims[1];
+    cx = (int)(key % nx);
+    const uint32_t t = key / nx;
+    cy = (int)(t % ny);
+    cz = (int)(t / ny);
+}
+
 __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restrict__ perm,
                           const uint32_t* __restrict__ key, uint32_t* __restrict__ skey, uint32_t n,
-                          int mhd, int charge) {
+                          int mhd, int charge, int mixed, Mixed X, Grid g) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t i = perm[k];
-    skey[k] = key[i];
-    dst.x[k] = src.x[i];
-    dst.y[k] = src.y[i];
-    dst.z[k] = src.z[i];
-    dst.vx[k] = src.vx[i];
-    dst.vy[k] = src.vy[i];
-    dst.vz[k] = src.vz[i];
+    const uint32_t c = key[i];
+    skey[k] = c;
+    const double x = src.x[i], y = src.y[i], z = src.z[i];
+    const double vx = src.vx[i], vy = src.vy[i], vz = src.vz[i];
+    const double m = src.m[i], h = src.h[i];
+    dst.x[k] = x;
+    dst.y[k] = y;
+    dst.z[k] = z;
+    dst.vx[k] = vx;
+    dst.vy[k] = vy;
+    dst.vz[k] = vz;
     dst.apx[k] = src.apx[i];
     dst.apy[k] = src.apy[i];
     dst.apz[k] = src.apz[i];
-    dst.m[k] = src.m[i];
-    dst.h[k] = src.h[i];
+    dst.m[k] = m;
+    dst.h[k] = h;
     dst.id[k] = src.id[i];
     dst.orig[k] = src.orig[i];
     if (mhd) {
@@ -255,26 +266,37 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
         dst.psi[k] = src.psi[i];
     }
     if (charge) dst.q[k] = src.q[i];
+    if (mixed) {
+        int cx, cy, cz;
+        decode_cell(c, g, cx, cy, cz);
+        X.xr[k] = (float)(x - (g.lo[0] + cx * g.w[0]));
+        X.yr[k] = (float)(y - (g.lo[1] + cy * g.w[1]));
+        X.zr[k] = (float)(z - (g.lo[2] + cz * g.w[2]));
+        X.vx[k] = (float)vx;
+        X.vy[k] = (float)vy;
+        X.vz[k] = (float)vz;
+        X.m[k] = (float)m;
+        X.h[k] = (float)h;
+    }
 }
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n, bool mhd,
-                    bool charge, hipStream_t s) {
+                    bool charge, bool mixed, const Mixed& X, const Grid& g, hipStream_t s) {
     if (n)
         hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, S.perm, S.key,
-                           S.skey, n, (int)mhd, (int)charge);
+                           S.skey, n, (int)mhd, (int)charge, (int)mixed, X, g);
 }
 
 // ---------------------------------------------- neighbour lists (a1, a2)
-// Visits the particles of the 27 cells around particle i's cell; with x the
-// fastest cell axis, the three cells of one (y, z) row are one contiguous
-// slice of the sorted arrays.  Periodic rows that wrap split in two slices.
+// Visits the 27 cells around particle i's cell in (z, y, x) order, calling
+// f(j, code) for every particle j of each; code packs the cell offset
+// (ox+1) | (oy+1) << 2 | (oz+1) << 4.  With x the fastest cell axis the
+// candidates come in ascending sorted index within each (y, z) row.
 template <typename F>
 __device__ __forceinline__ void for_each_candidate(const Grid& g, const uint32_t* __restrict__ start,
                                                    uint32_t key, F&& f) {
     const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
-    const int cx = (int)(key % (uint32_t)nx);
-    const uint32_t t = key / (uint32_t)nx;
-    const int cy = (int)(t % (uint32_t)ny);
-    const int cz = (int)(t / (uint32_t)ny);
+    int cx, cy, cz;
+    decode_cell(key, g, cx, cy, cz);
     for (int oz = -1; oz <= 1; ++oz) {
         int z = cz + oz;
         if (g.periodic) z = (z + nz) % nz;
@@ -284,22 +306,13 @@ __device__ __forceinline__ void for_each_candidate(const Grid& g, const uint32_t
             if (g.periodic) y = (y + ny) % ny;
             else if (y < 0 || y >= ny) continue;
             const uint32_t row = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
-            if (!g.periodic || (cx > 0 && cx < nx - 1)) {
-                const int x0 = max(cx - 1, 0), x1 = min(cx + 1, nx - 1);
-                const uint32_t b = start[row + x0], e = start[row + x1 + 1];
-                for (uint32_t j = b; j < e; ++j) f(j);
-            } else if (cx == 0) {
-                uint32_t b = start[row + nx - 1], e = start[row + nx];
-                for (uint32_t j = b; j < e; ++j) f(j);
-                b = start[row];
-                e = start[row + 2];
-                for (uint32_t j = b; j < e; ++j) f(j);
-            } else {  // cx == nx - 1
-                uint32_t b = start[row], e = start[row + 1];
-                for (uint32_t j = b; j < e; ++j) f(j);
-                b = start[row + nx - 2];
-                e = start[row + nx];
-                for (uint32_t j = b; j < e; ++j) f(j);
+            for (int ox = -1; ox <= 1; ++ox) {
+                int x = cx + ox;
+                if (g.periodic) x = (x + nx) % nx;
+                else if (x < 0 || x >= nx) continue;
+                const uint32_t code = (uint32_t)(ox + 1) | ((uint32_t)(oy + 1) << 2) | ((uint32_t)(oz + 1) << 4);
+                const uint32_t b = start[row + x], e = start[row + x + 1];
+                for (uint32_t j = b; j < e; ++j) f(j, code);
             }
         }
     }
@@ -324,7 +337,16 @@ __device__ __forceinline__ bool exact_neighbour(double dx, double dy, double dz,
     return sqrt(r2) < t;
 }
 
-__global__ void __launch_bounds__(kBlock) k_build_nbr(Particles P, uint32_t n, uint32_t cap, Grid g,
+// Cell offset code -> f32 offset of the neighbour's cell origin.
+__device__ __forceinline__ void code_offset(uint32_t code, float wx, float wy, float wz, float& ox,
+                                            float& oy, float& oz) {
+    ox = (float)((int)(code & 3u) - 1) * wx;
+    oy = (float)((int)((code >> 2) & 3u) - 1) * wy;
+    oz = (float)((int)((code >> 4) & 3u) - 1) * wz;
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kBlock) k_build_nbr(Particles P, Mixed X, uint32_t n, uint32_t cap, Grid g,
                                                        const uint32_t* __restrict__ key,
                                                        const uint32_t* __restrict__ start,
                                                        uint32_t* __restrict__ idx, uint32_t* __restrict__ cnt,
@@ -332,16 +354,32 @@ __global__ void __launch_bounds__(kBlock) k_build_nbr(Particles P, uint32_t n, u
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     uint32_t count = 0;
     if (i < n) {
-        const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
-        for_each_candidate(g, start, key[i], [&](uint32_t j) {
-            if (j == i) return;
-            double dx, dy, dz;
-            pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
-            if (exact_neighbour(dx, dy, dz, hi, P.h[j])) {
-                if (count < (uint32_t)kmax) idx[(size_t)count * cap + i] = j;
-                ++count;
-            }
-        });
+        auto push = [&](uint32_t j, uint32_t code) {
+            if (count < (uint32_t)kmax) idx[(size_t)count * cap + i] = j | (code << kIdxBits);
+            ++count;
+        };
+        if (!MIXED) {
+            const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
+            for_each_candidate(g, start, key[i], [&](uint32_t j, uint32_t code) {
+                if (j == i) return;
+                double dx, dy, dz;
+                pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+                if (exact_neighbour(dx, dy, dz, hi, P.h[j])) push(j, code);
+            });
+        } else {
+            // f32 predicate on cell-relative coordinates; d_ji = -d_ij exactly,
+            // so the lists stay symmetric.
+            const float xi = X.xr[i], yi = X.yr[i], zi = X.zr[i], hi = X.h[i];
+            const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
+            for_each_candidate(g, start, key[i], [&](uint32_t j, uint32_t code) {
+                if (j == i) return;
+                float ox, oy, oz;
+                code_offset(code, wx, wy, wz, ox, oy, oz);
+                const float dx = (X.xr[j] - xi) + ox, dy = (X.yr[j] - yi) + oy, dz = (X.zr[j] - zi) + oz;
+                const float t = 2.0f * (0.5f * (hi + X.h[j]));
+                if ((dx * dx + dy * dy) + dz * dz < t * t) push(j, code);
+            });
+        }
         cnt[i] = min(count, (uint32_t)kmax);
     }
     const uint32_t wmax = wave_max_u32(count);
@@ -351,9 +389,13 @@ __global__ void __launch_bounds__(kBlock) k_build_nbr(Particles P, uint32_t n, u
     }
 }
 void launch_build_nbr(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                      const NbrList& L, hipStream_t s) {
-    if (n)
-        hipLaunchKernelGGL(k_build_nbr, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, cap, g, S.skey,
+                      const NbrList& L, bool mixed, const Mixed& X, hipStream_t s) {
+    if (!n) return;
+    if (mixed)
+        hipLaunchKernelGGL(k_build_nbr<true>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, X, n, cap, g, S.skey,
+                           S.cell_start, L.idx, L.cnt, L.overflow, L.kmax);
+    else
+        hipLaunchKernelGGL(k_build_nbr<false>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, X, n, cap, g, S.skey,
                            S.cell_start, L.idx, L.cnt, L.overflow, L.kmax);
 }
 
@@ -382,7 +424,7 @@ __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint
     double rho = P.m[i] * (1.0 / (kPi * ((hi * hi) * hi)));
     const uint32_t c = cnt[i];
     for (uint32_t k = 0; k < c; ++k) {
-        const uint32_t j = idx[(size_t)k * cap + i];
+        const uint32_t j = idx[(size_t)k * cap + i] & kIdxMask;
         double dx, dy, dz;
         pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
         const double r = sqrt((dx * dx + dy * dy) + dz * dz);
@@ -393,11 +435,73 @@ __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint
     D.p[i] = p;
     D.pr2[i] = p / (rho * rho);
 }
+
+// f32 cubic spline and derivative (FP32-mixed form)
+__device__ __forceinline__ float kernel_w_f(float r, float inv_h) {
+    const float q = r * inv_h;
+    const float sigma = (inv_h * inv_h) * inv_h * (float)kInvPi;
+    if (q < 1.0f) return sigma * fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
+    if (q < 2.0f) {
+        const float t = 2.0f - q;
+        return (0.25f * sigma) * ((t * t) * t);
+    }
+    return 0.0f;
+}
+__device__ __forceinline__ float kernel_dw_f(float r, float inv_h) {
+    const float q = r * inv_h;
+    const float ih2 = inv_h * inv_h;
+    const float sigma = (ih2 * ih2) * (float)kInvPi;
+    if (q < 1.0f) return sigma * (q * fmaf(2.25f, q, -3.0f));
+    if (q < 2.0f) {
+        const float t = 2.0f - q;
+        return sigma * (-0.75f * (t * t));
+    }
+    return 0.0f;
+}
+__device__ __forceinline__ float inv_h_pair_f(float hi, float hj, float inv_hi) {
+    return (hi == hj) ? inv_hi : __builtin_amdgcn_rcpf(0.5f * (hi + hj));
+}
+
+// FP32-mixed density: f32 pair terms, f64 sum; rho, P and P/rho^2 in f64
+// with f32 copies for the force pass.
+__global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D, Mixed X, uint32_t n,
+                                                           uint32_t cap, Grid g,
+                                                           const uint32_t* __restrict__ idx,
+                                                           const uint32_t* __restrict__ cnt, KernelParams kp) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double hd = P.h[i];
+    double rho = P.m[i] * (1.0 / (kPi * ((hd * hd) * hd)));
+    const float xi = X.xr[i], yi = X.yr[i], zi = X.zr[i], hi = X.h[i];
+    const float inv_hi = 1.0f / hi;
+    const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
+    const uint32_t c = cnt[i];
+    for (uint32_t k = 0; k < c; ++k) {
+        const uint32_t e = idx[(size_t)k * cap + i];
+        const uint32_t j = e & kIdxMask;
+        float ox, oy, oz;
+        code_offset(e >> kIdxBits, wx, wy, wz, ox, oy, oz);
+        const float dx = (X.xr[j] - xi) + ox, dy = (X.yr[j] - yi) + oy, dz = (X.zr[j] - zi) + oz;
+        const float r = __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        rho += (double)(X.m[j] * kernel_w_f(r, inv_h_pair_f(hi, X.h[j], inv_hi)));
+    }
+    const double p = eos_pressure(kp, rho);
+    const double pr2 = p / (rho * rho);
+    D.rho[i] = rho;
+    D.p[i] = p;
+    D.pr2[i] = pr2;
+    X.rho[i] = (float)rho;
+    X.pr2[i] = (float)pr2;
+}
+
 void launch_density(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                    const NbrList& L, const KernelParams& kp, hipStream_t s) {
-    if (n)
-        hipLaunchKernelGGL(k_density, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, cap, g, L.idx,
+                    const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
+    if (!n) return;
+    if (mixed)
+        hipLaunchKernelGGL(k_density_mixed, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, X, n, cap, g, L.idx,
                            L.cnt, kp);
+    else
+        hipLaunchKernelGGL(k_density, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, cap, g, L.idx, L.cnt, kp);
 }
 
 // ------------------------- pressure + viscosity + Maxwell stress + walls (a6-a9, a13)
@@ -424,7 +528,7 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
     double fmx = 0, fmy = 0, fmz = 0;  // Maxwell stress
     const uint32_t c = cnt[i];
     for (uint32_t k = 0; k < c; ++k) {
-        const uint32_t j = idx[(size_t)k * cap + i];
+        const uint32_t j = idx[(size_t)k * cap + i] & kIdxMask;
         double dx, dy, dz;
         pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
         const double r2 = (dx * dx + dy * dy) + dz * dz;
@@ -501,15 +605,126 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
     D.az[i] = F[2] / mi;
 }
 
+// FP32-mixed force pass: pair terms in f32 on cell-relative coordinates, the
+// pressure / viscosity / stress sums in f64; same structure and epilogue as
+// k_forces, so VISC_ONLY reuse stays bit-identical within this form.
+template <bool VISC_ONLY, bool STRESS, bool CHARGE>
+__global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D, Mixed X, uint32_t n,
+                                                          uint32_t cap, Grid g,
+                                                          const uint32_t* __restrict__ idx,
+                                                          const uint32_t* __restrict__ cnt, KernelParams kp) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float xi = X.xr[i], yi = X.yr[i], zi = X.zr[i], hi = X.h[i];
+    const float vxi = X.vx[i], vyi = X.vy[i], vzi = X.vz[i];
+    const float mi = X.m[i], rhoi = X.rho[i], pr2i = X.pr2[i];
+    const float inv_hi = 1.0f / hi;
+    const float ac = (float)(-kp.alpha * kp.speed_of_sound);
+    const float beta = (float)kp.beta;
+    const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
+    const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
+    float bxi = 0, byi = 0, bzi = 0, irho2i = 0, hb2i = 0;
+    if (STRESS) {
+        bxi = (float)P.bx[i]; byi = (float)P.by[i]; bzi = (float)P.bz[i];
+        irho2i = 1.0f / (rhoi * rhoi);
+        hb2i = 0.5f * ((bxi * bxi + byi * byi) + bzi * bzi);
+    }
+    double fpx = 0, fpy = 0, fpz = 0;
+    double fvx = 0, fvy = 0, fvz = 0;
+    double fmx = 0, fmy = 0, fmz = 0;
+    const uint32_t c = cnt[i];
+    for (uint32_t k = 0; k < c; ++k) {
+        const uint32_t e = idx[(size_t)k * cap + i];
+        const uint32_t j = e & kIdxMask;
+        float ox, oy, oz;
+        code_offset(e >> kIdxBits, wx, wy, wz, ox, oy, oz);
+        const float dx = (X.xr[j] - xi) + ox, dy = (X.yr[j] - yi) + oy, dz = (X.zr[j] - zi) + oz;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (!(r2 > 1e-20f)) continue;  // r <= 1e-10: no force
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * inv_r;
+        const float hj = X.h[j];
+        const float inv_h = inv_h_pair_f(hi, hj, inv_hi);
+        const float dwr = kernel_dw_f(r, inv_h) * inv_r;
+        const float mm = mi * X.m[j];
+        if (!VISC_ONLY) {
+            const float s = (mm * (pr2i + X.pr2[j])) * dwr;
+            fpx += (double)(dx * s);
+            fpy += (double)(dy * s);
+            fpz += (double)(dz * s);
+        }
+        const float sdot = fmaf(vzi - X.vz[j], dz, fmaf(vyi - X.vy[j], dy, (vxi - X.vx[j]) * dx));
+        const bool active = ref_conv ? (sdot < 0.0f) : (sdot > 0.0f);
+        if (active) {
+            const float h_avg = 0.5f * (hi + hj);
+            const float rho_bar = 0.5f * (rhoi + X.rho[j]);
+            const float ss = ref_conv ? sdot : -sdot;
+            const float mu = (h_avg * ss) * __builtin_amdgcn_rcpf(fmaf(0.01f * h_avg, h_avg, r2));
+            const float pi_ij = fmaf(beta * mu, mu, ac * mu) * __builtin_amdgcn_rcpf(rho_bar);
+            const float s = (mm * pi_ij) * dwr;
+            fvx += (double)(dx * s);
+            fvy += (double)(dy * s);
+            fvz += (double)(dz * s);
+        }
+        if (STRESS) {
+            const float gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;
+            const float bxj = (float)P.bx[j], byj = (float)P.by[j], bzj = (float)P.bz[j];
+            const float rhoj = X.rho[j];
+            const float irho2j = 1.0f / (rhoj * rhoj);
+            const float hb2j = 0.5f * ((bxj * bxj + byj * byj) + bzj * bzj);
+            const float big = (bxi * gx + byi * gy) + bzi * gz;
+            const float bjg = (bxj * gx + byj * gy) + bzj * gz;
+            fmx += (double)(mm * ((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j));
+            fmy += (double)(mm * ((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j));
+            fmz += (double)(mm * ((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j));
+        }
+    }
+    const double mid = P.m[i];
+    double sx, sy, sz;
+    if (VISC_ONLY) {
+        sx = D.spx[i]; sy = D.spy[i]; sz = D.spz[i];
+    } else {
+        sx = mid * kp.g[0] + fpx;
+        sy = mid * kp.g[1] + fpy;
+        sz = mid * kp.g[2] + fpz;
+        D.spx[i] = sx; D.spy[i] = sy; D.spz[i] = sz;
+    }
+    const double xd = P.x[i], yd = P.y[i], zd = P.z[i];
+    if (CHARGE) {
+        const double q = P.q[i], vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+        const double bx = P.bx[i], by = P.by[i], bz = P.bz[i];
+        sx += q * (kp.e[0] + (vy * bz - vz * by));
+        sy += q * (kp.e[1] + (vz * bx - vx * bz));
+        sz += q * (kp.e[2] + (vx * by - vy * bx));
+    }
+    double F[3] = {sx + fvx + fmx, sy + fvy + fmy, sz + fvz + fmz};
+    if (kp.bnd & SPHX_BND_PENALTY_CUBE) {
+        const double c3[3] = {xd, yd, zd};
+        for (int a = 0; a < 3; ++a) {
+            const double d_low = c3[a] - (-kp.half_extent);
+            if (d_low < kp.act_dist && d_low >= 0.0) F[a] += kp.stiffness * (kp.act_dist - d_low);
+            const double d_high = kp.half_extent - c3[a];
+            if (d_high < kp.act_dist && d_high >= 0.0) F[a] -= kp.stiffness * (kp.act_dist - d_high);
+        }
+    }
+    D.ax[i] = F[0] / mid;
+    D.ay[i] = F[1] / mid;
+    D.az[i] = F[2] / mid;
+}
+
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
                    const NbrList& L, const KernelParams& kp, bool visc_only, bool stress, bool charge,
-                   hipStream_t s) {
+                   bool mixed, const Mixed& X, hipStream_t s) {
     if (!n) return;
     dim3 grid(nblocks(n)), block(kBlock);
-#define SPHX_FORCES(V, M, C)                                                                       \
-    if (visc_only == V && stress == M && charge == C) {                                           \
-        hipLaunchKernelGGL((k_forces<V, M, C>), grid, block, 0, s, P, D, n, cap, g, L.idx, L.cnt, kp); \
-        return;                                                                                    \
+#define SPHX_FORCES(V, M, C)                                                                           \
+    if (visc_only == V && stress == M && charge == C) {                                               \
+        if (mixed)                                                                                     \
+            hipLaunchKernelGGL((k_forces_mixed<V, M, C>), grid, block, 0, s, P, D, X, n, cap, g, L.idx, \
+                               L.cnt, kp);                                                             \
+        else                                                                                           \
+            hipLaunchKernelGGL((k_forces<V, M, C>), grid, block, 0, s, P, D, n, cap, g, L.idx, L.cnt, kp); \
+        return;                                                                                        \
     }
     SPHX_FORCES(false, false, false)
     SPHX_FORCES(true, false, false)
@@ -558,23 +773,32 @@ void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const 
 }
 
 // velocity_verlet_velocity_system (integrators.rs:92-104)
-__global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams kp) {
+__global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams kp, int mixed, Mixed X) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    P.vx[i] += (P.apx[i] + D.ax[i]) * kp.half_dt;
-    P.vy[i] += (P.apy[i] + D.ay[i]) * kp.half_dt;
-    P.vz[i] += (P.apz[i] + D.az[i]) * kp.half_dt;
+    const double vx = P.vx[i] + (P.apx[i] + D.ax[i]) * kp.half_dt;
+    const double vy = P.vy[i] + (P.apy[i] + D.ay[i]) * kp.half_dt;
+    const double vz = P.vz[i] + (P.apz[i] + D.az[i]) * kp.half_dt;
+    P.vx[i] = vx;
+    P.vy[i] = vy;
+    P.vz[i] = vz;
+    if (mixed) {
+        X.vx[i] = (float)vx;
+        X.vy[i] = (float)vy;
+        X.vz[i] = (float)vz;
+    }
 }
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
-                        hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_vv_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp);
+                        bool mixed, const Mixed& X, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_vv_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, (int)mixed, X);
 }
 
 // ---------------------------------------------------------------- XSPH (a16)
 // sph_xsph_system (sph.rs:349-405); Jacobi: writes the new velocity to tmp3.
 __global__ void __launch_bounds__(kBlock) k_xsph(Particles P, Derived D, uint32_t n, uint32_t cap, Grid g,
                                                   const uint32_t* __restrict__ idx,
-                                                  const uint32_t* __restrict__ cnt, KernelParams kp) {
+                                                  const uint32_t* __restrict__ cnt, KernelParams kp,
+                                                  int mixed, Mixed X) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], inv_hi = 1.0 / hi;
@@ -582,7 +806,7 @@ __global__ void __launch_bounds__(kBlock) k_xsph(Particles P, Derived D, uint32_
     double ux = 0, uy = 0, uz = 0;
     const uint32_t c = cnt[i];
     for (uint32_t k = 0; k < c; ++k) {
-        const uint32_t j = idx[(size_t)k * cap + i];
+        const uint32_t j = idx[(size_t)k * cap + i] & kIdxMask;
         double dx, dy, dz;
         pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
         const double r = sqrt((dx * dx + dy * dy) + dz * dz);
@@ -596,11 +820,17 @@ __global__ void __launch_bounds__(kBlock) k_xsph(Particles P, Derived D, uint32_
     D.tmp3[0][i] = vxi + ux;
     D.tmp3[1][i] = vyi + uy;
     D.tmp3[2][i] = vzi + uz;
+    if (mixed) {  // f32 copies are read by the next force pass, not by this one
+        X.vx[i] = (float)(vxi + ux);
+        X.vy[i] = (float)(vyi + uy);
+        X.vz[i] = (float)(vzi + uz);
+    }
 }
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                 const NbrList& L, const KernelParams& kp, hipStream_t s) {
+                 const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
     if (n)
-        hipLaunchKernelGGL(k_xsph, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, cap, g, L.idx, L.cnt, kp);
+        hipLaunchKernelGGL(k_xsph, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, cap, g, L.idx, L.cnt, kp,
+                           (int)mixed, X);
 }
 
 // ------------------------------------------------------ MHD induction (a11)
@@ -619,7 +849,7 @@ __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, ui
     double ux = 0, uy = 0, uz = 0;
     const uint32_t c = cnt[i];
     for (uint32_t k = 0; k < c; ++k) {
-        const uint32_t j = idx[(size_t)k * cap + i];
+        const uint32_t j = idx[(size_t)k * cap + i] & kIdxMask;
         double dx, dy, dz;
         pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
         const double r2 = (dx * dx + dy * dy) + dz * dz;
@@ -666,7 +896,7 @@ __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, ui
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
     const uint32_t c = cnt[i];
     for (uint32_t k = 0; k < c; ++k) {
-        const uint32_t j = idx[(size_t)k * cap + i];
+        const uint32_t j = idx[(size_t)k * cap + i] & kIdxMask;
         double dx, dy, dz;
         pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
         const double r = sqrt((dx * dx + dy * dy) + dz * dz);
@@ -703,7 +933,7 @@ __global__ void k_export_pairs(const uint32_t* __restrict__ id, uint32_t n, uint
     const uint32_t idi = id[i];
     const uint32_t c = cnt[i];
     for (uint32_t k = 0; k < c; ++k) {
-        const uint32_t idj = id[idx[(size_t)k * cap + i]];
+        const uint32_t idj = id[idx[(size_t)k * cap + i] & kIdxMask];
         if (idi < idj) {
             unsigned long long pos = atomicAdd(counter, 1ull);
             if (pos < out_cap) {

@@ -44,6 +44,7 @@ struct SphxCtx {
     Derived D{};
     Sort S{};
     NbrList L{};
+    Mixed X{};
     double* energy_partial = nullptr;
     uint32_t* pair_buf = nullptr;
     unsigned long long* pair_counter = nullptr;
@@ -61,6 +62,7 @@ struct SphxCtx {
     std::string err;
 
     Particles& P() { return A[cur]; }
+    bool mixed() const { return prm.precision == SPHX_PREC_FP32_MIXED; }
 };
 
 namespace {
@@ -309,13 +311,13 @@ int do_neighbours(SphxCtx* c) {
         launch_cell_hash(src, n, c->grid, c->S, c->stream);
         launch_scan(c->S, c->grid.ncell, c->stream);
         launch_scatter(c->S, n, c->stream);
-        launch_cell_order(src, c->S, c->grid.ncell, c->stream);
-        launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->stream);
+        launch_cell_order(c->S, c->grid.ncell, c->stream);
+        launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->mixed(), c->X, c->grid, c->stream);
         c->cur = 1 - c->cur;
     }
     {
         Timer t(c, kNbr);
-        launch_build_nbr(c->P(), n, c->cap, c->grid, c->S, c->L, c->stream);
+        launch_build_nbr(c->P(), n, c->cap, c->grid, c->S, c->L, c->mixed(), c->X, c->stream);
     }
     c->sorted = c->listed = true;
     c->dens = c->sp = c->acc = false;
@@ -332,7 +334,7 @@ int do_density(SphxCtx* c) {
     if ((rc = ensure_list(c))) return rc;
     {
         Timer t(c, kDensity);
-        launch_density(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->stream);
+        launch_density(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
     }
     c->dens = true;
     return launch_check(c);
@@ -347,7 +349,8 @@ int do_forces(SphxCtx* c, bool visc_only) {
     const bool charge = c->has_charge && c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_LORENTZ_CHARGE);
     {
         Timer t(c, visc_only ? kViscOnly : kForce);
-        launch_forces(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, visc_only, stress, charge, c->stream);
+        launch_forces(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, visc_only, stress, charge, c->mixed(),
+                      c->X, c->stream);
     }
     c->sp = true;
     c->acc = true;
@@ -368,7 +371,7 @@ int do_vv_velocity(SphxCtx* c) {
     if (!c->acc) return fail(c, SPHX_E_STATE, "vv_velocity needs accelerations (call sphx_forces)");
     {
         Timer t(c, kIntegrate);
-        launch_vv_velocity(c->P(), c->D, c->n, c->kp, c->stream);
+        launch_vv_velocity(c->P(), c->D, c->n, c->kp, c->mixed(), c->X, c->stream);
     }
     c->acc = false;  // a is a function of v, which just changed
     return launch_check(c);
@@ -388,7 +391,7 @@ int do_xsph(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_xsph(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->stream);
+        launch_xsph(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
     }
     Particles& P = c->P();
     swap3(P.vx, P.vy, P.vz, c->D.tmp3);
@@ -495,6 +498,7 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
         return SPHX_E_NODEVICE;
     }
     if (device < 0 || device >= ndev) return SPHX_E_NODEVICE;
+    if (capacity > kMaxParticles) return SPHX_E_CAPACITY;
     SphxCtx* c = new (std::nothrow) SphxCtx();
     if (!c) return SPHX_E_OOM;
     c->device = device;
@@ -524,6 +528,12 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
                      &D.tmp3[0], &D.tmp3[1], &D.tmp3[2], &D.tmp1};
     for (double** d : dd)
         if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
+    {
+        Mixed& X = c->X;
+        float** ff[] = {&X.xr, &X.yr, &X.zr, &X.vx, &X.vy, &X.vz, &X.m, &X.h, &X.rho, &X.pr2};
+        for (float** f : ff)
+            if (dalloc(c, f, N)) return bail(SPHX_E_OOM);
+    }
     if (dalloc(c, &c->S.key, N) || dalloc(c, &c->S.rank, N) || dalloc(c, &c->S.perm, N) ||
         dalloc(c, &c->S.skey, N))
         return bail(SPHX_E_OOM);
@@ -568,8 +578,10 @@ int sphx_set_params(SphxCtx* c, const SphxParams* p) {
     if (grid_change && c->n) c->grid_set = false;
     if (p->max_neighbours > 0 && p->max_neighbours != c->L.kmax)
         return fail(c, SPHX_E_INVALID, "max_neighbours is fixed at sphx_create");
-    // derived quantities depend on the EOS, viscosity and walls
+    // derived quantities depend on the EOS, viscosity and walls; the
+    // precision form also changes the neighbour predicate and f32 copies
     c->dens = c->sp = c->acc = false;
+    if (old.precision != p->precision) c->sorted = c->listed = false;
     return SPHX_OK;
 }
 

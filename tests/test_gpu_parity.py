@@ -310,3 +310,52 @@ def test_full_size_lattice_properties(require_gpu):
     assert (deg[interior] == 80).all()
     assert e["total_neighbours"] == 2 * len(pairs)
     assert e["max_neighbours"] == 80
+
+
+# ------------------------------------------------------------- FP32-mixed form
+def mixed(params):
+    params.precision = sphx.PREC_FP32_MIXED
+    return params
+
+
+def test_mixed_density_and_force_error(require_gpu):
+    # f32 pair terms on cell-relative coordinates, f64 sums: relative density
+    # error <= 1e-6 against the f64 oracle; accelerations within 1e-4 of max|a|.
+    st, p = scenarios.dam_break(nx=24, ny=24, nz=24)
+    st = with_velocity(st, 0.3)
+    st["m"] = st["m"] * 1.05
+    p.visc_alpha, p.visc_beta = 0.5, 0.3
+    with make_ctx(st, mixed(p)) as ctx:
+        ctx.forces()
+        g = ctx.download(sphx.F_ACC | sphx.F_RHO)
+    p.precision = sphx.PREC_FP64
+    s = O.State(st)
+    O.evaluate(s, p)
+    err_rho = np.abs(g["rho"] / s["rho"] - 1).max()
+    assert err_rho < 1e-6, err_rho
+    for k in ("ax", "ay", "az"):
+        assert np.abs(g[k] - s[k]).max() < 1e-4 * np.abs(s[k]).max(), k
+
+
+def test_mixed_visc_only_reuse_bit_identical(require_gpu):
+    st, p = scenarios.dam_break()
+    st = with_velocity(st, 0.2)
+    with make_ctx(st, mixed(p)) as ctx:
+        ctx.forces()
+        a1 = ctx.download(sphx.F_ACC)
+        ctx.forces(visc_only=True)
+        a2 = ctx.download(sphx.F_ACC)
+    for k in a1:
+        assert np.array_equal(a1[k], a2[k])
+
+
+def test_mixed_mhd_periodic_step(require_gpu):
+    st, p = scenarios.mhd_periodic(16)
+    with make_ctx(st, mixed(p)) as ctx:
+        ctx.step(5)
+        g = ctx.download(sphx.F_POS | sphx.F_VEL | sphx.F_B)
+    p.precision = sphx.PREC_FP64
+    s = O.State(st)
+    O.step(s, p, 5)
+    for k in ("vx", "vy", "vz", "bx", "by", "bz"):
+        assert np.abs(g[k] - s[k]).max() < 1e-5 * np.abs(s[k]).max(), k
