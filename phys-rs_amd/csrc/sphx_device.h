@@ -1,0 +1,396 @@
+// sphx_device.h — device helpers shared by the kernel translation units:
+// SPH kernel functions, pair vectors, the EOS, and the tile/stage framework
+// every neighbour pass is built on.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "sphx_internal.h"
+
+namespace sphx {
+
+static constexpr int kBlock = kTile;            // one thread per tile particle
+static constexpr int kStageCap = 384;           // candidates staged in LDS per chunk
+static constexpr double kInvPi = 0.31830988618379067154;  // 1/pi
+static constexpr double kPi = 3.141592653589793;           // std::f64::consts::PI
+
+static inline uint32_t nblocks(size_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
+
+// blockIdx -> logical block so that each XCD (blocks are dealt round-robin to
+// the 8 XCDs) works on one contiguous range of tiles / particles and its L2
+// holds that range's neighbours.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t xcd = b & 7u, local = b >> 3;
+    const uint32_t base = nb >> 3, rem = nb & 7u;
+    return xcd * base + min(xcd, rem) + local;
+}
+
+__device__ __forceinline__ void min_image(double& d, double L, double halfL) {
+    if (d > halfL) d -= L;
+    else if (d < -halfL) d += L;
+}
+
+// x_j - x_i as core/coordinates.rs:51-65 forms it in one sector (0 + (b - a)),
+// with the minimum image in a periodic box.
+__device__ __forceinline__ void pair_vec(const Grid& g, double xi, double yi, double zi, double xj,
+                                         double yj, double zj, double& dx, double& dy, double& dz) {
+    dx = xj - xi;
+    dy = yj - yi;
+    dz = zj - zi;
+    if (g.periodic) {
+        min_image(dx, g.L[0], g.halfL[0]);
+        min_image(dy, g.L[1], g.halfL[1]);
+        min_image(dz, g.L[2], g.halfL[2]);
+    }
+}
+
+__device__ __forceinline__ int cell_coord(double x, double lo, double inv_w, int dim) {
+    double u = (x - lo) * inv_w;
+    u = fmin(fmax(u, 0.0), (double)(dim - 1));
+    return (int)floor(u);
+}
+
+__device__ __forceinline__ void decode_cell(uint32_t key, const Grid& g, int& cx, int& cy, int& cz) {
+    const uint32_t nx = (uint32_t)g.dims[0], ny = (uint32_t)g.dims[1];
+    cx = (int)(key % nx);
+    const uint32_t t = key / nx;
+    cy = (int)(t % ny);
+    cz = (int)(t / ny);
+}
+
+// ---- cubic spline (sph.rs:29-64), f64 given 1/h --------------------------
+__device__ __forceinline__ double kernel_w(double r, double inv_h) {
+    const double q = r * inv_h;
+    const double sigma = (inv_h * inv_h) * inv_h * kInvPi;
+    if (q < 1.0) return sigma * fma(q * q, fma(0.75, q, -1.5), 1.0);
+    if (q < 2.0) {
+        const double t = 2.0 - q;
+        return (0.25 * sigma) * ((t * t) * t);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double kernel_dw(double r, double inv_h) {
+    const double q = r * inv_h;
+    const double ih2 = inv_h * inv_h;
+    const double sigma = (ih2 * ih2) * kInvPi;
+    if (q < 1.0) return sigma * (q * fma(2.25, q, -3.0));
+    if (q < 2.0) {
+        const double t = 2.0 - q;
+        return sigma * (-0.75 * (t * t));
+    }
+    return 0.0;
+}
+
+// d^2 W / dr^2 (for the adjoint of the force pass)
+__device__ __forceinline__ double kernel_d2w(double r, double inv_h) {
+    const double q = r * inv_h;
+    const double ih2 = inv_h * inv_h;
+    const double sigma = (ih2 * ih2) * inv_h * kInvPi;
+    if (q < 1.0) return sigma * fma(4.5, q, -3.0);
+    if (q < 2.0) return sigma * (1.5 * (2.0 - q));
+    return 0.0;
+}
+
+__device__ __forceinline__ double inv_h_pair(double hi, double hj, double inv_hi) {
+    return (hi == hj) ? inv_hi : 1.0 / (0.5 * (hi + hj));
+}
+
+// ---- f64 arithmetic for the pair passes ----------------------------------
+// The passes match the reference to a relative tolerance (the neighbour
+// predicate, which must round exactly, does not use these).  1/x: hardware
+// estimate and two Newton steps (within ~1 ulp).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+// sqrt(x) and 1/sqrt(x) for normal x > 0: Goldschmidt iterations on the
+// hardware rsqrt estimate (two steps take its ~2^-23 error below 1 ulp).
+__device__ __forceinline__ void sqrt_rsqrt(double x, double& sq, double& rs) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    const double d = fma(-g, g, x);
+    sq = fma(d, h, g);
+    rs = 2.0 * h;
+}
+// Cubic-spline shape functions without the normalisation: W = sigma_w * w_shape(q)
+// with sigma_w = 1/(pi h^3); dW/dr = sigma_g * dw_shape(q) with sigma_g = 1/(pi h^4).
+__device__ __forceinline__ double w_shape(double q) {
+    const double t = fmax(2.0 - q, 0.0);
+    const double a = fma(q * q, fma(0.75, q, -1.5), 1.0);
+    const double b = 0.25 * ((t * t) * t);
+    return q < 1.0 ? a : b;
+}
+__device__ __forceinline__ double dw_shape(double q) {
+    const double t = fmax(2.0 - q, 0.0);
+    const double a = q * fma(2.25, q, -3.0);
+    const double b = -0.75 * (t * t);
+    return q < 1.0 ? a : b;
+}
+__device__ __forceinline__ float w_shape_f(float q) {
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float a = fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
+    const float b = 0.25f * ((t * t) * t);
+    return q < 1.0f ? a : b;
+}
+__device__ __forceinline__ float dw_shape_f(float q) {
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float a = q * fmaf(2.25f, q, -3.0f);
+    const float b = -0.75f * (t * t);
+    return q < 1.0f ? a : b;
+}
+
+// ---- f32 forms (FP32-mixed) ---------------------------------------------
+__device__ __forceinline__ float kernel_w_f(float r, float inv_h) {
+    const float q = r * inv_h;
+    const float sigma = (inv_h * inv_h) * inv_h * (float)kInvPi;
+    if (q < 1.0f) return sigma * fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
+    if (q < 2.0f) {
+        const float t = 2.0f - q;
+        return (0.25f * sigma) * ((t * t) * t);
+    }
+    return 0.0f;
+}
+__device__ __forceinline__ float kernel_dw_f(float r, float inv_h) {
+    const float q = r * inv_h;
+    const float ih2 = inv_h * inv_h;
+    const float sigma = (ih2 * ih2) * (float)kInvPi;
+    if (q < 1.0f) return sigma * (q * fmaf(2.25f, q, -3.0f));
+    if (q < 2.0f) {
+        const float t = 2.0f - q;
+        return sigma * (-0.75f * (t * t));
+    }
+    return 0.0f;
+}
+__device__ __forceinline__ float inv_h_pair_f(float hi, float hj, float inv_hi) {
+    return (hi == hj) ? inv_hi : __builtin_amdgcn_rcpf(0.5f * (hi + hj));
+}
+
+// ---- equations of state (sph.rs:157-167; thermodynamics.rs:232-238) -----
+__device__ __forceinline__ double eos_pressure(const KernelParams& kp, double rho) {
+    if (kp.eos == SPHX_EOS_TAIT) {
+        double P = kp.tait_B * (pow(rho / kp.rest_density, kp.gamma) - 1.0);
+        return P < 0.0 ? 0.0 : P;
+    }
+    if (kp.eos == SPHX_EOS_POLYTROPIC) return kp.poly_k * pow(rho, kp.gamma);
+    return rho * kp.iso_cs2;
+}
+// dP/drho (0 where the Tait clamp is active)
+__device__ __forceinline__ double eos_dpdrho(const KernelParams& kp, double rho) {
+    if (kp.eos == SPHX_EOS_TAIT) {
+        const double ratio = rho / kp.rest_density;
+        if (kp.tait_B * (pow(ratio, kp.gamma) - 1.0) < 0.0) return 0.0;
+        return kp.tait_B * kp.gamma * pow(ratio, kp.gamma - 1.0) / kp.rest_density;
+    }
+    if (kp.eos == SPHX_EOS_POLYTROPIC) return kp.poly_k * kp.gamma * pow(rho, kp.gamma - 1.0);
+    return kp.iso_cs2;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+
+// =========================================================================
+// Tile / stage framework.
+//
+// A tile is <= kTile consecutive sorted particles of one cell row (fixed
+// cell y, z) whose cells span [cx_lo, cx_hi].  Stage s = (oy, oz) in
+// {-1,0,1}^2 covers the neighbour row (cy + oy, cz + oz) over cells
+// [cx_lo - 1, cx_hi + 1]: one contiguous slice of the sorted arrays, or two
+// when a periodic row wraps.  Its particles are staged into LDS in chunks of
+// kStageCap; "slot" = position in that staged sequence.  Every value here
+// is uniform across the workgroup.
+// =========================================================================
+struct TileCtx {
+    uint32_t p0, np;
+    int cy, cz, cx_lo, cx_hi;
+};
+
+struct Stage {
+    int nseg;
+    int c00, c01;               // physical cell x range of segment 0 (inclusive)
+    int c10, c11;               // ... of segment 1
+    uint32_t ps0, ps1;          // first particle of each segment
+    uint32_t off1;              // slot of segment 1's first particle (segment 0 starts at 0)
+    uint32_t total;             // staged candidates
+    uint32_t rowbase;           // linear index of cell (0, y, z)
+    float oy, oz;               // stage cell offsets times the cell width (mixed form)
+};
+
+__device__ __forceinline__ uint32_t tile_count(const Tiles& T) { return T.row_off[T.nrows]; }
+
+__device__ __forceinline__ TileCtx load_tile(uint32_t b, const Tiles& T, const Grid& g,
+                                             const uint32_t* __restrict__ skey) {
+    TileCtx tc;
+    tc.p0 = T.p0[b];
+    tc.np = T.np[b];
+    const uint32_t row = T.row[b];
+    const uint32_t ny = (uint32_t)g.dims[1], nx = (uint32_t)g.dims[0];
+    tc.cy = (int)(row % ny);
+    tc.cz = (int)(row / ny);
+    tc.cx_lo = (int)(skey[tc.p0] % nx);
+    tc.cx_hi = (int)(skey[tc.p0 + tc.np - 1] % nx);
+    return tc;
+}
+
+__device__ __forceinline__ void stage_segments(const Grid& g, const uint32_t* __restrict__ start,
+                                               const TileCtx& tc, int s, Stage& st) {
+    const int oy = s % 3 - 1, oz = s / 3 - 1;
+    const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+    st.nseg = 0;
+    st.total = 0;
+    st.off1 = 0;
+    st.ps0 = st.ps1 = 0;
+    st.c00 = st.c01 = st.c10 = st.c11 = 0;
+    st.oy = (float)(oy * g.w[1]);
+    st.oz = (float)(oz * g.w[2]);
+    st.rowbase = 0;
+    int y = tc.cy + oy, z = tc.cz + oz;
+    if (g.periodic) {
+        y = (y + ny) % ny;
+        z = (z + nz) % nz;
+    } else if (y < 0 || y >= ny || z < 0 || z >= nz) {
+        return;
+    }
+    st.rowbase = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+    const int L0 = tc.cx_lo - 1, L1 = tc.cx_hi + 1;
+    if (!g.periodic) {
+        st.nseg = 1; st.c00 = max(L0, 0); st.c01 = min(L1, nx - 1);
+    } else if (L1 - L0 + 1 >= nx) {
+        st.nseg = 1; st.c00 = 0; st.c01 = nx - 1;
+    } else if (L0 < 0) {
+        st.nseg = 2; st.c00 = L0 + nx; st.c01 = nx - 1; st.c10 = 0; st.c11 = L1;
+    } else if (L1 > nx - 1) {
+        st.nseg = 2; st.c00 = L0; st.c01 = nx - 1; st.c10 = 0; st.c11 = L1 - nx;
+    } else {
+        st.nseg = 1; st.c00 = L0; st.c01 = L1;
+    }
+    st.ps0 = start[st.rowbase + st.c00];
+    st.total = start[st.rowbase + st.c01 + 1] - st.ps0;
+    if (st.nseg == 2) {
+        st.ps1 = start[st.rowbase + st.c10];
+        st.off1 = st.total;
+        st.total += start[st.rowbase + st.c11 + 1] - st.ps1;
+    }
+}
+
+// slot -> sorted particle index
+__device__ __forceinline__ uint32_t slot_particle(const Stage& st, uint32_t slot) {
+    return (st.nseg == 2 && slot >= st.off1) ? st.ps1 + (slot - st.off1) : st.ps0 + slot;
+}
+
+// Slots of the cell x = cx + ox of a stage, as [a, b); empty if the cell is
+// outside a closed box.
+__device__ __forceinline__ void cell_slots(const Grid& g, const uint32_t* __restrict__ start, const Stage& st,
+                                           int cx, int ox, uint32_t& a, uint32_t& b) {
+    a = b = 0;
+    if (st.nseg == 0) return;
+    const int nx = g.dims[0];
+    int c = cx + ox;
+    if (g.periodic) c = (c + nx) % nx;
+    else if (c < 0 || c >= nx) return;
+    const bool second = st.nseg == 2 && c >= st.c10 && c <= st.c11;
+    const uint32_t base = second ? st.off1 - st.ps1 : 0u - st.ps0;  // slot = particle + base (mod 2^32)
+    a = start[st.rowbase + c] + base;
+    b = start[st.rowbase + c + 1] + base;
+}
+
+// Drives one neighbour pass over a tile.  For each of the 9 stages:
+// stage_fn(u, j) copies particle j's fields to LDS index u; pair_fn(u, ox, st)
+// processes one list entry.  A stage that fits one LDS chunk (the common case)
+// also copies the first kListRows rows of every thread's list into LDS
+// alongside the fields, so the entry loop reads no global memory; longer
+// lists and multi-chunk stages read the remaining entries from HBM.
+static constexpr int kListRows = 16;
+__device__ __forceinline__ uint16_t* list_base(const NbrList& L, uint32_t b, int s, uint32_t t) {
+    return L.idx + ((size_t)(b * kStages + s) * (uint32_t)L.ks) * kTile + t;
+}
+__device__ __forceinline__ uint8_t* count_ptr(const NbrList& L, uint32_t b, int s, uint32_t t) {
+    return L.cnt + (size_t)(b * kStages + s) * kTile + t;
+}
+
+template <typename StageF, typename PairF>
+__device__ __forceinline__ void tile_pass(const Grid& g, const uint32_t* __restrict__ start, const TileCtx& tc,
+                                          const NbrList& L, uint32_t b, bool valid,
+                                          uint16_t (*lst)[kTile], uint32_t* smax,
+                                          StageF&& stage_fn, PairF&& pair_fn) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t rows = (uint32_t)min(kListRows, L.ks);
+    // the workgroup's largest list per stage bounds the list-row loop below
+    if (t < kStages) smax[t] = 0u;
+    __syncthreads();
+    if (valid)
+        for (int s = 0; s < kStages; ++s) atomicMax(&smax[s], (uint32_t)*count_ptr(L, b, s, t));
+    for (int s = 0; s < kStages; ++s) {
+        Stage st;
+        stage_segments(g, start, tc, s, st);
+        const uint16_t* __restrict__ base = list_base(L, b, s, t);
+        const uint32_t cnt_s = valid ? *count_ptr(L, b, s, t) : 0u;
+        if (st.total <= (uint32_t)kStageCap) {
+            __syncthreads();  // smax complete; the previous stage is done with LDS
+            const uint32_t kmax = smax[s];
+            if (kmax == 0) continue;  // no thread of the tile has a neighbour in this row
+            for (uint32_t kb = 0; kb < kmax; kb += rows) {
+                if (kb) __syncthreads();  // previous list rows consumed
+                else
+                    for (uint32_t u = t; u < st.total; u += kBlock) stage_fn(u, slot_particle(st, u));
+                const uint32_t kend = min(cnt_s, kb + rows);
+                if (valid)
+                    for (uint32_t k = kb; k < kend; ++k) lst[k - kb][t] = base[(size_t)k * kTile];
+                __syncthreads();
+                for (uint32_t k = kb; k < kend; ++k) {
+                    const uint32_t e = lst[k - kb][t];
+                    pair_fn(e & kSlotMask, (int)(e >> kSlotBits) - 1, st);
+                }
+            }
+        } else {
+            uint32_t k = 0;
+            for (uint32_t cb = 0; cb < st.total; cb += kStageCap) {
+                const uint32_t ce = min(st.total, cb + (uint32_t)kStageCap);
+                __syncthreads();
+                for (uint32_t u = t; u < ce - cb; u += kBlock) stage_fn(u, slot_particle(st, cb + u));
+                __syncthreads();
+                while (k < cnt_s) {
+                    const uint32_t e = base[(size_t)k * kTile];
+                    const uint32_t slot = e & kSlotMask;
+                    if (slot >= ce) break;
+                    // lists ascend in slot, so slot >= cb; the test keeps LDS
+                    // reads in bounds even for a list the build flagged
+                    if (slot >= cb) pair_fn(slot - cb, (int)(e >> kSlotBits) - 1, st);
+                    ++k;
+                }
+            }
+        }
+    }
+}
+
+// Iterate over a particle's list entries of stage s whose slots fall in the
+// staged chunk [cb, ce); k carries across chunks.  f(slot - cb, ox).
+template <typename F>
+__device__ __forceinline__ void for_stage_entries(const NbrList& L, uint32_t b, uint32_t t, int s,
+                                                  uint32_t cnt_s, uint32_t& k, uint32_t cb, uint32_t ce, F&& f) {
+    const uint16_t* __restrict__ base = list_base(L, b, s, t);
+    while (k < cnt_s) {
+        const uint32_t e = base[(size_t)k * kTile];
+        const uint32_t slot = e & kSlotMask;
+        if (slot >= ce) break;
+        // lists are ascending in slot, so slot >= cb; the test keeps LDS reads
+        // in bounds even for a list the build flagged as overflowing
+        if (slot >= cb) f(slot - cb, (int)(e >> kSlotBits) - 1);
+        ++k;
+    }
+}
+
+}  // namespace sphx

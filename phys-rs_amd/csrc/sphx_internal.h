@@ -58,17 +58,37 @@ struct Sort {
     uint32_t cell_cap;
 };
 
-// Neighbour list entry: bits 0..25 the neighbour's sorted index, bits 26..31
-// its cell offset from the owner's cell, 2 bits per axis (offset + 1).
-static constexpr uint32_t kIdxBits = 26;
-static constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
-static constexpr uint32_t kMaxParticles = 1u << kIdxBits;
+// Tiles: every cell row (fixed y, z) is cut into runs of at most kTile
+// consecutive sorted particles.  One workgroup owns one tile; the particles
+// of a tile share their row, so the candidates of one neighbour row (one
+// "stage", 9 per tile) are one or two contiguous slices of the sorted arrays.
+static constexpr int kTile = 256;
+static constexpr int kStages = 9;
+struct Tiles {
+    uint32_t* row_count;       // tiles per row (nrows)
+    uint32_t* row_off;         // exclusive scan (nrows + 1); row_off[nrows] = tile count
+    uint32_t* scan_sums;
+    uint32_t *p0, *np, *row;   // per tile: first particle, particles, row index
+    uint32_t nrows, cap;       // rows of the grid; tile arrays capacity
+    uint32_t nrows_cap;        // rows the row arrays can hold
+};
+
+// Neighbour list, stored per tile so that one tile's lists are one contiguous
+// block (a few pages, not one page per row): for tile b, stage s, lane t,
+// cnt[(b * 9 + s) * kTile + t] entries at idx[((b * 9 + s) * ks + k) * kTile + t]
+// (lanes at equal k read 128 contiguous bytes).  An entry is the neighbour's
+// slot in the stage's staged candidate array (bits 0..13) and its cell x
+// offset + 1 (bits 14..15).  ntot[i] is particle i's total (sorted order).
+static constexpr uint32_t kSlotBits = 14;
+static constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
+static constexpr uint32_t kMaxParticles = 1u << 30;
 
 struct NbrList {
-    uint32_t* idx;             // column-major: idx[k * cap + i]
-    uint32_t* cnt;             // neighbours of i
-    uint32_t* overflow;        // [0]: max count seen, [1]: overflow flag
-    int kmax;
+    uint16_t* idx;
+    uint8_t* cnt;
+    uint16_t* ntot;
+    uint32_t* overflow;        // [0] max neighbours, [1] list overflow, [2] stage too wide
+    int ks;                    // entries per particle and stage (<= 255)
 };
 
 // FP32-mixed form: pair-side copies in f32.  Positions are offsets from the
@@ -92,36 +112,50 @@ struct KernelParams {          // what the kernels need from SphxParams, by valu
     double g[3], e[3];
     double eta, ch, sigma_c;
     int eos, visc_conv, bnd, mhd;
+    // set at upload when every particle has the same mass / smoothing radius:
+    // the pair passes then use these instead of staging m and h
+    int uniform;
+    double m0, h0, inv_h0;
 };
 
-// ---- kernel launchers (sphx_kernels.hip) --------------------------------
+// ---- kernel launchers -------------------------------------------------------
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_iota_u32(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_cell_hash(const Particles& P, uint32_t n, const Grid& g, const Sort& S, hipStream_t s);
-void launch_scan(const Sort& S, uint32_t ncell, hipStream_t s);
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, hipStream_t s);
 void launch_scatter(const Sort& S, uint32_t n, hipStream_t s);
 void launch_cell_order(const Sort& S, uint32_t ncell, hipStream_t s);
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n,
                     bool mhd, bool charge, bool mixed, const Mixed& X, const Grid& g, hipStream_t s);
+void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
+uint32_t tile_grid(uint32_t n, const Tiles& T);
 void launch_build_nbr(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                      const NbrList& L, bool mixed, const Mixed& X, hipStream_t s);
+                      const Tiles& T, const NbrList& L, bool mixed, const Mixed& X, hipStream_t s);
 void launch_density(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                    const Sort& S, const Tiles& T,
                     const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                    const Sort& S, const Tiles& T,
                    const NbrList& L, const KernelParams& kp, bool visc_only, bool mhd_stress,
                    bool charge, bool mixed, const Mixed& X, hipStream_t s);
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         hipStream_t s);
+void launch_vv_position_flip(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+                             uint8_t* flip, hipStream_t s);
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         bool mixed, const Mixed& X, hipStream_t s);
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                    const Sort& S, const Tiles& T,
                  const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                    const Sort& S, const Tiles& T,
                       const NbrList& L, const KernelParams& kp, hipStream_t s);
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
+                    const Sort& S, const Tiles& T,
                       const NbrList& L, const KernelParams& kp, hipStream_t s);
-void launch_export_pairs(const Particles& P, uint32_t n, uint32_t cap, const NbrList& L,
-                         uint32_t* out, unsigned long long* counter, uint64_t out_cap, hipStream_t s);
+void launch_export_pairs(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
+                         const Tiles& T, const NbrList& L, uint32_t* out, unsigned long long* counter,
+                         uint64_t out_cap, hipStream_t s);
 void launch_energy(const Particles& P, const Derived& D, uint32_t n, uint32_t cap,
                    const NbrList& L, const KernelParams& kp, bool mhd, double* partial,
                    uint32_t nblocks, hipStream_t s);

@@ -1,0 +1,122 @@
+// sphx_particles.hip — per-particle kernels: velocity-Verlet halves and the
+// energy/momentum reduction.  Streaming SoA f64 access; HBM-bound.
+
+#include "sphx_device.h"
+
+namespace sphx {
+
+// velocity_verlet_position_system (integrators.rs:63-83), then the box:
+// reflection mirrors x and flips v and the stored acceleration; periodic wraps.
+// When `flip` is set, bit k of flip[i] records a reflection on axis k (the
+// adjoint needs the sign of the map).
+__global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams kp, uint8_t* flip) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* xs[3] = {P.x, P.y, P.z};
+    double* vs[3] = {P.vx, P.vy, P.vz};
+    double* aps[3] = {P.apx, P.apy, P.apz};
+    const double* as[3] = {D.ax, D.ay, D.az};
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double a = as[k][i];
+        double v = vs[k][i];
+        double ap = a;
+        double x = xs[k][i] + (v * kp.dt + a * kp.half_dt_sq);
+        if (kp.bnd & SPHX_BND_REFLECT) {
+            const double lo = kp.box_lo[k], hi = kp.box_hi[k];
+            if (x < lo) { x = lo + (lo - x); v = -v; ap = -ap; bits |= 1u << k; }
+            else if (x > hi) { x = hi - (x - hi); v = -v; ap = -ap; bits |= 1u << k; }
+        } else if (kp.bnd & SPHX_BND_PERIODIC) {
+            const double lo = kp.box_lo[k], hi = kp.box_hi[k], L = hi - lo;
+            if (x >= hi) x -= L;
+            else if (x < lo) x += L;
+        }
+        xs[k][i] = x;
+        vs[k][i] = v;
+        aps[k][i] = ap;
+    }
+    if (flip) flip[i] = (uint8_t)bits;
+}
+void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+                        hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_vv_position, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, (uint8_t*)nullptr);
+}
+void launch_vv_position_flip(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+                             uint8_t* flip, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_vv_position, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, flip);
+}
+
+// velocity_verlet_velocity_system (integrators.rs:92-104)
+__global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams kp, int mixed, Mixed X) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double vx = P.vx[i] + (P.apx[i] + D.ax[i]) * kp.half_dt;
+    const double vy = P.vy[i] + (P.apy[i] + D.ay[i]) * kp.half_dt;
+    const double vz = P.vz[i] + (P.apz[i] + D.az[i]) * kp.half_dt;
+    P.vx[i] = vx;
+    P.vy[i] = vy;
+    P.vz[i] = vz;
+    if (mixed) {
+        X.vx[i] = (float)vx;
+        X.vy[i] = (float)vy;
+        X.vz[i] = (float)vz;
+    }
+}
+void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, bool mixed,
+                        const Mixed& X, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_vv_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, (int)mixed, X);
+}
+
+// ---------------------------------------------------------- energy
+// Per-block partial sums (deterministic), finished on the host.
+static constexpr int kEnergyVals = 9;
+uint32_t energy_blocks(uint32_t n) { return nblocks(n ? n : 1); }
+
+__device__ double specific_internal(const KernelParams& kp, double rho) {
+    if (kp.eos == SPHX_EOS_TAIT) {
+        const double rho0 = kp.rest_density, g = kp.gamma;
+        if (rho <= rho0) return 0.0;
+        return kp.tait_B * ((pow(rho / rho0, g - 1.0) - 1.0) / ((g - 1.0) * rho0) + 1.0 / rho - 1.0 / rho0);
+    }
+    if (kp.eos == SPHX_EOS_POLYTROPIC) return kp.poly_k * pow(rho, kp.gamma - 1.0) / (kp.gamma - 1.0);
+    return kp.iso_cs2 * log(rho / kp.rest_density);
+}
+
+__global__ void k_energy(Particles P, Derived D, uint32_t n, const uint16_t* __restrict__ ntot,
+                         KernelParams kp, int mhd, double* __restrict__ partial) {
+    __shared__ double lds[kEnergyVals][kBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[kEnergyVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const double m = P.m[i], vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+        v[0] = 0.5 * m * ((vx * vx + vy * vy) + vz * vz);
+        v[1] = -m * ((kp.g[0] * P.x[i] + kp.g[1] * P.y[i]) + kp.g[2] * P.z[i]);
+        v[2] = m * specific_internal(kp, D.rho[i]);
+        if (mhd) {
+            const double bx = P.bx[i], by = P.by[i], bz = P.bz[i];
+            v[3] = 0.5 * (m / D.rho[i]) * ((bx * bx + by * by) + bz * bz);
+        }
+        v[4] = m * vx; v[5] = m * vy; v[6] = m * vz;
+        const uint32_t c = ntot[i];
+        v[7] = (double)c;
+        v[8] = (double)c;
+    }
+    for (int k = 0; k < kEnergyVals; ++k) lds[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            for (int k = 0; k < 8; ++k) lds[k][threadIdx.x] += lds[k][threadIdx.x + off];
+            lds[8][threadIdx.x] = fmax(lds[8][threadIdx.x], lds[8][threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kEnergyVals) partial[(size_t)blockIdx.x * kEnergyVals + threadIdx.x] = lds[threadIdx.x][0];
+}
+void launch_energy(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const NbrList& L,
+                   const KernelParams& kp, bool mhd, double* partial, uint32_t nb, hipStream_t s) {
+    (void)cap;
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(kBlock), 0, s, P, D, n, L.ntot, kp, (int)mhd, partial);
+}
+
+}  // namespace sphx

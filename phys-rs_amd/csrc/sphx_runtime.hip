@@ -43,6 +43,7 @@ struct SphxCtx {
     int cur = 0;
     Derived D{};
     Sort S{};
+    Tiles T{};
     NbrList L{};
     Mixed X{};
     double* energy_partial = nullptr;
@@ -50,6 +51,8 @@ struct SphxCtx {
     unsigned long long* pair_counter = nullptr;
     uint64_t pair_buf_cap = 0;
     bool has_mhd = false, has_charge = false;
+    bool uniform = false;                  // all m equal and all h equal
+    double m0 = 0.0, h0 = 0.0;
     // validity of derived data for the current positions and sorted order
     bool sorted = false, listed = false, dens = false, sp = false, acc = false;
     // timing
@@ -156,6 +159,10 @@ void make_kernel_params(SphxCtx* c) {
     k.visc_conv = p.visc_convention;
     k.bnd = p.boundary_flags;
     k.mhd = p.mhd_flags;
+    k.uniform = c->uniform ? 1 : 0;
+    k.m0 = c->m0;
+    k.h0 = c->h0;
+    k.inv_h0 = c->uniform ? 1.0 / c->h0 : 0.0;
 }
 
 // Grid from the parameters and (for open boxes) the uploaded positions.
@@ -245,6 +252,23 @@ int setup_grid(SphxCtx* c, const SphxState* host) {
         if ((rc = dalloc(c, &c->S.block_sums, tiles))) return rc;
         c->S.cell_cap = g.ncell;
     }
+    const uint32_t nrows = (uint32_t)g.dims[1] * (uint32_t)g.dims[2];
+    const uint32_t tcap = c->cap / kTile + nrows + 2;
+    if (nrows > c->T.nrows_cap || tcap > c->T.cap) {
+        int rc;
+        if ((rc = dalloc(c, &c->T.row_count, nrows))) return rc;
+        if ((rc = dalloc(c, &c->T.row_off, (size_t)nrows + 1))) return rc;
+        if ((rc = dalloc(c, &c->T.scan_sums, ((size_t)nrows + 1023) / 1024 + 1))) return rc;
+        if ((rc = dalloc(c, &c->T.p0, tcap)) || (rc = dalloc(c, &c->T.np, tcap)) || (rc = dalloc(c, &c->T.row, tcap)))
+            return rc;
+        // neighbour lists are stored per tile
+        if ((rc = dalloc(c, &c->L.idx, (size_t)tcap * kStages * c->L.ks * kTile)) ||
+            (rc = dalloc(c, &c->L.cnt, (size_t)tcap * kStages * kTile)))
+            return rc;
+        c->T.cap = tcap;
+        c->T.nrows_cap = nrows;
+    }
+    c->T.nrows = nrows;
     c->grid = g;
     c->grid_set = true;
     return SPHX_OK;
@@ -309,15 +333,16 @@ int do_neighbours(SphxCtx* c) {
         Particles& src = c->A[c->cur];
         Particles& dst = c->A[1 - c->cur];
         launch_cell_hash(src, n, c->grid, c->S, c->stream);
-        launch_scan(c->S, c->grid.ncell, c->stream);
+        launch_scan(c->S.cell_count, c->S.cell_start, c->grid.ncell, c->S.block_sums, c->stream);
         launch_scatter(c->S, n, c->stream);
         launch_cell_order(c->S, c->grid.ncell, c->stream);
         launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->mixed(), c->X, c->grid, c->stream);
         c->cur = 1 - c->cur;
+        launch_tiles(c->grid, c->S, c->T, c->stream);
     }
     {
         Timer t(c, kNbr);
-        launch_build_nbr(c->P(), n, c->cap, c->grid, c->S, c->L, c->mixed(), c->X, c->stream);
+        launch_build_nbr(c->P(), n, c->cap, c->grid, c->S, c->T, c->L, c->mixed(), c->X, c->stream);
     }
     c->sorted = c->listed = true;
     c->dens = c->sp = c->acc = false;
@@ -334,7 +359,7 @@ int do_density(SphxCtx* c) {
     if ((rc = ensure_list(c))) return rc;
     {
         Timer t(c, kDensity);
-        launch_density(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
+        launch_density(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, c->mixed(), c->X, c->stream);
     }
     c->dens = true;
     return launch_check(c);
@@ -349,7 +374,7 @@ int do_forces(SphxCtx* c, bool visc_only) {
     const bool charge = c->has_charge && c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_LORENTZ_CHARGE);
     {
         Timer t(c, visc_only ? kViscOnly : kForce);
-        launch_forces(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, visc_only, stress, charge, c->mixed(),
+        launch_forces(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, visc_only, stress, charge, c->mixed(),
                       c->X, c->stream);
     }
     c->sp = true;
@@ -391,7 +416,7 @@ int do_xsph(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_xsph(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
+        launch_xsph(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, c->mixed(), c->X, c->stream);
     }
     Particles& P = c->P();
     swap3(P.vx, P.vy, P.vz, c->D.tmp3);
@@ -405,7 +430,7 @@ int do_induction(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_induction(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->stream);
+        launch_induction(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, c->stream);
     }
     Particles& P = c->P();
     swap3(P.bx, P.by, P.bz, c->D.tmp3);
@@ -419,7 +444,7 @@ int do_div_clean(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_div_clean(c->P(), c->D, c->n, c->cap, c->grid, c->L, c->kp, c->stream);
+        launch_div_clean(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, c->stream);
     }
     Particles& P = c->P();
     swap3(P.bx, P.by, P.bz, c->D.tmp3);
@@ -429,13 +454,17 @@ int do_div_clean(SphxCtx* c) {
 }
 
 int check_overflow(SphxCtx* c) {
-    uint32_t ovf[2] = {0, 0};
+    uint32_t ovf[3] = {0, 0, 0};
     HIP_TRY(c, hipMemcpyAsync(ovf, c->L.overflow, sizeof ovf, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (ovf[2])
+        return fail(c, SPHX_E_CAPACITY,
+                    "a tile's neighbour row holds more than %u candidates; the particle distribution is too "
+                    "sparse along x for this grid (set cell_size or grid_lo/grid_hi)", kSlotMask + 1u);
     if (ovf[1])
         return fail(c, SPHX_E_CAPACITY,
-                    "a particle has %u neighbours, more than max_neighbours = %d; raise it and upload again",
-                    ovf[0], c->L.kmax);
+                    "a particle has more than max_neighbours = %d neighbours in one neighbour cell row "
+                    "(%u in total); raise max_neighbours and upload again", c->L.ks, ovf[0]);
     return SPHX_OK;
 }
 
@@ -505,7 +534,11 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     c->cap = capacity ? capacity : 1;
     c->prm = *params;
     make_kernel_params(c);
-    c->L.kmax = params->max_neighbours > 0 ? params->max_neighbours : 192;
+    c->L.ks = params->max_neighbours > 0 ? params->max_neighbours : 64;
+    if (c->L.ks > 255) {
+        delete c;
+        return SPHX_E_INVALID;
+    }
     auto bail = [&](int code) {
         free_all(c);
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -537,13 +570,11 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     if (dalloc(c, &c->S.key, N) || dalloc(c, &c->S.rank, N) || dalloc(c, &c->S.perm, N) ||
         dalloc(c, &c->S.skey, N))
         return bail(SPHX_E_OOM);
-    if (dalloc(c, &c->L.idx, N * (size_t)c->L.kmax) || dalloc(c, &c->L.cnt, N) ||
-        dalloc(c, &c->L.overflow, 2))
-        return bail(SPHX_E_OOM);
+    if (dalloc(c, &c->L.ntot, N) || dalloc(c, &c->L.overflow, 4)) return bail(SPHX_E_OOM);
     if (dalloc(c, &c->energy_partial, (size_t)energy_blocks(c->cap) * 9)) return bail(SPHX_E_OOM);
     if (dalloc(c, &c->pair_counter, 1)) return bail(SPHX_E_OOM);
-    launch_fill_u32(c->L.overflow, 0u, 2, c->stream);
-    launch_fill_u32(c->L.cnt, 0u, N, c->stream);
+    launch_fill_u32(c->L.overflow, 0u, 4, c->stream);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(c->L.ntot), 0u, (N + 1) / 2, c->stream);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(SPHX_E_HIP);
     *out = c;
     return SPHX_OK;
@@ -576,7 +607,7 @@ int sphx_set_params(SphxCtx* c, const SphxParams* p) {
         grid_change |= old.box_lo[k] != p->box_lo[k] || old.box_hi[k] != p->box_hi[k] ||
                        old.grid_lo[k] != p->grid_lo[k] || old.grid_hi[k] != p->grid_hi[k];
     if (grid_change && c->n) c->grid_set = false;
-    if (p->max_neighbours > 0 && p->max_neighbours != c->L.kmax)
+    if (p->max_neighbours > 0 && p->max_neighbours != c->L.ks)
         return fail(c, SPHX_E_INVALID, "max_neighbours is fixed at sphx_create");
     // derived quantities depend on the EOS, viscosity and walls; the
     // precision form also changes the neighbour predicate and f32 copies
@@ -602,6 +633,12 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
     if (h->q && !mhd) return fail(c, SPHX_E_INVALID, "charge needs a magnetic field");
     HIP_TRY(c, hipSetDevice(c->device));
     c->n = h->n;
+    c->uniform = h->n > 0;
+    for (uint32_t i = 1; i < h->n && c->uniform; ++i)
+        c->uniform = h->m[i] == h->m[0] && h->h[i] == h->h[0];
+    c->m0 = h->n ? h->m[0] : 0.0;
+    c->h0 = h->n ? h->h[0] : 0.0;
+    make_kernel_params(c);
     int rc = setup_grid(c, h);
     if (rc) return rc;
     c->has_mhd = mhd;
@@ -628,7 +665,7 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
         launch_iota_u32(P.orig, h->n, c->stream);
         HIP_TRY(c, hipStreamSynchronize(c->stream));  // `zero` goes out of scope
     }
-    launch_fill_u32(c->L.overflow, 0u, 2, c->stream);
+    launch_fill_u32(c->L.overflow, 0u, 4, c->stream);
     c->sorted = c->listed = c->dens = c->sp = c->acc = false;
     return launch_check(c);
 }
@@ -740,17 +777,18 @@ int sphx_export_pairs(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t*
     if ((rc = ensure_list(c))) return rc;
     if ((rc = check_overflow(c))) return rc;
     // every directed neighbour appears once per side: total/2 undirected pairs
-    std::vector<uint32_t> cnt(c->n);
-    if (c->n) HIP_TRY(c, hipMemcpy(cnt.data(), c->L.cnt, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
+    std::vector<uint16_t> cnt(c->n);
+    if (c->n) HIP_TRY(c, hipMemcpy(cnt.data(), c->L.ntot, sizeof(uint16_t) * c->n, hipMemcpyDeviceToHost));
     uint64_t total = 0;
-    for (uint32_t v : cnt) total += v;
+    for (uint16_t v : cnt) total += v;
     const uint64_t np = total / 2;
     if (np > c->pair_buf_cap) {
         if ((rc = dalloc(c, &c->pair_buf, 2 * np))) return rc;
         c->pair_buf_cap = np;
     }
     launch_fill_u32(reinterpret_cast<uint32_t*>(c->pair_counter), 0u, 2, c->stream);
-    launch_export_pairs(c->P(), c->n, c->cap, c->L, c->pair_buf, c->pair_counter, c->pair_buf_cap, c->stream);
+    launch_export_pairs(c->P(), c->n, c->cap, c->grid, c->S, c->T, c->L, c->pair_buf, c->pair_counter,
+                        c->pair_buf_cap, c->stream);
     if ((rc = launch_check(c))) return rc;
     unsigned long long got = 0;
     HIP_TRY(c, hipMemcpyAsync(&got, c->pair_counter, sizeof got, hipMemcpyDeviceToHost, c->stream));

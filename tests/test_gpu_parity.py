@@ -68,6 +68,28 @@ def test_pairs_bit_exact_random_variable_h(require_gpu, periodic):
     assert np.array_equal(got, ref)
 
 
+def test_pairs_dense_periodic_chunked(require_gpu):
+    # cell_size 0.25 -> 4 cells per axis, ~156 particles per cell: a 256-particle
+    # tile spans 2-3 cells, so each neighbour row is the whole periodic grid row
+    # (~625 candidates), more than one LDS chunk (512): staging runs in two
+    # chunks.  h = 0.06 keeps ~72 neighbours per particle.
+    rng = scenarios.SplitMix64(33)
+    n = 10000
+    st = {k: rng.uniform(n) for k in ("x", "y", "z")}
+    st.update(vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n), m=np.full(n, 1.0 / n), h=np.full(n, 0.06))
+    p = sphx.make_params(1e-4, boundary_flags=sphx.BND_PERIODIC, box_lo=(0, 0, 0), box_hi=(1, 1, 1),
+                         cell_size=0.25, max_neighbours=160)
+    with make_ctx(st, p) as ctx:
+        got = ctx.export_pairs()
+        ctx.density()
+        rho = ctx.download(sphx.F_RHO)["rho"]
+    s = O.State(st)
+    pr = O.pairs(s, p, brute=True)
+    assert np.array_equal(got, O.id_pairs(s, pr))
+    O.density(s, p, pr)
+    assert rel_err(rho, s["rho"]) < 1e-13
+
+
 def test_pairs_exactly_at_cutoff(require_gpu):
     x, y, z = scenarios.lattice(6, 6, 6, 1.0)
     n = x.size
