@@ -34,6 +34,10 @@ void orc_mhd_induction(OrcState* s, const SphxParams* p, const uint32_t* pairs, 
 void orc_div_clean(OrcState* s, const SphxParams* p, const uint32_t* pairs, uint64_t np);
 void orc_step(OrcState* s, const SphxParams* p, uint32_t nsteps);
 void orc_energy(const OrcState* s, const SphxParams* p, double out[8]);
+/* sph_adjoint.c */
+int orc_adjoint(OrcState* s, const SphxParams* p, uint32_t nsteps, const double* target, double* loss,
+                double* gv, double* galpha);
+double orc_loss(OrcState* s, const SphxParams* p, uint32_t nsteps, const double* target);
 
 #ifdef __cplusplus
 }
