@@ -1,6 +1,6 @@
 // sphx_device.h — device helpers shared by the kernel translation units:
-// SPH kernel functions, pair vectors, the EOS, and the tile/stage framework
-// every neighbour pass is built on.
+// SPH kernel functions, pair vectors, the EOS, the tile/stage framework of
+// the neighbour build and the list walk of the neighbour passes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,7 +10,7 @@
 
 namespace sphx {
 
-static constexpr int kBlock = kTile;            // one thread per tile particle
+static constexpr int kBlock = kTile;            // one thread per particle
 static constexpr int kStageCap = 384;           // candidates staged in LDS per chunk
 static constexpr double kInvPi = 0.31830988618379067154;  // 1/pi
 static constexpr double kPi = 3.141592653589793;           // std::f64::consts::PI
@@ -307,90 +307,43 @@ __device__ __forceinline__ void cell_slots(const Grid& g, const uint32_t* __rest
     b = start[st.rowbase + c + 1] + base;
 }
 
-// Drives one neighbour pass over a tile.  For each of the 9 stages:
-// stage_fn(u, j) copies particle j's fields to LDS index u; pair_fn(u, ox, st)
-// processes one list entry.  A stage that fits one LDS chunk (the common case)
-// also copies the first kListRows rows of every thread's list into LDS
-// alongside the fields, so the entry loop reads no global memory; longer
-// lists and multi-chunk stages read the remaining entries from HBM.
-static constexpr int kListRows = 16;
-__device__ __forceinline__ uint16_t* list_base(const NbrList& L, uint32_t b, int s, uint32_t t) {
-    return L.idx + ((size_t)(b * kStages + s) * (uint32_t)L.ks) * kTile + t;
-}
-__device__ __forceinline__ uint8_t* count_ptr(const NbrList& L, uint32_t b, int s, uint32_t t) {
-    return L.cnt + (size_t)(b * kStages + s) * kTile + t;
+// =========================================================================
+// Neighbour lists (built per tile, walked per particle).
+// =========================================================================
+__device__ __forceinline__ uint32_t* nbr_base(const NbrList& L, uint32_t i) {
+    return L.idx + (((size_t)(i >> 6) * (uint32_t)L.ks) << 6) + (i & 63u);
 }
 
-template <typename StageF, typename PairF>
-__device__ __forceinline__ void tile_pass(const Grid& g, const uint32_t* __restrict__ start, const TileCtx& tc,
-                                          const NbrList& L, uint32_t b, bool valid,
-                                          uint16_t (*lst)[kTile], uint32_t* smax,
-                                          StageF&& stage_fn, PairF&& pair_fn) {
-    const uint32_t t = threadIdx.x;
-    const uint32_t rows = (uint32_t)min(kListRows, L.ks);
-    // the workgroup's largest list per stage bounds the list-row loop below
-    if (t < kStages) smax[t] = 0u;
-    __syncthreads();
-    if (valid)
-        for (int s = 0; s < kStages; ++s) atomicMax(&smax[s], (uint32_t)*count_ptr(L, b, s, t));
-    for (int s = 0; s < kStages; ++s) {
-        Stage st;
-        stage_segments(g, start, tc, s, st);
-        const uint16_t* __restrict__ base = list_base(L, b, s, t);
-        const uint32_t cnt_s = valid ? *count_ptr(L, b, s, t) : 0u;
-        if (st.total <= (uint32_t)kStageCap) {
-            __syncthreads();  // smax complete; the previous stage is done with LDS
-            const uint32_t kmax = smax[s];
-            if (kmax == 0) continue;  // no thread of the tile has a neighbour in this row
-            for (uint32_t kb = 0; kb < kmax; kb += rows) {
-                if (kb) __syncthreads();  // previous list rows consumed
-                else
-                    for (uint32_t u = t; u < st.total; u += kBlock) stage_fn(u, slot_particle(st, u));
-                const uint32_t kend = min(cnt_s, kb + rows);
-                if (valid)
-                    for (uint32_t k = kb; k < kend; ++k) lst[k - kb][t] = base[(size_t)k * kTile];
-                __syncthreads();
-                for (uint32_t k = kb; k < kend; ++k) {
-                    const uint32_t e = lst[k - kb][t];
-                    pair_fn(e & kSlotMask, (int)(e >> kSlotBits) - 1, st);
-                }
-            }
-        } else {
-            uint32_t k = 0;
-            for (uint32_t cb = 0; cb < st.total; cb += kStageCap) {
-                const uint32_t ce = min(st.total, cb + (uint32_t)kStageCap);
-                __syncthreads();
-                for (uint32_t u = t; u < ce - cb; u += kBlock) stage_fn(u, slot_particle(st, cb + u));
-                __syncthreads();
-                while (k < cnt_s) {
-                    const uint32_t e = base[(size_t)k * kTile];
-                    const uint32_t slot = e & kSlotMask;
-                    if (slot >= ce) break;
-                    // lists ascend in slot, so slot >= cb; the test keeps LDS
-                    // reads in bounds even for a list the build flagged
-                    if (slot >= cb) pair_fn(slot - cb, (int)(e >> kSlotBits) - 1, st);
-                    ++k;
-                }
-            }
-        }
-    }
-}
-
-// Iterate over a particle's list entries of stage s whose slots fall in the
-// staged chunk [cb, ce); k carries across chunks.  f(slot - cb, ox).
+// f(e) for every entry of sorted particle i, two entries in flight so that
+// the second's gathers overlap the first's arithmetic.
 template <typename F>
-__device__ __forceinline__ void for_stage_entries(const NbrList& L, uint32_t b, uint32_t t, int s,
-                                                  uint32_t cnt_s, uint32_t& k, uint32_t cb, uint32_t ce, F&& f) {
-    const uint16_t* __restrict__ base = list_base(L, b, s, t);
-    while (k < cnt_s) {
-        const uint32_t e = base[(size_t)k * kTile];
-        const uint32_t slot = e & kSlotMask;
-        if (slot >= ce) break;
-        // lists are ascending in slot, so slot >= cb; the test keeps LDS reads
-        // in bounds even for a list the build flagged as overflowing
-        if (slot >= cb) f(slot - cb, (int)(e >> kSlotBits) - 1);
-        ++k;
+__device__ __forceinline__ void for_neighbours(const NbrList& L, uint32_t i, F&& f) {
+    const uint32_t* __restrict__ e = nbr_base(L, i);
+    const uint32_t cnt = L.ntot[i];
+    uint32_t k = 0;
+    for (; k + 1 < cnt; k += 2) {
+        const uint32_t e0 = e[(size_t)k << 6];
+        const uint32_t e1 = e[(size_t)(k + 1) << 6];
+        f(e0);
+        f(e1);
     }
+    if (k < cnt) f(e[(size_t)k << 6]);
 }
+
+// Cell offset of an entry's neighbour times the cell widths (FP32-mixed pair
+// vectors are (xr_j - xr_i) + offset).  code = stage * 3 + ox + 1 < 27 and
+// stage = (oz + 1) * 3 + oy + 1; (c * 11) >> 5 == c / 3 for c < 27.
+__device__ __forceinline__ void entry_offset(uint32_t e, float wx, float wy, float wz, float& ox, float& oy,
+                                             float& oz) {
+    const uint32_t c = e >> kIdxBits;
+    const uint32_t s = (c * 11u) >> 5;
+    const uint32_t sz = (s * 11u) >> 5;
+    ox = (float)((int)(c - 3u * s) - 1) * wx;
+    oy = (float)((int)(s - 3u * sz) - 1) * wy;
+    oz = (float)((int)sz - 1) * wz;
+}
+
+// Sorted particle index of the calling thread in a one-thread-per-particle pass.
+__device__ __forceinline__ uint32_t particle_index() { return xcd_block() * kBlock + threadIdx.x; }
 
 }  // namespace sphx

@@ -1,5 +1,6 @@
 // sphx_internal.h — device buffers, grid description and kernel launchers
-// shared by the kernel TU (sphx_kernels.hip) and the runtime TU (sphx_runtime.hip).
+// shared by the kernel translation units (sphx_sort.hip, sphx_pairs.hip,
+// sphx_particles.hip, sphx_adjoint.hip) and the runtime (sphx_runtime.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -59,9 +60,9 @@ struct Sort {
 };
 
 // Tiles: every cell row (fixed y, z) is cut into runs of at most kTile
-// consecutive sorted particles.  One workgroup owns one tile; the particles
-// of a tile share their row, so the candidates of one neighbour row (one
-// "stage", 9 per tile) are one or two contiguous slices of the sorted arrays.
+// consecutive sorted particles.  The neighbour build runs one workgroup per
+// tile: the candidates of one neighbour row ("stage", 9 per tile) are one or
+// two contiguous slices of the sorted arrays, staged through LDS.
 static constexpr int kTile = 256;
 static constexpr int kStages = 9;
 struct Tiles {
@@ -73,22 +74,20 @@ struct Tiles {
     uint32_t nrows_cap;        // rows the row arrays can hold
 };
 
-// Neighbour list, stored per tile so that one tile's lists are one contiguous
-// block (a few pages, not one page per row): for tile b, stage s, lane t,
-// cnt[(b * 9 + s) * kTile + t] entries at idx[((b * 9 + s) * ks + k) * kTile + t]
-// (lanes at equal k read 128 contiguous bytes).  An entry is the neighbour's
-// slot in the stage's staged candidate array (bits 0..13) and its cell x
-// offset + 1 (bits 14..15).  ntot[i] is particle i's total (sorted order).
-static constexpr uint32_t kSlotBits = 14;
-static constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
-static constexpr uint32_t kMaxParticles = 1u << 30;
+// Neighbour list of sorted particle i: ntot[i] entries at
+// idx[((i / 64) * ks + k) * 64 + i % 64], k < ntot[i], so the 64 lanes of a
+// wave at equal k read 256 contiguous bytes.  An entry is the neighbour's
+// sorted index (bits 0..26) and its cell's offset from particle i's cell
+// (bits 27..31: stage * 3 + x offset + 1, stage = (oz + 1) * 3 + oy + 1).
+static constexpr uint32_t kIdxBits = 27;
+static constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+static constexpr uint32_t kMaxParticles = 1u << kIdxBits;
 
 struct NbrList {
-    uint16_t* idx;
-    uint8_t* cnt;
-    uint16_t* ntot;
-    uint32_t* overflow;        // [0] max neighbours, [1] list overflow, [2] stage too wide
-    int ks;                    // entries per particle and stage (<= 255)
+    uint32_t* idx;
+    uint16_t* ntot;            // entries per particle (<= ks)
+    uint32_t* overflow;        // [0] largest neighbour count seen, [1] some list overflowed
+    int ks;                    // list capacity per particle
 };
 
 // FP32-mixed form: pair-side copies in f32.  Positions are offsets from the
@@ -131,31 +130,25 @@ void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
 void launch_build_nbr(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
                       const Tiles& T, const NbrList& L, bool mixed, const Mixed& X, hipStream_t s);
-void launch_density(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                    const Sort& S, const Tiles& T,
-                    const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
-void launch_forces(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                    const Sort& S, const Tiles& T,
-                   const NbrList& L, const KernelParams& kp, bool visc_only, bool mhd_stress,
-                   bool charge, bool mixed, const Mixed& X, hipStream_t s);
+void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                    const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
+void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                   const KernelParams& kp, bool visc_only, bool mhd_stress, bool charge, bool mixed,
+                   const Mixed& X, hipStream_t s);
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         hipStream_t s);
 void launch_vv_position_flip(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                              uint8_t* flip, hipStream_t s);
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         bool mixed, const Mixed& X, hipStream_t s);
-void launch_xsph(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                    const Sort& S, const Tiles& T,
-                 const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
-void launch_induction(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                    const Sort& S, const Tiles& T,
-                      const NbrList& L, const KernelParams& kp, hipStream_t s);
-void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g,
-                    const Sort& S, const Tiles& T,
-                      const NbrList& L, const KernelParams& kp, hipStream_t s);
-void launch_export_pairs(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                         const Tiles& T, const NbrList& L, uint32_t* out, unsigned long long* counter,
-                         uint64_t out_cap, hipStream_t s);
+void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                 const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
+void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                      const KernelParams& kp, hipStream_t s);
+void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                      const KernelParams& kp, hipStream_t s);
+void launch_export_pairs(const Particles& P, uint32_t n, const NbrList& L, uint32_t* out,
+                         unsigned long long* counter, uint64_t out_cap, hipStream_t s);
 void launch_energy(const Particles& P, const Derived& D, uint32_t n, uint32_t cap,
                    const NbrList& L, const KernelParams& kp, bool mhd, double* partial,
                    uint32_t nblocks, hipStream_t s);

@@ -1,13 +1,19 @@
-// sphx_pairs.hip — the neighbour passes of the SPH step on gfx950.
+// sphx_pairs.hip — the neighbour build and the neighbour passes of the SPH
+// step on gfx950.
 //
-// Every pass runs one workgroup per tile (<= 256 sorted particles of one cell
-// row).  For each of the 9 neighbour rows ("stages") the workgroup copies the
-// candidates' fields from HBM into LDS with contiguous, coalesced loads, then
-// each thread walks its own neighbour list for that stage, reading neighbour
-// data from LDS.  Lists are column-major u16 LDS slots, so the 64 lanes of a
-// wave at equal k read 128 contiguous bytes.  One thread owns one particle
-// and gathers: no atomics on the floating-point path, every sum is
-// deterministic.  No MFMA: the path is not a dense contraction.
+// Build: one workgroup per tile (<= 256 sorted particles of one cell row).
+// For each of the 9 neighbour rows ("stages") the workgroup copies the
+// candidates' f32 cell-relative coordinates into LDS with contiguous loads
+// and each thread tests its 3-cell window, appending accepted neighbours to
+// one list per particle (all stages together).
+//
+// Passes: one thread per sorted particle walks its whole list and gathers
+// the neighbours' fields from the sorted SoA arrays (L2-resident: the
+// neighbours of a wave are a few cell rows).  One loop over all stages keeps
+// lanes busy (a per-stage loop idles every lane whose particle has no
+// neighbour in that row).  One thread owns one particle and gathers: no
+// atomics on the floating-point path, every sum is deterministic.  No MFMA:
+// the path is not a dense contraction.
 //
 // Compiled with -ffp-contract=off: the neighbour predicate rounds exactly
 // like the reference's sqrt((dx*dx + dy*dy) + dz*dz) < 2*h_avg
@@ -77,14 +83,13 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         }
     }
     uint32_t total = 0;
-    bool over = false, wide = false;
+    uint32_t* __restrict__ out = nbr_base(L, p);
     for (int s = 0; s < kStages; ++s) {
         Stage st;
         stage_segments(g, start, tc, s, st);
-        if (st.total > kSlotMask + 1u) wide = true;
-        // the three cells' slot ranges, ordered by slot so that every list
-        // is ascending within a stage (a periodic row spanning the whole
-        // grid stages in physical, not logical, cell order)
+        // the three cells' slot ranges, ordered by slot so that a stage's
+        // entries ascend in memory (a periodic row spanning the whole grid
+        // stages in physical, not logical, cell order)
         uint32_t ra[3], rb[3], ro[3];
         for (int o = 0; o < 3; ++o) {
             cell_slots(g, start, st, cx, o - 1, ra[o], rb[o]);
@@ -102,8 +107,6 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         SPHX_SWAP_RANGE(1, 2)
         SPHX_SWAP_RANGE(0, 1)
 #undef SPHX_SWAP_RANGE
-        uint32_t cnt_s = 0;
-        uint16_t* __restrict__ out = list_base(L, b_, s, t);
         for (uint32_t cb = 0; cb < st.total; cb += kStageCap) {
             const uint32_t ce = min(st.total, cb + kStageCap);
             __syncthreads();
@@ -137,25 +140,19 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                         in = exact_neighbour(ddx, ddy, ddz, hi, P.h[j]);
                     }
                     if (in) {
-                        if (cnt_s < (uint32_t)L.ks) out[(size_t)cnt_s * kTile] = (uint16_t)(slot | (o << kSlotBits));
-                        ++cnt_s;
+                        if (total < (uint32_t)L.ks)
+                            out[(size_t)total << 6] = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
+                        ++total;
                     }
                 }
             }
         }
-        if (valid) {
-            *count_ptr(L, b_, s, t) = (uint8_t)min(cnt_s, (uint32_t)L.ks);
-            over |= cnt_s > (uint32_t)L.ks;
-            total += cnt_s;
-        }
     }
-    if (valid) L.ntot[p] = (uint16_t)min(total, 65535u);
+    if (valid) L.ntot[p] = (uint16_t)min(total, (uint32_t)L.ks);
     const uint32_t wmax = wave_max_u32(total);
-    const uint32_t wover = wave_max_u32(over ? 1u : 0u);
-    if ((t & 63) == 0) {
-        if (wmax) atomicMax(&L.overflow[0], wmax);
-        if (wover) atomicOr(&L.overflow[1], 1u);
-        if (wide) atomicOr(&L.overflow[2], 1u);
+    if ((t & 63) == 0 && wmax) {
+        atomicMax(&L.overflow[0], wmax);
+        if (wmax > (uint32_t)L.ks) atomicOr(&L.overflow[1], 1u);
     }
 }
 
@@ -170,8 +167,8 @@ void launch_build_nbr(const Particles& P, uint32_t n, uint32_t cap, const Grid& 
 }
 
 // ------------------------------------------------------- density + EOS (a4, a5)
-// Pair geometry shared by the f64 passes: d = x_j - x_i, r^2, and the
-// smoothing length of the pair (1/h_avg), uniform-h case folded to constants.
+// Pair geometry shared by the f64 passes: the smoothing length of the pair
+// (1/h_avg) and the kernel normalisations, uniform-h case folded to constants.
 struct PairH {
     double inv_h, sig_w, sig_g, h_avg;
 };
@@ -191,110 +188,86 @@ __device__ __forceinline__ PairH pair_h(const KernelParams& kp, double hi, doubl
     return q;
 }
 
+#define SPHX_PASS_PROLOGUE()                \
+    const uint32_t i = particle_index();    \
+    if (i >= n) return
+
 template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, Tiles T, Grid g,
-                                                     const uint32_t* __restrict__ skey,
-                                                     const uint32_t* __restrict__ start, uint32_t cap,
-                                                     NbrList L, KernelParams kp) {
-    __shared__ double lds[5][kStageCap];
-    __shared__ uint16_t lst[kListRows][kTile];
-    __shared__ uint32_t smax[kStages];
-    SPHX_TILE_PROLOGUE();
-    double xi = 0, yi = 0, zi = 0, hi = 1, rho = 0;
-    if (valid) {
-        xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
-        rho = P.m[p] * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
-    }
-    constexpr bool uni = UNI;
+__global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                     KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
+    double rho = P.m[i] * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
     double acc = 0.0;
-    tile_pass(g, start, tc, L, b_, valid, lst, smax,
-        [&](uint32_t u, uint32_t j) {
-            lds[0][u] = P.x[j]; lds[1][u] = P.y[j]; lds[2][u] = P.z[j];
-            if (!uni) { lds[3][u] = P.m[j]; lds[4][u] = P.h[j]; }
-        },
-        [&](uint32_t u, int, const Stage&) {
-            double dx, dy, dz;
-            pair_vec(g, xi, yi, zi, lds[0][u], lds[1][u], lds[2][u], dx, dy, dz);
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            double r = 0.0, inv_r;
-            if (r2 > 0.0) sqrt_rsqrt(r2, r, inv_r);
-            const PairH ph = pair_h<UNI>(kp, hi, uni ? hi : lds[4][u]);
-            const double w = ph.sig_w * w_shape(r * ph.inv_h);
-            acc = fma(uni ? 1.0 : lds[3][u], w, acc);
-        });
-    if (!valid) return;
-    rho = uni ? fma(kp.m0, acc, rho) : rho + acc;
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        double dx, dy, dz;
+        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        double r = 0.0, inv_r;
+        if (r2 > 0.0) sqrt_rsqrt(r2, r, inv_r);
+        const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
+        const double w = ph.sig_w * w_shape(r * ph.inv_h);
+        acc = fma(UNI ? 1.0 : P.m[j], w, acc);
+    });
+    rho = UNI ? fma(kp.m0, acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
-    D.rho[p] = rho;
-    D.p[p] = pr;
-    D.pr2[p] = pr / (rho * rho);
+    D.rho[i] = rho;
+    D.p[i] = pr;
+    D.pr2[i] = pr / (rho * rho);
 }
 
 // FP32-mixed density: f32 pair terms on cell-relative coordinates, f64 sum.
 template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D, Mixed X, Tiles T, Grid g,
-                                                           const uint32_t* __restrict__ skey,
-                                                           const uint32_t* __restrict__ start, uint32_t cap,
+__global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
                                                            NbrList L, KernelParams kp) {
-    __shared__ float lds[5][kStageCap];
-    __shared__ uint16_t lst[kListRows][kTile];
-    __shared__ uint32_t smax[kStages];
-    SPHX_TILE_PROLOGUE();
-    float xi = 0, yi = 0, zi = 0, hi = 1;
-    double rho = 0;
-    if (valid) {
-        xi = X.xr[p]; yi = X.yr[p]; zi = X.zr[p]; hi = X.h[p];
-        const double hd = P.h[p];
-        rho = P.m[p] * (1.0 / (kPi * ((hd * hd) * hd)));
-    }
-    constexpr bool uni = UNI;
+    SPHX_PASS_PROLOGUE();
+    const float xi = X.xr[i], yi = X.yr[i], zi = X.zr[i], hi = X.h[i];
+    const double hd = P.h[i];
+    double rho = P.m[i] * (1.0 / (kPi * ((hd * hd) * hd)));
     const float inv_h0 = (float)kp.inv_h0;
     const float sig_w0 = (float)(kp.inv_h0 * kp.inv_h0 * kp.inv_h0 * kInvPi);
-    const float wx = (float)g.w[0];
+    const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double acc = 0.0;
-    tile_pass(g, start, tc, L, b_, valid, lst, smax,
-        [&](uint32_t u, uint32_t j) {
-            lds[0][u] = X.xr[j]; lds[1][u] = X.yr[j]; lds[2][u] = X.zr[j];
-            if (!uni) { lds[3][u] = X.m[j]; lds[4][u] = X.h[j]; }
-        },
-        [&](uint32_t u, int ox, const Stage& st) {
-            const float dx = (lds[0][u] - xi) + (float)ox * wx;
-            const float dy = (lds[1][u] - yi) + st.oy;
-            const float dz = (lds[2][u] - zi) + st.oz;
-            const float r = __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-            float inv_h = inv_h0, sig = sig_w0;
-            if (!uni) {
-                inv_h = __builtin_amdgcn_rcpf(0.5f * (hi + lds[4][u]));
-                sig = (inv_h * inv_h) * inv_h * (float)kInvPi;
-            }
-            const float w = sig * w_shape_f(r * inv_h);
-            acc += (double)(uni ? w : lds[3][u] * w);
-        });
-    if (!valid) return;
-    rho = uni ? fma(kp.m0, acc, rho) : rho + acc;
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        float ox, oy, oz;
+        entry_offset(e, wx, wy, wz, ox, oy, oz);
+        const float dx = (X.xr[j] - xi) + ox;
+        const float dy = (X.yr[j] - yi) + oy;
+        const float dz = (X.zr[j] - zi) + oz;
+        const float r = __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        float inv_h = inv_h0, sig = sig_w0;
+        if (!UNI) {
+            inv_h = __builtin_amdgcn_rcpf(0.5f * (hi + X.h[j]));
+            sig = (inv_h * inv_h) * inv_h * (float)kInvPi;
+        }
+        const float w = sig * w_shape_f(r * inv_h);
+        acc += (double)(UNI ? w : X.m[j] * w);
+    });
+    rho = UNI ? fma(kp.m0, acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
     const double pr2 = pr / (rho * rho);
-    D.rho[p] = rho;
-    D.p[p] = pr;
-    D.pr2[p] = pr2;
-    X.rho[p] = (float)rho;
-    X.pr2[p] = (float)pr2;
+    D.rho[i] = rho;
+    D.p[i] = pr;
+    D.pr2[i] = pr2;
+    X.rho[i] = (float)rho;
+    X.pr2[i] = (float)pr2;
 }
 
-void launch_density(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                    const Tiles& T, const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X,
-                    hipStream_t s) {
+void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                    const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
     if (!n) return;
-    const dim3 grid(tile_grid(n, T)), block(kBlock);
+    const dim3 grid(nblocks(n)), block(kBlock);
     const bool uni = kp.uniform != 0;
     if (mixed && uni)
-        hipLaunchKernelGGL(k_density_mixed<true>, grid, block, 0, s, P, D, X, T, g, S.skey, S.cell_start, cap, L, kp);
+        hipLaunchKernelGGL(k_density_mixed<true>, grid, block, 0, s, P, D, X, n, g, L, kp);
     else if (mixed)
-        hipLaunchKernelGGL(k_density_mixed<false>, grid, block, 0, s, P, D, X, T, g, S.skey, S.cell_start, cap, L, kp);
+        hipLaunchKernelGGL(k_density_mixed<false>, grid, block, 0, s, P, D, X, n, g, L, kp);
     else if (uni)
-        hipLaunchKernelGGL(k_density<true>, grid, block, 0, s, P, D, T, g, S.skey, S.cell_start, cap, L, kp);
+        hipLaunchKernelGGL(k_density<true>, grid, block, 0, s, P, D, n, g, L, kp);
     else
-        hipLaunchKernelGGL(k_density<false>, grid, block, 0, s, P, D, T, g, S.skey, S.cell_start, cap, L, kp);
+        hipLaunchKernelGGL(k_density<false>, grid, block, 0, s, P, D, n, g, L, kp);
 }
 
 // ------------------------- pressure + viscosity + Maxwell stress + walls (a6-a9, a13)
@@ -338,194 +311,155 @@ __device__ __forceinline__ void force_epilogue(const Particles& P, const Derived
     D.az[p] = F[2] / mi;
 }
 
-// LDS field slots of the force pass
-enum { FX = 0, FY, FZ, FVX, FVY, FVZ, FRHO, FPR2, FM, FH, FBX, FBY, FBZ, FNUM };
-
 template <bool VISC_ONLY, bool STRESS, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, Tiles T, Grid g,
-                                                    const uint32_t* __restrict__ skey,
-                                                    const uint32_t* __restrict__ start, uint32_t cap,
-                                                    NbrList L, KernelParams kp, int charge) {
-    __shared__ double lds[STRESS ? FNUM : FBX][kStageCap];
-    __shared__ uint16_t lst[kListRows][kTile];
-    __shared__ uint32_t smax[kStages];
-    SPHX_TILE_PROLOGUE();
-    double xi = 0, yi = 0, zi = 0, hi = 1, vxi = 0, vyi = 0, vzi = 0, mi = 0, rhoi = 1, pr2i = 0;
+__global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                    KernelParams kp, int charge) {
+    SPHX_PASS_PROLOGUE();
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
+    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
+    const double mi = P.m[i], rhoi = D.rho[i], pr2i = D.pr2[i];
     double bxi = 0, byi = 0, bzi = 0, irho2i = 0, hb2i = 0;
-    if (valid) {
-        xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
-        vxi = P.vx[p]; vyi = P.vy[p]; vzi = P.vz[p];
-        mi = P.m[p]; rhoi = D.rho[p]; pr2i = D.pr2[p];
-        if (STRESS) {
-            bxi = P.bx[p]; byi = P.by[p]; bzi = P.bz[p];
-            irho2i = 1.0 / (rhoi * rhoi);
-            hb2i = 0.5 * ((bxi * bxi + byi * byi) + bzi * bzi);
-        }
+    if (STRESS) {
+        bxi = P.bx[i]; byi = P.by[i]; bzi = P.bz[i];
+        irho2i = 1.0 / (rhoi * rhoi);
+        hb2i = 0.5 * ((bxi * bxi + byi * byi) + bzi * bzi);
     }
-    constexpr bool uni = UNI;
     const double ac = -kp.alpha * kp.speed_of_sound;
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
-    tile_pass(g, start, tc, L, b_, valid, lst, smax,
-        [&](uint32_t u, uint32_t j) {
-            lds[FX][u] = P.x[j]; lds[FY][u] = P.y[j]; lds[FZ][u] = P.z[j];
-            lds[FVX][u] = P.vx[j]; lds[FVY][u] = P.vy[j]; lds[FVZ][u] = P.vz[j];
-            lds[FRHO][u] = D.rho[j];
-            if (!VISC_ONLY) lds[FPR2][u] = D.pr2[j];
-            if (!uni) { lds[FM][u] = P.m[j]; lds[FH][u] = P.h[j]; }
-            if (STRESS) { lds[FBX % (STRESS ? FNUM : FBX)][u] = P.bx[j]; lds[(FBY) % (STRESS ? FNUM : FBX)][u] = P.by[j]; lds[(FBZ) % (STRESS ? FNUM : FBX)][u] = P.bz[j]; }
-        },
-        [&](uint32_t u, int, const Stage&) {
-            double dx, dy, dz;
-            pair_vec(g, xi, yi, zi, lds[FX][u], lds[FY][u], lds[FZ][u], dx, dy, dz);
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            if (!(r2 > 1e-20)) return;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
-            double r, inv_r;
-            sqrt_rsqrt(r2, r, inv_r);
-            const PairH ph = pair_h<UNI>(kp, hi, uni ? hi : lds[FH][u]);
-            const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;  // (dW/dr) / r
-            const double mm = uni ? kp.m0 * kp.m0 : mi * lds[FM][u];
-            if (!VISC_ONLY) {  // sph.rs:208-221
-                const double sc = (mm * (pr2i + lds[FPR2][u])) * dwr;
-                fp[0] = fma(dx, sc, fp[0]);
-                fp[1] = fma(dy, sc, fp[1]);
-                fp[2] = fma(dz, sc, fp[2]);
-            }
-            // sph.rs:292-314: (v_i - v_j).(x_j - x_i); REFERENCE convention active when < 0
-            const double sdot = fma(vzi - lds[FVZ][u], dz, fma(vyi - lds[FVY][u], dy, (vxi - lds[FVX][u]) * dx));
-            if (ref_conv ? (sdot < 0.0) : (sdot > 0.0)) {
-                const double rho_bar = 0.5 * (rhoi + lds[FRHO][u]);
-                const double ss = ref_conv ? sdot : -sdot;
-                const double mu = (ph.h_avg * ss) * rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
-                const double pi_ij = fma(kp.beta * mu, mu, ac * mu) * rcp_nr(rho_bar);
-                const double sc = (mm * pi_ij) * dwr;
-                fv[0] = fma(dx, sc, fv[0]);
-                fv[1] = fma(dy, sc, fv[1]);
-                fv[2] = fma(dz, sc, fv[2]);
-            }
-            if (STRESS) {
-                constexpr int NB = STRESS ? FNUM : FBX;
-                const double gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;  // grad_i W
-                const double bxj = lds[FBX % NB][u], byj = lds[FBY % NB][u], bzj = lds[FBZ % NB][u];
-                const double rhoj = lds[FRHO][u];
-                const double irho2j = rcp_nr(rhoj * rhoj);
-                const double hb2j = 0.5 * ((bxj * bxj + byj * byj) + bzj * bzj);
-                const double big = (bxi * gx + byi * gy) + bzi * gz;
-                const double bjg = (bxj * gx + byj * gy) + bzj * gz;
-                fm[0] += mm * ((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j);
-                fm[1] += mm * ((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j);
-                fm[2] += mm * ((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j);
-            }
-        });
-    if (valid) force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, p, fp, fv, fm);
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        double dx, dy, dz;
+        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        if (!(r2 > 1e-20)) return;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
+        double r, inv_r;
+        sqrt_rsqrt(r2, r, inv_r);
+        const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
+        const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;  // (dW/dr) / r
+        const double mm = UNI ? kp.m0 * kp.m0 : mi * P.m[j];
+        if (!VISC_ONLY) {  // sph.rs:208-221
+            const double sc = (mm * (pr2i + D.pr2[j])) * dwr;
+            fp[0] = fma(dx, sc, fp[0]);
+            fp[1] = fma(dy, sc, fp[1]);
+            fp[2] = fma(dz, sc, fp[2]);
+        }
+        // sph.rs:292-314: (v_i - v_j).(x_j - x_i); REFERENCE convention active when < 0
+        const double sdot = fma(vzi - P.vz[j], dz, fma(vyi - P.vy[j], dy, (vxi - P.vx[j]) * dx));
+        const double rhoj = D.rho[j];
+        if (ref_conv ? (sdot < 0.0) : (sdot > 0.0)) {
+            const double rho_bar = 0.5 * (rhoi + rhoj);
+            const double ss = ref_conv ? sdot : -sdot;
+            const double mu = (ph.h_avg * ss) * rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
+            const double pi_ij = fma(kp.beta * mu, mu, ac * mu) * rcp_nr(rho_bar);
+            const double sc = (mm * pi_ij) * dwr;
+            fv[0] = fma(dx, sc, fv[0]);
+            fv[1] = fma(dy, sc, fv[1]);
+            fv[2] = fma(dz, sc, fv[2]);
+        }
+        if (STRESS) {
+            const double gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;  // grad_i W
+            const double bxj = P.bx[j], byj = P.by[j], bzj = P.bz[j];
+            const double irho2j = rcp_nr(rhoj * rhoj);
+            const double hb2j = 0.5 * ((bxj * bxj + byj * byj) + bzj * bzj);
+            const double big = (bxi * gx + byi * gy) + bzi * gz;
+            const double bjg = (bxj * gx + byj * gy) + bzj * gz;
+            fm[0] += mm * ((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j);
+            fm[1] += mm * ((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j);
+            fm[2] += mm * ((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j);
+        }
+    });
+    force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
 // FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue.
 template <bool VISC_ONLY, bool STRESS, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D, Mixed X, Tiles T, Grid g,
-                                                          const uint32_t* __restrict__ skey,
-                                                          const uint32_t* __restrict__ start, uint32_t cap,
+__global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
                                                           NbrList L, KernelParams kp, int charge) {
-    constexpr int NB = STRESS ? FNUM : FBX;
-    __shared__ float lds[NB][kStageCap];
-    __shared__ uint16_t lst[kListRows][kTile];
-    __shared__ uint32_t smax[kStages];
-    SPHX_TILE_PROLOGUE();
-    float xi = 0, yi = 0, zi = 0, hi = 1, vxi = 0, vyi = 0, vzi = 0, mi = 0, rhoi = 1, pr2i = 0;
+    SPHX_PASS_PROLOGUE();
+    const float xi = X.xr[i], yi = X.yr[i], zi = X.zr[i], hi = X.h[i];
+    const float vxi = X.vx[i], vyi = X.vy[i], vzi = X.vz[i];
+    const float mi = X.m[i], rhoi = X.rho[i], pr2i = X.pr2[i];
     float bxi = 0, byi = 0, bzi = 0, irho2i = 0, hb2i = 0;
-    if (valid) {
-        xi = X.xr[p]; yi = X.yr[p]; zi = X.zr[p]; hi = X.h[p];
-        vxi = X.vx[p]; vyi = X.vy[p]; vzi = X.vz[p];
-        mi = X.m[p]; rhoi = X.rho[p]; pr2i = X.pr2[p];
-        if (STRESS) {
-            bxi = (float)P.bx[p]; byi = (float)P.by[p]; bzi = (float)P.bz[p];
-            irho2i = 1.0f / (rhoi * rhoi);
-            hb2i = 0.5f * ((bxi * bxi + byi * byi) + bzi * bzi);
-        }
+    if (STRESS) {
+        bxi = (float)P.bx[i]; byi = (float)P.by[i]; bzi = (float)P.bz[i];
+        irho2i = 1.0f / (rhoi * rhoi);
+        hb2i = 0.5f * ((bxi * bxi + byi * byi) + bzi * bzi);
     }
-    constexpr bool uni = UNI;
     const float inv_h0 = (float)kp.inv_h0, h0 = (float)kp.h0;
     const float sig_g0 = (float)(kp.inv_h0 * kp.inv_h0 * kp.inv_h0 * kp.inv_h0 * kInvPi);
     const float mm0 = (float)(kp.m0 * kp.m0);
     const float ac = (float)(-kp.alpha * kp.speed_of_sound);
     const float beta = (float)kp.beta;
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
-    const float wx = (float)g.w[0];
+    const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
-    tile_pass(g, start, tc, L, b_, valid, lst, smax,
-        [&](uint32_t u, uint32_t j) {
-            lds[FX][u] = X.xr[j]; lds[FY][u] = X.yr[j]; lds[FZ][u] = X.zr[j];
-            lds[FVX][u] = X.vx[j]; lds[FVY][u] = X.vy[j]; lds[FVZ][u] = X.vz[j];
-            lds[FRHO][u] = X.rho[j];
-            if (!VISC_ONLY) lds[FPR2][u] = X.pr2[j];
-            if (!uni) { lds[FM][u] = X.m[j]; lds[FH][u] = X.h[j]; }
-            if (STRESS) { lds[FBX % NB][u] = (float)P.bx[j]; lds[FBY % NB][u] = (float)P.by[j]; lds[FBZ % NB][u] = (float)P.bz[j]; }
-        },
-        [&](uint32_t u, int ox, const Stage& st) {
-            const float dx = (lds[FX][u] - xi) + (float)ox * wx;
-            const float dy = (lds[FY][u] - yi) + st.oy;
-            const float dz = (lds[FZ][u] - zi) + st.oz;
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            if (!(r2 > 1e-20f)) return;
-            const float inv_r = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * inv_r;
-            float inv_h = inv_h0, sig = sig_g0, h_avg = h0;
-            if (!uni) {
-                h_avg = 0.5f * (hi + lds[FH][u]);
-                inv_h = __builtin_amdgcn_rcpf(h_avg);
-                const float ih2 = inv_h * inv_h;
-                sig = (ih2 * ih2) * (float)kInvPi;
-            }
-            const float dwr = (sig * dw_shape_f(r * inv_h)) * inv_r;
-            const float mm = uni ? mm0 : mi * lds[FM][u];
-            if (!VISC_ONLY) {
-                const float sc = (mm * (pr2i + lds[FPR2][u])) * dwr;
-                fp[0] += (double)(dx * sc);
-                fp[1] += (double)(dy * sc);
-                fp[2] += (double)(dz * sc);
-            }
-            const float sdot = fmaf(vzi - lds[FVZ][u], dz, fmaf(vyi - lds[FVY][u], dy, (vxi - lds[FVX][u]) * dx));
-            if (ref_conv ? (sdot < 0.0f) : (sdot > 0.0f)) {
-                const float rho_bar = 0.5f * (rhoi + lds[FRHO][u]);
-                const float ss = ref_conv ? sdot : -sdot;
-                const float mu = (h_avg * ss) * __builtin_amdgcn_rcpf(fmaf(0.01f * h_avg, h_avg, r2));
-                const float pi_ij = fmaf(beta * mu, mu, ac * mu) * __builtin_amdgcn_rcpf(rho_bar);
-                const float sc = (mm * pi_ij) * dwr;
-                fv[0] += (double)(dx * sc);
-                fv[1] += (double)(dy * sc);
-                fv[2] += (double)(dz * sc);
-            }
-            if (STRESS) {
-                const float gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;
-                const float bxj = lds[FBX % NB][u], byj = lds[FBY % NB][u], bzj = lds[FBZ % NB][u];
-                const float rhoj = lds[FRHO][u];
-                const float irho2j = __builtin_amdgcn_rcpf(rhoj * rhoj);
-                const float hb2j = 0.5f * ((bxj * bxj + byj * byj) + bzj * bzj);
-                const float big = (bxi * gx + byi * gy) + bzi * gz;
-                const float bjg = (bxj * gx + byj * gy) + bzj * gz;
-                fm[0] += (double)(mm * ((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j));
-                fm[1] += (double)(mm * ((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j));
-                fm[2] += (double)(mm * ((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j));
-            }
-        });
-    if (valid) force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, p, fp, fv, fm);
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        float ox, oy, oz;
+        entry_offset(e, wx, wy, wz, ox, oy, oz);
+        const float dx = (X.xr[j] - xi) + ox;
+        const float dy = (X.yr[j] - yi) + oy;
+        const float dz = (X.zr[j] - zi) + oz;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (!(r2 > 1e-20f)) return;
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * inv_r;
+        float inv_h = inv_h0, sig = sig_g0, h_avg = h0;
+        if (!UNI) {
+            h_avg = 0.5f * (hi + X.h[j]);
+            inv_h = __builtin_amdgcn_rcpf(h_avg);
+            const float ih2 = inv_h * inv_h;
+            sig = (ih2 * ih2) * (float)kInvPi;
+        }
+        const float dwr = (sig * dw_shape_f(r * inv_h)) * inv_r;
+        const float mm = UNI ? mm0 : mi * X.m[j];
+        if (!VISC_ONLY) {
+            const float sc = (mm * (pr2i + X.pr2[j])) * dwr;
+            fp[0] += (double)(dx * sc);
+            fp[1] += (double)(dy * sc);
+            fp[2] += (double)(dz * sc);
+        }
+        const float sdot = fmaf(vzi - X.vz[j], dz, fmaf(vyi - X.vy[j], dy, (vxi - X.vx[j]) * dx));
+        const float rhoj = X.rho[j];
+        if (ref_conv ? (sdot < 0.0f) : (sdot > 0.0f)) {
+            const float rho_bar = 0.5f * (rhoi + rhoj);
+            const float ss = ref_conv ? sdot : -sdot;
+            const float mu = (h_avg * ss) * __builtin_amdgcn_rcpf(fmaf(0.01f * h_avg, h_avg, r2));
+            const float pi_ij = fmaf(beta * mu, mu, ac * mu) * __builtin_amdgcn_rcpf(rho_bar);
+            const float sc = (mm * pi_ij) * dwr;
+            fv[0] += (double)(dx * sc);
+            fv[1] += (double)(dy * sc);
+            fv[2] += (double)(dz * sc);
+        }
+        if (STRESS) {
+            const float gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;
+            const float bxj = (float)P.bx[j], byj = (float)P.by[j], bzj = (float)P.bz[j];
+            const float irho2j = __builtin_amdgcn_rcpf(rhoj * rhoj);
+            const float hb2j = 0.5f * ((bxj * bxj + byj * byj) + bzj * bzj);
+            const float big = (bxi * gx + byi * gy) + bzi * gz;
+            const float bjg = (bxj * gx + byj * gy) + bzj * gz;
+            fm[0] += (double)(mm * ((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j));
+            fm[1] += (double)(mm * ((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j));
+            fm[2] += (double)(mm * ((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j));
+        }
+    });
+    force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
-void launch_forces(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                   const Tiles& T, const NbrList& L, const KernelParams& kp, bool visc_only, bool stress,
-                   bool charge, bool mixed, const Mixed& X, hipStream_t s) {
+void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                   const KernelParams& kp, bool visc_only, bool stress, bool charge, bool mixed, const Mixed& X,
+                   hipStream_t s) {
     if (!n) return;
-    const dim3 grid(tile_grid(n, T)), block(kBlock);
+    const dim3 grid(nblocks(n)), block(kBlock);
     const int ch = charge ? 1 : 0;
     const bool uni = kp.uniform != 0;
 #define SPHX_FORCES(V, M, U)                                                                              \
     if (visc_only == V && stress == M && uni == U) {                                                     \
         if (mixed)                                                                                        \
-            hipLaunchKernelGGL((k_forces_mixed<V, M, U>), grid, block, 0, s, P, D, X, T, g, S.skey,       \
-                               S.cell_start, cap, L, kp, ch);                                             \
+            hipLaunchKernelGGL((k_forces_mixed<V, M, U>), grid, block, 0, s, P, D, X, n, g, L, kp, ch);  \
         else                                                                                              \
-            hipLaunchKernelGGL((k_forces<V, M, U>), grid, block, 0, s, P, D, T, g, S.skey, S.cell_start,  \
-                               cap, L, kp, ch);                                                           \
+            hipLaunchKernelGGL((k_forces<V, M, U>), grid, block, 0, s, P, D, n, g, L, kp, ch);           \
         return;                                                                                           \
     }
     SPHX_FORCES(false, false, false)
@@ -541,216 +475,146 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, uint32_t ca
 
 // ---------------------------------------------------------------- XSPH (a16)
 // sph_xsph_system (sph.rs:349-405); Jacobi: the new velocity goes to tmp3.
-__global__ void __launch_bounds__(kBlock) k_xsph(Particles P, Derived D, Mixed X, Tiles T, Grid g,
-                                                  const uint32_t* __restrict__ skey,
-                                                  const uint32_t* __restrict__ start, uint32_t cap, NbrList L,
+__global__ void __launch_bounds__(kBlock) k_xsph(Particles P, Derived D, Mixed X, uint32_t n, Grid g, NbrList L,
                                                   KernelParams kp, int mixed) {
-    __shared__ double lds[9][kStageCap];
-    __shared__ uint16_t lst[kListRows][kTile];
-    __shared__ uint32_t smax[kStages];
-    SPHX_TILE_PROLOGUE();
-    double xi = 0, yi = 0, zi = 0, hi = 1, vxi = 0, vyi = 0, vzi = 0, rhoi = 1;
-    if (valid) {
-        xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
-        vxi = P.vx[p]; vyi = P.vy[p]; vzi = P.vz[p]; rhoi = D.rho[p];
-    }
+    SPHX_PASS_PROLOGUE();
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
+    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
     double ux = 0, uy = 0, uz = 0;
-    tile_pass(g, start, tc, L, b_, valid, lst, smax,
-        [&](uint32_t u, uint32_t j) {
-            lds[0][u] = P.x[j]; lds[1][u] = P.y[j]; lds[2][u] = P.z[j];
-            lds[3][u] = P.vx[j]; lds[4][u] = P.vy[j]; lds[5][u] = P.vz[j];
-            lds[6][u] = P.m[j]; lds[7][u] = P.h[j]; lds[8][u] = D.rho[j];
-        },
-        [&](uint32_t u, int, const Stage&) {
-            double dx, dy, dz;
-            pair_vec(g, xi, yi, zi, lds[0][u], lds[1][u], lds[2][u], dx, dy, dz);
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            double r = 0.0, inv_r;
-            if (r2 > 0.0) sqrt_rsqrt(r2, r, inv_r);
-            const PairH ph = pair_h<false>(kp, hi, lds[7][u]);
-            const double w = ph.sig_w * w_shape(r * ph.inv_h);
-            const double rho_bar = 0.5 * (rhoi + lds[8][u]);
-            const double f = (kp.xsph_eps * (lds[6][u] * rcp_nr(rho_bar))) * w;
-            ux = fma(f, lds[3][u] - vxi, ux);
-            uy = fma(f, lds[4][u] - vyi, uy);
-            uz = fma(f, lds[5][u] - vzi, uz);
-        });
-    if (!valid) return;
-    D.tmp3[0][p] = vxi + ux;
-    D.tmp3[1][p] = vyi + uy;
-    D.tmp3[2][p] = vzi + uz;
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        double dx, dy, dz;
+        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        double r = 0.0, inv_r;
+        if (r2 > 0.0) sqrt_rsqrt(r2, r, inv_r);
+        const PairH ph = pair_h<false>(kp, hi, P.h[j]);
+        const double w = ph.sig_w * w_shape(r * ph.inv_h);
+        const double rho_bar = 0.5 * (rhoi + D.rho[j]);
+        const double f = (kp.xsph_eps * (P.m[j] * rcp_nr(rho_bar))) * w;
+        ux = fma(f, P.vx[j] - vxi, ux);
+        uy = fma(f, P.vy[j] - vyi, uy);
+        uz = fma(f, P.vz[j] - vzi, uz);
+    });
+    D.tmp3[0][i] = vxi + ux;
+    D.tmp3[1][i] = vyi + uy;
+    D.tmp3[2][i] = vzi + uz;
     if (mixed) {  // f32 copies are read by the next force pass, not by this one
-        X.vx[p] = (float)(vxi + ux);
-        X.vy[p] = (float)(vyi + uy);
-        X.vz[p] = (float)(vzi + uz);
+        X.vx[i] = (float)(vxi + ux);
+        X.vy[i] = (float)(vyi + uy);
+        X.vz[i] = (float)(vzi + uz);
     }
 }
-void launch_xsph(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                 const Tiles& T, const NbrList& L, const KernelParams& kp, bool mixed, const Mixed& X,
-                 hipStream_t s) {
-    if (n)
-        hipLaunchKernelGGL(k_xsph, dim3(tile_grid(n, T)), dim3(kBlock), 0, s, P, D, X, T, g, S.skey,
-                           S.cell_start, cap, L, kp, (int)mixed);
+void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                 const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_xsph, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, X, n, g, L, kp, (int)mixed);
 }
 
 // ------------------------------------------------------ MHD induction (a11)
 // mhd_induction_system (mhd.rs:68-152) in gather form.  The reference adds
 // the pair term to entity_a and its negation to entity_b (mhd.rs:121-122,
 // 130-131), so particle i's term carries +1 when id_i < id_j, -1 otherwise.
-__global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, Tiles T, Grid g,
-                                                       const uint32_t* __restrict__ skey,
-                                                       const uint32_t* __restrict__ start, uint32_t cap,
-                                                       NbrList L, KernelParams kp) {
-    __shared__ double lds[12][kStageCap];
-    __shared__ uint32_t lid[kStageCap];
-    __shared__ uint16_t lst[kListRows][kTile];
-    __shared__ uint32_t smax[kStages];
-    SPHX_TILE_PROLOGUE();
-    double xi = 0, yi = 0, zi = 0, hi = 1, vxi = 0, vyi = 0, vzi = 0, rhoi = 1, bxi = 0, byi = 0, bzi = 0;
-    uint32_t idi = 0;
-    if (valid) {
-        xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
-        vxi = P.vx[p]; vyi = P.vy[p]; vzi = P.vz[p]; rhoi = D.rho[p];
-        bxi = P.bx[p]; byi = P.by[p]; bzi = P.bz[p];
-        idi = P.id[p];
-    }
+__global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                       KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
+    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
+    const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i];
+    const uint32_t idi = P.id[i];
     double ux = 0, uy = 0, uz = 0;
-    tile_pass(g, start, tc, L, b_, valid, lst, smax,
-        [&](uint32_t u, uint32_t j) {
-            lds[0][u] = P.x[j]; lds[1][u] = P.y[j]; lds[2][u] = P.z[j];
-            lds[3][u] = P.vx[j]; lds[4][u] = P.vy[j]; lds[5][u] = P.vz[j];
-            lds[6][u] = P.m[j]; lds[7][u] = P.h[j]; lds[8][u] = D.rho[j];
-            lds[9][u] = P.bx[j]; lds[10][u] = P.by[j]; lds[11][u] = P.bz[j];
-            lid[u] = P.id[j];
-        },
-        [&](uint32_t u, int, const Stage&) {
-            double dx, dy, dz;
-            pair_vec(g, xi, yi, zi, lds[0][u], lds[1][u], lds[2][u], dx, dy, dz);
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            if (!(r2 > 1e-20)) return;
-            double r, inv_r;
-            sqrt_rsqrt(r2, r, inv_r);
-            const PairH ph = pair_h<false>(kp, hi, lds[7][u]);
-            const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;
-            const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;  // r_hat dW/dr
-            const double vx = vxi - lds[3][u], vy = vyi - lds[4][u], vz = vzi - lds[5][u];
-            const double rhoj = lds[8][u], mj = lds[6][u];
-            const double v_dot_g = (vx * gx + vy * gy) + vz * gz;
-            const double b_dot_g = (bxi * gx + byi * gy) + bzi * gz;
-            const double f = mj * rcp_nr(rhoi * rhoj);
-            const double r_dot_g = (dx * gx + dy * gy) + dz * gz;
-            const double res = (2.0 * kp.eta * (mj * rcp_nr(rhoj))) * r_dot_g *
-                               rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
-            const double sg = (idi < lid[u]) ? kp.dt : -kp.dt;
-            ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (lds[9][u] - bxi));
-            uy += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (lds[10][u] - byi));
-            uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (lds[11][u] - bzi));
-        });
-    if (!valid) return;
-    D.tmp3[0][p] = bxi + ux;
-    D.tmp3[1][p] = byi + uy;
-    D.tmp3[2][p] = bzi + uz;
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        double dx, dy, dz;
+        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        if (!(r2 > 1e-20)) return;
+        double r, inv_r;
+        sqrt_rsqrt(r2, r, inv_r);
+        const PairH ph = pair_h<false>(kp, hi, P.h[j]);
+        const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;
+        const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;  // r_hat dW/dr
+        const double vx = vxi - P.vx[j], vy = vyi - P.vy[j], vz = vzi - P.vz[j];
+        const double rhoj = D.rho[j], mj = P.m[j];
+        const double v_dot_g = (vx * gx + vy * gy) + vz * gz;
+        const double b_dot_g = (bxi * gx + byi * gy) + bzi * gz;
+        const double f = mj * rcp_nr(rhoi * rhoj);
+        const double r_dot_g = (dx * gx + dy * gy) + dz * gz;
+        const double res = (2.0 * kp.eta * (mj * rcp_nr(rhoj))) * r_dot_g *
+                           rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
+        const double sg = (idi < P.id[j]) ? kp.dt : -kp.dt;
+        ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (P.bx[j] - bxi));
+        uy += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (P.by[j] - byi));
+        uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (P.bz[j] - bzi));
+    });
+    D.tmp3[0][i] = bxi + ux;
+    D.tmp3[1][i] = byi + uy;
+    D.tmp3[2][i] = bzi + uz;
 }
-void launch_induction(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                      const Tiles& T, const NbrList& L, const KernelParams& kp, hipStream_t s) {
-    if (n)
-        hipLaunchKernelGGL(k_induction, dim3(tile_grid(n, T)), dim3(kBlock), 0, s, P, D, T, g, S.skey,
-                           S.cell_start, cap, L, kp);
+void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                      const KernelParams& kp, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_induction, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
 }
 
 // ---------------------------------------------- divergence cleaning (a12)
 // divergence_cleaning_system (mhd.rs:172-259): Jacobi over B and psi, then
 // psi *= exp(-sigma c_h dt / h) for every particle.
-__global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, Tiles T, Grid g,
-                                                       const uint32_t* __restrict__ skey,
-                                                       const uint32_t* __restrict__ start, uint32_t cap,
-                                                       NbrList L, KernelParams kp) {
-    __shared__ double lds[10][kStageCap];
-    __shared__ uint16_t lst[kListRows][kTile];
-    __shared__ uint32_t smax[kStages];
-    SPHX_TILE_PROLOGUE();
-    double xi = 0, yi = 0, zi = 0, hi = 1, bxi = 0, byi = 0, bzi = 0, psii = 0;
-    if (valid) {
-        xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
-        bxi = P.bx[p]; byi = P.by[p]; bzi = P.bz[p]; psii = P.psi[p];
-    }
+__global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                       KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
+    const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i], psii = P.psi[i];
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
-    tile_pass(g, start, tc, L, b_, valid, lst, smax,
-        [&](uint32_t u, uint32_t j) {
-            lds[0][u] = P.x[j]; lds[1][u] = P.y[j]; lds[2][u] = P.z[j];
-            lds[3][u] = P.m[j]; lds[4][u] = P.h[j]; lds[5][u] = D.rho[j];
-            lds[6][u] = P.bx[j]; lds[7][u] = P.by[j]; lds[8][u] = P.bz[j]; lds[9][u] = P.psi[j];
-        },
-        [&](uint32_t u, int, const Stage&) {
-            double dx, dy, dz;
-            pair_vec(g, xi, yi, zi, lds[0][u], lds[1][u], lds[2][u], dx, dy, dz);
-            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            if (!(r2 > 1e-20)) return;
-            double r, inv_r;
-            sqrt_rsqrt(r2, r, inv_r);
-            const PairH ph = pair_h<false>(kp, hi, lds[4][u]);
-            const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;
-            const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;
-            const double vol = lds[3][u] * rcp_nr(lds[5][u]);
-            div += vol * (((lds[6][u] - bxi) * gx + (lds[7][u] - byi) * gy) + (lds[8][u] - bzi) * gz);
-            const double dp = vol * (lds[9][u] - psii);
-            gpx = fma(dp, gx, gpx);
-            gpy = fma(dp, gy, gpy);
-            gpz = fma(dp, gz, gpz);
-        });
-    if (!valid) return;
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        double dx, dy, dz;
+        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        if (!(r2 > 1e-20)) return;
+        double r, inv_r;
+        sqrt_rsqrt(r2, r, inv_r);
+        const PairH ph = pair_h<false>(kp, hi, P.h[j]);
+        const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;
+        const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;
+        const double vol = P.m[j] * rcp_nr(D.rho[j]);
+        div += vol * (((P.bx[j] - bxi) * gx + (P.by[j] - byi) * gy) + (P.bz[j] - bzi) * gz);
+        const double dp = vol * (P.psi[j] - psii);
+        gpx = fma(dp, gx, gpx);
+        gpy = fma(dp, gy, gpy);
+        gpz = fma(dp, gz, gpz);
+    });
     const double dt = kp.dt;
-    D.tmp3[0][p] = bxi - gpx * dt;
-    D.tmp3[1][p] = byi - gpy * dt;
-    D.tmp3[2][p] = bzi - gpz * dt;
+    D.tmp3[0][i] = bxi - gpx * dt;
+    D.tmp3[1][i] = byi - gpy * dt;
+    D.tmp3[2][i] = bzi - gpz * dt;
     const double decay = exp(-kp.sigma_c * kp.ch * dt / hi);
-    D.tmp1[p] = (psii - (kp.ch * kp.ch) * div * dt) * decay;
+    D.tmp1[i] = (psii - (kp.ch * kp.ch) * div * dt) * decay;
 }
-void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                      const Tiles& T, const NbrList& L, const KernelParams& kp, hipStream_t s) {
-    if (n)
-        hipLaunchKernelGGL(k_div_clean, dim3(tile_grid(n, T)), dim3(kBlock), 0, s, P, D, T, g, S.skey,
-                           S.cell_start, cap, L, kp);
+void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                      const KernelParams& kp, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_div_clean, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
 }
 
 // ---------------------------------------------------------- pair export
-__global__ void __launch_bounds__(kBlock) k_export_pairs(Particles P, Tiles T, Grid g,
-                                                          const uint32_t* __restrict__ skey,
-                                                          const uint32_t* __restrict__ start, uint32_t cap,
-                                                          NbrList L, uint32_t* __restrict__ out,
+__global__ void __launch_bounds__(kBlock) k_export_pairs(Particles P, uint32_t n, NbrList L,
+                                                          uint32_t* __restrict__ out,
                                                           unsigned long long* counter, uint64_t out_cap) {
-    __shared__ uint32_t lid[kStageCap];
-    SPHX_TILE_PROLOGUE();
-    const uint32_t idi = valid ? P.id[p] : 0u;
-    for (int s = 0; s < kStages; ++s) {
-        Stage st;
-        stage_segments(g, start, tc, s, st);
-        const uint32_t cnt_s = valid ? *count_ptr(L, b_, s, t) : 0u;
-        uint32_t k = 0;
-        for (uint32_t cb = 0; cb < st.total; cb += kStageCap) {
-            const uint32_t ce = min(st.total, cb + kStageCap);
-            __syncthreads();
-            SPHX_STAGE_CHUNK({ lid[u] = P.id[j]; });
-            __syncthreads();
-            for_stage_entries(L, b_, t, s, cnt_s, k, cb, ce, [&](uint32_t u, int) {
-                const uint32_t idj = lid[u];
-                if (idi < idj) {
-                    const unsigned long long pos = atomicAdd(counter, 1ull);
-                    if (pos < out_cap) {
-                        out[2 * pos] = idi;
-                        out[2 * pos + 1] = idj;
-                    }
-                }
-            });
+    SPHX_PASS_PROLOGUE();
+    const uint32_t idi = P.id[i];
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t idj = P.id[e & kIdxMask];
+        if (idi < idj) {
+            const unsigned long long pos = atomicAdd(counter, 1ull);
+            if (pos < out_cap) {
+                out[2 * pos] = idi;
+                out[2 * pos + 1] = idj;
+            }
         }
-    }
+    });
 }
-void launch_export_pairs(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S, const Tiles& T,
-                         const NbrList& L, uint32_t* out, unsigned long long* counter, uint64_t out_cap,
-                         hipStream_t s) {
+void launch_export_pairs(const Particles& P, uint32_t n, const NbrList& L, uint32_t* out,
+                         unsigned long long* counter, uint64_t out_cap, hipStream_t s) {
     if (n)
-        hipLaunchKernelGGL(k_export_pairs, dim3(tile_grid(n, T)), dim3(kBlock), 0, s, P, T, g, S.skey,
-                           S.cell_start, cap, L, out, counter, out_cap);
+        hipLaunchKernelGGL(k_export_pairs, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, L, out, counter, out_cap);
 }
 
 }  // namespace sphx

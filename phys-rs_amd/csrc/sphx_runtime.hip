@@ -22,6 +22,9 @@ using namespace sphx;
 
 namespace {
 
+constexpr int kDefaultNeighbours = 192;  // list capacity per particle when max_neighbours <= 0
+constexpr int kMaxNeighbours = 4096;
+
 enum Slot { kSort = 0, kNbr, kDensity, kForce, kViscOnly, kIntegrate, kMhd, kNumSlots };
 
 struct TimedRange {
@@ -261,10 +264,6 @@ int setup_grid(SphxCtx* c, const SphxState* host) {
         if ((rc = dalloc(c, &c->T.scan_sums, ((size_t)nrows + 1023) / 1024 + 1))) return rc;
         if ((rc = dalloc(c, &c->T.p0, tcap)) || (rc = dalloc(c, &c->T.np, tcap)) || (rc = dalloc(c, &c->T.row, tcap)))
             return rc;
-        // neighbour lists are stored per tile
-        if ((rc = dalloc(c, &c->L.idx, (size_t)tcap * kStages * c->L.ks * kTile)) ||
-            (rc = dalloc(c, &c->L.cnt, (size_t)tcap * kStages * kTile)))
-            return rc;
         c->T.cap = tcap;
         c->T.nrows_cap = nrows;
     }
@@ -359,7 +358,7 @@ int do_density(SphxCtx* c) {
     if ((rc = ensure_list(c))) return rc;
     {
         Timer t(c, kDensity);
-        launch_density(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, c->mixed(), c->X, c->stream);
+        launch_density(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
     }
     c->dens = true;
     return launch_check(c);
@@ -374,7 +373,7 @@ int do_forces(SphxCtx* c, bool visc_only) {
     const bool charge = c->has_charge && c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_LORENTZ_CHARGE);
     {
         Timer t(c, visc_only ? kViscOnly : kForce);
-        launch_forces(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, visc_only, stress, charge, c->mixed(),
+        launch_forces(c->P(), c->D, c->n, c->grid, c->L, c->kp, visc_only, stress, charge, c->mixed(),
                       c->X, c->stream);
     }
     c->sp = true;
@@ -416,7 +415,7 @@ int do_xsph(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_xsph(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, c->mixed(), c->X, c->stream);
+        launch_xsph(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
     }
     Particles& P = c->P();
     swap3(P.vx, P.vy, P.vz, c->D.tmp3);
@@ -430,7 +429,7 @@ int do_induction(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_induction(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, c->stream);
+        launch_induction(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->stream);
     }
     Particles& P = c->P();
     swap3(P.bx, P.by, P.bz, c->D.tmp3);
@@ -444,7 +443,7 @@ int do_div_clean(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_div_clean(c->P(), c->D, c->n, c->cap, c->grid, c->S, c->T, c->L, c->kp, c->stream);
+        launch_div_clean(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->stream);
     }
     Particles& P = c->P();
     swap3(P.bx, P.by, P.bz, c->D.tmp3);
@@ -454,17 +453,13 @@ int do_div_clean(SphxCtx* c) {
 }
 
 int check_overflow(SphxCtx* c) {
-    uint32_t ovf[3] = {0, 0, 0};
+    uint32_t ovf[2] = {0, 0};
     HIP_TRY(c, hipMemcpyAsync(ovf, c->L.overflow, sizeof ovf, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (ovf[2])
-        return fail(c, SPHX_E_CAPACITY,
-                    "a tile's neighbour row holds more than %u candidates; the particle distribution is too "
-                    "sparse along x for this grid (set cell_size or grid_lo/grid_hi)", kSlotMask + 1u);
     if (ovf[1])
         return fail(c, SPHX_E_CAPACITY,
-                    "a particle has more than max_neighbours = %d neighbours in one neighbour cell row "
-                    "(%u in total); raise max_neighbours and upload again", c->L.ks, ovf[0]);
+                    "a particle has %u neighbours, more than max_neighbours = %d; create the context with a "
+                    "larger max_neighbours", ovf[0], c->L.ks);
     return SPHX_OK;
 }
 
@@ -534,8 +529,8 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     c->cap = capacity ? capacity : 1;
     c->prm = *params;
     make_kernel_params(c);
-    c->L.ks = params->max_neighbours > 0 ? params->max_neighbours : 64;
-    if (c->L.ks > 255) {
+    c->L.ks = params->max_neighbours > 0 ? params->max_neighbours : kDefaultNeighbours;
+    if (c->L.ks > kMaxNeighbours) {
         delete c;
         return SPHX_E_INVALID;
     }
@@ -570,7 +565,9 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     if (dalloc(c, &c->S.key, N) || dalloc(c, &c->S.rank, N) || dalloc(c, &c->S.perm, N) ||
         dalloc(c, &c->S.skey, N))
         return bail(SPHX_E_OOM);
-    if (dalloc(c, &c->L.ntot, N) || dalloc(c, &c->L.overflow, 4)) return bail(SPHX_E_OOM);
+    if (dalloc(c, &c->L.ntot, N) || dalloc(c, &c->L.overflow, 4) ||
+        dalloc(c, &c->L.idx, ((N + 63) / 64) * 64 * (size_t)c->L.ks))
+        return bail(SPHX_E_OOM);
     if (dalloc(c, &c->energy_partial, (size_t)energy_blocks(c->cap) * 9)) return bail(SPHX_E_OOM);
     if (dalloc(c, &c->pair_counter, 1)) return bail(SPHX_E_OOM);
     launch_fill_u32(c->L.overflow, 0u, 4, c->stream);
@@ -787,7 +784,7 @@ int sphx_export_pairs(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t*
         c->pair_buf_cap = np;
     }
     launch_fill_u32(reinterpret_cast<uint32_t*>(c->pair_counter), 0u, 2, c->stream);
-    launch_export_pairs(c->P(), c->n, c->cap, c->grid, c->S, c->T, c->L, c->pair_buf, c->pair_counter,
+    launch_export_pairs(c->P(), c->n, c->L, c->pair_buf, c->pair_counter,
                         c->pair_buf_cap, c->stream);
     if ((rc = launch_check(c))) return rc;
     unsigned long long got = 0;

@@ -71,7 +71,7 @@ def test_pairs_bit_exact_random_variable_h(require_gpu, periodic):
 def test_pairs_dense_periodic_chunked(require_gpu):
     # cell_size 0.25 -> 4 cells per axis, ~156 particles per cell: a 256-particle
     # tile spans 2-3 cells, so each neighbour row is the whole periodic grid row
-    # (~625 candidates), more than one LDS chunk (512): staging runs in two
+    # (~625 candidates), more than one LDS chunk (384) of the neighbour build: staging runs in two
     # chunks.  h = 0.06 keeps ~72 neighbours per particle.
     rng = scenarios.SplitMix64(33)
     n = 10000
