@@ -197,6 +197,20 @@ __device__ __forceinline__ double eos_dpdrho(const KernelParams& kp, double rho)
     return kp.iso_cs2;
 }
 
+// v of particle i in the force pass's gather record
+__device__ __forceinline__ void set_record_velocity(const Derived& D, uint32_t i, bool mixed, double vx, double vy,
+                                                    double vz) {
+    if (mixed) {
+        D.r32[i].vx = (float)vx;
+        D.r32[i].vy = (float)vy;
+        D.r32[i].vz = (float)vz;
+    } else {
+        D.r64[i].vx = vx;
+        D.r64[i].vy = vy;
+        D.r64[i].vz = vz;
+    }
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
     return v;

@@ -41,8 +41,24 @@ struct Particles {
     uint32_t *id, *orig;       // ECS order id; upload slot
 };
 
+// Gather records of the force pass: one particle's neighbour-side fields in
+// one 64-byte (f64) or 32-byte (FP32-mixed) block, so a neighbour costs 4 or
+// 2 wide loads from one cache line instead of 8 gathers from 8 arrays.
+// Written by the density pass (x, rho, P/rho^2 and v) and kept current by
+// every later writer of v (vv_velocity, XSPH).
+struct alignas(64) Rec64 {
+    double x, y, z, rho;
+    double vx, vy, vz, pr2;
+};
+struct alignas(32) Rec32 {
+    float xr, yr, zr, rho;     // cell-relative position
+    float vx, vy, vz, pr2;
+};
+
 // Per-particle quantities that are recomputed after every sort.
 struct Derived {
+    Rec64* r64;
+    Rec32* r32;
     double *rho, *p, *pr2;     // density, pressure, P / rho^2
     double *ax, *ay, *az;      // acceleration
     double *spx, *spy, *spz;   // m g + sum of pressure terms (reused by VISC_ONLY)
@@ -90,14 +106,14 @@ struct NbrList {
     int ks;                    // list capacity per particle
 };
 
-// FP32-mixed form: pair-side copies in f32.  Positions are offsets from the
-// particle's own cell origin (computed in f64, rounded once), so a pair vector
-// is (xr_j - xr_i) + (cell offset) * w, never a difference of large numbers.
+// f32 copies in sorted order.  Positions are offsets from the particle's own
+// cell origin (computed in f64, rounded once), so a pair vector is
+// (xr_j - xr_i) + (cell offset) * w, never a difference of large numbers.
+// The neighbour build prefilters with them in both forms; the FP32-mixed
+// passes compute with them.
 struct Mixed {
     float *xr, *yr, *zr;
-    float *vx, *vy, *vz;
     float *m, *h;
-    float *rho, *pr2;
 };
 
 struct KernelParams {          // what the kernels need from SphxParams, by value
@@ -123,13 +139,13 @@ void launch_iota_u32(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_cell_hash(const Particles& P, uint32_t n, const Grid& g, const Sort& S, hipStream_t s);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, hipStream_t s);
 void launch_scatter(const Sort& S, uint32_t n, hipStream_t s);
-void launch_cell_order(const Sort& S, uint32_t ncell, hipStream_t s);
+void launch_cell_order(const Sort& S, uint32_t ncell, const double* x, hipStream_t s);
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n,
                     bool mhd, bool charge, bool mixed, const Mixed& X, const Grid& g, hipStream_t s);
 void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
-void launch_build_nbr(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S,
-                      const Tiles& T, const NbrList& L, bool mixed, const Mixed& X, hipStream_t s);
+void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
+                      const NbrList& L, bool mixed, const Mixed& X, double h_max, hipStream_t s);
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                     const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,

@@ -62,88 +62,129 @@ __device__ __forceinline__ bool exact_neighbour(double dx, double dy, double dz,
 // f64 predicate on the f64 positions.  Owners further than 2 cells from
 // their cell origin (particles clamped into the grid) test every candidate
 // exactly.
+//
+// Cells are sorted by x and a row's cells are consecutive in the sorted
+// order, so a thread's candidates in one neighbour row -- the three cells
+// around its own, cut to |dx| < reach = 2 h_max (1 + 1e-4) -- are one
+// contiguous slot range, found by binary search on the staged f32 x at both
+// ends.  All threads of a wave then walk ranges of about two cells in
+// lockstep.  The f32 error of dx (<~1e-7 w for |xr| <= 2w) is far inside the
+// 1e-4 margin, so no neighbour is cut; owners outside that bound do not cut.
+// Rows whose three cells are not consecutive (a periodic wrap) walk the
+// cells one at a time.
 template <bool MIXED>
 __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T, Grid g,
                                                    const uint32_t* __restrict__ skey,
-                                                   const uint32_t* __restrict__ start, uint32_t cap,
-                                                   NbrList L) {
+                                                   const uint32_t* __restrict__ start, NbrList L,
+                                                   float reach) {
     __shared__ float lds[4][kStageCap];
     SPHX_TILE_PROLOGUE();
-    (void)cap;
     const int cx = valid ? (int)(skey[p] % (uint32_t)g.dims[0]) : 0;
     float xf = 0, yf = 0, zf = 0, hf = 0;
     double xi = 0, yi = 0, zi = 0, hi = 0;
     bool exact_all = false;
     if (valid) {
         xf = X.xr[p]; yf = X.yr[p]; zf = X.zr[p]; hf = X.h[p];
-        if (!MIXED) {
-            xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
-            const float lim = 2.0f * (float)fmax(g.w[0], fmax(g.w[1], g.w[2]));
-            exact_all = fabsf(xf) > lim || fabsf(yf) > lim || fabsf(zf) > lim;
-        }
+        const float lim = 2.0f * (float)fmax(g.w[0], fmax(g.w[1], g.w[2]));
+        exact_all = fabsf(xf) > lim || fabsf(yf) > lim || fabsf(zf) > lim;
+        if (!MIXED) { xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p]; }
     }
+    const bool trim = !exact_all;
+    const float wx = (float)g.w[0];
     uint32_t total = 0;
     uint32_t* __restrict__ out = nbr_base(L, p);
     for (int s = 0; s < kStages; ++s) {
         Stage st;
         stage_segments(g, start, tc, s, st);
-        // the three cells' slot ranges, ordered by slot so that a stage's
-        // entries ascend in memory (a periodic row spanning the whole grid
-        // stages in physical, not logical, cell order)
-        uint32_t ra[3], rb[3], ro[3];
-        for (int o = 0; o < 3; ++o) {
-            cell_slots(g, start, st, cx, o - 1, ra[o], rb[o]);
-            ro[o] = (uint32_t)o;
-        }
+        uint32_t ra[3], rb[3];
+        for (int o = 0; o < 3; ++o) cell_slots(g, start, st, cx, o - 1, ra[o], rb[o]);
         uint32_t self = 0xffffffffu;
         if (s == 4 && valid) self = ra[1] + (p - start[st.rowbase + cx]);
+        const bool e0 = ra[0] == rb[0], e2 = ra[2] == rb[2];
+        const bool merged = (e0 || rb[0] == ra[1]) && (e2 || rb[1] == ra[2]);
+        const bool single = st.total <= (uint32_t)kStageCap;
+        // candidate test of slot (cell x offset o - 1); u = slot - cb
+        auto test = [&](uint32_t slot, uint32_t u, uint32_t o) {
+            if (slot == self) return;
+            // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
+            const float dx = (lds[0][u] - xf) + (float)((int)o - 1) * wx, dy = (lds[1][u] - yf) + st.oy,
+                        dz = (lds[2][u] - zf) + st.oz;
+            const float tt = hf + lds[3][u];  // 2 h_avg
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float t2 = tt * tt;
+            bool in;
+            if (MIXED) {
+                in = r2 < t2;
+            } else if (!exact_all && r2 < t2 * (1.0f - 1e-5f)) {
+                in = true;
+            } else if (!exact_all && r2 > t2 * (1.0f + 1e-5f)) {
+                in = false;
+            } else {
+                const uint32_t j = slot_particle(st, slot);
+                double ddx, ddy, ddz;
+                pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], ddx, ddy, ddz);
+                in = exact_neighbour(ddx, ddy, ddz, hi, P.h[j]);
+            }
+            if (in) {
+                if (total < (uint32_t)L.ks)
+                    out[(size_t)total << 6] = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
+                ++total;
+            }
+        };
+        // first slot of [a, e) (ascending x, whole stage staged) with
+        // (x_slot - x_i) + off beyond the bound: below -reach (left cell) or
+        // at least reach (right cell)
+        auto cut = [&](uint32_t a, uint32_t e, float off, bool left) {
+            while (a < e) {
+                const uint32_t mid = (a + e) >> 1;
+                const float d = (lds[0][mid] - xf) + off;
+                if (left ? d <= -reach : d < reach) a = mid + 1;
+                else e = mid;
+            }
+            return a;
+        };
+        // non-merged rows: the cells in slot order (a periodic row spanning
+        // the whole grid stages in physical, not logical, cell order)
+        uint32_t qa[3] = {ra[0], ra[1], ra[2]}, qb[3] = {rb[0], rb[1], rb[2]}, qo[3] = {0, 1, 2};
+        if (!merged) {
 #define SPHX_SWAP_RANGE(a_, b_)                                                            \
-        if (ra[b_] < ra[a_]) {                                                             \
-            uint32_t t0 = ra[a_]; ra[a_] = ra[b_]; ra[b_] = t0;                            \
-            t0 = rb[a_]; rb[a_] = rb[b_]; rb[b_] = t0;                                     \
-            t0 = ro[a_]; ro[a_] = ro[b_]; ro[b_] = t0;                                     \
-        }
-        SPHX_SWAP_RANGE(0, 1)
-        SPHX_SWAP_RANGE(1, 2)
-        SPHX_SWAP_RANGE(0, 1)
+            if (qa[b_] < qa[a_]) {                                                         \
+                uint32_t t0 = qa[a_]; qa[a_] = qa[b_]; qa[b_] = t0;                        \
+                t0 = qb[a_]; qb[a_] = qb[b_]; qb[b_] = t0;                                 \
+                t0 = qo[a_]; qo[a_] = qo[b_]; qo[b_] = t0;                                 \
+            }
+            SPHX_SWAP_RANGE(0, 1)
+            SPHX_SWAP_RANGE(1, 2)
+            SPHX_SWAP_RANGE(0, 1)
 #undef SPHX_SWAP_RANGE
+        }
         for (uint32_t cb = 0; cb < st.total; cb += kStageCap) {
             const uint32_t ce = min(st.total, cb + kStageCap);
             __syncthreads();
             SPHX_STAGE_CHUNK({ lds[0][u] = X.xr[j]; lds[1][u] = X.yr[j]; lds[2][u] = X.zr[j]; lds[3][u] = X.h[j]; });
             __syncthreads();
             if (!valid) continue;
-            for (int q = 0; q < 3; ++q) {
-                const uint32_t a = max(ra[q], cb), e = min(rb[q], ce);
-                const uint32_t o = ro[q];
-                const float ox = (float)(((int)o - 1) * g.w[0]);
-                for (uint32_t slot = a; slot < e; ++slot) {
-                    if (slot == self) continue;
-                    const uint32_t u = slot - cb;
-                    // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
-                    const float dx = (lds[0][u] - xf) + ox, dy = (lds[1][u] - yf) + st.oy,
-                                dz = (lds[2][u] - zf) + st.oz;
-                    const float tt = hf + lds[3][u];  // 2 h_avg
-                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float t2 = tt * tt;
-                    bool in;
-                    if (MIXED) {
-                        in = r2 < t2;
-                    } else if (!exact_all && r2 < t2 * (1.0f - 1e-5f)) {
-                        in = true;
-                    } else if (!exact_all && r2 > t2 * (1.0f + 1e-5f)) {
-                        in = false;
-                    } else {
-                        const uint32_t j = slot_particle(st, slot);
-                        double ddx, ddy, ddz;
-                        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], ddx, ddy, ddz);
-                        in = exact_neighbour(ddx, ddy, ddz, hi, P.h[j]);
+            if (merged) {
+                uint32_t A = e0 ? ra[1] : ra[0], E = e2 ? rb[1] : rb[2];
+                if (trim && single) {
+                    if (!e0) A = cut(ra[0], rb[0], -wx, true);
+                    if (!e2) E = cut(ra[2], rb[2], wx, false);
+                }
+                A = max(A, cb);
+                E = min(E, ce);
+                for (uint32_t slot = A; slot < E; ++slot)
+                    test(slot, slot - cb, slot < ra[1] ? 0u : (slot < rb[1] ? 1u : 2u));
+            } else {
+                for (int q = 0; q < 3; ++q) {
+                    uint32_t a = qa[q], e = qb[q];
+                    const uint32_t o = qo[q];
+                    if (trim && single && o != 1u && a < e) {
+                        if (o == 0u) a = cut(a, e, -wx, true);
+                        else e = cut(a, e, wx, false);
                     }
-                    if (in) {
-                        if (total < (uint32_t)L.ks)
-                            out[(size_t)total << 6] = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
-                        ++total;
-                    }
+                    a = max(a, cb);
+                    e = min(e, ce);
+                    for (uint32_t slot = a; slot < e; ++slot) test(slot, slot - cb, o);
                 }
             }
         }
@@ -156,14 +197,15 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
     }
 }
 
-void launch_build_nbr(const Particles& P, uint32_t n, uint32_t cap, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, bool mixed, const Mixed& X, hipStream_t s) {
+void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
+                      const NbrList& L, bool mixed, const Mixed& X, double h_max, hipStream_t s) {
     if (!n) return;
     const dim3 grid(tile_grid(n, T)), block(kBlock);
+    const float reach = (float)(2.0 * h_max * (1.0 + 1e-4));
     if (mixed)
-        hipLaunchKernelGGL(k_build<true>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, cap, L);
+        hipLaunchKernelGGL(k_build<true>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach);
     else
-        hipLaunchKernelGGL(k_build<false>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, cap, L);
+        hipLaunchKernelGGL(k_build<false>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach);
 }
 
 // ------------------------------------------------------- density + EOS (a4, a5)
@@ -212,9 +254,15 @@ __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint
     });
     rho = UNI ? fma(kp.m0, acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
+    const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
     D.p[i] = pr;
-    D.pr2[i] = pr / (rho * rho);
+    D.pr2[i] = pr2;
+    double2* rec = reinterpret_cast<double2*>(D.r64 + i);
+    rec[0] = make_double2(xi, yi);
+    rec[1] = make_double2(zi, rho);
+    rec[2] = make_double2(P.vx[i], P.vy[i]);
+    rec[3] = make_double2(P.vz[i], pr2);
 }
 
 // FP32-mixed density: f32 pair terms on cell-relative coordinates, f64 sum.
@@ -251,8 +299,9 @@ __global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D
     D.rho[i] = rho;
     D.p[i] = pr;
     D.pr2[i] = pr2;
-    X.rho[i] = (float)rho;
-    X.pr2[i] = (float)pr2;
+    float4* rec = reinterpret_cast<float4*>(D.r32 + i);
+    rec[0] = make_float4(xi, yi, zi, (float)rho);
+    rec[1] = make_float4((float)P.vx[i], (float)P.vy[i], (float)P.vz[i], (float)pr2);
 }
 
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
@@ -327,10 +376,12 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
     const double ac = -kp.alpha * kp.speed_of_sound;
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
+    const double2* __restrict__ R = reinterpret_cast<const double2*>(D.r64);
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
+        const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 > 1e-20)) return;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
         double r, inv_r;
@@ -339,14 +390,14 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
         const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;  // (dW/dr) / r
         const double mm = UNI ? kp.m0 * kp.m0 : mi * P.m[j];
         if (!VISC_ONLY) {  // sph.rs:208-221
-            const double sc = (mm * (pr2i + D.pr2[j])) * dwr;
+            const double sc = (mm * (pr2i + vzp.y)) * dwr;
             fp[0] = fma(dx, sc, fp[0]);
             fp[1] = fma(dy, sc, fp[1]);
             fp[2] = fma(dz, sc, fp[2]);
         }
         // sph.rs:292-314: (v_i - v_j).(x_j - x_i); REFERENCE convention active when < 0
-        const double sdot = fma(vzi - P.vz[j], dz, fma(vyi - P.vy[j], dy, (vxi - P.vx[j]) * dx));
-        const double rhoj = D.rho[j];
+        const double sdot = fma(vzi - vzp.x, dz, fma(vyi - vxy.y, dy, (vxi - vxy.x) * dx));
+        const double rhoj = zr.y;
         if (ref_conv ? (sdot < 0.0) : (sdot > 0.0)) {
             const double rho_bar = 0.5 * (rhoi + rhoj);
             const double ss = ref_conv ? sdot : -sdot;
@@ -377,9 +428,11 @@ template <bool VISC_ONLY, bool STRESS, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
                                                           NbrList L, KernelParams kp, int charge) {
     SPHX_PASS_PROLOGUE();
-    const float xi = X.xr[i], yi = X.yr[i], zi = X.zr[i], hi = X.h[i];
-    const float vxi = X.vx[i], vyi = X.vy[i], vzi = X.vz[i];
-    const float mi = X.m[i], rhoi = X.rho[i], pr2i = X.pr2[i];
+    const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
+    const float4 own0 = R[2 * i], own1 = R[2 * i + 1];
+    const float xi = own0.x, yi = own0.y, zi = own0.z, rhoi = own0.w;
+    const float vxi = own1.x, vyi = own1.y, vzi = own1.z, pr2i = own1.w;
+    const float hi = X.h[i], mi = UNI ? 0.0f : X.m[i];
     float bxi = 0, byi = 0, bzi = 0, irho2i = 0, hb2i = 0;
     if (STRESS) {
         bxi = (float)P.bx[i]; byi = (float)P.by[i]; bzi = (float)P.bz[i];
@@ -396,11 +449,12 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
+        const float4 a = R[2 * j], b = R[2 * j + 1];
         float ox, oy, oz;
         entry_offset(e, wx, wy, wz, ox, oy, oz);
-        const float dx = (X.xr[j] - xi) + ox;
-        const float dy = (X.yr[j] - yi) + oy;
-        const float dz = (X.zr[j] - zi) + oz;
+        const float dx = (a.x - xi) + ox;
+        const float dy = (a.y - yi) + oy;
+        const float dz = (a.z - zi) + oz;
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         if (!(r2 > 1e-20f)) return;
         const float inv_r = __builtin_amdgcn_rsqf(r2);
@@ -415,13 +469,13 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
         const float dwr = (sig * dw_shape_f(r * inv_h)) * inv_r;
         const float mm = UNI ? mm0 : mi * X.m[j];
         if (!VISC_ONLY) {
-            const float sc = (mm * (pr2i + X.pr2[j])) * dwr;
+            const float sc = (mm * (pr2i + b.w)) * dwr;
             fp[0] += (double)(dx * sc);
             fp[1] += (double)(dy * sc);
             fp[2] += (double)(dz * sc);
         }
-        const float sdot = fmaf(vzi - X.vz[j], dz, fmaf(vyi - X.vy[j], dy, (vxi - X.vx[j]) * dx));
-        const float rhoj = X.rho[j];
+        const float sdot = fmaf(vzi - b.z, dz, fmaf(vyi - b.y, dy, (vxi - b.x) * dx));
+        const float rhoj = a.w;
         if (ref_conv ? (sdot < 0.0f) : (sdot > 0.0f)) {
             const float rho_bar = 0.5f * (rhoi + rhoj);
             const float ss = ref_conv ? sdot : -sdot;
@@ -499,11 +553,8 @@ __global__ void __launch_bounds__(kBlock) k_xsph(Particles P, Derived D, Mixed X
     D.tmp3[0][i] = vxi + ux;
     D.tmp3[1][i] = vyi + uy;
     D.tmp3[2][i] = vzi + uz;
-    if (mixed) {  // f32 copies are read by the next force pass, not by this one
-        X.vx[i] = (float)(vxi + ux);
-        X.vy[i] = (float)(vyi + uy);
-        X.vz[i] = (float)(vzi + uz);
-    }
+    // the record is read by the next force pass, not by this one (Jacobi)
+    set_record_velocity(D, i, mixed != 0, vxi + ux, vyi + uy, vzi + uz);
 }
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                  const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {

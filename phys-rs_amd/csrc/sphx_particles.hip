@@ -48,7 +48,8 @@ void launch_vv_position_flip(const Particles& P, const Derived& D, uint32_t n, c
 }
 
 // velocity_verlet_velocity_system (integrators.rs:92-104)
-__global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams kp, int mixed, Mixed X) {
+// The force pass's gather record is updated with the new velocity.
+__global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams kp, int mixed) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double vx = P.vx[i] + (P.apx[i] + D.ax[i]) * kp.half_dt;
@@ -57,15 +58,12 @@ __global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams k
     P.vx[i] = vx;
     P.vy[i] = vy;
     P.vz[i] = vz;
-    if (mixed) {
-        X.vx[i] = (float)vx;
-        X.vy[i] = (float)vy;
-        X.vz[i] = (float)vz;
-    }
+    set_record_velocity(D, i, mixed != 0, vx, vy, vz);
 }
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, bool mixed,
                         const Mixed& X, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_vv_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, (int)mixed, X);
+    (void)X;
+    if (n) hipLaunchKernelGGL(k_vv_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, (int)mixed);
 }
 
 // ---------------------------------------------------------- energy

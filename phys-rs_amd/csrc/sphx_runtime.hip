@@ -56,6 +56,7 @@ struct SphxCtx {
     bool has_mhd = false, has_charge = false;
     bool uniform = false;                  // all m equal and all h equal
     double m0 = 0.0, h0 = 0.0;
+    double hmax = 0.0;                     // largest smoothing radius (h is fixed per particle)
     // validity of derived data for the current positions and sorted order
     bool sorted = false, listed = false, dens = false, sp = false, acc = false;
     // timing
@@ -174,6 +175,7 @@ int setup_grid(SphxCtx* c, const SphxState* host) {
     double hmax = 0.0;
     for (uint32_t i = 0; i < host->n; ++i) hmax = std::max(hmax, host->h[i]);
     if (host->n && !(hmax > 0.0)) return fail(c, SPHX_E_INVALID, "smoothing radius must be > 0");
+    c->hmax = hmax;
     double cs = 2.0 * hmax * (1.0 + 1e-9);
     if (p.cell_size > cs) cs = p.cell_size;
     if (!(cs > 0.0)) cs = 1.0;
@@ -334,14 +336,14 @@ int do_neighbours(SphxCtx* c) {
         launch_cell_hash(src, n, c->grid, c->S, c->stream);
         launch_scan(c->S.cell_count, c->S.cell_start, c->grid.ncell, c->S.block_sums, c->stream);
         launch_scatter(c->S, n, c->stream);
-        launch_cell_order(c->S, c->grid.ncell, c->stream);
+        launch_cell_order(c->S, c->grid.ncell, src.x, c->stream);
         launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->mixed(), c->X, c->grid, c->stream);
         c->cur = 1 - c->cur;
         launch_tiles(c->grid, c->S, c->T, c->stream);
     }
     {
         Timer t(c, kNbr);
-        launch_build_nbr(c->P(), n, c->cap, c->grid, c->S, c->T, c->L, c->mixed(), c->X, c->stream);
+        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->mixed(), c->X, c->hmax, c->stream);
     }
     c->sorted = c->listed = true;
     c->dens = c->sp = c->acc = false;
@@ -556,9 +558,10 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
                      &D.tmp3[0], &D.tmp3[1], &D.tmp3[2], &D.tmp1};
     for (double** d : dd)
         if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
+    if (dalloc(c, &D.r64, N) || dalloc(c, &D.r32, N)) return bail(SPHX_E_OOM);
     {
         Mixed& X = c->X;
-        float** ff[] = {&X.xr, &X.yr, &X.zr, &X.vx, &X.vy, &X.vz, &X.m, &X.h, &X.rho, &X.pr2};
+        float** ff[] = {&X.xr, &X.yr, &X.zr, &X.m, &X.h};
         for (float** f : ff)
             if (dalloc(c, f, N)) return bail(SPHX_E_OOM);
     }

@@ -127,26 +127,31 @@ void launch_scatter(const Sort& S, uint32_t n, hipStream_t s) {
                            S.cell_start, n, S.perm);
 }
 
-// Atomic ranks arrive in any order; sorting each cell's slice of perm by the
-// previous index makes the order deterministic (and stable with respect to
-// the previous step, which keeps the reorder gather nearly sequential).
-__global__ void k_cell_order(const uint32_t* __restrict__ start, uint32_t ncell,
-                             uint32_t* __restrict__ perm) {
+// Atomic ranks arrive in any order; sorting each cell's slice of perm by x
+// (ties by the previous index) makes the order deterministic, lets the
+// neighbour build cut the two side cells of a row to the reach in x, and
+// stays nearly sorted from one step to the next (insertion sort is cheap).
+__global__ void k_cell_order(const uint32_t* __restrict__ start, uint32_t ncell, uint32_t* __restrict__ perm,
+                             const double* __restrict__ x) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const uint32_t b = start[c], e = start[c + 1];
     for (uint32_t k = b + 1; k < e; ++k) {
-        uint32_t v = perm[k];
+        const uint32_t v = perm[k];
+        const double xv = x[v];
         uint32_t t = k;
-        while (t > b && perm[t - 1] > v) {
-            perm[t] = perm[t - 1];
+        while (t > b) {
+            const uint32_t u = perm[t - 1];
+            const double xu = x[u];
+            if (xu < xv || (xu == xv && u < v)) break;
+            perm[t] = u;
             --t;
         }
         perm[t] = v;
     }
 }
-void launch_cell_order(const Sort& S, uint32_t ncell, hipStream_t s) {
-    hipLaunchKernelGGL(k_cell_order, dim3(nblocks(ncell)), dim3(kBlock), 0, s, S.cell_start, ncell, S.perm);
+void launch_cell_order(const Sort& S, uint32_t ncell, const double* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_order, dim3(nblocks(ncell)), dim3(kBlock), 0, s, S.cell_start, ncell, S.perm, x);
 }
 
 // Gather of all persistent state into the new order; in the mixed form also
@@ -190,12 +195,7 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
     X.yr[k] = (float)(y - (g.lo[1] + cy * g.w[1]));
     X.zr[k] = (float)(z - (g.lo[2] + cz * g.w[2]));
     X.h[k] = (float)h;
-    if (mixed) {
-        X.vx[k] = (float)vx;
-        X.vy[k] = (float)vy;
-        X.vz[k] = (float)vz;
-        X.m[k] = (float)m;
-    }
+    if (mixed) X.m[k] = (float)m;
 }
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n, bool mhd,
                     bool charge, bool mixed, const Mixed& X, const Grid& g, hipStream_t s) {
