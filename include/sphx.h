@@ -182,6 +182,25 @@ int sphx_div_clean(SphxCtx* ctx);
  * nsteps times: forces(x_n, v_n) -> vv_position -> neighbours/density/
  * forces(x_n+1, v_n) -> vv_velocity -> xsph -> induction -> cleaning.     */
 int sphx_step(SphxCtx* ctx, uint32_t nsteps);
+
+/* ---- differentiable step (new; the reference has no adjoint) ----------- */
+/* Runs nsteps ticks of sphx_step from the current state and returns
+ *   L = sum_i |x_i(nsteps) - target_i|^2 / (3 n)
+ * with its gradient with respect to the initial velocities (grad_v*), the
+ * initial positions (grad_x*, optional: all three NULL to skip) and the
+ * viscosity alpha, by a hand-written reverse pass over the same schedule
+ * (main.rs:103-157; pressure sph.rs:177-228, viscosity sph.rs:260-322,
+ * density sph.rs:72-116, EOS sph.rs:149-169, integrators.rs:63-104).
+ * The forward keeps a checkpoint (x, v) every checkpoint_every ticks (0:
+ * every nsteps) and recomputes each segment's tape during the reverse pass.
+ * Arrays are in upload order.  On return the context holds the state after
+ * nsteps ticks, as sphx_step would leave it.  FP64 form only; no XSPH, MHD,
+ * penalty walls or periodic boxes (SPHX_E_INVALID).  The neighbour sets and
+ * the viscosity switch are those of the forward trajectory.              */
+int sphx_adjoint(SphxCtx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
+                 const double* target_x, const double* target_y, const double* target_z,
+                 double* loss, double* grad_x, double* grad_y, double* grad_z,
+                 double* grad_vx, double* grad_vy, double* grad_vz, double* grad_alpha);
 int sphx_synchronize(SphxCtx* ctx);
 
 /* ---- test and measurement -------------------------------------------- */

@@ -53,7 +53,7 @@ def lib() -> C.CDLL:
         L.orc_energy.restype = None
         L.orc_energy.argtypes = [C.POINTER(OrcState), C.c_void_p, PD]
         L.orc_adjoint.restype = C.c_int
-        L.orc_adjoint.argtypes = [C.POINTER(OrcState), C.c_void_p, C.c_uint32, PD, PD, PD, PD]
+        L.orc_adjoint.argtypes = [C.POINTER(OrcState), C.c_void_p, C.c_uint32, PD, PD, PD, PD, PD]
         L.orc_loss.restype = C.c_double
         L.orc_loss.argtypes = [C.POINTER(OrcState), C.c_void_p, C.c_uint32, PD]
         _lib = L
@@ -176,16 +176,20 @@ def energy(st: State, params) -> Dict[str, float]:
             "momentum": out[4:7].tolist()}
 
 
-def adjoint(st: State, params, nsteps: int, target: np.ndarray):
-    """Loss, dL/dv0 (n x 3) and dL/dalpha for L = mean |x_N - target|^2; st ends at x_N."""
+def adjoint(st: State, params, nsteps: int, target: np.ndarray, with_gx: bool = False):
+    """Loss, dL/dv0 (n x 3) and dL/dalpha for L = mean |x_N - target|^2 (and
+    dL/dx0 when with_gx); st ends at x_N."""
     tgt = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
     loss = C.c_double(0.0)
     ga = C.c_double(0.0)
     gv = np.zeros(3 * st.n)
+    gx = np.zeros(3 * st.n)
     rc = lib().orc_adjoint(st.ref(), _pp(params), nsteps, tgt.ctypes.data_as(PD), C.byref(loss),
-                           gv.ctypes.data_as(PD), C.byref(ga))
+                           gx.ctypes.data_as(PD), gv.ctypes.data_as(PD), C.byref(ga))
     if rc != 0:
         raise ValueError("orc_adjoint: unsupported options")
+    if with_gx:
+        return loss.value, gx.reshape(-1, 3), gv.reshape(-1, 3), ga.value
     return loss.value, gv.reshape(-1, 3), ga.value
 
 

@@ -154,11 +154,12 @@ static void snap_free(Snap* S) {
     free(S->x); free(S->y); free(S->z); free(S->vx); free(S->vy); free(S->vz);
 }
 
-/* Gradient of L with respect to the initial velocities (gv: 3n interleaved)
- * and the viscosity alpha; s holds the initial state and is left at x_N.
- * target: 3n interleaved.  Returns 0, or -1 for unsupported options. */
+/* Gradient of L with respect to the initial positions (gx, may be NULL) and
+ * velocities (gv; both 3n interleaved) and the viscosity alpha; s holds the
+ * initial state and is left at x_N.  target: 3n interleaved.  Returns 0, or
+ * -1 for unsupported options. */
 int orc_adjoint(OrcState* s, const SphxParams* p, uint32_t nsteps, const double* target, double* loss,
-                double* gv, double* galpha) {
+                double* gx, double* gv, double* galpha) {
     if ((p->boundary_flags & (SPHX_BND_PENALTY_CUBE | SPHX_BND_PERIODIC)) || s->bx || s->q ||
         p->xsph_epsilon != 0.0)
         return -1;
@@ -242,6 +243,8 @@ int orc_adjoint(OrcState* s, const SphxParams* p, uint32_t nsteps, const double*
         free(pairs);
     }
     for (uint32_t k = 0; k < 3 * n; ++k) gv[k] = vb[k];
+    if (gx)
+        for (uint32_t k = 0; k < 3 * n; ++k) gx[k] = xb[k];
     *galpha = alphab;
     snap_load(&A[nsteps], s);
     for (uint32_t t = 0; t <= nsteps; ++t) snap_free(&A[t]);

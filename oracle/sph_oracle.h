@@ -36,7 +36,7 @@ void orc_step(OrcState* s, const SphxParams* p, uint32_t nsteps);
 void orc_energy(const OrcState* s, const SphxParams* p, double out[8]);
 /* sph_adjoint.c */
 int orc_adjoint(OrcState* s, const SphxParams* p, uint32_t nsteps, const double* target, double* loss,
-                double* gv, double* galpha);
+                double* gx, double* gv, double* galpha);
 double orc_loss(OrcState* s, const SphxParams* p, uint32_t nsteps, const double* target);
 
 #ifdef __cplusplus

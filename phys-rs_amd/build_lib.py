@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["sphx_sort.hip", "sphx_pairs.hip", "sphx_particles.hip", "sphx_runtime.hip"]
+SOURCES = ["sphx_sort.hip", "sphx_pairs.hip", "sphx_particles.hip", "sphx_adjoint.hip", "sphx_runtime.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -1,0 +1,232 @@
+// sphx_adjoint.hip — reverse-mode gradient of the velocity-Verlet SPH tick.
+//
+// The tick (main.rs:103-157) is x_{n+1} = R(x_n + v_n dt + a_n dt^2/2),
+// v' = S v_n, a_prev = S a_n, a* = A(x_{n+1}, v'), v_{n+1} = v' + (a_prev + a*) dt/2,
+// with a_n = A(x_n, v_n), R the wall reflection and S = diag(+-1) its sign.
+// A = gravity + pressure (sph.rs:177-228) + viscosity (sph.rs:260-322) on
+// the summation density (sph.rs:72-116) and its EOS (sph.rs:149-169).
+//
+// The vector-Jacobian product of A at (x, v) with a cotangent abar is, per
+// undirected pair (i, j) with d = x_j - x_i, u = abar_i/m_i - abar_j/m_j,
+// c = u.d and G = (dW/dr)/r, the derivative of c * m_i m_j (T + Pi) G
+// (T = P_i/rho_i^2 + P_j/rho_j^2) in d, v, rho and alpha; rhobar then flows
+// through the density sum.  Both passes run in gather form (one thread per
+// particle, each pair visited from both ends, each end adding its own share),
+// so no atomics and every sum is deterministic.  The neighbour sets and the
+// viscosity switch are those of the forward state (the map is piecewise
+// smooth; this is the gradient of the active piece).
+//
+// Adjoint vectors live in upload order (index = SphxState slot), which does
+// not change when the particles are re-sorted; kernels gather and scatter
+// through Particles::orig.
+
+#include "sphx_device.h"
+
+namespace sphx {
+
+// cotangent of the acceleration, gathered into sorted order (ab_s) for the
+// pair pass; apb keeps hdt * vbar for the reflection step
+__global__ void k_adj_seed(const uint32_t* __restrict__ orig, uint32_t n, double hdt, Vec3 vb, Vec3 apb, Vec3 ab_s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = orig[i];
+    const double x = hdt * vb.x[u], y = hdt * vb.y[u], z = hdt * vb.z[u];
+    apb.x[u] = x; apb.y[u] = y; apb.z[u] = z;
+    ab_s.x[i] = x; ab_s.y[i] = y; ab_s.z[i] = z;
+}
+void launch_adj_seed(const uint32_t* orig, uint32_t n, double hdt, const Vec3& vb, const Vec3& apb, const Vec3& ab_s,
+                     hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_adj_seed, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, hdt, vb, apb, ab_s);
+}
+
+// dst (sorted) = src (upload order) through orig
+__global__ void k_gather3(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, Vec3 dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = orig[i];
+    dst.x[i] = src.x[u]; dst.y[i] = src.y[u]; dst.z[i] = src.z[u];
+}
+void launch_gather3(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_gather3, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, dst);
+}
+
+// dst (upload order) = src (sorted) through orig
+__global__ void k_scatter3(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, Vec3 dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = orig[i];
+    dst.x[u] = src.x[i]; dst.y[u] = src.y[i]; dst.z[u] = src.z[i];
+}
+void launch_scatter3(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_scatter3, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, dst);
+}
+
+// Reverse of the position update and wall reflection (integrators.rs:63-83):
+// ybar = S xbar; vbar = S vbar' + dt ybar; abar = S abar_prev + dt^2/2 ybar;
+// xbar = ybar.  flip bit k of particle u set: S_k = -1.
+__global__ void k_adj_flip(uint32_t n, const uint8_t* __restrict__ flip, double dt, double hdt2, Vec3 xb, Vec3 vb,
+                           Vec3 apb, Vec3 ab) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const uint32_t f = flip[u];
+    double* xs[3] = {xb.x, xb.y, xb.z};
+    double* vs[3] = {vb.x, vb.y, vb.z};
+    double* ps[3] = {apb.x, apb.y, apb.z};
+    double* as[3] = {ab.x, ab.y, ab.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double sg = ((f >> k) & 1u) ? -1.0 : 1.0;
+        const double y = sg * xs[k][u];
+        vs[k][u] = sg * vs[k][u] + dt * y;
+        as[k][u] = sg * ps[k][u] + hdt2 * y;
+        xs[k][u] = y;
+    }
+}
+void launch_adj_flip(uint32_t n, const uint8_t* flip, double dt, double hdt2, const Vec3& xb, const Vec3& vb,
+                     const Vec3& apb, const Vec3& ab, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_adj_flip, dim3(nblocks(n)), dim3(kBlock), 0, s, n, flip, dt, hdt2, xb, vb, apb, ab);
+}
+
+// Block sum of one double per thread; thread 0 gets the total.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+        __syncthreads();
+    }
+    return lds[0];
+}
+
+// L = sum |x - target|^2 / (3n) seeds xbar = dL/dx_N (upload order) and
+// leaves one partial sum of squares per block.
+__global__ void k_adj_loss(Particles P, uint32_t n, Vec3 target, Vec3 xb, double* __restrict__ part) {
+    __shared__ double lds[kBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e2 = 0.0;
+    if (i < n) {
+        const uint32_t u = P.orig[i];
+        const double s = 2.0 / (3.0 * (double)n);
+        const double ex = P.x[i] - target.x[u], ey = P.y[i] - target.y[u], ez = P.z[i] - target.z[u];
+        xb.x[u] = s * ex; xb.y[u] = s * ey; xb.z[u] = s * ez;
+        e2 = (ex * ex + ey * ey) + ez * ez;
+    }
+    const double t = block_sum(e2, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const Vec3& xb, double* part,
+                     hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_adj_loss, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, target, xb, part);
+}
+
+// VJP of the pressure + viscosity acceleration at the current (sorted)
+// state.  ab_s: abar in sorted order.  Adds dL/dx and dL/dv into xb, vb
+// (upload order), writes rhobar (sorted) and adds this block's share of
+// dL/dalpha to alpha_acc[block].
+__global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                        KernelParams kp, Vec3 ab_s, Vec3 xb, Vec3 vb,
+                                                        double* __restrict__ rhob, double* __restrict__ alpha_acc) {
+    __shared__ double lds[kBlock];
+    const uint32_t i = particle_index();
+    double da = 0.0;
+    if (i < n) {
+        const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i];
+        const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
+        const double rhoi = D.rho[i], pr2i = D.pr2[i];
+        const double inv_mi = 1.0 / mi;
+        const double uxi = ab_s.x[i] * inv_mi, uyi = ab_s.y[i] * inv_mi, uzi = ab_s.z[i] * inv_mi;
+        // d(P_i/rho_i^2)/d(rho_i)
+        const double dTi = eos_dpdrho(kp, rhoi) / (rhoi * rhoi) - 2.0 * pr2i / rhoi;
+        const double cs = kp.speed_of_sound, alpha = kp.alpha, beta = kp.beta;
+        const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
+        const double sc = ref_conv ? 1.0 : -1.0;
+        double gx = 0, gy = 0, gz = 0, gvx = 0, gvy = 0, gvz = 0, grho = 0;
+        for_neighbours(L, i, [&](uint32_t e) {
+            const uint32_t j = e & kIdxMask;
+            double dx, dy, dz;
+            pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            if (!(r2 > 1e-20)) return;  // r <= 1e-10: no pressure force (sph.rs:52)
+            double r, inv_r;
+            sqrt_rsqrt(r2, r, inv_r);
+            const double h = 0.5 * (hi + P.h[j]);
+            const double inv_h = rcp_nr(h);
+            const double G = kernel_dw(r, inv_h) * inv_r;
+            const double Gp = (kernel_d2w(r, inv_h) - G) * inv_r;  // dG/dr
+            const double mj = P.m[j], mm = mi * mj;
+            const double inv_mj = 1.0 / mj;
+            const double ux = uxi - ab_s.x[j] * inv_mj, uy = uyi - ab_s.y[j] * inv_mj, uz = uzi - ab_s.z[j] * inv_mj;
+            const double c = fma(uz, dz, fma(uy, dy, ux * dx));
+            const double rhoj = D.rho[j];
+            const double T = pr2i + D.pr2[j];
+            const double vx = vxi - P.vx[j], vy = vyi - P.vy[j], vz = vzi - P.vz[j];
+            const double sdot = fma(vz, dz, fma(vy, dy, vx * dx));
+            const bool act = ref_conv ? (sdot < 0.0) : (sdot > 0.0);
+            double Pi = 0.0, Pi_mu = 0.0, mu = 0.0, Dn = 1.0, rho_bar = 1.0;
+            if (act) {
+                Dn = fma(0.01 * h, h, r2);
+                mu = (h * (sc * sdot)) / Dn;
+                rho_bar = 0.5 * (rhoi + rhoj);
+                Pi = (-alpha * cs * mu + beta * (mu * mu)) / rho_bar;
+                Pi_mu = (-alpha * cs + 2.0 * beta * mu) / rho_bar;
+            }
+            const double phi = mm * (T + Pi) * G;
+            const double q = c * mm * (T + Pi) * Gp * inv_r;
+            const double cg = c * mm * G;
+            const double w = act ? cg * Pi_mu / Dn : 0.0;  // d(c phi)/d(mu) * dmu/d(.) prefactor
+            // dL/dd = phi u + q d + w (h sc v - 2 mu d); particle i is the pair's "a": xbar_i -= dL/dd
+            gx -= phi * ux + q * dx + w * (h * sc * vx - 2.0 * mu * dx);
+            gy -= phi * uy + q * dy + w * (h * sc * vy - 2.0 * mu * dy);
+            gz -= phi * uz + q * dz + w * (h * sc * vz - 2.0 * mu * dz);
+            // dL/dv_i
+            gvx += w * h * sc * dx;
+            gvy += w * h * sc * dy;
+            gvz += w * h * sc * dz;
+            grho += cg * (dTi + (act ? -Pi / (2.0 * rho_bar) : 0.0));
+            if (act) da += 0.5 * cg * (-cs * mu / rho_bar);  // each pair is visited from both ends
+        });
+        const uint32_t u = P.orig[i];
+        xb.x[u] += gx; xb.y[u] += gy; xb.z[u] += gz;
+        vb.x[u] += gvx; vb.y[u] += gvy; vb.z[u] += gvz;
+        rhob[i] = grho;
+    }
+    const double t = block_sum(da, lds);
+    if (threadIdx.x == 0) alpha_acc[xcd_block()] += t;
+}
+
+// VJP of the summation density: rho_i = m_i W(0, h_i) + sum_j m_j W(r_ij, h_ij)
+// gives xbar_i -= sum_j (rhobar_i m_j + rhobar_j m_i) G d.
+__global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n, Grid g, NbrList L,
+                                                         const double* __restrict__ rhob, Vec3 xb) {
+    const uint32_t i = particle_index();
+    if (i >= n) return;
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i], rbi = rhob[i];
+    double gx = 0, gy = 0, gz = 0;
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        double dx, dy, dz;
+        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        if (!(r2 > 0.0)) return;
+        double r, inv_r;
+        sqrt_rsqrt(r2, r, inv_r);
+        const double inv_h = rcp_nr(0.5 * (hi + P.h[j]));
+        const double coef = (rbi * P.m[j] + rhob[j] * mi) * (kernel_dw(r, inv_h) * inv_r);
+        gx = fma(-coef, dx, gx);
+        gy = fma(-coef, dy, gy);
+        gz = fma(-coef, dz, gz);
+    });
+    const uint32_t u = P.orig[i];
+    xb.x[u] += gx; xb.y[u] += gy; xb.z[u] += gz;
+}
+
+void launch_vjp(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                const KernelParams& kp, const Vec3& ab_s, const Vec3& xb, const Vec3& vb, double* rhob,
+                double* alpha_acc, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_vjp_forces, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp, ab_s, xb, vb, rhob,
+                       alpha_acc);
+    hipLaunchKernelGGL(k_vjp_density, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, g, L, (const double*)rhob, xb);
+}
+
+}  // namespace sphx

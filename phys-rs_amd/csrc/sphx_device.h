@@ -15,7 +15,6 @@ static constexpr int kStageCap = 384;           // candidates staged in LDS per 
 static constexpr double kInvPi = 0.31830988618379067154;  // 1/pi
 static constexpr double kPi = 3.141592653589793;           // std::f64::consts::PI
 
-static inline uint32_t nblocks(size_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 // blockIdx -> logical block so that each XCD (blocks are dealt round-robin to
 // the 8 XCDs) works on one contiguous range of tiles / particles and its L2
@@ -213,6 +212,10 @@ __device__ __forceinline__ void set_record_velocity(const Derived& D, uint32_t i
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
     return v;
 }
 

@@ -81,6 +81,8 @@ struct Sort {
 // two contiguous slices of the sorted arrays, staged through LDS.
 static constexpr int kTile = 256;
 static constexpr int kStages = 9;
+// blocks of a one-thread-per-particle launch (kTile threads per block)
+static inline uint32_t nblocks(size_t n, int b = kTile) { return (uint32_t)((n + b - 1) / b); }
 struct Tiles {
     uint32_t* row_count;       // tiles per row (nrows)
     uint32_t* row_off;         // exclusive scan (nrows + 1); row_off[nrows] = tile count
@@ -114,6 +116,10 @@ struct NbrList {
 struct Mixed {
     float *xr, *yr, *zr;
     float *m, *h;
+};
+
+struct Vec3 {
+    double *x, *y, *z;
 };
 
 struct KernelParams {          // what the kernels need from SphxParams, by value
@@ -169,5 +175,19 @@ void launch_energy(const Particles& P, const Derived& D, uint32_t n, uint32_t ca
                    const NbrList& L, const KernelParams& kp, bool mhd, double* partial,
                    uint32_t nblocks, hipStream_t s);
 uint32_t energy_blocks(uint32_t n);
+
+// adjoint (sphx_adjoint.hip); Vec3 arrays named *_s are in sorted order,
+// the others in upload order
+void launch_adj_seed(const uint32_t* orig, uint32_t n, double hdt, const Vec3& vb, const Vec3& apb,
+                     const Vec3& ab_s, hipStream_t s);
+void launch_gather3(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst_s, hipStream_t s);
+void launch_scatter3(const uint32_t* orig, uint32_t n, const Vec3& src_s, const Vec3& dst, hipStream_t s);
+void launch_adj_flip(uint32_t n, const uint8_t* flip, double dt, double hdt2, const Vec3& xb, const Vec3& vb,
+                     const Vec3& apb, const Vec3& ab, hipStream_t s);
+void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const Vec3& xb, double* part,
+                     hipStream_t s);
+void launch_vjp(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
+                const KernelParams& kp, const Vec3& ab_s, const Vec3& xb, const Vec3& vb, double* rhob,
+                double* alpha_acc, hipStream_t s);
 
 }  // namespace sphx

@@ -7,8 +7,8 @@ namespace sphx {
 
 // velocity_verlet_position_system (integrators.rs:63-83), then the box:
 // reflection mirrors x and flips v and the stored acceleration; periodic wraps.
-// When `flip` is set, bit k of flip[i] records a reflection on axis k (the
-// adjoint needs the sign of the map).
+// When `flip` is set, bit k of flip[orig[i]] (upload order) records a
+// reflection on axis k (the adjoint needs the sign of the map).
 __global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams kp, uint8_t* flip) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -36,7 +36,7 @@ __global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams k
         vs[k][i] = v;
         aps[k][i] = ap;
     }
-    if (flip) flip[i] = (uint8_t)bits;
+    if (flip) flip[P.orig[i]] = (uint8_t)bits;
 }
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         hipStream_t s) {

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sphx_internal.h"
@@ -33,6 +34,30 @@ struct TimedRange {
 };
 
 }  // namespace
+
+// Device memory of sphx_adjoint, kept between calls and grown on demand.
+// Tapes and adjoint vectors are in upload order.
+struct AdjBuffers {
+    uint32_t n = 0, k = 0, m = 0;  // particles, ticks per segment, checkpoints
+    std::vector<void*> allocs;
+    double* ckpt = nullptr;        // m x 6n: x, y, z, vx, vy, vz at ticks 0, k, 2k, ...
+    double* tx = nullptr;          // (k + 1) x 3n: x_t of the current segment
+    double* tv = nullptr;          // (k + 1) x 3n: v_t
+    double* tvp = nullptr;         // k x 3n: v after the position update (v')
+    uint8_t* flip = nullptr;       // k x n: wall reflections of each tick
+    double* fin = nullptr;         // 6n: state after the last tick
+    double* adj = nullptr;         // 4 x 3n: xbar, vbar, abar, abar_prev
+    double* ab_s = nullptr;        // 3n: abar in sorted order
+    double* rhob = nullptr;        // n: rhobar in sorted order
+    double* target = nullptr;      // 3n
+    double* part = nullptr;        // per-block loss partial sums
+    double* alpha_acc = nullptr;   // per-block dL/dalpha
+    void release() {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        n = k = m = 0;
+    }
+};
 
 struct SphxCtx {
     int device = 0;
@@ -67,6 +92,7 @@ struct SphxCtx {
     uint64_t launches[kNumSlots] = {0};
     std::vector<void*> allocs;
     std::string err;
+    AdjBuffers adj;
 
     Particles& P() { return A[cur]; }
     bool mixed() const { return prm.precision == SPHX_PREC_FP32_MIXED; }
@@ -454,6 +480,79 @@ int do_div_clean(SphxCtx* c) {
     return launch_check(c);
 }
 
+// ---- adjoint ---------------------------------------------------------------
+Vec3 vec3(double* base, uint32_t n) { return Vec3{base, base + n, base + 2 * (size_t)n}; }
+Vec3 pos3(Particles& P) { return Vec3{P.x, P.y, P.z}; }
+Vec3 vel3(Particles& P) { return Vec3{P.vx, P.vy, P.vz}; }
+
+int adj_alloc(SphxCtx* c, uint32_t n, uint32_t k, uint32_t m) {
+    AdjBuffers& A = c->adj;
+    if (n <= A.n && k <= A.k && m <= A.m) return SPHX_OK;
+    A.release();
+    const size_t N = n, nb = nblocks(n);
+    auto get = [&](auto** p, size_t count) -> int {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(**p));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(c, SPHX_E_OOM, "adjoint: hipMalloc of %zu bytes failed", count * sizeof(**p));
+        }
+        A.allocs.push_back(q);
+        *p = static_cast<std::remove_reference_t<decltype(*p)>>(q);
+        return SPHX_OK;
+    };
+    int rc;
+    if ((rc = get(&A.ckpt, (size_t)m * 6 * N)) || (rc = get(&A.tx, (size_t)(k + 1) * 3 * N)) ||
+        (rc = get(&A.tv, (size_t)(k + 1) * 3 * N)) || (rc = get(&A.tvp, (size_t)k * 3 * N)) ||
+        (rc = get(&A.flip, (size_t)k * N)) || (rc = get(&A.fin, 6 * N)) || (rc = get(&A.adj, 12 * N)) ||
+        (rc = get(&A.ab_s, 3 * N)) || (rc = get(&A.rhob, N)) || (rc = get(&A.target, 3 * N)) ||
+        (rc = get(&A.part, nb)) || (rc = get(&A.alpha_acc, nb))) {
+        A.release();
+        return rc;
+    }
+    A.n = n;
+    A.k = k;
+    A.m = m;
+    return SPHX_OK;
+}
+
+// state (upload order) -> working arrays (current sorted order); x may be NULL
+void adj_load(SphxCtx* c, const double* x3, const double* v3) {
+    const uint32_t n = c->n;
+    Particles& P = c->P();
+    if (x3) {
+        launch_gather3(P.orig, n, vec3(const_cast<double*>(x3), n), pos3(P), c->stream);
+        c->sorted = c->listed = c->dens = c->sp = c->acc = false;
+    }
+    launch_gather3(P.orig, n, vec3(const_cast<double*>(v3), n), vel3(P), c->stream);
+    c->acc = false;
+}
+void adj_save(SphxCtx* c, double* x3, double* v3) {
+    const uint32_t n = c->n;
+    Particles& P = c->P();
+    if (x3) launch_scatter3(P.orig, n, pos3(P), vec3(x3, n), c->stream);
+    if (v3) launch_scatter3(P.orig, n, vel3(P), vec3(v3, n), c->stream);
+}
+
+// One tick of sphx_step (no XSPH / MHD); records the reflections and v' when asked.
+int adj_tick(SphxCtx* c, uint8_t* flip, double* vp3) {
+    int rc;
+    if ((rc = do_forces(c, c->sp && c->dens))) return rc;
+    if (!c->acc) return fail(c, SPHX_E_STATE, "vv_position needs accelerations");
+    {
+        Timer t(c, kIntegrate);
+        if (flip) launch_vv_position_flip(c->P(), c->D, c->n, c->kp, flip, c->stream);
+        else launch_vv_position(c->P(), c->D, c->n, c->kp, c->stream);
+    }
+    c->sorted = c->listed = c->dens = c->sp = c->acc = false;
+    if ((rc = launch_check(c))) return rc;
+    if (vp3) adj_save(c, nullptr, vp3);
+    if ((rc = do_neighbours(c))) return rc;
+    if ((rc = do_density(c))) return rc;
+    if ((rc = do_forces(c, false))) return rc;
+    return do_vv_velocity(c);
+}
+
 int check_overflow(SphxCtx* c) {
     uint32_t ovf[2] = {0, 0};
     HIP_TRY(c, hipMemcpyAsync(ovf, c->L.overflow, sizeof ovf, hipMemcpyDeviceToHost, c->stream));
@@ -589,6 +688,7 @@ void sphx_destroy(SphxCtx* c) {
         (void)hipEventDestroy(t.b);
     }
     for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
+    c->adj.release();
     free_all(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -761,6 +861,97 @@ int sphx_step(SphxCtx* c, uint32_t nsteps) {
         if (c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_CLEANING) && (rc = do_div_clean(c))) return rc;
     }
     return check_overflow(c);
+}
+
+int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx, const double* tgy,
+                 const double* tgz, double* loss, double* gx, double* gy, double* gz, double* gvx, double* gvy,
+                 double* gvz, double* galpha) {
+    SPHX_ENTER(c);
+    if (!tgx || !tgy || !tgz || !loss || !gvx || !gvy || !gvz || !galpha)
+        return fail(c, SPHX_E_INVALID, "target, loss, grad_v and grad_alpha are required");
+    if ((gx || gy || gz) && !(gx && gy && gz)) return fail(c, SPHX_E_INVALID, "grad_x, grad_y, grad_z go together");
+    if (nsteps == 0 || c->n == 0) return fail(c, SPHX_E_INVALID, "sphx_adjoint needs particles and nsteps >= 1");
+    if (c->mixed()) return fail(c, SPHX_E_INVALID, "sphx_adjoint runs in the FP64 form only");
+    if (c->prm.boundary_flags & (SPHX_BND_PENALTY_CUBE | SPHX_BND_PERIODIC))
+        return fail(c, SPHX_E_INVALID, "sphx_adjoint supports open and reflective boxes only");
+    if (c->has_mhd || c->prm.xsph_epsilon != 0.0)
+        return fail(c, SPHX_E_INVALID, "sphx_adjoint does not differentiate XSPH or MHD");
+    const uint32_t n = c->n;
+    const uint32_t k = (every == 0 || every > nsteps) ? nsteps : every;
+    const uint32_t M = (nsteps + k - 1) / k;
+    int rc;
+    if ((rc = adj_alloc(c, n, k, M))) return rc;
+    AdjBuffers& A = c->adj;
+    const size_t n3 = 3 * (size_t)n;
+    HIP_TRY(c, hipMemcpyAsync(A.target, tgx, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(A.target + n, tgy, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(A.target + 2 * (size_t)n, tgz, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    const Vec3 xb = vec3(A.adj, n), vb = vec3(A.adj + n3, n), ab = vec3(A.adj + 2 * n3, n),
+               apb = vec3(A.adj + 3 * n3, n), ab_s = vec3(A.ab_s, n);
+    const uint32_t last0 = (M - 1) * k;  // first tick of the last segment
+    // forward: checkpoints every k ticks, the last segment's tape kept
+    for (uint32_t t = 0; t < nsteps; ++t) {
+        const uint32_t s = t % k;
+        if (s == 0) adj_save(c, A.ckpt + (size_t)(t / k) * 6 * n, A.ckpt + (size_t)(t / k) * 6 * n + n3);
+        if (t >= last0) {
+            adj_save(c, A.tx + (t - last0) * n3, A.tv + (t - last0) * n3);
+            rc = adj_tick(c, A.flip + (size_t)(t - last0) * n, A.tvp + (t - last0) * n3);
+        } else {
+            rc = adj_tick(c, nullptr, nullptr);
+        }
+        if (rc) return rc;
+    }
+    if ((rc = check_overflow(c))) return rc;
+    adj_save(c, A.fin, A.fin + n3);
+    // loss and dL/dx_N
+    launch_adj_loss(c->P(), n, vec3(A.target, n), xb, A.part, c->stream);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(vb.x), 0u, 2 * n3, c->stream);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(A.alpha_acc), 0u, 2 * (size_t)nblocks(n), c->stream);
+    const double dt = c->kp.dt, hdt = c->kp.half_dt, hdt2 = c->kp.half_dt_sq;
+    // reverse, one segment at a time
+    for (uint32_t m = M; m-- > 0;) {
+        const uint32_t t0 = m * k, len = std::min(k, nsteps - t0);
+        if (m + 1 < M) {  // recompute the segment's tape from its checkpoint
+            adj_load(c, A.ckpt + (size_t)m * 6 * n, A.ckpt + (size_t)m * 6 * n + n3);
+            for (uint32_t s = 0; s < len; ++s) {
+                adj_save(c, A.tx + s * n3, A.tv + s * n3);
+                if ((rc = adj_tick(c, A.flip + (size_t)s * n, A.tvp + s * n3))) return rc;
+            }
+        }
+        for (uint32_t s = len; s-- > 0;) {
+            // at (x_{t+1}, v'_t): positions sorted, listed and dense already
+            adj_load(c, nullptr, A.tvp + s * n3);
+            launch_adj_seed(c->P().orig, n, hdt, vb, apb, ab_s, c->stream);
+            launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, ab_s, xb, vb, A.rhob, A.alpha_acc, c->stream);
+            launch_adj_flip(n, A.flip + (size_t)s * n, dt, hdt2, xb, vb, apb, ab, c->stream);
+            // at (x_t, v_t)
+            adj_load(c, A.tx + s * n3, A.tv + s * n3);
+            if ((rc = do_neighbours(c)) || (rc = do_density(c))) return rc;
+            launch_gather3(c->P().orig, n, ab, ab_s, c->stream);
+            launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, ab_s, xb, vb, A.rhob, A.alpha_acc, c->stream);
+            if ((rc = launch_check(c))) return rc;
+        }
+    }
+    if ((rc = check_overflow(c))) return rc;
+    // results, then the state after the last tick back in place
+    const uint32_t nb = nblocks(n);
+    std::vector<double> part(nb), acc(nb);
+    HIP_TRY(c, hipMemcpyAsync(part.data(), A.part, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(acc.data(), A.alpha_acc, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    double* outs[6] = {gvx, gvy, gvz, gx, gy, gz};
+    const double* srcs[6] = {vb.x, vb.y, vb.z, xb.x, xb.y, xb.z};
+    for (int q = 0; q < 6; ++q)
+        if (outs[q]) HIP_TRY(c, hipMemcpyAsync(outs[q], srcs[q], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    adj_load(c, A.fin, A.fin + n3);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    double l = 0.0, ga = 0.0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        l += part[b];
+        ga += acc[b];
+    }
+    *loss = l / (3.0 * (double)n);
+    *galpha = ga;
+    return launch_check(c);
 }
 
 int sphx_synchronize(SphxCtx* c) {

@@ -240,6 +240,27 @@ class Context:
     def synchronize(self) -> None:
         self._check(self._lib.sphx_synchronize(self._h))
 
+    def adjoint(self, nsteps: int, target: np.ndarray, checkpoint_every: int = 0,
+                with_grad_x: bool = False) -> Dict[str, object]:
+        """Loss mean|x_N - target|^2 and its gradient in the initial velocities
+        (n x 3), the viscosity alpha and (with_grad_x) the initial positions.
+        target is n x 3 in upload order; the context ends at x_N."""
+        n = self.n
+        tgt = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(n, 3).T)
+        gv = np.zeros((3, n))
+        gx = np.zeros((3, n)) if with_grad_x else None
+        loss = C.c_double(0.0)
+        ga = C.c_double(0.0)
+        d = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        self._check(self._lib.sphx_adjoint(
+            self._h, nsteps, checkpoint_every, d(tgt[0]), d(tgt[1]), d(tgt[2]), C.byref(loss),
+            d(gx[0]) if gx is not None else None, d(gx[1]) if gx is not None else None,
+            d(gx[2]) if gx is not None else None, d(gv[0]), d(gv[1]), d(gv[2]), C.byref(ga)))
+        out = {"loss": loss.value, "grad_v": gv.T.copy(), "grad_alpha": ga.value}
+        if gx is not None:
+            out["grad_x"] = gx.T.copy()
+        return out
+
     # -- measurement
     def export_pairs(self) -> np.ndarray:
         count = C.c_uint64(0)

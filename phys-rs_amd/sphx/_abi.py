@@ -92,6 +92,7 @@ SIGNATURES = {
     "sphx_mhd_induction": (C.c_int, [C.c_void_p]),
     "sphx_div_clean": (C.c_int, [C.c_void_p]),
     "sphx_step": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "sphx_adjoint": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32] + [C.POINTER(C.c_double)] * 11),
     "sphx_synchronize": (C.c_int, [C.c_void_p]),
     "sphx_export_pairs": (C.c_int, [C.c_void_p, PU, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sphx_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),

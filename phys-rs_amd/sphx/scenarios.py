@@ -57,12 +57,16 @@ def lattice(nx: int, ny: int, nz: int, spacing: float, origin=(0.0, 0.0, 0.0)) -
 
 def dam_break(nx: int = 32, ny: int = 32, nz: int = 16, spacing: float = 0.01,
               box=(0.8, 0.4, 0.4), jitter: float = 0.01, seed: int = 0, alpha: float = 0.1,
-              dt: float = 1e-4) -> Tuple[Dict[str, np.ndarray], object]:
+              dt: float = 1e-4, v_sigma: float = 0.0) -> Tuple[Dict[str, np.ndarray], object]:
     """Dam-break block in a reflective box (the 16,384 / 65,536 / 1,048,576 workloads).
 
     Lattice nx*ny*nz at `spacing` from the box corner, uniform jitter of
     +-jitter*spacing per axis, h = 1.3*spacing, m = rho0*spacing^3, Tait
     rho0 = 1000, gamma = 7, c0 = 20, Monaghan alpha (beta = 0), g = -9.81 z.
+    Particles start at rest unless v_sigma > 0 (seeded N(0, v_sigma^2) per
+    component; the differentiable form uses it: at rest every pair sits on
+    the viscosity switch (v_ij . x_ij = 0), where the step is not
+    differentiable).
     """
     rng = SplitMix64(seed)
     x, y, z = lattice(nx, ny, nz, spacing)
@@ -70,10 +74,14 @@ def dam_break(nx: int = 32, ny: int = 32, nz: int = 16, spacing: float = 0.01,
     x = x + (2.0 * rng.uniform(n) - 1.0) * jitter * spacing
     y = y + (2.0 * rng.uniform(n) - 1.0) * jitter * spacing
     z = z + (2.0 * rng.uniform(n) - 1.0) * jitter * spacing
+    if v_sigma > 0.0:
+        vx, vy, vz = (v_sigma * rng.normal(n) for _ in range(3))
+    else:
+        vx, vy, vz = np.zeros(n), np.zeros(n), np.zeros(n)
     rho0 = 1000.0
     state = {
         "x": x, "y": y, "z": z,
-        "vx": np.zeros(n), "vy": np.zeros(n), "vz": np.zeros(n),
+        "vx": vx, "vy": vy, "vz": vz,
         "m": np.full(n, rho0 * spacing ** 3), "h": np.full(n, 1.3 * spacing),
         "id": np.arange(n, dtype=np.uint32),
     }

@@ -115,6 +115,28 @@ def test_adjoint_matches_finite_differences(eos, conv):
     assert min(errs) <= 3e-6, errs
 
 
+@pytest.mark.parametrize("eos", ["tait", "polytropic"])
+def test_adjoint_position_gradient_matches_finite_differences(eos):
+    nsteps = 6
+    st, prm, target = blob_case(eos=eos)
+    _, gx, _, _ = O.adjoint(O.State(st), prm, nsteps, target, with_gx=True)
+    n = len(st["x"])
+    d = np.random.default_rng(12).standard_normal((n, 3))
+    an = float(np.sum(gx * d))
+
+    def loss_dx(eps):
+        s = dict(st)
+        s["x"] = st["x"] + eps * d[:, 0]
+        s["y"] = st["y"] + eps * d[:, 1]
+        s["z"] = st["z"] + eps * d[:, 2]
+        return O.loss(O.State(s), prm, nsteps, target)
+
+    errs = []
+    for eps in (1e-7, 3e-8, 1e-8):
+        errs.append(abs((loss_dx(eps) - loss_dx(-eps)) / (2 * eps) - an) / abs(an))
+    assert min(errs) <= 3e-6, errs
+
+
 def test_adjoint_single_particle_free_fall():
     dt, nsteps = 1e-3, 7
     st = {"x": np.array([0.1]), "y": np.array([0.2]), "z": np.array([0.3]),
