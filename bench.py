@@ -6,7 +6,16 @@ Workload: 128 x 128 x 64 lattice, spacing 0.01 m, +-1 % seeded jitter, in a
 gamma = 7, c0 = 20, Monaghan alpha = 0.1 (beta = 0), g = -9.81 z, dt = 1e-4.
 A step is one velocity-Verlet tick of the reference schedule (main.rs:103-157)
 over all particles, state resident in HBM (no host copies inside the timed
-region).  The FP64 drop-in is `value`; the FP32-mixed form is reported beside it.
+region).  The FP64 drop-in is `value`; beside it the line reports
+  * "mixed": the FP32-mixed form on the same workload (ms per step, speed-up)
+    and its error against FP64 at the same positions (the state the FP64 run
+    ends in): max and RMS relative error of density and acceleration;
+  * "adjoint": the differentiable step on the 65,536-particle dam break
+    (64 x 32 x 32, seeded initial velocities sigma 0.05 m/s, 100 steps,
+    checkpoint every 10, target = the same start run with alpha = 0.15):
+    forward and adjoint (recompute + reverse) milliseconds per step.
+--precision-report runs the 500-step FP64 / FP32-mixed comparison (errors at
+the FP64 end state, total-energy drift of each form) and prints it instead.
 
 Prints one JSON line (rank 0).  --gpus N > 1 is not supported: this build runs
 on one GPU.
@@ -43,6 +52,9 @@ STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
 STEP_FLOPS_PER_PAIR = 26.0 + 58.0 + 45.0
 
 
+END_STATE = {}
+
+
 def sync_all(ctx) -> None:
     ctx.synchronize()
     try:
@@ -73,8 +85,110 @@ def run_gpu(precision: str, steps: int, warmup: int):
     t1 = time.perf_counter()
     times = ctx.kernel_times(reset=True)
     e = ctx.energy()
+    end = download_state(ctx)
+    end.update(m=st["m"], h=st["h"], id=st["id"])
+    END_STATE[precision] = end
     ctx.close()
     return n, t1 - t0, times, e
+
+
+def eval_errors(st):
+    """Density and acceleration of the FP32-mixed form against FP64 at one state."""
+    import numpy as np
+    import sphx
+    from sphx import scenarios
+
+    _, p = scenarios.dam_break(nx=128, ny=128, nz=64, spacing=0.01, box=(3.2, 1.6, 1.6))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        p.precision = sphx.PREC_FP64 if prec == "fp64" else sphx.PREC_FP32_MIXED
+        with sphx.Context(len(st["x"]), p) as ctx:
+            ctx.upload(st)
+            ctx.forces()
+            out[prec] = ctx.download(sphx.F_RHO | sphx.F_ACC)
+    a, b = out["mixed"], out["fp64"]
+    drho = np.abs(a["rho"] - b["rho"]) / np.abs(b["rho"])
+    acc64 = np.stack([b["ax"], b["ay"], b["az"]], axis=1)
+    acc32 = np.stack([a["ax"], a["ay"], a["az"]], axis=1)
+    # acceleration error relative to the largest acceleration in the field
+    # (componentwise relative error is meaningless where a ~ 0)
+    dacc = np.linalg.norm(acc32 - acc64, axis=1) / np.max(np.linalg.norm(acc64, axis=1))
+    return {"density_rel_max": float(drho.max()), "density_rel_rms": float(np.sqrt(np.mean(drho ** 2))),
+            "accel_rel_max": float(dacc.max()), "accel_rel_rms": float(np.sqrt(np.mean(dacc ** 2)))}
+
+
+def download_state(ctx):
+    import sphx
+    s = ctx.download(sphx.F_POS | sphx.F_VEL)
+    return s
+
+
+def run_adjoint(steps: int = 100, every: int = 10, reps: int = 3):
+    """Forward vs adjoint time on the 65,536-particle differentiable dam break."""
+    import numpy as np
+    import sphx
+    from sphx import scenarios
+
+    st, p = scenarios.dam_break(nx=64, ny=32, nz=32, v_sigma=0.05, seed=0)
+    n = len(st["x"])
+    p15 = scenarios.dam_break(nx=64, ny=32, nz=32, v_sigma=0.05, seed=0, alpha=0.15)[1]
+    with sphx.Context(n, p15) as ctx:
+        ctx.upload(st)
+        ctx.step(steps)
+        f = ctx.download(sphx.F_POS)
+    target = np.stack([f["x"], f["y"], f["z"]], axis=1)
+    with sphx.Context(n, p) as ctx:
+        ctx.upload(st)
+        ctx.adjoint(steps, target, checkpoint_every=every)  # warm-up (allocates the tape)
+        fwd, adj = [], []
+        for _ in range(reps):
+            ctx.upload(st)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.step(steps)
+            ctx.synchronize()
+            fwd.append(time.perf_counter() - t0)
+            ctx.upload(st)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            out = ctx.adjoint(steps, target, checkpoint_every=every)
+            adj.append(time.perf_counter() - t0)
+    tf, ta = min(fwd), min(adj)
+    return {"workload": "dam_break_65k_adjoint", "particles": n, "steps": steps, "checkpoint_every": every,
+            "forward_ms_per_step": round(tf / steps * 1e3, 4),
+            "forward_plus_adjoint_ms_per_step": round(ta / steps * 1e3, 4),
+            "adjoint_over_forward": round((ta - tf) / tf, 3),
+            "loss": out["loss"], "grad_alpha": out["grad_alpha"],
+            "note": "adjoint = checkpoint recompute + reverse pass (the call also runs the forward it "
+                    "differentiates, subtracted here); host-timed, state resident"}
+
+
+def precision_report(steps: int = 500):
+    import numpy as np
+    import sphx
+    from sphx import scenarios
+
+    st, p = scenarios.dam_break(nx=128, ny=128, nz=64, spacing=0.01, box=(3.2, 1.6, 1.6))
+    n = len(st["x"])
+    res = {}
+    end = None
+    for prec in ("fp64", "mixed"):
+        p.precision = sphx.PREC_FP64 if prec == "fp64" else sphx.PREC_FP32_MIXED
+        with sphx.Context(n, p) as ctx:
+            ctx.upload(st)
+            ctx.forces()
+            e0 = ctx.energy()
+            ctx.step(steps)
+            ctx.forces()
+            e1 = ctx.energy()
+            tot = lambda e: e["kinetic"] + e["potential"] + e["internal"]
+            res[prec] = {"energy_start": tot(e0), "energy_end": tot(e1),
+                         "energy_drift_rel": (tot(e1) - tot(e0)) / abs(tot(e0))}
+            if prec == "fp64":
+                end = download_state(ctx)
+                end.update(m=st["m"], h=st["h"], id=st["id"])
+    res["errors_at_fp64_end_state"] = eval_errors(end)
+    return {"report": "precision", "workload": "dam_break_1M", "steps": steps, **res}
 
 
 def roofline(n: int, times: dict, pairs: int, steps: int):
@@ -138,6 +252,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the mixed and adjoint side runs")
+    ap.add_argument("--precision-report", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (FETCH/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -155,6 +271,9 @@ def main() -> int:
               "(the multi-GPU domain decomposition is a design, not part of this build)", file=sys.stderr)
         return 2
     if int(os.environ.get("RANK", "0")) != 0:
+        return 0
+    if args.precision_report:
+        print(json.dumps(precision_report()), flush=True)
         return 0
 
     n, secs, times, e = run_gpu(args.precision, args.steps, args.warmup)
@@ -188,6 +307,14 @@ def main() -> int:
         },
         "kernels": kern,
     }
+    if not args.no_extras and args.precision == "fp64":
+        n2, secs2, _, _ = run_gpu("mixed", args.steps, args.warmup)
+        mixed = {"ms_per_step": round(secs2 / args.steps * 1e3, 4),
+                 "value": round(n2 * args.steps / secs2, 1),
+                 "speedup_vs_fp64": round(secs / secs2, 3)}
+        mixed.update(eval_errors(END_STATE["fp64"]))
+        out["mixed"] = mixed
+        out["adjoint"] = run_adjoint()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -159,3 +159,52 @@ def test_adjoint_rejects_unsupported(require_gpu):
         with pytest.raises(sphx.SphxError) as e:
             ctx.adjoint(2, np.zeros((len(st2["x"]), 3)))
         assert e.value.code == 4
+
+
+def test_adjoint_65k_dam_break_32_directions(require_gpu):
+    # the differentiable workload at full size: 64 x 32 x 32 dam break, seeded
+    # initial velocities, 100 steps, checkpoint every 10, target from the same
+    # start with alpha = 0.15; central differences (eps = 1e-6) on 32 seeded
+    # unit directions of the initial velocities, and on alpha
+    nsteps, every, eps = 100, 10, 1e-6
+    st, prm = scenarios.dam_break(nx=64, ny=32, nz=32, v_sigma=0.05, seed=0)
+    p15 = scenarios.dam_break(nx=64, ny=32, nz=32, v_sigma=0.05, seed=0, alpha=0.15)[1]
+    n = len(st["x"])
+    with sphx.Context(n, prm) as ctx:
+        ctx.set_params(p15)
+        ctx.upload(st)
+        ctx.step(nsteps)
+        f = ctx.download(sphx.F_POS)
+        target = np.stack([f["x"], f["y"], f["z"]], axis=1)
+        ctx.set_params(prm)
+        ctx.upload(st)
+        out = ctx.adjoint(nsteps, target, checkpoint_every=every)
+
+        def loss(dv=None, p=prm):
+            s = dict(st)
+            if dv is not None:
+                s["vx"], s["vy"], s["vz"] = st["vx"] + dv[:, 0], st["vy"] + dv[:, 1], st["vz"] + dv[:, 2]
+            ctx.set_params(p)
+            ctx.upload(s)
+            ctx.step(nsteps)
+            r = ctx.download(sphx.F_POS)
+            x = np.stack([r["x"], r["y"], r["z"]], axis=1)
+            return float(np.sum((x - target) ** 2) / (3 * n))
+
+        assert loss() == pytest.approx(out["loss"], rel=1e-12)
+        rng = np.random.default_rng(2024)
+        errs = []
+        for _ in range(32):
+            d = rng.standard_normal((n, 3))
+            d /= np.linalg.norm(d)
+            fd = (loss(eps * d) - loss(-eps * d)) / (2 * eps)
+            an = float(np.sum(out["grad_v"] * d))
+            errs.append(abs(fd - an) / abs(an))
+        p_hi = scenarios.dam_break(nx=64, ny=32, nz=32, v_sigma=0.05, seed=0, alpha=0.1 + eps)[1]
+        p_lo = scenarios.dam_break(nx=64, ny=32, nz=32, v_sigma=0.05, seed=0, alpha=0.1 - eps)[1]
+        fda = (loss(p=p_hi) - loss(p=p_lo)) / (2 * eps)
+    print("65k adjoint: max rel err over 32 directions %.3g, median %.3g; alpha %.3g vs %.3g"
+          % (max(errs), float(np.median(errs)), out["grad_alpha"], fda))
+    assert max(errs) < 1e-3, errs
+    assert float(np.median(errs)) < 1e-5, errs
+    assert abs(fda - out["grad_alpha"]) <= 1e-3 * abs(out["grad_alpha"])
