@@ -16,6 +16,9 @@ region).  The FP64 drop-in is `value`; beside it the line reports
     forward and adjoint (recompute + reverse) milliseconds per step.
 --precision-report runs the 500-step FP64 / FP32-mixed comparison (errors at
 the FP64 end state, total-energy drift of each form) and prints it instead.
+--mhd-report runs the 4,194,304-particle periodic MHD box (seeded uniform
+random fill, <N_nb> = 40, induction + cleaning + Maxwell stress) in both
+forms and prints the per-kernel breakdown instead (--mhd-particles to resize).
 
 Prints one JSON line (rank 0).  --gpus N > 1 is not supported: this build runs
 on one GPU.
@@ -163,6 +166,45 @@ def run_adjoint(steps: int = 100, every: int = 10, reps: int = 3):
                     "differentiates, subtracted here); host-timed, state resident"}
 
 
+def mhd_report(n_fill: int, steps: int, warmup: int):
+    """Per-kernel breakdown of the periodic MHD box in the FP64 and FP32-mixed forms."""
+    import sphx
+    from sphx import scenarios
+
+    st, p = scenarios.mhd_periodic(random_fill=n_fill)
+    n = len(st["x"])
+    out = {"report": "mhd", "workload": f"mhd_periodic_random_{n}", "particles": n, "steps": steps,
+           "h": float(st["h"][0])}
+    for prec in ("fp64", "mixed"):
+        p.precision = sphx.PREC_FP64 if prec == "fp64" else sphx.PREC_FP32_MIXED
+        with sphx.Context(n, p) as ctx:
+            ctx.upload(st)
+            ctx.step(warmup)
+            sync_all(ctx)
+            ctx.enable_timing(True)
+            ctx.kernel_times(reset=True)
+            t0 = time.perf_counter()
+            ctx.step(steps)
+            sync_all(ctx)
+            secs = time.perf_counter() - t0
+            times = ctx.kernel_times(reset=True)
+            e = ctx.energy()
+        pairs = e["total_neighbours"]
+        kern = {}
+        for k, v in times.items():
+            if v["launches"] == 0:
+                continue
+            avg_s = v["ms"] / v["launches"] * 1e-3
+            kern[k] = {"ms_per_step": round(v["ms"] / steps, 4), "avg_ms": round(avg_s * 1e3, 4),
+                       "launches": v["launches"],
+                       "alg_GBs": round(ALG[k]["bytes"] * n / avg_s / 1e9, 1),
+                       "alg_TFLOPs": round(ALG[k]["flops_pair"] * pairs / avg_s / 1e12, 3)}
+        out[prec] = {"ms_per_step": round(secs / steps * 1e3, 4), "particle_steps_per_s": round(n * steps / secs, 1),
+                     "directed_neighbour_pairs": pairs, "max_neighbours": e["max_neighbours"], "kernels": kern}
+    out["mixed_speedup"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    return out
+
+
 def precision_report(steps: int = 500):
     import numpy as np
     import sphx
@@ -254,6 +296,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the mixed and adjoint side runs")
     ap.add_argument("--precision-report", action="store_true")
+    ap.add_argument("--mhd-report", action="store_true")
+    ap.add_argument("--mhd-particles", type=int, default=4194304)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (FETCH/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -274,6 +318,9 @@ def main() -> int:
         return 0
     if args.precision_report:
         print(json.dumps(precision_report()), flush=True)
+        return 0
+    if args.mhd_report:
+        print(json.dumps(mhd_report(args.mhd_particles, min(args.steps, 20), min(args.warmup, 2))), flush=True)
         return 0
 
     n, secs, times, e = run_gpu(args.precision, args.steps, args.warmup)

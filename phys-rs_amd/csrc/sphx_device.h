@@ -214,10 +214,7 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
     return v;
 }
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
-    return v;
-}
+
 
 // =========================================================================
 // Tile / stage framework.
@@ -331,8 +328,8 @@ __device__ __forceinline__ uint32_t* nbr_base(const NbrList& L, uint32_t i) {
     return L.idx + (((size_t)(i >> 6) * (uint32_t)L.ks) << 6) + (i & 63u);
 }
 
-// f(e) for every entry of sorted particle i, two entries in flight so that
-// the second's gathers overlap the first's arithmetic.
+// f(e) for every entry of sorted particle i, two entries per iteration so
+// that the second's gathers overlap the first's arithmetic.
 template <typename F>
 __device__ __forceinline__ void for_neighbours(const NbrList& L, uint32_t i, F&& f) {
     const uint32_t* __restrict__ e = nbr_base(L, i);

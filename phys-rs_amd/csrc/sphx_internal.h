@@ -32,6 +32,7 @@ struct Grid {
 // Persistent per-particle state, in sorted (cell) order.  Two copies: the
 // counting sort gathers from one into the other.
 struct Particles {
+    double4* pos;              // (x, y, z, h) packed, written by the reorder: one 32-byte gather per neighbour
     double *x, *y, *z;
     double *vx, *vy, *vz;
     double *apx, *apy, *apz;   // PreviousAcceleration
@@ -151,7 +152,7 @@ void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, u
 void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, bool mixed, const Mixed& X, double h_max, hipStream_t s);
+                      const NbrList& L, bool mixed, bool uniform, const Mixed& X, double h_max, hipStream_t s);
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                     const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,

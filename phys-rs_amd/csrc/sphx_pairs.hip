@@ -72,7 +72,7 @@ __device__ __forceinline__ bool exact_neighbour(double dx, double dy, double dz,
 // 1e-4 margin, so no neighbour is cut; owners outside that bound do not cut.
 // Rows whose three cells are not consecutive (a periodic wrap) walk the
 // cells one at a time.
-template <bool MIXED>
+template <bool MIXED, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T, Grid g,
                                                    const uint32_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ start, NbrList L,
@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
             const float dx = (lds[0][u] - xf) + (float)((int)o - 1) * wx, dy = (lds[1][u] - yf) + st.oy,
                         dz = (lds[2][u] - zf) + st.oz;
-            const float tt = hf + lds[3][u];  // 2 h_avg
+            const float tt = UNI ? hf + hf : hf + lds[3][u];  // 2 h_avg
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
             const float t2 = tt * tt;
             bool in;
@@ -161,7 +161,10 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         for (uint32_t cb = 0; cb < st.total; cb += kStageCap) {
             const uint32_t ce = min(st.total, cb + kStageCap);
             __syncthreads();
-            SPHX_STAGE_CHUNK({ lds[0][u] = X.xr[j]; lds[1][u] = X.yr[j]; lds[2][u] = X.zr[j]; lds[3][u] = X.h[j]; });
+            SPHX_STAGE_CHUNK({
+                lds[0][u] = X.xr[j]; lds[1][u] = X.yr[j]; lds[2][u] = X.zr[j];
+                if (!UNI) lds[3][u] = X.h[j];
+            });
             __syncthreads();
             if (!valid) continue;
             if (merged) {
@@ -198,14 +201,20 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
 }
 
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, bool mixed, const Mixed& X, double h_max, hipStream_t s) {
+                      const NbrList& L, bool mixed, bool uniform, const Mixed& X, double h_max, hipStream_t s) {
     if (!n) return;
     const dim3 grid(tile_grid(n, T)), block(kBlock);
     const float reach = (float)(2.0 * h_max * (1.0 + 1e-4));
-    if (mixed)
-        hipLaunchKernelGGL(k_build<true>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach);
-    else
-        hipLaunchKernelGGL(k_build<false>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach);
+#define SPHX_BUILD(M, U)                                                                                   \
+    if (mixed == M && uniform == U) {                                                                      \
+        hipLaunchKernelGGL((k_build<M, U>), grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach); \
+        return;                                                                                            \
+    }
+    SPHX_BUILD(false, false)
+    SPHX_BUILD(false, true)
+    SPHX_BUILD(true, false)
+    SPHX_BUILD(true, true)
+#undef SPHX_BUILD
 }
 
 // ------------------------------------------------------- density + EOS (a4, a5)
@@ -241,14 +250,16 @@ __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     double rho = P.m[i] * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
     double acc = 0.0;
+    const double2* __restrict__ pos = reinterpret_cast<const double2*>(P.pos);
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
+        const double2 xy = pos[2 * j], zh = pos[2 * j + 1];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        pair_vec(g, xi, yi, zi, xy.x, xy.y, zh.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         double r = 0.0, inv_r;
         if (r2 > 0.0) sqrt_rsqrt(r2, r, inv_r);
-        const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
+        const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : zh.y);
         const double w = ph.sig_w * w_shape(r * ph.inv_h);
         acc = fma(UNI ? 1.0 : P.m[j], w, acc);
     });

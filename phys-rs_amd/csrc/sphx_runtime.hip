@@ -369,7 +369,8 @@ int do_neighbours(SphxCtx* c) {
     }
     {
         Timer t(c, kNbr);
-        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->mixed(), c->X, c->hmax, c->stream);
+        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->mixed(), c->kp.uniform != 0, c->X, c->hmax,
+                         c->stream);
     }
     c->sorted = c->listed = true;
     c->dens = c->sp = c->acc = false;
@@ -650,7 +651,7 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
                          &P.m, &P.h, &P.bx, &P.by, &P.bz, &P.psi, &P.q};
         for (double** d : dd)
             if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
-        if (dalloc(c, &P.id, N) || dalloc(c, &P.orig, N)) return bail(SPHX_E_OOM);
+        if (dalloc(c, &P.id, N) || dalloc(c, &P.orig, N) || dalloc(c, &P.pos, N)) return bail(SPHX_E_OOM);
     }
     Derived& D = c->D;
     double** dd[] = {&D.rho, &D.p, &D.pr2, &D.ax, &D.ay, &D.az, &D.spx, &D.spy, &D.spz,

@@ -154,8 +154,9 @@ void launch_cell_order(const Sort& S, uint32_t ncell, const double* x, hipStream
     hipLaunchKernelGGL(k_cell_order, dim3(nblocks(ncell)), dim3(kBlock), 0, s, S.cell_start, ncell, S.perm, x);
 }
 
-// Gather of all persistent state into the new order; in the mixed form also
-// the f32 pair-side copies, positions taken relative to the cell origin.
+// Gather of all persistent state into the new order, plus the sorted cell
+// key, the packed position record and the f32 copies (positions relative to
+// the cell origin).
 __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restrict__ perm,
                           const uint32_t* __restrict__ key, uint32_t* __restrict__ skey, uint32_t n,
                           int mhd, int charge, int mixed, Mixed X, Grid g) {
@@ -178,6 +179,9 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
     dst.apz[k] = src.apz[i];
     dst.m[k] = m;
     dst.h[k] = h;
+    double2* pos = reinterpret_cast<double2*>(dst.pos + k);
+    pos[0] = make_double2(x, y);
+    pos[1] = make_double2(z, h);
     dst.id[k] = src.id[i];
     dst.orig[k] = src.orig[i];
     if (mhd) {
