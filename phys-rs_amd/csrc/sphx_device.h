@@ -196,14 +196,16 @@ __device__ __forceinline__ double eos_dpdrho(const KernelParams& kp, double rho)
     return kp.iso_cs2;
 }
 
-// v of particle i in the force pass's gather record
-__device__ __forceinline__ void set_record_velocity(const Derived& D, uint32_t i, bool mixed, double vx, double vy,
-                                                    double vz) {
+// v of particle i in the gather records (the f32 one in the mixed form, the
+// f64 one whenever it is kept current)
+__device__ __forceinline__ void set_record_velocity(const Derived& D, const KernelParams& kp, uint32_t i, bool mixed,
+                                                    double vx, double vy, double vz) {
     if (mixed) {
         D.r32[i].vx = (float)vx;
         D.r32[i].vy = (float)vy;
         D.r32[i].vz = (float)vz;
-    } else {
+    }
+    if (kp.rec64) {
         D.r64[i].vx = vx;
         D.r64[i].vy = vy;
         D.r64[i].vz = vz;

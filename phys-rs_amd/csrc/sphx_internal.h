@@ -60,6 +60,8 @@ struct alignas(32) Rec32 {
 struct Derived {
     Rec64* r64;
     Rec32* r32;
+    double4* recb;             // (bx, by, bz, psi) in sorted order, written by the reorder and the MHD passes
+    double4* recb_next;        // the MHD passes' output (Jacobi), swapped with recb afterwards
     double *rho, *p, *pr2;     // density, pressure, P / rho^2
     double *ax, *ay, *az;      // acceleration
     double *spx, *spy, *spz;   // m g + sum of pressure terms (reused by VISC_ONLY)
@@ -138,6 +140,7 @@ struct KernelParams {          // what the kernels need from SphxParams, by valu
     // the pair passes then use these instead of staging m and h
     int uniform;
     double m0, h0, inv_h0;
+    int rec64;                 // keep the f64 gather record current (FP64 form, or MHD in either form)
 };
 
 // ---- kernel launchers -------------------------------------------------------
@@ -148,7 +151,8 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, 
 void launch_scatter(const Sort& S, uint32_t n, hipStream_t s);
 void launch_cell_order(const Sort& S, uint32_t ncell, const double* x, hipStream_t s);
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n,
-                    bool mhd, bool charge, bool mixed, const Mixed& X, const Grid& g, hipStream_t s);
+                    bool mhd, bool charge, bool mixed, const Mixed& X, const Grid& g, double4* recb,
+                    hipStream_t s);
 void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,

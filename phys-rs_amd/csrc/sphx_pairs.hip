@@ -311,8 +311,16 @@ __global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D
     D.p[i] = pr;
     D.pr2[i] = pr2;
     float4* rec = reinterpret_cast<float4*>(D.r32 + i);
+    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
     rec[0] = make_float4(xi, yi, zi, (float)rho);
-    rec[1] = make_float4((float)P.vx[i], (float)P.vy[i], (float)P.vz[i], (float)pr2);
+    rec[1] = make_float4((float)vxi, (float)vyi, (float)vzi, (float)pr2);
+    if (kp.rec64) {  // the f64 MHD passes gather the f64 record
+        double2* r = reinterpret_cast<double2*>(D.r64 + i);
+        r[0] = make_double2(P.x[i], P.y[i]);
+        r[1] = make_double2(P.z[i], rho);
+        r[2] = make_double2(vxi, vyi);
+        r[3] = make_double2(vzi, pr2);
+    }
 }
 
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
@@ -388,6 +396,7 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     const double2* __restrict__ R = reinterpret_cast<const double2*>(D.r64);
+    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
@@ -421,7 +430,8 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
         }
         if (STRESS) {
             const double gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;  // grad_i W
-            const double bxj = P.bx[j], byj = P.by[j], bzj = P.bz[j];
+            const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
+            const double bxj = bxy.x, byj = bxy.y, bzj = bzp.x;
             const double irho2j = rcp_nr(rhoj * rhoj);
             const double hb2j = 0.5 * ((bxj * bxj + byj * byj) + bzj * bzj);
             const double big = (bxi * gx + byi * gy) + bzi * gz;
@@ -440,6 +450,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
                                                           NbrList L, KernelParams kp, int charge) {
     SPHX_PASS_PROLOGUE();
     const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
+    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     const float4 own0 = R[2 * i], own1 = R[2 * i + 1];
     const float xi = own0.x, yi = own0.y, zi = own0.z, rhoi = own0.w;
     const float vxi = own1.x, vyi = own1.y, vzi = own1.z, pr2i = own1.w;
@@ -499,7 +510,8 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
         }
         if (STRESS) {
             const float gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;
-            const float bxj = (float)P.bx[j], byj = (float)P.by[j], bzj = (float)P.bz[j];
+            const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
+            const float bxj = (float)bxy.x, byj = (float)bxy.y, bzj = (float)bzp.x;
             const float irho2j = __builtin_amdgcn_rcpf(rhoj * rhoj);
             const float hb2j = 0.5f * ((bxj * bxj + byj * byj) + bzj * bzj);
             const float big = (bxi * gx + byi * gy) + bzi * gz;
@@ -565,7 +577,7 @@ __global__ void __launch_bounds__(kBlock) k_xsph(Particles P, Derived D, Mixed X
     D.tmp3[1][i] = vyi + uy;
     D.tmp3[2][i] = vzi + uz;
     // the record is read by the next force pass, not by this one (Jacobi)
-    set_record_velocity(D, i, mixed != 0, vxi + ux, vyi + uy, vzi + uz);
+    set_record_velocity(D, kp, i, mixed != 0, vxi + ux, vyi + uy, vzi + uz);
 }
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                  const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
@@ -576,6 +588,9 @@ void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g
 // mhd_induction_system (mhd.rs:68-152) in gather form.  The reference adds
 // the pair term to entity_a and its negation to entity_b (mhd.rs:121-122,
 // 130-131), so particle i's term carries +1 when id_i < id_j, -1 otherwise.
+// Neighbours come from the f64 record (x, rho, v) and the B record; the new
+// B goes to tmp3 and, with psi, to recb_next (Jacobi).
+template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                        KernelParams kp) {
     SPHX_PASS_PROLOGUE();
@@ -583,77 +598,104 @@ __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, ui
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i];
     const uint32_t idi = P.id[i];
+    const double2* __restrict__ R = reinterpret_cast<const double2*>(D.r64);
+    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
+    const double inv_rhoi = rcp_nr(rhoi);
     double ux = 0, uy = 0, uz = 0;
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
+        const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
+        const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 > 1e-20)) return;
         double r, inv_r;
         sqrt_rsqrt(r2, r, inv_r);
-        const PairH ph = pair_h<false>(kp, hi, P.h[j]);
+        const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
         const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;
         const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;  // r_hat dW/dr
-        const double vx = vxi - P.vx[j], vy = vyi - P.vy[j], vz = vzi - P.vz[j];
-        const double rhoj = D.rho[j], mj = P.m[j];
+        const double vx = vxi - vxy.x, vy = vyi - vxy.y, vz = vzi - vzp.x;
+        const double rhoj = zr.y, mj = UNI ? kp.m0 : P.m[j];
         const double v_dot_g = (vx * gx + vy * gy) + vz * gz;
         const double b_dot_g = (bxi * gx + byi * gy) + bzi * gz;
-        const double f = mj * rcp_nr(rhoi * rhoj);
+        const double inv_rhoj = rcp_nr(rhoj);
+        const double f = mj * (inv_rhoi * inv_rhoj);
         const double r_dot_g = (dx * gx + dy * gy) + dz * gz;
-        const double res = (2.0 * kp.eta * (mj * rcp_nr(rhoj))) * r_dot_g *
-                           rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
+        const double res = (2.0 * kp.eta * (mj * inv_rhoj)) * r_dot_g * rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
         const double sg = (idi < P.id[j]) ? kp.dt : -kp.dt;
-        ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (P.bx[j] - bxi));
-        uy += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (P.by[j] - byi));
-        uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (P.bz[j] - bzi));
+        ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (bxy.x - bxi));
+        uy += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (bxy.y - byi));
+        uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (bzp.x - bzi));
     });
-    D.tmp3[0][i] = bxi + ux;
-    D.tmp3[1][i] = byi + uy;
-    D.tmp3[2][i] = bzi + uz;
+    const double bx = bxi + ux, by = byi + uy, bz = bzi + uz;
+    D.tmp3[0][i] = bx;
+    D.tmp3[1][i] = by;
+    D.tmp3[2][i] = bz;
+    double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
+    rb[0] = make_double2(bx, by);
+    rb[1] = make_double2(bz, P.psi[i]);
 }
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                       const KernelParams& kp, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_induction, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
+    if (!n) return;
+    if (kp.uniform)
+        hipLaunchKernelGGL(k_induction<true>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
+    else
+        hipLaunchKernelGGL(k_induction<false>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
 }
 
 // ---------------------------------------------- divergence cleaning (a12)
 // divergence_cleaning_system (mhd.rs:172-259): Jacobi over B and psi, then
 // psi *= exp(-sigma c_h dt / h) for every particle.
+template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                        KernelParams kp) {
     SPHX_PASS_PROLOGUE();
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i], psii = P.psi[i];
+    const double2* __restrict__ R = reinterpret_cast<const double2*>(D.r64);
+    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
+        const double2 xy = R[4 * j], zr = R[4 * j + 1];
+        const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 > 1e-20)) return;
         double r, inv_r;
         sqrt_rsqrt(r2, r, inv_r);
-        const PairH ph = pair_h<false>(kp, hi, P.h[j]);
+        const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
         const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;
         const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;
-        const double vol = P.m[j] * rcp_nr(D.rho[j]);
-        div += vol * (((P.bx[j] - bxi) * gx + (P.by[j] - byi) * gy) + (P.bz[j] - bzi) * gz);
-        const double dp = vol * (P.psi[j] - psii);
+        const double vol = (UNI ? kp.m0 : P.m[j]) * rcp_nr(zr.y);
+        div += vol * (((bxy.x - bxi) * gx + (bxy.y - byi) * gy) + (bzp.x - bzi) * gz);
+        const double dp = vol * (bzp.y - psii);
         gpx = fma(dp, gx, gpx);
         gpy = fma(dp, gy, gpy);
         gpz = fma(dp, gz, gpz);
     });
     const double dt = kp.dt;
-    D.tmp3[0][i] = bxi - gpx * dt;
-    D.tmp3[1][i] = byi - gpy * dt;
-    D.tmp3[2][i] = bzi - gpz * dt;
+    const double bx = bxi - gpx * dt, by = byi - gpy * dt, bz = bzi - gpz * dt;
+    D.tmp3[0][i] = bx;
+    D.tmp3[1][i] = by;
+    D.tmp3[2][i] = bz;
     const double decay = exp(-kp.sigma_c * kp.ch * dt / hi);
-    D.tmp1[i] = (psii - (kp.ch * kp.ch) * div * dt) * decay;
+    const double psi = (psii - (kp.ch * kp.ch) * div * dt) * decay;
+    D.tmp1[i] = psi;
+    double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
+    rb[0] = make_double2(bx, by);
+    rb[1] = make_double2(bz, psi);
 }
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                       const KernelParams& kp, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_div_clean, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
+    if (!n) return;
+    if (kp.uniform)
+        hipLaunchKernelGGL(k_div_clean<true>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
+    else
+        hipLaunchKernelGGL(k_div_clean<false>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
 }
 
 // ---------------------------------------------------------- pair export

@@ -58,7 +58,7 @@ __global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams k
     P.vx[i] = vx;
     P.vy[i] = vy;
     P.vz[i] = vz;
-    set_record_velocity(D, i, mixed != 0, vx, vy, vz);
+    set_record_velocity(D, kp, i, mixed != 0, vx, vy, vz);
 }
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, bool mixed,
                         const Mixed& X, hipStream_t s) {

@@ -190,6 +190,7 @@ void make_kernel_params(SphxCtx* c) {
     k.bnd = p.boundary_flags;
     k.mhd = p.mhd_flags;
     k.uniform = c->uniform ? 1 : 0;
+    k.rec64 = (!c->mixed() || c->has_mhd) ? 1 : 0;
     k.m0 = c->m0;
     k.h0 = c->h0;
     k.inv_h0 = c->uniform ? 1.0 / c->h0 : 0.0;
@@ -363,7 +364,8 @@ int do_neighbours(SphxCtx* c) {
         launch_scan(c->S.cell_count, c->S.cell_start, c->grid.ncell, c->S.block_sums, c->stream);
         launch_scatter(c->S, n, c->stream);
         launch_cell_order(c->S, c->grid.ncell, src.x, c->stream);
-        launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->mixed(), c->X, c->grid, c->stream);
+        launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->mixed(), c->X, c->grid, c->D.recb,
+                       c->stream);
         c->cur = 1 - c->cur;
         launch_tiles(c->grid, c->S, c->T, c->stream);
     }
@@ -462,6 +464,7 @@ int do_induction(SphxCtx* c) {
     }
     Particles& P = c->P();
     swap3(P.bx, P.by, P.bz, c->D.tmp3);
+    std::swap(c->D.recb, c->D.recb_next);
     c->acc = false;
     return launch_check(c);
 }
@@ -477,6 +480,7 @@ int do_div_clean(SphxCtx* c) {
     Particles& P = c->P();
     swap3(P.bx, P.by, P.bz, c->D.tmp3);
     std::swap(P.psi, c->D.tmp1);
+    std::swap(c->D.recb, c->D.recb_next);
     c->acc = false;
     return launch_check(c);
 }
@@ -658,7 +662,8 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
                      &D.tmp3[0], &D.tmp3[1], &D.tmp3[2], &D.tmp1};
     for (double** d : dd)
         if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
-    if (dalloc(c, &D.r64, N) || dalloc(c, &D.r32, N)) return bail(SPHX_E_OOM);
+    if (dalloc(c, &D.r64, N) || dalloc(c, &D.r32, N) || dalloc(c, &D.recb, N) || dalloc(c, &D.recb_next, N))
+        return bail(SPHX_E_OOM);
     {
         Mixed& X = c->X;
         float** ff[] = {&X.xr, &X.yr, &X.zr, &X.m, &X.h};
@@ -739,10 +744,10 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
         c->uniform = h->m[i] == h->m[0] && h->h[i] == h->h[0];
     c->m0 = h->n ? h->m[0] : 0.0;
     c->h0 = h->n ? h->h[0] : 0.0;
+    c->has_mhd = mhd;
     make_kernel_params(c);
     int rc = setup_grid(c, h);
     if (rc) return rc;
-    c->has_mhd = mhd;
     c->has_charge = h->q != nullptr;
     c->cur = 0;
     Particles& P = c->A[0];

@@ -159,7 +159,7 @@ void launch_cell_order(const Sort& S, uint32_t ncell, const double* x, hipStream
 // the cell origin).
 __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restrict__ perm,
                           const uint32_t* __restrict__ key, uint32_t* __restrict__ skey, uint32_t n,
-                          int mhd, int charge, int mixed, Mixed X, Grid g) {
+                          int mhd, int charge, int mixed, Mixed X, Grid g, double4* __restrict__ recb) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t i = perm[k];
@@ -185,10 +185,14 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
     dst.id[k] = src.id[i];
     dst.orig[k] = src.orig[i];
     if (mhd) {
-        dst.bx[k] = src.bx[i];
-        dst.by[k] = src.by[i];
-        dst.bz[k] = src.bz[i];
-        dst.psi[k] = src.psi[i];
+        const double bx = src.bx[i], by = src.by[i], bz = src.bz[i], psi = src.psi[i];
+        dst.bx[k] = bx;
+        dst.by[k] = by;
+        dst.bz[k] = bz;
+        dst.psi[k] = psi;
+        double2* rb = reinterpret_cast<double2*>(recb + k);
+        rb[0] = make_double2(bx, by);
+        rb[1] = make_double2(bz, psi);
     }
     if (charge) dst.q[k] = src.q[i];
     // f32 cell-relative coordinates and h: the neighbour build's prefilter in
@@ -202,10 +206,10 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
     if (mixed) X.m[k] = (float)m;
 }
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n, bool mhd,
-                    bool charge, bool mixed, const Mixed& X, const Grid& g, hipStream_t s) {
+                    bool charge, bool mixed, const Mixed& X, const Grid& g, double4* recb, hipStream_t s) {
     if (n)
         hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, S.perm, S.key,
-                           S.skey, n, (int)mhd, (int)charge, (int)mixed, X, g);
+                           S.skey, n, (int)mhd, (int)charge, (int)mixed, X, g, recb);
 }
 
 // ---- tiles: each row cut into runs of <= kTile sorted particles ---------
