@@ -109,6 +109,10 @@ typedef struct SphxParams {
     double grid_hi[3];         /* uploaded positions (padded), or from box_lo/hi when periodic   */
     int32_t precision;         /* enum SphxPrecision                                             */
     int32_t max_neighbours;    /* per-particle neighbour-list capacity; <= 0: 192                */
+    double neighbour_skin;     /* list radius 2*h_avg + skin, rebuilt only once some particle has
+                                  moved skin/2 since the last build; < 0: 0.04 * max(h) (default);
+                                  0: rebuilt every tick.  Every pass still uses exactly the pairs
+                                  with r < 2*h_avg (pairs beyond contribute exactly zero). */
 } SphxParams;
 
 /* Fill *p with the reference defaults: TaitEquationConfig::default (sph.rs:134-142),

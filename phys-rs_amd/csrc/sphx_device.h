@@ -59,6 +59,20 @@ __device__ __forceinline__ void decode_cell(uint32_t key, const Grid& g, int& cx
     cz = (int)(t / ny);
 }
 
+// Exact membership r < 2*h_avg with r = sqrt((dx*dx + dy*dy) + dz*dz) rounded
+// as the reference rounds it.  r2 far from the threshold (relative 1e-12,
+// about 5000 ulp) decides without the square root; the band in between takes
+// the correctly rounded sqrt and the reference comparison.
+__device__ __forceinline__ bool exact_neighbour(double dx, double dy, double dz, double hi, double hj) {
+    const double r2 = (dx * dx + dy * dy) + dz * dz;
+    const double h_avg = 0.5 * (hi + hj);
+    const double t = 2.0 * h_avg;
+    const double t2 = t * t;
+    if (r2 < t2 * (1.0 - 1e-12)) return true;
+    if (!(r2 <= t2 * (1.0 + 1e-12))) return false;
+    return sqrt(r2) < t;
+}
+
 // ---- cubic spline (sph.rs:29-64), f64 given 1/h --------------------------
 __device__ __forceinline__ double kernel_w(double r, double inv_h) {
     const double q = r * inv_h;
@@ -344,6 +358,20 @@ __device__ __forceinline__ void for_neighbours(const NbrList& L, uint32_t i, F&&
         f(e1);
     }
     if (k < cnt) f(e[(size_t)k << 6]);
+}
+
+// Sorted particle i's exact neighbour count r < 2*h_avg (its list may also
+// hold pairs out to 2*h_avg + skin).
+__device__ __forceinline__ uint32_t exact_count(const Particles& P, const Grid& g, const NbrList& L, uint32_t i) {
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
+    uint32_t c = 0;
+    for_neighbours(L, i, [&](uint32_t e) {
+        const uint32_t j = e & kIdxMask;
+        double dx, dy, dz;
+        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        c += exact_neighbour(dx, dy, dz, hi, P.h[j]) ? 1u : 0u;
+    });
+    return c;
 }
 
 // Cell offset of an entry's neighbour times the cell widths (FP32-mixed pair

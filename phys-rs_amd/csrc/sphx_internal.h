@@ -32,7 +32,7 @@ struct Grid {
 // Persistent per-particle state, in sorted (cell) order.  Two copies: the
 // counting sort gathers from one into the other.
 struct Particles {
-    double4* pos;              // (x, y, z, h) packed, written by the reorder: one 32-byte gather per neighbour
+    double4* pos;              // (x, y, z, h) packed, kept current with x: one 32-byte gather per neighbour
     double *x, *y, *z;
     double *vx, *vy, *vz;
     double *apx, *apy, *apz;   // PreviousAcceleration
@@ -62,6 +62,7 @@ struct Derived {
     Rec32* r32;
     double4* recb;             // (bx, by, bz, psi) in sorted order, written by the reorder and the MHD passes
     double4* recb_next;        // the MHD passes' output (Jacobi), swapped with recb afterwards
+    double4* xref;             // positions at the last neighbour build (sorted order)
     double *rho, *p, *pr2;     // density, pressure, P / rho^2
     double *ax, *ay, *az;      // acceleration
     double *spx, *spy, *spz;   // m g + sum of pressure terms (reused by VISC_ONLY)
@@ -71,6 +72,7 @@ struct Derived {
 
 struct Sort {
     uint32_t *key, *rank, *perm;
+    uint32_t* tmp;             // arrival-order scatter (cell order before the deterministic ranking)
     uint32_t* skey;            // key in sorted order (written by the reorder)
     uint32_t *cell_count;      // ncell
     uint32_t *cell_start;      // ncell + 1
@@ -107,7 +109,8 @@ static constexpr uint32_t kMaxParticles = 1u << kIdxBits;
 struct NbrList {
     uint32_t* idx;
     uint16_t* ntot;            // entries per particle (<= ks)
-    uint32_t* overflow;        // [0] largest neighbour count seen, [1] some list overflowed
+    uint32_t* overflow;        // [0] largest list seen, [1] some list overflowed, [2] a particle moved
+                               // more than skin/2 since the build
     int ks;                    // list capacity per particle
 };
 
@@ -141,6 +144,8 @@ struct KernelParams {          // what the kernels need from SphxParams, by valu
     int uniform;
     double m0, h0, inv_h0;
     int rec64;                 // keep the f64 gather record current (FP64 form, or MHD in either form)
+    double skin;               // neighbour-list skin (list radius 2 h_avg + skin)
+    double half_skin_sq;       // (skin / 2)^2: displacement since the last build that forces a rebuild
 };
 
 // ---- kernel launchers -------------------------------------------------------
@@ -149,23 +154,25 @@ void launch_iota_u32(uint32_t* p, uint32_t n, hipStream_t s);
 void launch_cell_hash(const Particles& P, uint32_t n, const Grid& g, const Sort& S, hipStream_t s);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, hipStream_t s);
 void launch_scatter(const Sort& S, uint32_t n, hipStream_t s);
-void launch_cell_order(const Sort& S, uint32_t ncell, const double* x, hipStream_t s);
+void launch_cell_order(const Sort& S, uint32_t n, const double* x, hipStream_t s);
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n,
                     bool mhd, bool charge, bool mixed, const Mixed& X, const Grid& g, double4* recb,
-                    hipStream_t s);
+                    double4* xref, hipStream_t s);
 void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, bool mixed, bool uniform, const Mixed& X, double h_max, hipStream_t s);
+                      const NbrList& L, bool uniform, const Mixed& X, double h_max, double skin, hipStream_t s);
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                     const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                    const KernelParams& kp, bool visc_only, bool mhd_stress, bool charge, bool mixed,
                    const Mixed& X, hipStream_t s);
-void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
+                        const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint8_t* flip,
                         hipStream_t s);
-void launch_vv_position_flip(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
-                             uint8_t* flip, hipStream_t s);
+void launch_refresh_positions(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+                              const Grid& g, const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved,
+                              hipStream_t s);
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         bool mixed, const Mixed& X, hipStream_t s);
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
@@ -174,9 +181,9 @@ void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Gr
                       const KernelParams& kp, hipStream_t s);
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                       const KernelParams& kp, hipStream_t s);
-void launch_export_pairs(const Particles& P, uint32_t n, const NbrList& L, uint32_t* out,
+void launch_export_pairs(const Particles& P, uint32_t n, const Grid& g, const NbrList& L, uint32_t* out,
                          unsigned long long* counter, uint64_t out_cap, hipStream_t s);
-void launch_energy(const Particles& P, const Derived& D, uint32_t n, uint32_t cap,
+void launch_energy(const Particles& P, const Derived& D, uint32_t n, const Grid& g,
                    const NbrList& L, const KernelParams& kp, bool mhd, double* partial,
                    uint32_t nblocks, hipStream_t s);
 uint32_t energy_blocks(uint32_t n);

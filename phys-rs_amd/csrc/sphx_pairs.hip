@@ -40,43 +40,30 @@ namespace sphx {
     }
 
 // ------------------------------------------------ neighbour lists (a1, a2)
-// Exact membership r < 2*h_avg with r = sqrt((dx*dx + dy*dy) + dz*dz) rounded
-// as the reference rounds it.  r2 far from the threshold (relative 1e-12,
-// about 5000 ulp) decides without the square root; the band in between takes
-// the correctly rounded sqrt and the reference comparison.
-__device__ __forceinline__ bool exact_neighbour(double dx, double dy, double dz, double hi, double hj) {
-    const double r2 = (dx * dx + dy * dy) + dz * dz;
-    const double h_avg = 0.5 * (hi + hj);
-    const double t = 2.0 * h_avg;
-    const double t2 = t * t;
-    if (r2 < t2 * (1.0 - 1e-12)) return true;
-    if (!(r2 <= t2 * (1.0 + 1e-12))) return false;
-    return sqrt(r2) < t;
-}
 
-// Both forms stage f32 cell-relative coordinates (16 B per candidate).  The
-// mixed form decides with the f32 predicate.  The f64 form uses it as a
-// prefilter: |xr| <= 2w for the owner bounds the f32 error of r^2 by ~2e-6
-// relative, so anything outside a 1e-5 band around (2 h_avg)^2 is decided,
-// and candidates inside the band (about 1e-5 of neighbours) take the exact
-// f64 predicate on the f64 positions.  Owners further than 2 cells from
-// their cell origin (particles clamped into the grid) test every candidate
-// exactly.
+// The list holds every pair with r < 2 h_avg + skin (a superset: each pass
+// gets exactly the pairs r < 2 h_avg, because the cubic spline and its
+// derivatives are exactly zero from q = 2 on, and the pair export and the
+// neighbour counts apply the reference predicate itself).  The test is f32
+// on staged cell-relative coordinates with a 1e-5 relative margin on r^2:
+// |xr| <= 2w for the owner bounds the f32 error of r^2 by ~2e-6 relative,
+// so no pair inside the radius is missed.  Owners further than 2 cells
+// from their cell origin (particles clamped into the grid) test in f64.
 //
 // Cells are sorted by x and a row's cells are consecutive in the sorted
 // order, so a thread's candidates in one neighbour row -- the three cells
-// around its own, cut to |dx| < reach = 2 h_max (1 + 1e-4) -- are one
+// around its own, cut to |dx| < reach = 2 h_max (1 + 1e-4) + skin -- are one
 // contiguous slot range, found by binary search on the staged f32 x at both
 // ends.  All threads of a wave then walk ranges of about two cells in
 // lockstep.  The f32 error of dx (<~1e-7 w for |xr| <= 2w) is far inside the
 // 1e-4 margin, so no neighbour is cut; owners outside that bound do not cut.
 // Rows whose three cells are not consecutive (a periodic wrap) walk the
 // cells one at a time.
-template <bool MIXED, bool UNI>
+template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T, Grid g,
                                                    const uint32_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ start, NbrList L,
-                                                   float reach) {
+                                                   float reach, double skin) {
     __shared__ float lds[4][kStageCap];
     SPHX_TILE_PROLOGUE();
     const int cx = valid ? (int)(skey[p] % (uint32_t)g.dims[0]) : 0;
@@ -87,8 +74,9 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         xf = X.xr[p]; yf = X.yr[p]; zf = X.zr[p]; hf = X.h[p];
         const float lim = 2.0f * (float)fmax(g.w[0], fmax(g.w[1], g.w[2]));
         exact_all = fabsf(xf) > lim || fabsf(yf) > lim || fabsf(zf) > lim;
-        if (!MIXED) { xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p]; }
+        xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
     }
+    const float skin_f = (float)skin;
     const bool trim = !exact_all;
     const float wx = (float)g.w[0];
     uint32_t total = 0;
@@ -109,21 +97,17 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
             const float dx = (lds[0][u] - xf) + (float)((int)o - 1) * wx, dy = (lds[1][u] - yf) + st.oy,
                         dz = (lds[2][u] - zf) + st.oz;
-            const float tt = UNI ? hf + hf : hf + lds[3][u];  // 2 h_avg
+            const float tt = (UNI ? hf + hf : hf + lds[3][u]) + skin_f;  // 2 h_avg + skin
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float t2 = tt * tt;
             bool in;
-            if (MIXED) {
-                in = r2 < t2;
-            } else if (!exact_all && r2 < t2 * (1.0f - 1e-5f)) {
-                in = true;
-            } else if (!exact_all && r2 > t2 * (1.0f + 1e-5f)) {
-                in = false;
+            if (!exact_all) {
+                in = r2 < (tt * tt) * (1.0f + 1e-5f);
             } else {
                 const uint32_t j = slot_particle(st, slot);
                 double ddx, ddy, ddz;
                 pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], ddx, ddy, ddz);
-                in = exact_neighbour(ddx, ddy, ddz, hi, P.h[j]);
+                const double t = (hi + P.h[j]) + skin;
+                in = (ddx * ddx + ddy * ddy) + ddz * ddz < (t * t) * (1.0 + 1e-12);
             }
             if (in) {
                 if (total < (uint32_t)L.ks)
@@ -201,20 +185,14 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
 }
 
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, bool mixed, bool uniform, const Mixed& X, double h_max, hipStream_t s) {
+                      const NbrList& L, bool uniform, const Mixed& X, double h_max, double skin, hipStream_t s) {
     if (!n) return;
     const dim3 grid(tile_grid(n, T)), block(kBlock);
-    const float reach = (float)(2.0 * h_max * (1.0 + 1e-4));
-#define SPHX_BUILD(M, U)                                                                                   \
-    if (mixed == M && uniform == U) {                                                                      \
-        hipLaunchKernelGGL((k_build<M, U>), grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach); \
-        return;                                                                                            \
-    }
-    SPHX_BUILD(false, false)
-    SPHX_BUILD(false, true)
-    SPHX_BUILD(true, false)
-    SPHX_BUILD(true, true)
-#undef SPHX_BUILD
+    const float reach = (float)(2.0 * h_max * (1.0 + 1e-4) + skin);
+    if (uniform)
+        hipLaunchKernelGGL(k_build<true>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach, skin);
+    else
+        hipLaunchKernelGGL(k_build<false>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach, skin);
 }
 
 // ------------------------------------------------------- density + EOS (a4, a5)
@@ -699,14 +677,18 @@ void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Gr
 }
 
 // ---------------------------------------------------------- pair export
-__global__ void __launch_bounds__(kBlock) k_export_pairs(Particles P, uint32_t n, NbrList L,
+__global__ void __launch_bounds__(kBlock) k_export_pairs(Particles P, uint32_t n, Grid g, NbrList L,
                                                           uint32_t* __restrict__ out,
                                                           unsigned long long* counter, uint64_t out_cap) {
     SPHX_PASS_PROLOGUE();
     const uint32_t idi = P.id[i];
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     for_neighbours(L, i, [&](uint32_t e) {
-        const uint32_t idj = P.id[e & kIdxMask];
-        if (idi < idj) {
+        const uint32_t j = e & kIdxMask;
+        const uint32_t idj = P.id[j];
+        double dx, dy, dz;
+        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        if (idi < idj && exact_neighbour(dx, dy, dz, hi, P.h[j])) {
             const unsigned long long pos = atomicAdd(counter, 1ull);
             if (pos < out_cap) {
                 out[2 * pos] = idi;
@@ -715,10 +697,11 @@ __global__ void __launch_bounds__(kBlock) k_export_pairs(Particles P, uint32_t n
         }
     });
 }
-void launch_export_pairs(const Particles& P, uint32_t n, const NbrList& L, uint32_t* out,
+void launch_export_pairs(const Particles& P, uint32_t n, const Grid& g, const NbrList& L, uint32_t* out,
                          unsigned long long* counter, uint64_t out_cap, hipStream_t s) {
     if (n)
-        hipLaunchKernelGGL(k_export_pairs, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, L, out, counter, out_cap);
+        hipLaunchKernelGGL(k_export_pairs, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, g, L, out, counter,
+                           out_cap);
 }
 
 }  // namespace sphx

@@ -5,46 +5,102 @@
 
 namespace sphx {
 
+// After particle i's position changed to x: the packed position record and,
+// in the mixed form, the f32 position relative to the particle's cell at the
+// last neighbour build follow x.  Returns whether i has moved more than
+// skin/2 since that build.
+__device__ __forceinline__ bool track_position(const Particles& P, const Derived& D, const KernelParams& kp,
+                                               const Grid& g, const uint32_t* __restrict__ skey, int mixed,
+                                               const Mixed& X, uint32_t i, const double x[3]) {
+    double2* pos = reinterpret_cast<double2*>(P.pos + i);
+    pos[0] = make_double2(x[0], x[1]);
+    pos[1] = make_double2(x[2], P.h[i]);
+    const double2* ref = reinterpret_cast<const double2*>(D.xref + i);
+    const double2 r01 = ref[0], r2_ = ref[1];
+    double dx, dy, dz;
+    pair_vec(g, r01.x, r01.y, r2_.x, x[0], x[1], x[2], dx, dy, dz);
+    if (mixed) {
+        int cx, cy, cz;
+        decode_cell(skey[i], g, cx, cy, cz);
+        double d[3] = {x[0] - (g.lo[0] + cx * g.w[0]), x[1] - (g.lo[1] + cy * g.w[1]),
+                       x[2] - (g.lo[2] + cz * g.w[2])};
+        if (g.periodic)
+            for (int k = 0; k < 3; ++k) min_image(d[k], g.L[k], g.halfL[k]);
+        X.xr[i] = (float)d[0];
+        X.yr[i] = (float)d[1];
+        X.zr[i] = (float)d[2];
+    }
+    return (dx * dx + dy * dy) + dz * dz > kp.half_skin_sq;
+}
+
+// Positions replaced from outside (the adjoint's tape): same bookkeeping.
+__global__ void k_refresh_positions(Particles P, Derived D, uint32_t n, KernelParams kp, Grid g,
+                                    const uint32_t* __restrict__ skey, int mixed, Mixed X, uint32_t* moved) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool far = false;
+    if (i < n) {
+        const double x[3] = {P.x[i], P.y[i], P.z[i]};
+        far = track_position(P, D, kp, g, skey, mixed, X, i, x);
+    }
+    if (__ballot(far) && (threadIdx.x & 63u) == 0u) atomicOr(moved, 1u);
+}
+void launch_refresh_positions(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+                              const Grid& g, const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved,
+                              hipStream_t s) {
+    if (n)
+        hipLaunchKernelGGL(k_refresh_positions, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, g, skey,
+                           (int)mixed, X, moved);
+}
+
 // velocity_verlet_position_system (integrators.rs:63-83), then the box:
 // reflection mirrors x and flips v and the stored acceleration; periodic wraps.
 // When `flip` is set, bit k of flip[orig[i]] (upload order) records a
-// reflection on axis k (the adjoint needs the sign of the map).
-__global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams kp, uint8_t* flip) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double* xs[3] = {P.x, P.y, P.z};
-    double* vs[3] = {P.vx, P.vy, P.vz};
-    double* aps[3] = {P.apx, P.apy, P.apz};
-    const double* as[3] = {D.ax, D.ay, D.az};
-    uint32_t bits = 0;
+// reflection on axis k (the adjoint needs the sign of the map).  A particle
+// that has moved more than skin/2 since the last neighbour build raises
+// *moved (see track_position).
+__global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams kp, Grid g,
+                              const uint32_t* __restrict__ skey, int mixed, Mixed X, uint32_t* moved,
+                              uint8_t* flip) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool far = false;
+    if (i < n) {
+        double* xs[3] = {P.x, P.y, P.z};
+        double* vs[3] = {P.vx, P.vy, P.vz};
+        double* aps[3] = {P.apx, P.apy, P.apz};
+        const double* as[3] = {D.ax, D.ay, D.az};
+        double nx[3];
+        uint32_t bits = 0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const double a = as[k][i];
-        double v = vs[k][i];
-        double ap = a;
-        double x = xs[k][i] + (v * kp.dt + a * kp.half_dt_sq);
-        if (kp.bnd & SPHX_BND_REFLECT) {
-            const double lo = kp.box_lo[k], hi = kp.box_hi[k];
-            if (x < lo) { x = lo + (lo - x); v = -v; ap = -ap; bits |= 1u << k; }
-            else if (x > hi) { x = hi - (x - hi); v = -v; ap = -ap; bits |= 1u << k; }
-        } else if (kp.bnd & SPHX_BND_PERIODIC) {
-            const double lo = kp.box_lo[k], hi = kp.box_hi[k], L = hi - lo;
-            if (x >= hi) x -= L;
-            else if (x < lo) x += L;
+        for (int k = 0; k < 3; ++k) {
+            const double a = as[k][i];
+            double v = vs[k][i];
+            double ap = a;
+            double x = xs[k][i] + (v * kp.dt + a * kp.half_dt_sq);
+            if (kp.bnd & SPHX_BND_REFLECT) {
+                const double lo = kp.box_lo[k], hi = kp.box_hi[k];
+                if (x < lo) { x = lo + (lo - x); v = -v; ap = -ap; bits |= 1u << k; }
+                else if (x > hi) { x = hi - (x - hi); v = -v; ap = -ap; bits |= 1u << k; }
+            } else if (kp.bnd & SPHX_BND_PERIODIC) {
+                const double lo = kp.box_lo[k], hi = kp.box_hi[k], L = hi - lo;
+                if (x >= hi) x -= L;
+                else if (x < lo) x += L;
+            }
+            xs[k][i] = x;
+            vs[k][i] = v;
+            aps[k][i] = ap;
+            nx[k] = x;
         }
-        xs[k][i] = x;
-        vs[k][i] = v;
-        aps[k][i] = ap;
+        if (flip) flip[P.orig[i]] = (uint8_t)bits;
+        far = track_position(P, D, kp, g, skey, mixed, X, i, nx);
     }
-    if (flip) flip[P.orig[i]] = (uint8_t)bits;
+    if (__ballot(far) && (threadIdx.x & 63u) == 0u) atomicOr(moved, 1u);
 }
-void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
+void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
+                        const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint8_t* flip,
                         hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_vv_position, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, (uint8_t*)nullptr);
-}
-void launch_vv_position_flip(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
-                             uint8_t* flip, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_vv_position, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, flip);
+    if (n)
+        hipLaunchKernelGGL(k_vv_position, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, g, skey, (int)mixed, X,
+                           moved, flip);
 }
 
 // velocity_verlet_velocity_system (integrators.rs:92-104)
@@ -81,8 +137,8 @@ __device__ double specific_internal(const KernelParams& kp, double rho) {
     return kp.iso_cs2 * log(rho / kp.rest_density);
 }
 
-__global__ void k_energy(Particles P, Derived D, uint32_t n, const uint16_t* __restrict__ ntot,
-                         KernelParams kp, int mhd, double* __restrict__ partial) {
+__global__ void k_energy(Particles P, Derived D, uint32_t n, Grid g, NbrList L, KernelParams kp, int mhd,
+                         double* __restrict__ partial) {
     __shared__ double lds[kEnergyVals][kBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[kEnergyVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -96,7 +152,7 @@ __global__ void k_energy(Particles P, Derived D, uint32_t n, const uint16_t* __r
             v[3] = 0.5 * (m / D.rho[i]) * ((bx * bx + by * by) + bz * bz);
         }
         v[4] = m * vx; v[5] = m * vy; v[6] = m * vz;
-        const uint32_t c = ntot[i];
+        const uint32_t c = exact_count(P, g, L, i);
         v[7] = (double)c;
         v[8] = (double)c;
     }
@@ -111,10 +167,9 @@ __global__ void k_energy(Particles P, Derived D, uint32_t n, const uint16_t* __r
     }
     if (threadIdx.x < kEnergyVals) partial[(size_t)blockIdx.x * kEnergyVals + threadIdx.x] = lds[threadIdx.x][0];
 }
-void launch_energy(const Particles& P, const Derived& D, uint32_t n, uint32_t cap, const NbrList& L,
+void launch_energy(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                    const KernelParams& kp, bool mhd, double* partial, uint32_t nb, hipStream_t s) {
-    (void)cap;
-    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(kBlock), 0, s, P, D, n, L.ntot, kp, (int)mhd, partial);
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(kBlock), 0, s, P, D, n, g, L, kp, (int)mhd, partial);
 }
 
 }  // namespace sphx

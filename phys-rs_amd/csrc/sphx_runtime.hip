@@ -191,6 +191,8 @@ void make_kernel_params(SphxCtx* c) {
     k.mhd = p.mhd_flags;
     k.uniform = c->uniform ? 1 : 0;
     k.rec64 = (!c->mixed() || c->has_mhd) ? 1 : 0;
+    k.skin = p.neighbour_skin < 0.0 ? 0.04 * c->hmax : p.neighbour_skin;
+    k.half_skin_sq = 0.25 * k.skin * k.skin;
     k.m0 = c->m0;
     k.h0 = c->h0;
     k.inv_h0 = c->uniform ? 1.0 / c->h0 : 0.0;
@@ -203,7 +205,8 @@ int setup_grid(SphxCtx* c, const SphxState* host) {
     for (uint32_t i = 0; i < host->n; ++i) hmax = std::max(hmax, host->h[i]);
     if (host->n && !(hmax > 0.0)) return fail(c, SPHX_E_INVALID, "smoothing radius must be > 0");
     c->hmax = hmax;
-    double cs = 2.0 * hmax * (1.0 + 1e-9);
+    make_kernel_params(c);  // the skin follows h_max
+    double cs = (2.0 * hmax + c->kp.skin) * (1.0 + 1e-9);
     if (p.cell_size > cs) cs = p.cell_size;
     if (!(cs > 0.0)) cs = 1.0;
     Grid g{};
@@ -363,15 +366,16 @@ int do_neighbours(SphxCtx* c) {
         launch_cell_hash(src, n, c->grid, c->S, c->stream);
         launch_scan(c->S.cell_count, c->S.cell_start, c->grid.ncell, c->S.block_sums, c->stream);
         launch_scatter(c->S, n, c->stream);
-        launch_cell_order(c->S, c->grid.ncell, src.x, c->stream);
+        launch_cell_order(c->S, n, src.x, c->stream);
         launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->mixed(), c->X, c->grid, c->D.recb,
-                       c->stream);
+                       c->D.xref, c->stream);
         c->cur = 1 - c->cur;
         launch_tiles(c->grid, c->S, c->T, c->stream);
     }
     {
         Timer t(c, kNbr);
-        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->mixed(), c->kp.uniform != 0, c->X, c->hmax,
+        launch_fill_u32(c->L.overflow + 2, 0u, 1, c->stream);  // displacements count from this build
+        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->kp.uniform != 0, c->X, c->hmax, c->kp.skin,
                          c->stream);
     }
     c->sorted = c->listed = true;
@@ -412,14 +416,27 @@ int do_forces(SphxCtx* c, bool visc_only) {
     return launch_check(c);
 }
 
-int do_vv_position(SphxCtx* c) {
+// Position update; the neighbour list stays valid while no particle has
+// moved more than skin/2 since it was built (the kernel raises a flag).
+int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr) {
     if (!c->acc) return fail(c, SPHX_E_STATE, "vv_position needs accelerations (call sphx_forces)");
     {
         Timer t(c, kIntegrate);
-        launch_vv_position(c->P(), c->D, c->n, c->kp, c->stream);
+        launch_vv_position(c->P(), c->D, c->n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, c->L.overflow + 2, flip,
+                           c->stream);
     }
-    c->sorted = c->listed = c->dens = c->sp = c->acc = false;
-    return launch_check(c);
+    int rc;
+    if ((rc = launch_check(c))) return rc;
+    c->dens = c->sp = c->acc = false;
+    bool rebuild = c->kp.skin <= 0.0 || !c->listed;
+    if (!rebuild) {
+        uint32_t moved = 0;
+        HIP_TRY(c, hipMemcpyAsync(&moved, c->L.overflow + 2, sizeof moved, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        rebuild = moved != 0;
+    }
+    if (rebuild) c->sorted = c->listed = false;
+    return SPHX_OK;
 }
 
 int do_vv_velocity(SphxCtx* c) {
@@ -521,16 +538,30 @@ int adj_alloc(SphxCtx* c, uint32_t n, uint32_t k, uint32_t m) {
     return SPHX_OK;
 }
 
-// state (upload order) -> working arrays (current sorted order); x may be NULL
-void adj_load(SphxCtx* c, const double* x3, const double* v3) {
+// state (upload order) -> working arrays (current sorted order); x may be
+// NULL.  The neighbour list is kept while no particle is further than skin/2
+// from where it was at the list's build.
+int adj_load(SphxCtx* c, const double* x3, const double* v3) {
     const uint32_t n = c->n;
     Particles& P = c->P();
     if (x3) {
         launch_gather3(P.orig, n, vec3(const_cast<double*>(x3), n), pos3(P), c->stream);
-        c->sorted = c->listed = c->dens = c->sp = c->acc = false;
+        c->dens = c->sp = c->acc = false;
+        if (c->listed && c->kp.skin > 0.0) {
+            launch_fill_u32(c->L.overflow + 2, 0u, 1, c->stream);
+            launch_refresh_positions(P, c->D, n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, c->L.overflow + 2,
+                                     c->stream);
+            uint32_t moved = 0;
+            HIP_TRY(c, hipMemcpyAsync(&moved, c->L.overflow + 2, sizeof moved, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            if (moved) c->sorted = c->listed = false;
+        } else {
+            c->sorted = c->listed = false;
+        }
     }
     launch_gather3(P.orig, n, vec3(const_cast<double*>(v3), n), vel3(P), c->stream);
     c->acc = false;
+    return launch_check(c);
 }
 void adj_save(SphxCtx* c, double* x3, double* v3) {
     const uint32_t n = c->n;
@@ -543,16 +574,9 @@ void adj_save(SphxCtx* c, double* x3, double* v3) {
 int adj_tick(SphxCtx* c, uint8_t* flip, double* vp3) {
     int rc;
     if ((rc = do_forces(c, c->sp && c->dens))) return rc;
-    if (!c->acc) return fail(c, SPHX_E_STATE, "vv_position needs accelerations");
-    {
-        Timer t(c, kIntegrate);
-        if (flip) launch_vv_position_flip(c->P(), c->D, c->n, c->kp, flip, c->stream);
-        else launch_vv_position(c->P(), c->D, c->n, c->kp, c->stream);
-    }
-    c->sorted = c->listed = c->dens = c->sp = c->acc = false;
-    if ((rc = launch_check(c))) return rc;
+    if ((rc = do_vv_position(c, flip))) return rc;
     if (vp3) adj_save(c, nullptr, vp3);
-    if ((rc = do_neighbours(c))) return rc;
+    if ((rc = ensure_list(c))) return rc;
     if ((rc = do_density(c))) return rc;
     if ((rc = do_forces(c, false))) return rc;
     return do_vv_velocity(c);
@@ -597,6 +621,7 @@ void sphx_default_params(SphxParams* p) {
     p->cleaning_damping = 0.5;
     p->precision = SPHX_PREC_FP64;
     p->max_neighbours = 0;
+    p->neighbour_skin = -1.0;
 }
 
 int sphx_device_count(int* count) {
@@ -662,7 +687,8 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
                      &D.tmp3[0], &D.tmp3[1], &D.tmp3[2], &D.tmp1};
     for (double** d : dd)
         if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
-    if (dalloc(c, &D.r64, N) || dalloc(c, &D.r32, N) || dalloc(c, &D.recb, N) || dalloc(c, &D.recb_next, N))
+    if (dalloc(c, &D.r64, N) || dalloc(c, &D.r32, N) || dalloc(c, &D.recb, N) || dalloc(c, &D.recb_next, N) ||
+        dalloc(c, &D.xref, N))
         return bail(SPHX_E_OOM);
     {
         Mixed& X = c->X;
@@ -671,7 +697,7 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
             if (dalloc(c, f, N)) return bail(SPHX_E_OOM);
     }
     if (dalloc(c, &c->S.key, N) || dalloc(c, &c->S.rank, N) || dalloc(c, &c->S.perm, N) ||
-        dalloc(c, &c->S.skey, N))
+        dalloc(c, &c->S.skey, N) || dalloc(c, &c->S.tmp, N))
         return bail(SPHX_E_OOM);
     if (dalloc(c, &c->L.ntot, N) || dalloc(c, &c->L.overflow, 4) ||
         dalloc(c, &c->L.idx, ((N + 63) / 64) * 64 * (size_t)c->L.ks))
@@ -708,7 +734,8 @@ int sphx_set_params(SphxCtx* c, const SphxParams* p) {
     c->prm = *p;
     make_kernel_params(c);
     // a change of grid-relevant parameters needs a new upload
-    bool grid_change = old.boundary_flags != p->boundary_flags || old.cell_size != p->cell_size;
+    bool grid_change = old.boundary_flags != p->boundary_flags || old.cell_size != p->cell_size ||
+                       old.neighbour_skin != p->neighbour_skin;
     for (int k = 0; k < 3; ++k)
         grid_change |= old.box_lo[k] != p->box_lo[k] || old.box_hi[k] != p->box_hi[k] ||
                        old.grid_lo[k] != p->grid_lo[k] || old.grid_hi[k] != p->grid_hi[k];
@@ -858,7 +885,7 @@ int sphx_step(SphxCtx* c, uint32_t nsteps) {
         // previous tick's second evaluation unless positions were replaced.
         if ((rc = do_forces(c, c->sp && c->dens))) return rc;
         if ((rc = do_vv_position(c))) return rc;
-        if ((rc = do_neighbours(c))) return rc;
+        if ((rc = ensure_list(c))) return rc;
         if ((rc = do_density(c))) return rc;
         if ((rc = do_forces(c, false))) return rc;  // a* = A(x_{n+1}, v_n)
         if ((rc = do_vv_velocity(c))) return rc;
@@ -918,7 +945,7 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
     for (uint32_t m = M; m-- > 0;) {
         const uint32_t t0 = m * k, len = std::min(k, nsteps - t0);
         if (m + 1 < M) {  // recompute the segment's tape from its checkpoint
-            adj_load(c, A.ckpt + (size_t)m * 6 * n, A.ckpt + (size_t)m * 6 * n + n3);
+            if ((rc = adj_load(c, A.ckpt + (size_t)m * 6 * n, A.ckpt + (size_t)m * 6 * n + n3))) return rc;
             for (uint32_t s = 0; s < len; ++s) {
                 adj_save(c, A.tx + s * n3, A.tv + s * n3);
                 if ((rc = adj_tick(c, A.flip + (size_t)s * n, A.tvp + s * n3))) return rc;
@@ -926,13 +953,13 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
         }
         for (uint32_t s = len; s-- > 0;) {
             // at (x_{t+1}, v'_t): positions sorted, listed and dense already
-            adj_load(c, nullptr, A.tvp + s * n3);
+            if ((rc = adj_load(c, nullptr, A.tvp + s * n3))) return rc;
             launch_adj_seed(c->P().orig, n, hdt, vb, apb, ab_s, c->stream);
             launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, ab_s, xb, vb, A.rhob, A.alpha_acc, c->stream);
             launch_adj_flip(n, A.flip + (size_t)s * n, dt, hdt2, xb, vb, apb, ab, c->stream);
             // at (x_t, v_t)
-            adj_load(c, A.tx + s * n3, A.tv + s * n3);
-            if ((rc = do_neighbours(c)) || (rc = do_density(c))) return rc;
+            if ((rc = adj_load(c, A.tx + s * n3, A.tv + s * n3))) return rc;
+            if ((rc = ensure_list(c)) || (rc = do_density(c))) return rc;
             launch_gather3(c->P().orig, n, ab, ab_s, c->stream);
             launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, ab_s, xb, vb, A.rhob, A.alpha_acc, c->stream);
             if ((rc = launch_check(c))) return rc;
@@ -948,7 +975,7 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
     const double* srcs[6] = {vb.x, vb.y, vb.z, xb.x, xb.y, xb.z};
     for (int q = 0; q < 6; ++q)
         if (outs[q]) HIP_TRY(c, hipMemcpyAsync(outs[q], srcs[q], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    adj_load(c, A.fin, A.fin + n3);
+    if ((rc = adj_load(c, A.fin, A.fin + n3))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     double l = 0.0, ga = 0.0;
     for (uint32_t b = 0; b < nb; ++b) {
@@ -973,7 +1000,8 @@ int sphx_export_pairs(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t*
     int rc;
     if ((rc = ensure_list(c))) return rc;
     if ((rc = check_overflow(c))) return rc;
-    // every directed neighbour appears once per side: total/2 undirected pairs
+    // the lists hold every neighbour on both sides (and skin pairs): total/2
+    // bounds the undirected pairs
     std::vector<uint16_t> cnt(c->n);
     if (c->n) HIP_TRY(c, hipMemcpy(cnt.data(), c->L.ntot, sizeof(uint16_t) * c->n, hipMemcpyDeviceToHost));
     uint64_t total = 0;
@@ -984,23 +1012,23 @@ int sphx_export_pairs(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t*
         c->pair_buf_cap = np;
     }
     launch_fill_u32(reinterpret_cast<uint32_t*>(c->pair_counter), 0u, 2, c->stream);
-    launch_export_pairs(c->P(), c->n, c->L, c->pair_buf, c->pair_counter,
+    launch_export_pairs(c->P(), c->n, c->grid, c->L, c->pair_buf, c->pair_counter,
                         c->pair_buf_cap, c->stream);
     if ((rc = launch_check(c))) return rc;
     unsigned long long got = 0;
     HIP_TRY(c, hipMemcpyAsync(&got, c->pair_counter, sizeof got, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (got != np) return fail(c, SPHX_E_INVALID, "asymmetric neighbour lists: %llu pairs from i<j, %llu expected", got, (unsigned long long)np);
-    *count = np;
-    if (np > cap_pairs) return fail(c, SPHX_E_CAPACITY, "%llu pairs do not fit in %llu", (unsigned long long)np, (unsigned long long)cap_pairs);
-    if (!np) return SPHX_OK;
+    const uint64_t ng = got;  // pairs with r < 2 h_avg (<= np)
+    *count = ng;
+    if (ng > cap_pairs) return fail(c, SPHX_E_CAPACITY, "%llu pairs do not fit in %llu", (unsigned long long)ng, (unsigned long long)cap_pairs);
+    if (!ng) return SPHX_OK;
     if (!pairs) return fail(c, SPHX_E_INVALID, "pairs is NULL");
-    std::vector<uint64_t> packed(np);
-    std::vector<uint32_t> raw(2 * np);
-    HIP_TRY(c, hipMemcpy(raw.data(), c->pair_buf, sizeof(uint32_t) * 2 * np, hipMemcpyDeviceToHost));
-    for (uint64_t k = 0; k < np; ++k) packed[k] = ((uint64_t)raw[2 * k] << 32) | raw[2 * k + 1];
+    std::vector<uint64_t> packed(ng);
+    std::vector<uint32_t> raw(2 * ng);
+    HIP_TRY(c, hipMemcpy(raw.data(), c->pair_buf, sizeof(uint32_t) * 2 * ng, hipMemcpyDeviceToHost));
+    for (uint64_t k = 0; k < ng; ++k) packed[k] = ((uint64_t)raw[2 * k] << 32) | raw[2 * k + 1];
     std::sort(packed.begin(), packed.end());
-    for (uint64_t k = 0; k < np; ++k) {
+    for (uint64_t k = 0; k < ng; ++k) {
         pairs[2 * k] = (uint32_t)(packed[k] >> 32);
         pairs[2 * k + 1] = (uint32_t)packed[k];
     }
@@ -1038,7 +1066,8 @@ int sphx_energy(SphxCtx* c, SphxEnergy* out) {
     int rc;
     if (!c->dens && (rc = do_density(c))) return rc;
     const uint32_t nb = energy_blocks(c->n);
-    launch_energy(c->P(), c->D, c->n, c->cap, c->L, c->kp, c->has_mhd, c->energy_partial, nb, c->stream);
+    if (!c->listed) return fail(c, SPHX_E_STATE, "energy needs the neighbour list (call sphx_neighbours)");
+    launch_energy(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->has_mhd, c->energy_partial, nb, c->stream);
     if ((rc = launch_check(c))) return rc;
     std::vector<double> part((size_t)nb * 9);
     HIP_TRY(c, hipMemcpyAsync(part.data(), c->energy_partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));

@@ -27,18 +27,35 @@ void launch_iota_u32(uint32_t* p, uint32_t n, hipStream_t s) {
 }
 
 // key = linear cell index (x fastest); rank = arrival order inside the cell.
+// Particles arrive in the previous step's sorted order, so neighbouring
+// lanes mostly share a cell: each run of equal keys in a wave takes one
+// atomic for the whole run.
 __global__ void k_cell_hash(const double* __restrict__ x, const double* __restrict__ y,
                             const double* __restrict__ z, uint32_t n, Grid g,
                             uint32_t* __restrict__ key, uint32_t* __restrict__ rank,
                             uint32_t* __restrict__ count) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int cx = cell_coord(x[i], g.lo[0], g.inv_w[0], g.dims[0]);
-    int cy = cell_coord(y[i], g.lo[1], g.inv_w[1], g.dims[1]);
-    int cz = cell_coord(z[i], g.lo[2], g.inv_w[2], g.dims[2]);
-    uint32_t c = ((uint32_t)cz * (uint32_t)g.dims[1] + (uint32_t)cy) * (uint32_t)g.dims[0] + (uint32_t)cx;
-    key[i] = c;
-    rank[i] = atomicAdd(&count[c], 1u);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t c = 0xffffffffu;
+    if (valid) {
+        const int cx = cell_coord(x[i], g.lo[0], g.inv_w[0], g.dims[0]);
+        const int cy = cell_coord(y[i], g.lo[1], g.inv_w[1], g.dims[1]);
+        const int cz = cell_coord(z[i], g.lo[2], g.inv_w[2], g.dims[2]);
+        c = ((uint32_t)cz * (uint32_t)g.dims[1] + (uint32_t)cy) * (uint32_t)g.dims[0] + (uint32_t)cx;
+        key[i] = c;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = (uint32_t)__shfl_up((int)c, 1);
+    const bool head = lane == 0u || prev != c;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+    const uint32_t h = 63u - (uint32_t)__clzll(heads & upto);            // this lane's run head
+    const unsigned long long after = h == 63u ? 0ull : (heads & ~((2ull << h) - 1ull));
+    const uint32_t end = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;  // one past the run
+    uint32_t base = 0;
+    if (valid && lane == h) base = atomicAdd(&count[c], end - h);
+    base = (uint32_t)__shfl((int)base, (int)h);
+    if (valid) rank[i] = base + (lane - h);
 }
 void launch_cell_hash(const Particles& P, uint32_t n, const Grid& g, const Sort& S, hipStream_t s) {
     launch_fill_u32(S.cell_count, 0u, g.ncell, s);
@@ -117,6 +134,7 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, 
 
 __global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ rank,
                           const uint32_t* __restrict__ start, uint32_t n, uint32_t* __restrict__ perm) {
+    // perm here is the arrival-order scatter; k_cell_rank makes it deterministic
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     perm[start[key[i]] + rank[i]] = i;
@@ -124,34 +142,34 @@ __global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __re
 void launch_scatter(const Sort& S, uint32_t n, hipStream_t s) {
     if (n)
         hipLaunchKernelGGL(k_scatter, dim3(nblocks(n)), dim3(kBlock), 0, s, S.key, S.rank,
-                           S.cell_start, n, S.perm);
+                           S.cell_start, n, S.tmp);
 }
 
-// Atomic ranks arrive in any order; sorting each cell's slice of perm by x
-// (ties by the previous index) makes the order deterministic, lets the
-// neighbour build cut the two side cells of a row to the reach in x, and
-// stays nearly sorted from one step to the next (insertion sort is cheap).
-__global__ void k_cell_order(const uint32_t* __restrict__ start, uint32_t ncell, uint32_t* __restrict__ perm,
-                             const double* __restrict__ x) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+// Atomic ranks arrive in any order.  Each particle's final place in its
+// cell is the number of cell members before it by (x, previous index): a
+// deterministic order, sorted by x so that the neighbour build can cut each
+// row to the reach in x.  One thread per particle scans its own cell
+// (members are L1-resident, the cell's threads read the same lines).
+__global__ void k_cell_rank(const uint32_t* __restrict__ key, const uint32_t* __restrict__ start,
+                            const uint32_t* __restrict__ arrival, const double* __restrict__ x, uint32_t n,
+                            uint32_t* __restrict__ perm) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = key[i];
     const uint32_t b = start[c], e = start[c + 1];
-    for (uint32_t k = b + 1; k < e; ++k) {
-        const uint32_t v = perm[k];
-        const double xv = x[v];
-        uint32_t t = k;
-        while (t > b) {
-            const uint32_t u = perm[t - 1];
-            const double xu = x[u];
-            if (xu < xv || (xu == xv && u < v)) break;
-            perm[t] = u;
-            --t;
-        }
-        perm[t] = v;
+    const double xv = x[i];
+    uint32_t r = 0;
+    for (uint32_t m = b; m < e; ++m) {
+        const uint32_t j = arrival[m];
+        const double xj = x[j];
+        r += (xj < xv || (xj == xv && j < i)) ? 1u : 0u;
     }
+    perm[b + r] = i;
 }
-void launch_cell_order(const Sort& S, uint32_t ncell, const double* x, hipStream_t s) {
-    hipLaunchKernelGGL(k_cell_order, dim3(nblocks(ncell)), dim3(kBlock), 0, s, S.cell_start, ncell, S.perm, x);
+void launch_cell_order(const Sort& S, uint32_t n, const double* x, hipStream_t s) {
+    if (n)
+        hipLaunchKernelGGL(k_cell_rank, dim3(nblocks(n)), dim3(kBlock), 0, s, S.key, S.cell_start, S.tmp, x, n,
+                           S.perm);
 }
 
 // Gather of all persistent state into the new order, plus the sorted cell
@@ -159,7 +177,8 @@ void launch_cell_order(const Sort& S, uint32_t ncell, const double* x, hipStream
 // the cell origin).
 __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restrict__ perm,
                           const uint32_t* __restrict__ key, uint32_t* __restrict__ skey, uint32_t n,
-                          int mhd, int charge, int mixed, Mixed X, Grid g, double4* __restrict__ recb) {
+                          int mhd, int charge, int mixed, Mixed X, Grid g, double4* __restrict__ recb,
+                          double4* __restrict__ xref) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t i = perm[k];
@@ -182,6 +201,9 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
     double2* pos = reinterpret_cast<double2*>(dst.pos + k);
     pos[0] = make_double2(x, y);
     pos[1] = make_double2(z, h);
+    double2* ref = reinterpret_cast<double2*>(xref + k);
+    ref[0] = make_double2(x, y);
+    ref[1] = make_double2(z, 0.0);
     dst.id[k] = src.id[i];
     dst.orig[k] = src.orig[i];
     if (mhd) {
@@ -206,10 +228,11 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
     if (mixed) X.m[k] = (float)m;
 }
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n, bool mhd,
-                    bool charge, bool mixed, const Mixed& X, const Grid& g, double4* recb, hipStream_t s) {
+                    bool charge, bool mixed, const Mixed& X, const Grid& g, double4* recb, double4* xref,
+                    hipStream_t s) {
     if (n)
         hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, S.perm, S.key,
-                           S.skey, n, (int)mhd, (int)charge, (int)mixed, X, g, recb);
+                           S.skey, n, (int)mhd, (int)charge, (int)mixed, X, g, recb, xref);
 }
 
 // ---- tiles: each row cut into runs of <= kTile sorted particles ---------

@@ -112,6 +112,7 @@ def _fill_defaults(p: SphxParams) -> None:
     p.half_extent, p.activation_distance, p.stiffness = 10.0, 0.5, 1e6
     p.resistivity, p.cleaning_speed, p.cleaning_damping = 1e-4, 100.0, 0.5
     p.precision, p.max_neighbours = PREC_FP64, 0
+    p.neighbour_skin = -1.0
 
 
 # ---- particle state --------------------------------------------------------

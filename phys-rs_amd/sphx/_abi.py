@@ -40,7 +40,7 @@ class SphxParams(C.Structure):
         ("box_lo", D3), ("box_hi", D3), ("gravity", D3), ("e_field", D3),
         ("resistivity", C.c_double), ("cleaning_speed", C.c_double), ("cleaning_damping", C.c_double),
         ("cell_size", C.c_double), ("grid_lo", D3), ("grid_hi", D3),
-        ("precision", C.c_int32), ("max_neighbours", C.c_int32),
+        ("precision", C.c_int32), ("max_neighbours", C.c_int32), ("neighbour_skin", C.c_double),
     ]
 
 
