@@ -90,6 +90,34 @@ def test_pairs_dense_periodic_chunked(require_gpu):
     assert rel_err(rho, s["rho"]) < 1e-13
 
 
+def test_skin_lists_keep_the_exact_neighbour_set(require_gpu):
+    # a large skin keeps one list for many ticks of a moving fluid: at every
+    # later state the exported set is still exactly {r < 2 h_avg}, and the
+    # trajectory equals the one with a list rebuilt every tick
+    st, p = scenarios.dam_break(nx=16, ny=16, nz=8, v_sigma=0.3, seed=7)
+    n = len(st["x"])
+    p.neighbour_skin = 0.3 * 0.013
+    with make_ctx(st, p) as ctx:
+        ctx.enable_timing(True)
+        for _ in range(4):
+            ctx.step(5)
+            cur = ctx.download(sphx.F_POS)
+            s = dict(st, x=cur["x"], y=cur["y"], z=cur["z"])
+            assert np.array_equal(ctx.export_pairs(), oracle_pairs_ids(s, p, brute=True))
+        builds = ctx.kernel_times()["neighbour"]["launches"]
+        a = ctx.download(sphx.F_POS | sphx.F_VEL)
+    assert 1 <= builds < 20  # the list outlived several ticks
+    p0 = scenarios.dam_break(nx=16, ny=16, nz=8, v_sigma=0.3, seed=7)[1]
+    p0.neighbour_skin = 0.0
+    with make_ctx(st, p0) as ctx:
+        ctx.step(20)
+        b = ctx.download(sphx.F_POS | sphx.F_VEL)
+    for k in ("x", "y", "z"):
+        assert rel_err(a[k], b[k]) < 1e-12, k
+    for k in ("vx", "vy", "vz"):
+        assert rel_err(a[k], b[k]) < 1e-9, k
+
+
 def test_pairs_exactly_at_cutoff(require_gpu):
     x, y, z = scenarios.lattice(6, 6, 6, 1.0)
     n = x.size
