@@ -224,9 +224,11 @@ void launch_vjp(const Particles& P, const Derived& D, uint32_t n, const Grid& g,
                 const KernelParams& kp, const Vec3& ab_s, const Vec3& xb, const Vec3& vb, double* rhob,
                 double* alpha_acc, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(k_vjp_forces, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp, ab_s, xb, vb, rhob,
+    hipLaunchKernelGGL(k_vjp_forces, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L, kp, ab_s, xb,
+                       vb, rhob,
                        alpha_acc);
-    hipLaunchKernelGGL(k_vjp_density, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, g, L, (const double*)rhob, xb);
+    hipLaunchKernelGGL(k_vjp_density, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(4), s, P, n, g, L,
+                       (const double*)rhob, xb);
 }
 
 }  // namespace sphx

@@ -16,6 +16,14 @@ static constexpr double kInvPi = 0.31830988618379067154;  // 1/pi
 static constexpr double kPi = 3.141592653589793;           // std::f64::consts::PI
 
 
+// Dynamic LDS that caps a gather pass at `blocks` workgroups per CU (160 KiB
+// of LDS per CU; the LDS itself is not used).  The gather passes are bound
+// by L1 misses of their neighbour loads: fewer waves per CU keep more of each
+// wave's neighbour lines in L1 between consecutive list entries (measured on
+// the 1M dam break: FP64 force pass 0.72 -> 0.62 ms at 3 workgroups per CU).
+static constexpr size_t kLdsPerCU = 160 * 1024;
+static inline size_t occupancy_lds(int blocks) { return (kLdsPerCU / (size_t)blocks) & ~(size_t)255; }
+
 // blockIdx -> logical block so that each XCD (blocks are dealt round-robin to
 // the 8 XCDs) works on one contiguous range of tiles / particles and its L2
 // holds that range's neighbours.

@@ -311,9 +311,9 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
     else if (mixed)
         hipLaunchKernelGGL(k_density_mixed<false>, grid, block, 0, s, P, D, X, n, g, L, kp);
     else if (uni)
-        hipLaunchKernelGGL(k_density<true>, grid, block, 0, s, P, D, n, g, L, kp);
+        hipLaunchKernelGGL(k_density<true>, grid, block, occupancy_lds(4), s, P, D, n, g, L, kp);
     else
-        hipLaunchKernelGGL(k_density<false>, grid, block, 0, s, P, D, n, g, L, kp);
+        hipLaunchKernelGGL(k_density<false>, grid, block, occupancy_lds(4), s, P, D, n, g, L, kp);
 }
 
 // ------------------------- pressure + viscosity + Maxwell stress + walls (a6-a9, a13)
@@ -512,9 +512,11 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
 #define SPHX_FORCES(V, M, U)                                                                              \
     if (visc_only == V && stress == M && uni == U) {                                                     \
         if (mixed)                                                                                        \
-            hipLaunchKernelGGL((k_forces_mixed<V, M, U>), grid, block, 0, s, P, D, X, n, g, L, kp, ch);  \
+            hipLaunchKernelGGL((k_forces_mixed<V, M, U>), grid, block, occupancy_lds(4), s, P, D, X, n, g, L, \
+                               kp, ch);                                                                   \
         else                                                                                              \
-            hipLaunchKernelGGL((k_forces<V, M, U>), grid, block, 0, s, P, D, n, g, L, kp, ch);           \
+            hipLaunchKernelGGL((k_forces<V, M, U>), grid, block, M ? 0 : occupancy_lds(3), s, P, D, n, g, L, kp, \
+                               ch);                                                                       \
         return;                                                                                           \
     }
     SPHX_FORCES(false, false, false)
@@ -618,9 +620,10 @@ void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Gr
                       const KernelParams& kp, hipStream_t s) {
     if (!n) return;
     if (kp.uniform)
-        hipLaunchKernelGGL(k_induction<true>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
+        hipLaunchKernelGGL(k_induction<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L, kp);
     else
-        hipLaunchKernelGGL(k_induction<false>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
+        hipLaunchKernelGGL(k_induction<false>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L,
+                           kp);
 }
 
 // ---------------------------------------------- divergence cleaning (a12)
@@ -671,9 +674,10 @@ void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Gr
                       const KernelParams& kp, hipStream_t s) {
     if (!n) return;
     if (kp.uniform)
-        hipLaunchKernelGGL(k_div_clean<true>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
+        hipLaunchKernelGGL(k_div_clean<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L, kp);
     else
-        hipLaunchKernelGGL(k_div_clean<false>, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, g, L, kp);
+        hipLaunchKernelGGL(k_div_clean<false>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L,
+                           kp);
 }
 
 // ---------------------------------------------------------- pair export
