@@ -22,7 +22,8 @@ The FP64 drop-in is `value`; beside it the line reports
 the FP64 end state, total-energy drift of each form) and prints it instead.
 --mhd-report runs the 4,194,304-particle periodic MHD box (seeded uniform
 random fill, <N_nb> = 40, induction + cleaning + Maxwell stress) in both
-forms and prints the per-kernel breakdown instead (--mhd-particles to resize).
+forms and prints the per-kernel breakdown instead (--mhd-particles to resize;
+--mhd-lattice 64 --steps 100 for the 262,144-particle jittered lattice).
 
 Prints one JSON line (rank 0).  --gpus N > 1 is not supported: this build runs
 on one GPU.
@@ -172,14 +173,19 @@ def run_adjoint(steps: int = 100, every: int = 10, reps: int = 3):
                     "differentiates, subtracted here); host-timed, state resident"}
 
 
-def mhd_report(n_fill: int, steps: int, warmup: int):
-    """Per-kernel breakdown of the periodic MHD box in the FP64 and FP32-mixed forms."""
+def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
+    """Per-kernel breakdown of the periodic MHD box in the FP64 and FP32-mixed forms
+    (seeded random fill of n_fill particles, or the lattice^3 jittered lattice)."""
     import sphx
     from sphx import scenarios
 
-    st, p = scenarios.mhd_periodic(random_fill=n_fill)
+    if lattice:
+        st, p = scenarios.mhd_periodic(n_side=lattice)
+    else:
+        st, p = scenarios.mhd_periodic(random_fill=n_fill)
     n = len(st["x"])
-    out = {"report": "mhd", "workload": f"mhd_periodic_random_{n}", "particles": n, "steps": steps,
+    kind = f"lattice_{lattice}^3" if lattice else "random"
+    out = {"report": "mhd", "workload": f"mhd_periodic_{kind}_{n}", "particles": n, "steps": steps,
            "h": float(st["h"][0])}
     for prec in ("fp64", "mixed"):
         p.precision = sphx.PREC_FP64 if prec == "fp64" else sphx.PREC_FP32_MIXED
@@ -304,6 +310,8 @@ def main() -> int:
     ap.add_argument("--precision-report", action="store_true")
     ap.add_argument("--mhd-report", action="store_true")
     ap.add_argument("--mhd-particles", type=int, default=4194304)
+    ap.add_argument("--mhd-lattice", type=int, default=0,
+                    help="use the n^3 jittered lattice (64: the 262,144-particle box) instead of the random fill")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device-to-device copy (FETCH/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -326,7 +334,8 @@ def main() -> int:
         print(json.dumps(precision_report()), flush=True)
         return 0
     if args.mhd_report:
-        print(json.dumps(mhd_report(args.mhd_particles, min(args.steps, 20), min(args.warmup, 2))), flush=True)
+        steps = args.steps if args.mhd_lattice else min(args.steps, 20)
+        print(json.dumps(mhd_report(args.mhd_particles, steps, min(args.warmup, 2), args.mhd_lattice)), flush=True)
         return 0
 
     n, secs, times, e = run_gpu(args.precision, args.steps, args.warmup)

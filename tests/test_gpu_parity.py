@@ -255,6 +255,27 @@ def test_step_parity_dam_break(require_gpu):
         assert np.abs(g[k] - s[k]).max() < 1e-9 * max(np.abs(s[k]).max(), 1e-3), k
 
 
+def test_c0_dam_break_200_steps(require_gpu):
+    # the full 16,384-particle dam break: 200 velocity-Verlet ticks against the
+    # oracle (about 15 s of CPU); differences stay at round-off level
+    st, p = scenarios.dam_break()
+    nsteps = 200
+    with make_ctx(st, p) as ctx:
+        ctx.step(nsteps)
+        g = ctx.download(sphx.F_POS | sphx.F_VEL)
+        e_gpu = ctx.energy()
+    s = O.State(st)
+    O.step(s, p, nsteps)
+    O.evaluate(s, p)
+    e_orc = O.energy(s, p)
+    for k in ("x", "y", "z"):
+        assert np.abs(g[k] - s[k]).max() < 1e-10, k
+    for k in ("vx", "vy", "vz"):
+        assert np.abs(g[k] - s[k]).max() < 1e-7 * max(np.abs(s[k]).max(), 1e-3), k
+    for k in ("kinetic", "potential", "internal"):
+        assert e_gpu[k] == pytest.approx(e_orc[k], rel=1e-9, abs=1e-12), k
+
+
 def test_step_parity_mhd(require_gpu):
     st, p = scenarios.mhd_periodic(16)
     nsteps = 5
