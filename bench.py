@@ -309,6 +309,7 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="skip the mixed and adjoint side runs")
     ap.add_argument("--precision-report", action="store_true")
     ap.add_argument("--mhd-report", action="store_true")
+    ap.add_argument("--adjoint-report", action="store_true", help="only the 65,536-particle adjoint/forward run")
     ap.add_argument("--mhd-particles", type=int, default=4194304)
     ap.add_argument("--mhd-lattice", type=int, default=0,
                     help="use the n^3 jittered lattice (64: the 262,144-particle box) instead of the random fill")
@@ -332,6 +333,9 @@ def main() -> int:
         return 0
     if args.precision_report:
         print(json.dumps(precision_report()), flush=True)
+        return 0
+    if args.adjoint_report:
+        print(json.dumps(run_adjoint()), flush=True)
         return 0
     if args.mhd_report:
         steps = args.steps if args.mhd_lattice else min(args.steps, 20)

@@ -24,19 +24,49 @@
 
 namespace sphx {
 
-// cotangent of the acceleration, gathered into sorted order (ab_s) for the
-// pair pass; apb keeps hdt * vbar for the reflection step
-__global__ void k_adj_seed(const uint32_t* __restrict__ orig, uint32_t n, double hdt, Vec3 vb, Vec3 apb, Vec3 ab_s) {
+// cotangent of the acceleration, gathered into sorted order (ab4: one
+// 32-byte record per particle) for the pair pass; apb keeps hdt * vbar for
+// the reflection step
+__global__ void k_adj_seed(const uint32_t* __restrict__ orig, uint32_t n, double hdt, Vec3 vb, Vec3 apb,
+                           double4* __restrict__ ab4) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t u = orig[i];
     const double x = hdt * vb.x[u], y = hdt * vb.y[u], z = hdt * vb.z[u];
     apb.x[u] = x; apb.y[u] = y; apb.z[u] = z;
-    ab_s.x[i] = x; ab_s.y[i] = y; ab_s.z[i] = z;
+    double2* r = reinterpret_cast<double2*>(ab4 + i);
+    r[0] = make_double2(x, y);
+    r[1] = make_double2(z, 0.0);
 }
-void launch_adj_seed(const uint32_t* orig, uint32_t n, double hdt, const Vec3& vb, const Vec3& apb, const Vec3& ab_s,
+void launch_adj_seed(const uint32_t* orig, uint32_t n, double hdt, const Vec3& vb, const Vec3& apb, double4* ab4,
                      hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_adj_seed, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, hdt, vb, apb, ab_s);
+    if (n) hipLaunchKernelGGL(k_adj_seed, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, hdt, vb, apb, ab4);
+}
+
+// ab4 (sorted records) = src (upload order) through orig
+__global__ void k_gather_ab(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, double4* __restrict__ ab4) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = orig[i];
+    double2* r = reinterpret_cast<double2*>(ab4 + i);
+    r[0] = make_double2(src.x[u], src.y[u]);
+    r[1] = make_double2(src.z[u], 0.0);
+}
+void launch_gather_ab(const uint32_t* orig, uint32_t n, const Vec3& src, double4* ab4, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_gather_ab, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, ab4);
+}
+
+// velocities (upload order) -> sorted state and the f64 gather record
+__global__ void k_load_velocity(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, Particles P, Derived D) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = orig[i];
+    const double vx = src.x[u], vy = src.y[u], vz = src.z[u];
+    P.vx[i] = vx; P.vy[i] = vy; P.vz[i] = vz;
+    D.r64[i].vx = vx; D.r64[i].vy = vy; D.r64[i].vz = vz;
+}
+void launch_load_velocity(const Particles& P, const Derived& D, uint32_t n, const Vec3& src, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_load_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, P.orig, n, src, P, D);
 }
 
 // dst (sorted) = src (upload order) through orig
@@ -120,12 +150,15 @@ void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const V
 }
 
 // VJP of the pressure + viscosity acceleration at the current (sorted)
-// state.  ab_s: abar in sorted order.  Adds dL/dx and dL/dv into xb, vb
+// state.  ab4: abar records in sorted order; neighbours' x, rho, v, P/rho^2
+// come from the f64 gather record.  Adds dL/dx and dL/dv into xb, vb
 // (upload order), writes rhobar (sorted) and adds this block's share of
 // dL/dalpha to alpha_acc[block].
+template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                        KernelParams kp, Vec3 ab_s, Vec3 xb, Vec3 vb,
-                                                        double* __restrict__ rhob, double* __restrict__ alpha_acc) {
+                                                        KernelParams kp, const double4* __restrict__ ab4, Vec3 xb,
+                                                        Vec3 vb, double* __restrict__ rhob,
+                                                        double* __restrict__ alpha_acc) {
     __shared__ double lds[kBlock];
     const uint32_t i = particle_index();
     double da = 0.0;
@@ -134,7 +167,10 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
         const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
         const double rhoi = D.rho[i], pr2i = D.pr2[i];
         const double inv_mi = 1.0 / mi;
-        const double uxi = ab_s.x[i] * inv_mi, uyi = ab_s.y[i] * inv_mi, uzi = ab_s.z[i] * inv_mi;
+        const double2* __restrict__ AB = reinterpret_cast<const double2*>(ab4);
+        const double2* __restrict__ R = reinterpret_cast<const double2*>(D.r64);
+        const double2 ab01 = AB[2 * i], ab2 = AB[2 * i + 1];
+        const double uxi = ab01.x * inv_mi, uyi = ab01.y * inv_mi, uzi = ab2.x * inv_mi;
         // d(P_i/rho_i^2)/d(rho_i)
         const double dTi = eos_dpdrho(kp, rhoi) / (rhoi * rhoi) - 2.0 * pr2i / rhoi;
         const double cs = kp.speed_of_sound, alpha = kp.alpha, beta = kp.beta;
@@ -143,23 +179,25 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
         double gx = 0, gy = 0, gz = 0, gvx = 0, gvy = 0, gvz = 0, grho = 0;
         for_neighbours(L, i, [&](uint32_t e) {
             const uint32_t j = e & kIdxMask;
+            const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
+            const double2 a01 = AB[2 * j], a2 = AB[2 * j + 1];
             double dx, dy, dz;
-            pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+            pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
             if (!(r2 > 1e-20)) return;  // r <= 1e-10: no pressure force (sph.rs:52)
             double r, inv_r;
             sqrt_rsqrt(r2, r, inv_r);
-            const double h = 0.5 * (hi + P.h[j]);
-            const double inv_h = rcp_nr(h);
+            const double h = UNI ? kp.h0 : 0.5 * (hi + P.h[j]);
+            const double inv_h = UNI ? kp.inv_h0 : rcp_nr(h);
             const double G = kernel_dw(r, inv_h) * inv_r;
             const double Gp = (kernel_d2w(r, inv_h) - G) * inv_r;  // dG/dr
-            const double mj = P.m[j], mm = mi * mj;
-            const double inv_mj = 1.0 / mj;
-            const double ux = uxi - ab_s.x[j] * inv_mj, uy = uyi - ab_s.y[j] * inv_mj, uz = uzi - ab_s.z[j] * inv_mj;
+            const double mj = UNI ? kp.m0 : P.m[j], mm = mi * mj;
+            const double inv_mj = UNI ? inv_mi : 1.0 / mj;
+            const double ux = uxi - a01.x * inv_mj, uy = uyi - a01.y * inv_mj, uz = uzi - a2.x * inv_mj;
             const double c = fma(uz, dz, fma(uy, dy, ux * dx));
-            const double rhoj = D.rho[j];
-            const double T = pr2i + D.pr2[j];
-            const double vx = vxi - P.vx[j], vy = vyi - P.vy[j], vz = vzi - P.vz[j];
+            const double rhoj = zr.y;
+            const double T = pr2i + vzp.y;
+            const double vx = vxi - vxy.x, vy = vyi - vxy.y, vz = vzi - vzp.x;
             const double sdot = fma(vz, dz, fma(vy, dy, vx * dx));
             const bool act = ref_conv ? (sdot < 0.0) : (sdot > 0.0);
             double Pi = 0.0, Pi_mu = 0.0, mu = 0.0, Dn = 1.0, rho_bar = 1.0;
@@ -196,22 +234,26 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
 
 // VJP of the summation density: rho_i = m_i W(0, h_i) + sum_j m_j W(r_ij, h_ij)
 // gives xbar_i -= sum_j (rhobar_i m_j + rhobar_j m_i) G d.
+template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n, Grid g, NbrList L,
-                                                         const double* __restrict__ rhob, Vec3 xb) {
+                                                         KernelParams kp, const double* __restrict__ rhob, Vec3 xb) {
     const uint32_t i = particle_index();
     if (i >= n) return;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i], rbi = rhob[i];
+    const double2* __restrict__ pos = reinterpret_cast<const double2*>(P.pos);
     double gx = 0, gy = 0, gz = 0;
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
+        const double2 xy = pos[2 * j], zh = pos[2 * j + 1];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+        pair_vec(g, xi, yi, zi, xy.x, xy.y, zh.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 > 0.0)) return;
         double r, inv_r;
         sqrt_rsqrt(r2, r, inv_r);
-        const double inv_h = rcp_nr(0.5 * (hi + P.h[j]));
-        const double coef = (rbi * P.m[j] + rhob[j] * mi) * (kernel_dw(r, inv_h) * inv_r);
+        const double inv_h = UNI ? kp.inv_h0 : rcp_nr(0.5 * (hi + zh.y));
+        const double coef = UNI ? (rbi + rhob[j]) * mi * (kernel_dw(r, inv_h) * inv_r)
+                                : (rbi * P.m[j] + rhob[j] * mi) * (kernel_dw(r, inv_h) * inv_r);
         gx = fma(-coef, dx, gx);
         gy = fma(-coef, dy, gy);
         gz = fma(-coef, dz, gz);
@@ -221,14 +263,21 @@ __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n,
 }
 
 void launch_vjp(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                const KernelParams& kp, const Vec3& ab_s, const Vec3& xb, const Vec3& vb, double* rhob,
+                const KernelParams& kp, const double4* ab4, const Vec3& xb, const Vec3& vb, double* rhob,
                 double* alpha_acc, hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(k_vjp_forces, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L, kp, ab_s, xb,
-                       vb, rhob,
-                       alpha_acc);
-    hipLaunchKernelGGL(k_vjp_density, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(4), s, P, n, g, L,
-                       (const double*)rhob, xb);
+    const dim3 grid(nblocks(n)), block(kBlock);
+    if (kp.uniform) {
+        hipLaunchKernelGGL(k_vjp_forces<true>, grid, block, occupancy_lds(3), s, P, D, n, g, L, kp, ab4, xb, vb, rhob,
+                           alpha_acc);
+        hipLaunchKernelGGL(k_vjp_density<true>, grid, block, occupancy_lds(4), s, P, n, g, L, kp,
+                           (const double*)rhob, xb);
+    } else {
+        hipLaunchKernelGGL(k_vjp_forces<false>, grid, block, occupancy_lds(3), s, P, D, n, g, L, kp, ab4, xb, vb,
+                           rhob, alpha_acc);
+        hipLaunchKernelGGL(k_vjp_density<false>, grid, block, occupancy_lds(4), s, P, n, g, L, kp,
+                           (const double*)rhob, xb);
+    }
 }
 
 }  // namespace sphx

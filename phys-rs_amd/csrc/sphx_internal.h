@@ -191,7 +191,9 @@ uint32_t energy_blocks(uint32_t n);
 // adjoint (sphx_adjoint.hip); Vec3 arrays named *_s are in sorted order,
 // the others in upload order
 void launch_adj_seed(const uint32_t* orig, uint32_t n, double hdt, const Vec3& vb, const Vec3& apb,
-                     const Vec3& ab_s, hipStream_t s);
+                     double4* ab4, hipStream_t s);
+void launch_gather_ab(const uint32_t* orig, uint32_t n, const Vec3& src, double4* ab4, hipStream_t s);
+void launch_load_velocity(const Particles& P, const Derived& D, uint32_t n, const Vec3& src, hipStream_t s);
 void launch_gather3(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst_s, hipStream_t s);
 void launch_scatter3(const uint32_t* orig, uint32_t n, const Vec3& src_s, const Vec3& dst, hipStream_t s);
 void launch_adj_flip(uint32_t n, const uint8_t* flip, double dt, double hdt2, const Vec3& xb, const Vec3& vb,
@@ -199,7 +201,7 @@ void launch_adj_flip(uint32_t n, const uint8_t* flip, double dt, double hdt2, co
 void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const Vec3& xb, double* part,
                      hipStream_t s);
 void launch_vjp(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                const KernelParams& kp, const Vec3& ab_s, const Vec3& xb, const Vec3& vb, double* rhob,
+                const KernelParams& kp, const double4* ab4, const Vec3& xb, const Vec3& vb, double* rhob,
                 double* alpha_acc, hipStream_t s);
 
 }  // namespace sphx

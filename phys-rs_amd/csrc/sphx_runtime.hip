@@ -47,7 +47,7 @@ struct AdjBuffers {
     uint8_t* flip = nullptr;       // k x n: wall reflections of each tick
     double* fin = nullptr;         // 6n: state after the last tick
     double* adj = nullptr;         // 4 x 3n: xbar, vbar, abar, abar_prev
-    double* ab_s = nullptr;        // 3n: abar in sorted order
+    double4* ab4 = nullptr;        // n: abar records in sorted order
     double* rhob = nullptr;        // n: rhobar in sorted order
     double* target = nullptr;      // 3n
     double* part = nullptr;        // per-block loss partial sums
@@ -527,7 +527,7 @@ int adj_alloc(SphxCtx* c, uint32_t n, uint32_t k, uint32_t m) {
     if ((rc = get(&A.ckpt, (size_t)m * 6 * N)) || (rc = get(&A.tx, (size_t)(k + 1) * 3 * N)) ||
         (rc = get(&A.tv, (size_t)(k + 1) * 3 * N)) || (rc = get(&A.tvp, (size_t)k * 3 * N)) ||
         (rc = get(&A.flip, (size_t)k * N)) || (rc = get(&A.fin, 6 * N)) || (rc = get(&A.adj, 12 * N)) ||
-        (rc = get(&A.ab_s, 3 * N)) || (rc = get(&A.rhob, N)) || (rc = get(&A.target, 3 * N)) ||
+        (rc = get(&A.ab4, N)) || (rc = get(&A.rhob, N)) || (rc = get(&A.target, 3 * N)) ||
         (rc = get(&A.part, nb)) || (rc = get(&A.alpha_acc, nb))) {
         A.release();
         return rc;
@@ -559,7 +559,7 @@ int adj_load(SphxCtx* c, const double* x3, const double* v3) {
             c->sorted = c->listed = false;
         }
     }
-    launch_gather3(P.orig, n, vec3(const_cast<double*>(v3), n), vel3(P), c->stream);
+    launch_load_velocity(P, c->D, n, vec3(const_cast<double*>(v3), n), c->stream);
     c->acc = false;
     return launch_check(c);
 }
@@ -920,7 +920,7 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
     HIP_TRY(c, hipMemcpyAsync(A.target + n, tgy, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(A.target + 2 * (size_t)n, tgz, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     const Vec3 xb = vec3(A.adj, n), vb = vec3(A.adj + n3, n), ab = vec3(A.adj + 2 * n3, n),
-               apb = vec3(A.adj + 3 * n3, n), ab_s = vec3(A.ab_s, n);
+               apb = vec3(A.adj + 3 * n3, n);
     const uint32_t last0 = (M - 1) * k;  // first tick of the last segment
     // forward: checkpoints every k ticks, the last segment's tape kept
     for (uint32_t t = 0; t < nsteps; ++t) {
@@ -954,14 +954,14 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
         for (uint32_t s = len; s-- > 0;) {
             // at (x_{t+1}, v'_t): positions sorted, listed and dense already
             if ((rc = adj_load(c, nullptr, A.tvp + s * n3))) return rc;
-            launch_adj_seed(c->P().orig, n, hdt, vb, apb, ab_s, c->stream);
-            launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, ab_s, xb, vb, A.rhob, A.alpha_acc, c->stream);
+            launch_adj_seed(c->P().orig, n, hdt, vb, apb, A.ab4, c->stream);
+            launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, A.ab4, xb, vb, A.rhob, A.alpha_acc, c->stream);
             launch_adj_flip(n, A.flip + (size_t)s * n, dt, hdt2, xb, vb, apb, ab, c->stream);
             // at (x_t, v_t)
             if ((rc = adj_load(c, A.tx + s * n3, A.tv + s * n3))) return rc;
             if ((rc = ensure_list(c)) || (rc = do_density(c))) return rc;
-            launch_gather3(c->P().orig, n, ab, ab_s, c->stream);
-            launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, ab_s, xb, vb, A.rhob, A.alpha_acc, c->stream);
+            launch_gather_ab(c->P().orig, n, ab, A.ab4, c->stream);
+            launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, A.ab4, xb, vb, A.rhob, A.alpha_acc, c->stream);
             if ((rc = launch_check(c))) return rc;
         }
     }
