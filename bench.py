@@ -57,6 +57,15 @@ ALG = {
     "mhd":       {"bytes": 200, "flops_pair": 0.0},
 }
 STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
+
+# HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a
+# 256 MiB copy) of the FP64 kernels on this workload, from the rocprofv3
+# counter runs summarised in profiles/dam_break_1M_fp64_mixed_kernels.md.
+MEASURED_TRAFFIC = {
+    "force": (604.6 + 50.3) * 1e6,
+    "visc_only": (621.4 + 25.2) * 1e6,
+    "density": (502.2 + 95.4) * 1e6,
+}
 STEP_FLOPS_PER_PAIR = 26.0 + 58.0 + 45.0
 
 
@@ -271,13 +280,13 @@ def roofline(n: int, times: dict, pairs: int, steps: int):
         # and no MFMA instruction is used.
         ach = f / avg_s / 1e12
         rf = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
-              "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+              "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": MEASURED_TRAFFIC.get(dom),
               "note": "FP64 vector ALU (no MFMA); peak = MI355X FP64 78.6 TFLOP/s",
               "hbm_GBs": round(b / avg_s / 1e9, 1)}
     else:
         ach = b / avg_s / 1e9
         rf = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-              "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+              "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get(dom)}
     return rf, kern
 
 
