@@ -11,6 +11,7 @@
 namespace sphx {
 
 static constexpr int kBlock = kTile;            // one thread per particle
+static constexpr int kStagedBlock = 512;        // workgroup of the brick-staged passes
 static constexpr int kStageCap = 384;           // candidates staged in LDS per chunk
 static constexpr double kInvPi = 0.31830988618379067154;  // 1/pi
 static constexpr double kPi = 3.141592653589793;           // std::f64::consts::PI
@@ -397,5 +398,278 @@ __device__ __forceinline__ void entry_offset(uint32_t e, float wx, float wy, flo
 
 // Sorted particle index of the calling thread in a one-thread-per-particle pass.
 __device__ __forceinline__ uint32_t particle_index() { return xcd_block() * kBlock + threadIdx.x; }
+
+// Gather record of list entry e: NQ 16-byte quads of type Q at record j.
+template <class Q, int NQ>
+struct GlobalFetch {
+    const Q* __restrict__ glob;
+    __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NQ]) const {
+        const Q* p = glob + (size_t)(e & kIdxMask) * NQ;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) q[k] = p[k];
+    }
+};
+
+// body(e, record) for every list entry of sorted particle i, records from
+// global memory.
+template <class Q, int NQ, class Body>
+__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalFetch<Q, NQ>& F, Body&& body) {
+    for_neighbours(L, i, [&](uint32_t e) {
+        Q q[NQ];
+        F(e, q);
+        body(e, q);
+    });
+}
+
+// =========================================================================
+// Brick-staged passes (the Bricks decomposition).
+// =========================================================================
+struct BrickHdr {
+    uint4 halo[kMaxHaloRows];           // per halo row: {first sorted index, staged count, LDS record base, 0}
+    uint32_t own_start[kMaxOwnRows];    // per brick row: its first sorted particle
+    uint32_t own_cum[kMaxOwnRows + 1];  // exclusive prefix of the brick rows' particle counts
+};
+
+// The active bricks are split into 8 contiguous ranges, one per XCD
+// (workgroups are dealt to the XCDs round-robin; the grid is a multiple of 8),
+// and the workgroups of an XCD take its range's bricks in turn.
+struct BrickIter {
+    uint32_t b, end, step;
+};
+__device__ __forceinline__ BrickIter brick_iter(const Bricks& B) {
+    const uint32_t na = B.pos[B.count];
+    const uint32_t xcd = blockIdx.x & 7u, k = blockIdx.x >> 3, K = gridDim.x >> 3;
+    const uint32_t base = na >> 3, rem = na & 7u;
+    const uint32_t lo = xcd * base + min(xcd, rem);
+    return {lo + k, lo + base + (xcd < rem ? 1u : 0u), K};
+}
+
+// Brick `id`'s halo rows (cells x0 - 1 .. x0 + b[0] of rows y0 - 1 .. y0 + b[1],
+// z0 - 1 .. z0 + b[2], wrapped in a periodic box, clamped in a closed one;
+// the first `cap` records are staged) and its own rows.  A periodic row that
+// wraps in x stages only its unwrapped part.  Needs blockDim.x >= 128.
+__device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const Grid& g,
+                                            const uint32_t* __restrict__ start, uint32_t cap, BrickHdr& H) {
+    const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+    const int BX = B.b[0], BY = B.b[1], BZ = B.b[2];
+    const uint32_t nbx = (uint32_t)B.nb[0], nby = (uint32_t)B.nb[1];
+    const int x0 = (int)(id % nbx) * BX, y0 = (int)((id / nbx) % nby) * BY, z0 = (int)(id / (nbx * nby)) * BZ;
+    const uint32_t t = threadIdx.x;
+    const int lane = (int)(t & 63u);
+    if (t < 64) {
+        const int hy = BY + 2, nh = hy * (BZ + 2);
+        uint32_t gs = 0, gl = 0;
+        if (lane < nh) {
+            int y = y0 - 1 + lane % hy, z = z0 - 1 + lane / hy;
+            bool ok = true;
+            if (g.periodic) {
+                y = (y % ny + ny) % ny;
+                z = (z % nz + nz) % nz;
+            } else {
+                ok = y >= 0 && y < ny && z >= 0 && z < nz;
+            }
+            if (ok) {
+                int xa = x0 - 1, xe = x0 + BX;
+                if (g.periodic && xe - xa + 1 >= nx) { xa = 0; xe = nx - 1; }
+                xa = max(xa, 0);
+                xe = min(xe, nx - 1);
+                const uint32_t row = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+                gs = start[row + (uint32_t)xa];
+                gl = start[row + (uint32_t)xe + 1u] - gs;
+            }
+        }
+        uint32_t incl = gl;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        const uint32_t base = incl - gl;
+        const uint32_t staged = base >= cap ? 0u : min(gl, cap - base);
+        if (lane < nh) H.halo[lane] = make_uint4(gs, staged, base, 0u);
+    } else if (t < 128) {
+        const int no = BY * BZ;
+        uint32_t s0 = 0, cnt = 0;
+        if (lane < no) {
+            const int y = y0 + lane % BY, z = z0 + lane / BY;
+            if (y < ny && z < nz) {
+                const uint32_t row = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+                s0 = start[row + (uint32_t)x0];
+                cnt = start[row + (uint32_t)min(x0 + BX, nx)] - s0;
+            }
+        }
+        uint32_t incl = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (lane < no) {
+            H.own_start[lane] = s0;
+            H.own_cum[lane] = incl - cnt;
+        }
+        if (lane == no - 1) H.own_cum[no] = incl;
+    }
+    __syncthreads();
+}
+
+// Copy the staged records (NQ 16-byte quads each) into LDS, quad-major: quad
+// k of staged record r at dst[k * cap + r], so that one quad of 64 random
+// records spreads over all LDS banks (record-major, a 64-byte record stride
+// would put quad k of every record in the same 16 of the 64 banks).  Wave w
+// copies halo rows w, w + waves, ..., each one contiguous range.
+template <int NQ>
+__device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uint4* __restrict__ src,
+                                            uint4* __restrict__ dst, uint32_t cap) {
+    const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (int r = w; r < nh; r += nw) {
+        const uint4 h = H.halo[r];
+        const uint32_t nq = h.y * NQ;
+        const uint4* __restrict__ s = src + (size_t)h.x * NQ;
+        uint4* __restrict__ d = dst + h.z;
+        uint32_t q = lane;
+        for (; q + 192u < nq; q += 256u) {
+            const uint4 a = s[q], b = s[q + 64u], c = s[q + 128u], e = s[q + 192u];
+            d[(q % NQ) * cap + q / NQ] = a;
+            d[((q + 64u) % NQ) * cap + (q + 64u) / NQ] = b;
+            d[((q + 128u) % NQ) * cap + (q + 128u) / NQ] = c;
+            d[((q + 192u) % NQ) * cap + (q + 192u) / NQ] = e;
+        }
+        for (; q < nq; q += 64u) d[(q % NQ) * cap + q / NQ] = s[q];
+    }
+    __syncthreads();
+}
+
+// LDS pointers carry their address space so that reads through them are
+// ds_read (a generic pointer would turn them into flat loads).
+#define SPHX_LDS __attribute__((address_space(3)))
+// 16-byte LDS word, read as a builtin vector (HIP's vector classes cannot be
+// copied out of an address-space-qualified object)
+typedef uint32_t lds_quad_t __attribute__((ext_vector_type(4)));
+template <class Q>
+__device__ __forceinline__ Q lds_quad(const SPHX_LDS lds_quad_t* p) {
+    const lds_quad_t v = *p;
+    Q q;
+    __builtin_memcpy(&q, &v, sizeof q);
+    return q;
+}
+
+// Record of list entry e of a particle in brick row (ly, lz), base = ly + hy * lz
+// with hy = b[1] + 2: the neighbour lies in halo row base + sy + hy * sz, where
+// (sy, sz) = (oy + 1, oz + 1) is the entry's stage; from LDS when staged.
+template <class Q, int NQ, int W_ = 2, bool PF_ = false>
+struct BrickFetch {
+    static_assert(sizeof(Q) == 16, "records are made of 16-byte quads");
+    static constexpr int W = W_;
+    static constexpr bool PF = PF_;
+    const SPHX_LDS lds_quad_t* lds;
+    const SPHX_LDS lds_quad_t* halo;
+    const Q* __restrict__ glob;
+    uint32_t base, hy, cap;
+    // LDS record slot of entry e; false if it is not staged
+    __device__ __forceinline__ bool locate(uint32_t e, uint32_t& slot) const {
+        const uint32_t j = e & kIdxMask, c = e >> kIdxBits;
+        const uint32_t s = (c * 11u) >> 5, sz = (s * 11u) >> 5;
+        const lds_quad_t h = halo[base + (s - 3u * sz) + hy * sz];
+        const uint32_t d = j - h.x;
+        slot = h.z + d;
+        return d < h.y;
+    }
+    __device__ __forceinline__ void from_lds(uint32_t slot, Q (&q)[NQ]) const {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) q[k] = lds_quad<Q>(lds + slot + k * cap);
+    }
+    __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NQ]) const {
+        uint32_t slot;
+        if (locate(e, slot)) {
+            from_lds(slot, q);
+        } else {
+            const Q* p = glob + (size_t)(e & kIdxMask) * NQ;
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) q[k] = p[k];
+        }
+    }
+};
+
+// body(e, record) for every list entry of sorted particle i, records from the
+// brick's LDS: W (2 or 4) entries per iteration, with PF the next W loaded
+// ahead.  When all W neighbours of every active lane are staged (a
+// wave-uniform test) the records are read with no per-lane branch, so the W
+// bodies, written branch-free, form one basic block the compiler can
+// interleave.
+template <class Q, int NQ, int W, bool PF, class Body>
+__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF>& F,
+                                     Body&& body) {
+    static_assert(W == 2 || W == 4, "two or four entries per iteration");
+    const uint32_t* __restrict__ e = nbr_base(L, i);
+    const uint32_t cnt = L.ntot[i];
+    uint32_t a[W];
+    if (PF) {
+#pragma unroll
+        for (int u = 0; u < W; ++u) a[u] = (uint32_t)u < cnt ? e[(size_t)u << 6] : 0u;
+    }
+    uint32_t k = 0;
+    for (; k + (W - 1) < cnt; k += W) {
+        uint32_t c[W], slot[W];
+        bool staged = true;
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            if (PF) {
+                // unconditional (clamped to the last entry) so that the load
+                // has its own registers and nothing waits for it this iteration
+                c[u] = a[u];
+                a[u] = e[(size_t)min(k + W + u, cnt - 1u) << 6];
+            } else {
+                c[u] = e[(size_t)(k + u) << 6];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) staged = F.locate(c[u], slot[u]) && staged;
+        Q q[W][NQ];
+        if (__all(staged)) {
+#pragma unroll
+            for (int u = 0; u < W; ++u) F.from_lds(slot[u], q[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < W; ++u) F(c[u], q[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) body(c[u], q[u]);
+    }
+    Q q[NQ];
+#pragma unroll
+    for (int u = 0; u < W - 1; ++u) {
+        if (k + u < cnt) {
+            const uint32_t c = PF ? a[u] : e[(size_t)(k + u) << 6];
+            F(c, q);
+            body(c, q);
+        }
+    }
+}
+
+// for every brick of this workgroup: set up, stage the records of `src`
+// (NQ quads each), then body(i, fetch) for each of the brick's particles
+template <class Q, int NQ, int W, bool PF, class Body>
+__device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const uint32_t* __restrict__ start,
+                                           uint32_t cap, const Q* __restrict__ src, BrickHdr& H,
+                                           uint4* __restrict__ sh, Body&& body) {
+    const uint32_t hy = (uint32_t)B.b[1] + 2u, nh = hy * ((uint32_t)B.b[2] + 2u);
+    const uint32_t BY = (uint32_t)B.b[1], no = BY * (uint32_t)B.b[2];
+    const SPHX_LDS BrickHdr* HL = (const SPHX_LDS BrickHdr*)&H;
+    for (BrickIter it = brick_iter(B); it.b < it.end; it.b += it.step) {
+        __syncthreads();  // the previous brick's walks are done with H and the records
+        brick_setup(B.active[it.b], B, g, start, cap, H);
+        brick_stage<NQ>(H, (int)nh, reinterpret_cast<const uint4*>(src), sh, cap);
+        const BrickFetch<Q, NQ, W, PF> base_fetch{(const SPHX_LDS lds_quad_t*)sh, (const SPHX_LDS lds_quad_t*)HL->halo,
+                                           src, 0u, hy, cap};
+        const uint32_t nown = HL->own_cum[no];
+        uint32_t k = 0;
+        for (uint32_t u = threadIdx.x; u < nown; u += blockDim.x) {
+            while (u >= HL->own_cum[k + 1]) ++k;
+            BrickFetch<Q, NQ, W, PF> f = base_fetch;
+            f.base = k % BY + hy * (k / BY);
+            body(HL->own_start[k] + (u - HL->own_cum[k]), f);
+        }
+    }
+}
 
 }  // namespace sphx

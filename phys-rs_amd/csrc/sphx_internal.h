@@ -114,6 +114,29 @@ struct NbrList {
     int ks;                    // list capacity per particle
 };
 
+// Bricks of b[0] x b[1] x b[2] cells for the staged passes.  A workgroup
+// takes one brick: it copies the gather records of the (b[1]+2) x (b[2]+2)
+// cell rows around the brick, cells b.x - 1 .. b.x + b[0], into LDS (each row
+// one contiguous range of the sorted order) and walks its particles' lists
+// against LDS.  Every list entry of a brick particle lies in those rows; the
+// entries that are not staged (a periodic row that wraps in x, LDS full) are
+// read from global memory, so the result never depends on the staging.
+static constexpr int kMaxBrickYZ = 4;
+static constexpr int kMaxHaloRows = (kMaxBrickYZ + 2) * (kMaxBrickYZ + 2);
+static constexpr int kMaxOwnRows = kMaxBrickYZ * kMaxBrickYZ;
+struct Bricks {
+    int b[3];                  // cells per brick along x, y, z (b[1], b[2] <= kMaxBrickYZ)
+    int nb[3];                 // bricks along x, y, z
+    uint32_t count;            // nb[0] * nb[1] * nb[2]
+    uint32_t cap;              // capacity of the arrays below
+    uint32_t* flag;            // 1 if the brick holds particles [cap]
+    uint32_t* pos;             // exclusive scan of flag [cap + 1]; pos[count] = active bricks
+    uint32_t* sums;            // scan scratch
+    uint32_t* active;          // ids of the bricks that hold particles, in brick order [cap]
+    int cus;                   // compute units: the staged passes run cus * k workgroups
+    int on;                    // staged passes enabled
+};
+
 // f32 copies in sorted order.  Positions are offsets from the particle's own
 // cell origin (computed in f64, rounded once), so a pair vector is
 // (xr_j - xr_i) + (cell offset) * w, never a difference of large numbers.
@@ -159,14 +182,16 @@ void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, u
                     bool mhd, bool charge, bool mixed, const Mixed& X, const Grid& g, double4* recb,
                     double4* xref, hipStream_t s);
 void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
+void launch_bricks(const Grid& g, const Sort& S, const Bricks& B, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
                       const NbrList& L, bool uniform, const Mixed& X, double h_max, double skin, hipStream_t s);
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                    const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
+                    const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
+                    const uint32_t* cell_start, hipStream_t s);
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                    const KernelParams& kp, bool visc_only, bool mhd_stress, bool charge, bool mixed,
-                   const Mixed& X, hipStream_t s);
+                   const Mixed& X, const Bricks& B, const uint32_t* cell_start, hipStream_t s);
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
                         const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint8_t* flip,
                         hipStream_t s);

@@ -7,13 +7,16 @@
 // and each thread tests its 3-cell window, appending accepted neighbours to
 // one list per particle (all stages together).
 //
-// Passes: one thread per sorted particle walks its whole list and gathers
-// the neighbours' fields from the sorted SoA arrays (L2-resident: the
-// neighbours of a wave are a few cell rows).  One loop over all stages keeps
-// lanes busy (a per-stage loop idles every lane whose particle has no
-// neighbour in that row).  One thread owns one particle and gathers: no
-// atomics on the floating-point path, every sum is deterministic.  No MFMA:
-// the path is not a dense contraction.
+// Passes: one thread per sorted particle walks its whole list (one loop over
+// all stages keeps lanes busy; a per-stage loop would idle every lane whose
+// particle has no neighbour in that row).  The density, force and mixed force
+// passes run brick-staged: a workgroup per brick of cells copies the
+// neighbours' gather records around the brick into LDS with contiguous loads
+// and reads them from there (gathering them from L2 was bound by L1 misses,
+// ~0.7 L2 requests per pair).  The other passes gather from global memory.
+// One thread owns one particle and gathers: no atomics on the floating-point
+// path, every sum is deterministic, staged or not.  No MFMA: the path is not a
+// dense contraction.
 //
 // Compiled with -ffp-contract=off: the neighbour predicate rounds exactly
 // like the reference's sqrt((dx*dx + dy*dy) + dz*dz) < 2*h_avg
@@ -221,25 +224,24 @@ __device__ __forceinline__ PairH pair_h(const KernelParams& kp, double hi, doubl
     const uint32_t i = particle_index();    \
     if (i >= n) return
 
-template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                     KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
+template <bool UNI, class Fetch>
+__device__ __forceinline__ void density_particle(const Particles& P, const Derived& D, const Grid& g,
+                                                 const NbrList& L, const KernelParams& kp, uint32_t i,
+                                                 const Fetch& fetch) {
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     double rho = P.m[i] * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
     double acc = 0.0;
-    const double2* __restrict__ pos = reinterpret_cast<const double2*>(P.pos);
-    for_neighbours(L, i, [&](uint32_t e) {
-        const uint32_t j = e & kIdxMask;
-        const double2 xy = pos[2 * j], zh = pos[2 * j + 1];
+    walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[2]) {
+        const double2 xy = q[0], zh = q[1];
         double dx, dy, dz;
         pair_vec(g, xi, yi, zi, xy.x, xy.y, zh.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        double r = 0.0, inv_r;
-        if (r2 > 0.0) sqrt_rsqrt(r2, r, inv_r);
+        double r, inv_r;
+        sqrt_rsqrt(r2 > 0.0 ? r2 : 1.0, r, inv_r);
+        r = r2 > 0.0 ? r : 0.0;
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : zh.y);
         const double w = ph.sig_w * w_shape(r * ph.inv_h);
-        acc = fma(UNI ? 1.0 : P.m[j], w, acc);
+        acc = fma(UNI ? 1.0 : P.m[e & kIdxMask], w, acc);
     });
     rho = UNI ? fma(kp.m0, acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
@@ -252,6 +254,27 @@ __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint
     rec[1] = make_double2(zi, rho);
     rec[2] = make_double2(P.vx[i], P.vy[i]);
     rec[3] = make_double2(P.vz[i], pr2);
+}
+
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                     KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    density_particle<UNI>(P, D, g, L, kp, i, GlobalFetch<double2, 2>{reinterpret_cast<const double2*>(P.pos)});
+}
+
+// Brick-staged forms: the neighbours' records come from LDS (bit-identical
+// results: same records, same order, same arithmetic).
+template <bool UNI, int W, bool PF>
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4))) k_density_staged(Particles P, Derived D, Grid g, NbrList L,
+                                                                 KernelParams kp, Bricks B,
+                                                                 const uint32_t* __restrict__ start, uint32_t cap) {
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    brick_pass<double2, 2, W, PF>(B, g, start, cap, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
+                               [&](uint32_t i, const BrickFetch<double2, 2, W, PF>& f) {
+                               density_particle<UNI>(P, D, g, L, kp, i, f);
+                           });
 }
 
 // FP32-mixed density: f32 pair terms on cell-relative coordinates, f64 sum.
@@ -301,11 +324,41 @@ __global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D
     }
 }
 
+// Staged launch with `per_cu` workgroups per CU and records of rec_bytes:
+// grid size (a multiple of 8, see brick_iter), dynamic LDS and record capacity.
+struct StagedLaunch {
+    dim3 grid, block;
+    size_t dyn;
+    uint32_t cap;
+};
+static StagedLaunch staged_launch(const Bricks& B, int per_cu, size_t rec_bytes) {
+    StagedLaunch l;
+    const size_t per_block = kLdsPerCU / (size_t)per_cu - sizeof(BrickHdr) - 512;
+    l.cap = (uint32_t)(per_block / rec_bytes);
+    l.dyn = (size_t)l.cap * rec_bytes;
+    l.grid = dim3((uint32_t)((B.cus * per_cu + 7) / 8 * 8));
+    l.block = dim3(kStagedBlock);
+    return l;
+}
+
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                    const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
+                    const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
+                    const uint32_t* cell_start, hipStream_t s) {
     if (!n) return;
     const dim3 grid(nblocks(n)), block(kBlock);
     const bool uni = kp.uniform != 0;
+    if (!mixed && B.on) {
+        const StagedLaunch sl = staged_launch(B, 2, 32);
+// walk widths measured on the 1M dam break: four entries with the next four
+        // loaded ahead for the uniform density (0.27 -> 0.24 ms)
+        if (uni)
+            hipLaunchKernelGGL((k_density_staged<true, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B,
+                               cell_start, sl.cap);
+        else
+            hipLaunchKernelGGL((k_density_staged<false, 2, false>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B,
+                               cell_start, sl.cap);
+        return;
+    }
     if (mixed && uni)
         hipLaunchKernelGGL(k_density_mixed<true>, grid, block, 0, s, P, D, X, n, g, L, kp);
     else if (mixed)
@@ -357,10 +410,10 @@ __device__ __forceinline__ void force_epilogue(const Particles& P, const Derived
     D.az[p] = F[2] / mi;
 }
 
-template <bool VISC_ONLY, bool STRESS, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                    KernelParams kp, int charge) {
-    SPHX_PASS_PROLOGUE();
+template <bool VISC_ONLY, bool STRESS, bool UNI, class Fetch>
+__device__ __forceinline__ void force_particle(const Particles& P, const Derived& D, const Grid& g,
+                                               const NbrList& L, const KernelParams& kp, int charge, uint32_t i,
+                                               const Fetch& fetch) {
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
     const double mi = P.m[i], rhoi = D.rho[i], pr2i = D.pr2[i];
@@ -373,19 +426,20 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
     const double ac = -kp.alpha * kp.speed_of_sound;
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
-    const double2* __restrict__ R = reinterpret_cast<const double2*>(D.r64);
     const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
-    for_neighbours(L, i, [&](uint32_t e) {
+    // The pair body is branch-free: a pair without a term adds +-0, which
+    // leaves a sum unchanged (the sums start at +0 and never become -0).
+    walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[4]) {
         const uint32_t j = e & kIdxMask;
-        const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
+        const double2 xy = q[0], zr = q[1], vxy = q[2], vzp = q[3];
         double dx, dy, dz;
         pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        if (!(r2 > 1e-20)) return;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
+        const bool ok = r2 > 1e-20;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
         double r, inv_r;
-        sqrt_rsqrt(r2, r, inv_r);
+        sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
-        const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;  // (dW/dr) / r
+        const double dwr = ok ? (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r : 0.0;  // (dW/dr) / r
         const double mm = UNI ? kp.m0 * kp.m0 : mi * P.m[j];
         if (!VISC_ONLY) {  // sph.rs:208-221
             const double sc = (mm * (pr2i + vzp.y)) * dwr;
@@ -396,12 +450,13 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
         // sph.rs:292-314: (v_i - v_j).(x_j - x_i); REFERENCE convention active when < 0
         const double sdot = fma(vzi - vzp.x, dz, fma(vyi - vxy.y, dy, (vxi - vxy.x) * dx));
         const double rhoj = zr.y;
-        if (ref_conv ? (sdot < 0.0) : (sdot > 0.0)) {
+        {
+            const bool act = ref_conv ? (sdot < 0.0) : (sdot > 0.0);
             const double rho_bar = 0.5 * (rhoi + rhoj);
             const double ss = ref_conv ? sdot : -sdot;
             const double mu = (ph.h_avg * ss) * rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
             const double pi_ij = fma(kp.beta * mu, mu, ac * mu) * rcp_nr(rho_bar);
-            const double sc = (mm * pi_ij) * dwr;
+            const double sc = act ? (mm * pi_ij) * dwr : 0.0;
             fv[0] = fma(dx, sc, fv[0]);
             fv[1] = fma(dy, sc, fv[1]);
             fv[2] = fma(dz, sc, fv[2]);
@@ -422,11 +477,31 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
-// FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue.
 template <bool VISC_ONLY, bool STRESS, bool UNI>
-__global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
-                                                          NbrList L, KernelParams kp, int charge) {
+__global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                    KernelParams kp, int charge) {
     SPHX_PASS_PROLOGUE();
+    force_particle<VISC_ONLY, STRESS, UNI>(P, D, g, L, kp, charge, i,
+                                           GlobalFetch<double2, 4>{reinterpret_cast<const double2*>(D.r64)});
+}
+
+template <bool VISC_ONLY, bool UNI>
+__global__ void __launch_bounds__(kStagedBlock) k_forces_staged(Particles P, Derived D, Grid g, NbrList L,
+                                                                KernelParams kp, int charge, Bricks B,
+                                                                const uint32_t* __restrict__ start, uint32_t cap) {
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    brick_pass<double2, 4, 4, true>(B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
+                               [&](uint32_t i, const BrickFetch<double2, 4, 4, true>& f) {
+                               force_particle<VISC_ONLY, false, UNI>(P, D, g, L, kp, charge, i, f);
+                           });
+}
+
+// FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue.
+template <bool VISC_ONLY, bool STRESS, bool UNI, class Fetch>
+__device__ __forceinline__ void force_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
+                                                     const Grid& g, const NbrList& L, const KernelParams& kp,
+                                                     int charge, uint32_t i, const Fetch& fetch) {
     const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
     const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     const float4 own0 = R[2 * i], own1 = R[2 * i + 1];
@@ -447,17 +522,18 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
-    for_neighbours(L, i, [&](uint32_t e) {
+    // branch-free pair body, as in the f64 pass
+    walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[2]) {
         const uint32_t j = e & kIdxMask;
-        const float4 a = R[2 * j], b = R[2 * j + 1];
+        const float4 a = q[0], b = q[1];
         float ox, oy, oz;
         entry_offset(e, wx, wy, wz, ox, oy, oz);
         const float dx = (a.x - xi) + ox;
         const float dy = (a.y - yi) + oy;
         const float dz = (a.z - zi) + oz;
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        if (!(r2 > 1e-20f)) return;
-        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const bool ok = r2 > 1e-20f;
+        const float inv_r = __builtin_amdgcn_rsqf(ok ? r2 : 1.0f);
         const float r = r2 * inv_r;
         float inv_h = inv_h0, sig = sig_g0, h_avg = h0;
         if (!UNI) {
@@ -466,7 +542,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
             const float ih2 = inv_h * inv_h;
             sig = (ih2 * ih2) * (float)kInvPi;
         }
-        const float dwr = (sig * dw_shape_f(r * inv_h)) * inv_r;
+        const float dwr = ok ? (sig * dw_shape_f(r * inv_h)) * inv_r : 0.0f;
         const float mm = UNI ? mm0 : mi * X.m[j];
         if (!VISC_ONLY) {
             const float sc = (mm * (pr2i + b.w)) * dwr;
@@ -476,12 +552,13 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
         }
         const float sdot = fmaf(vzi - b.z, dz, fmaf(vyi - b.y, dy, (vxi - b.x) * dx));
         const float rhoj = a.w;
-        if (ref_conv ? (sdot < 0.0f) : (sdot > 0.0f)) {
+        {
+            const bool act = ref_conv ? (sdot < 0.0f) : (sdot > 0.0f);
             const float rho_bar = 0.5f * (rhoi + rhoj);
             const float ss = ref_conv ? sdot : -sdot;
             const float mu = (h_avg * ss) * __builtin_amdgcn_rcpf(fmaf(0.01f * h_avg, h_avg, r2));
             const float pi_ij = fmaf(beta * mu, mu, ac * mu) * __builtin_amdgcn_rcpf(rho_bar);
-            const float sc = (mm * pi_ij) * dwr;
+            const float sc = act ? (mm * pi_ij) * dwr : 0.0f;
             fv[0] += (double)(dx * sc);
             fv[1] += (double)(dy * sc);
             fv[2] += (double)(dz * sc);
@@ -502,13 +579,54 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
+template <bool VISC_ONLY, bool STRESS, bool UNI>
+__global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
+                                                          NbrList L, KernelParams kp, int charge) {
+    SPHX_PASS_PROLOGUE();
+    force_mixed_particle<VISC_ONLY, STRESS, UNI>(P, D, X, g, L, kp, charge, i,
+                                                 GlobalFetch<float4, 2>{reinterpret_cast<const float4*>(D.r32)});
+}
+
+template <bool VISC_ONLY, bool UNI, int W, bool PF>
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4))) k_forces_mixed_staged(Particles P, Derived D, Mixed X, Grid g,
+                                                                      NbrList L, KernelParams kp, int charge,
+                                                                      Bricks B, const uint32_t* __restrict__ start,
+                                                                      uint32_t cap) {
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    brick_pass<float4, 2, W, PF>(B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+                              [&](uint32_t i, const BrickFetch<float4, 2, W, PF>& f) {
+                              force_mixed_particle<VISC_ONLY, false, UNI>(P, D, X, g, L, kp, charge, i, f);
+                          });
+}
+
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                    const KernelParams& kp, bool visc_only, bool stress, bool charge, bool mixed, const Mixed& X,
-                   hipStream_t s) {
+                   const Bricks& B, const uint32_t* cell_start, hipStream_t s) {
     if (!n) return;
     const dim3 grid(nblocks(n)), block(kBlock);
     const int ch = charge ? 1 : 0;
     const bool uni = kp.uniform != 0;
+    if (B.on && !stress) {
+#define SPHX_FORCES_STAGED(V, U)                                                                            \
+        if (visc_only == V && uni == U) {                                                                   \
+            if (mixed) {                                                                                    \
+                const StagedLaunch sl = staged_launch(B, 2, 32);                                            \
+                hipLaunchKernelGGL((k_forces_mixed_staged<V, U, 4, false>), sl.grid, sl.block, sl.dyn, s, P, D, X, \
+                                   g, L, kp, ch, B, cell_start, sl.cap);                                    \
+            } else {                                                                                        \
+                const StagedLaunch sl = staged_launch(B, 1, 64);                                            \
+                hipLaunchKernelGGL((k_forces_staged<V, U>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, ch, \
+                                   B, cell_start, sl.cap);                                                  \
+            }                                                                                               \
+            return;                                                                                         \
+        }
+        SPHX_FORCES_STAGED(false, false)
+        SPHX_FORCES_STAGED(true, false)
+        SPHX_FORCES_STAGED(false, true)
+        SPHX_FORCES_STAGED(true, true)
+#undef SPHX_FORCES_STAGED
+    }
 #define SPHX_FORCES(V, M, U)                                                                              \
     if (visc_only == V && stress == M && uni == U) {                                                     \
         if (mixed)                                                                                        \

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -72,6 +73,7 @@ struct SphxCtx {
     Derived D{};
     Sort S{};
     Tiles T{};
+    Bricks B{};
     NbrList L{};
     Mixed X{};
     double* energy_partial = nullptr;
@@ -198,6 +200,24 @@ void make_kernel_params(SphxCtx* c) {
     k.inv_h0 = c->uniform ? 1.0 / c->h0 : 0.0;
 }
 
+// Brick shape of the staged passes: 3 x 3 x 3 cells (the 5 x 5 x 5-cell halo of
+// a dense fluid at ~19 particles per cell fills the 160 KiB LDS of a CU with
+// 64-byte records).  SPHX_BRICKS=off runs the unstaged passes; SPHX_BRICKS=bx,by,bz
+// sets another shape (by, bz <= 4).
+void brick_defaults(SphxCtx* c) {
+    Bricks& B = c->B;
+    B.b[0] = B.b[1] = B.b[2] = 3;
+    B.on = 1;
+    const char* e = getenv("SPHX_BRICKS");
+    if (!e) return;
+    int bx, by, bz;
+    if (!strcmp(e, "off")) B.on = 0;
+    else if (sscanf(e, "%d,%d,%d", &bx, &by, &bz) == 3 && bx >= 1 && by >= 1 && bz >= 1 && by <= kMaxBrickYZ &&
+             bz <= kMaxBrickYZ) {
+        B.b[0] = bx; B.b[1] = by; B.b[2] = bz;
+    }
+}
+
 // Grid from the parameters and (for open boxes) the uploaded positions.
 int setup_grid(SphxCtx* c, const SphxState* host) {
     const SphxParams& p = c->prm;
@@ -300,6 +320,22 @@ int setup_grid(SphxCtx* c, const SphxState* host) {
         c->T.nrows_cap = nrows;
     }
     c->T.nrows = nrows;
+    {
+        Bricks& B = c->B;
+        uint64_t count = 1;
+        for (int k = 0; k < 3; ++k) {
+            B.nb[k] = (g.dims[k] + B.b[k] - 1) / B.b[k];
+            count *= (uint64_t)B.nb[k];
+        }
+        B.count = (uint32_t)count;
+        if (B.on && B.count > B.cap) {
+            int rc;
+            if ((rc = dalloc(c, &B.flag, B.count)) || (rc = dalloc(c, &B.pos, (size_t)B.count + 1)) ||
+                (rc = dalloc(c, &B.sums, ((size_t)B.count + 1023) / 1024 + 1)) || (rc = dalloc(c, &B.active, B.count)))
+                return rc;
+            B.cap = B.count;
+        }
+    }
     c->grid = g;
     c->grid_set = true;
     return SPHX_OK;
@@ -371,6 +407,7 @@ int do_neighbours(SphxCtx* c) {
                        c->D.xref, c->stream);
         c->cur = 1 - c->cur;
         launch_tiles(c->grid, c->S, c->T, c->stream);
+        launch_bricks(c->grid, c->S, c->B, c->stream);
     }
     {
         Timer t(c, kNbr);
@@ -393,7 +430,8 @@ int do_density(SphxCtx* c) {
     if ((rc = ensure_list(c))) return rc;
     {
         Timer t(c, kDensity);
-        launch_density(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
+        launch_density(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->mixed(), c->X, c->B, c->S.cell_start,
+                       c->stream);
     }
     c->dens = true;
     return launch_check(c);
@@ -409,7 +447,7 @@ int do_forces(SphxCtx* c, bool visc_only) {
     {
         Timer t(c, visc_only ? kViscOnly : kForce);
         launch_forces(c->P(), c->D, c->n, c->grid, c->L, c->kp, visc_only, stress, charge, c->mixed(),
-                      c->X, c->stream);
+                      c->X, c->B, c->S.cell_start, c->stream);
     }
     c->sp = true;
     c->acc = true;
@@ -673,6 +711,9 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     };
     if (hipSetDevice(device) != hipSuccess) return bail(SPHX_E_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(SPHX_E_HIP);
+    if (hipDeviceGetAttribute(&c->B.cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return bail(SPHX_E_HIP);
+    brick_defaults(c);
     const size_t N = c->cap;
     for (int b = 0; b < 2; ++b) {
         Particles& P = c->A[b];

@@ -268,6 +268,35 @@ void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s) {
                        nx, T.row_off, T.p0, T.np, T.row);
 }
 
+// ---- bricks: the ids of the bricks that hold particles, in brick order ----
+__global__ void k_brick_flag(const uint32_t* __restrict__ start, Grid g, Bricks B) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B.count) return;
+    const uint32_t nbx = (uint32_t)B.nb[0], nby = (uint32_t)B.nb[1];
+    const int x0 = (int)(b % nbx) * B.b[0], y0 = (int)((b / nbx) % nby) * B.b[1], z0 = (int)(b / (nbx * nby)) * B.b[2];
+    const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+    const int x1 = min(x0 + B.b[0], nx);
+    uint32_t any = 0;
+    for (int z = z0; z < min(z0 + B.b[2], nz); ++z)
+        for (int y = y0; y < min(y0 + B.b[1], ny); ++y) {
+            const uint32_t row = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+            any |= start[row + x1] - start[row + x0];
+        }
+    B.flag[b] = any ? 1u : 0u;
+}
+
+__global__ void k_brick_compact(Bricks B) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B.count && B.flag[b]) B.active[B.pos[b]] = b;
+}
+
+void launch_bricks(const Grid& g, const Sort& S, const Bricks& B, hipStream_t s) {
+    if (!B.on) return;
+    hipLaunchKernelGGL(k_brick_flag, dim3(nblocks(B.count)), dim3(kBlock), 0, s, S.cell_start, g, B);
+    launch_scan(B.flag, B.pos, B.count, B.sums, s);
+    hipLaunchKernelGGL(k_brick_compact, dim3(nblocks(B.count)), dim3(kBlock), 0, s, B);
+}
+
 // Upper bound of the tile count: sum over rows of ceil(n_r / kTile).
 uint32_t tile_grid(uint32_t n, const Tiles& T) {
     uint64_t bound = (uint64_t)n / kTile + std::min<uint64_t>(T.nrows, n);
