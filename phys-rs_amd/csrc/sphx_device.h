@@ -42,12 +42,15 @@ __device__ __forceinline__ void min_image(double& d, double L, double halfL) {
 
 // x_j - x_i as core/coordinates.rs:51-65 forms it in one sector (0 + (b - a)),
 // with the minimum image in a periodic box.
+// PER: 1 periodic, 0 closed box, -1 read from the grid (a compile-time PER
+// keeps a pair body free of the branch).
+template <int PER = -1>
 __device__ __forceinline__ void pair_vec(const Grid& g, double xi, double yi, double zi, double xj,
                                          double yj, double zj, double& dx, double& dy, double& dz) {
     dx = xj - xi;
     dy = yj - yi;
     dz = zj - zi;
-    if (g.periodic) {
+    if (PER == 1 || (PER < 0 && g.periodic)) {
         min_image(dx, g.L[0], g.halfL[0]);
         min_image(dy, g.L[1], g.halfL[1]);
         min_image(dz, g.L[2], g.halfL[2]);

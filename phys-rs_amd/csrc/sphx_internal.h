@@ -137,14 +137,14 @@ struct Bricks {
     int on;                    // staged passes enabled
 };
 
-// f32 copies in sorted order.  Positions are offsets from the particle's own
-// cell origin (computed in f64, rounded once), so a pair vector is
-// (xr_j - xr_i) + (cell offset) * w, never a difference of large numbers.
-// The neighbour build prefilters with them in both forms; the FP32-mixed
-// passes compute with them.
+// f32 copies in sorted order: p4 = (xr, yr, zr, h), the position as an offset
+// from the particle's own cell origin (computed in f64, rounded once), so a
+// pair vector is (xr_j - xr_i) + (cell offset) * w, never a difference of
+// large numbers.  The neighbour build prefilters with them in both forms; the
+// FP32-mixed passes compute with them (one 16-byte record per neighbour).
 struct Mixed {
-    float *xr, *yr, *zr;
-    float *m, *h;
+    float4* p4;
+    float* m;
 };
 
 struct Vec3 {

@@ -74,7 +74,8 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
     double xi = 0, yi = 0, zi = 0, hi = 0;
     bool exact_all = false;
     if (valid) {
-        xf = X.xr[p]; yf = X.yr[p]; zf = X.zr[p]; hf = X.h[p];
+        const float4 o = X.p4[p];
+        xf = o.x; yf = o.y; zf = o.z; hf = o.w;
         const float lim = 2.0f * (float)fmax(g.w[0], fmax(g.w[1], g.w[2]));
         exact_all = fabsf(xf) > lim || fabsf(yf) > lim || fabsf(zf) > lim;
         xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
@@ -149,8 +150,9 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             const uint32_t ce = min(st.total, cb + kStageCap);
             __syncthreads();
             SPHX_STAGE_CHUNK({
-                lds[0][u] = X.xr[j]; lds[1][u] = X.yr[j]; lds[2][u] = X.zr[j];
-                if (!UNI) lds[3][u] = X.h[j];
+                const float4 v = X.p4[j];
+                lds[0][u] = v.x; lds[1][u] = v.y; lds[2][u] = v.z;
+                if (!UNI) lds[3][u] = v.w;
             });
             __syncthreads();
             if (!valid) continue;
@@ -224,7 +226,7 @@ __device__ __forceinline__ PairH pair_h(const KernelParams& kp, double hi, doubl
     const uint32_t i = particle_index();    \
     if (i >= n) return
 
-template <bool UNI, class Fetch>
+template <bool UNI, int PER, class Fetch>
 __device__ __forceinline__ void density_particle(const Particles& P, const Derived& D, const Grid& g,
                                                  const NbrList& L, const KernelParams& kp, uint32_t i,
                                                  const Fetch& fetch) {
@@ -234,16 +236,19 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
     walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[2]) {
         const double2 xy = q[0], zh = q[1];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, xy.x, xy.y, zh.x, dx, dy, dz);
+        pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zh.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         double r, inv_r;
         sqrt_rsqrt(r2 > 0.0 ? r2 : 1.0, r, inv_r);
         r = r2 > 0.0 ? r : 0.0;
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : zh.y);
-        const double w = ph.sig_w * w_shape(r * ph.inv_h);
-        acc = fma(UNI ? 1.0 : P.m[e & kIdxMask], w, acc);
+        if (UNI) {  // m and sigma_w are folded out of the sum
+            acc += w_shape(r * ph.inv_h);
+        } else {
+            acc = fma(P.m[e & kIdxMask], ph.sig_w * w_shape(r * ph.inv_h), acc);
+        }
     });
-    rho = UNI ? fma(kp.m0, acc, rho) : rho + acc;
+    rho = UNI ? fma(kp.m0 * (((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi), acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
@@ -260,12 +265,12 @@ template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                      KernelParams kp) {
     SPHX_PASS_PROLOGUE();
-    density_particle<UNI>(P, D, g, L, kp, i, GlobalFetch<double2, 2>{reinterpret_cast<const double2*>(P.pos)});
+    density_particle<UNI, -1>(P, D, g, L, kp, i, GlobalFetch<double2, 2>{reinterpret_cast<const double2*>(P.pos)});
 }
 
 // Brick-staged forms: the neighbours' records come from LDS (bit-identical
 // results: same records, same order, same arithmetic).
-template <bool UNI, int W, bool PF>
+template <bool UNI, int PER, int W, bool PF>
 __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4))) k_density_staged(Particles P, Derived D, Grid g, NbrList L,
                                                                  KernelParams kp, Bricks B,
                                                                  const uint32_t* __restrict__ start, uint32_t cap) {
@@ -273,37 +278,37 @@ __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_
     extern __shared__ uint4 sh_dyn[];
     brick_pass<double2, 2, W, PF>(B, g, start, cap, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
                                [&](uint32_t i, const BrickFetch<double2, 2, W, PF>& f) {
-                               density_particle<UNI>(P, D, g, L, kp, i, f);
+                               density_particle<UNI, PER>(P, D, g, L, kp, i, f);
                            });
 }
 
 // FP32-mixed density: f32 pair terms on cell-relative coordinates, f64 sum.
-template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
-                                                           NbrList L, KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
-    const float xi = X.xr[i], yi = X.yr[i], zi = X.zr[i], hi = X.h[i];
+template <bool UNI, class Fetch>
+__device__ __forceinline__ void density_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
+                                                       const Grid& g, const NbrList& L, const KernelParams& kp,
+                                                       uint32_t i, const Fetch& fetch) {
+    const float4 own = X.p4[i];
+    const float xi = own.x, yi = own.y, zi = own.z, hi = own.w;
     const double hd = P.h[i];
     double rho = P.m[i] * (1.0 / (kPi * ((hd * hd) * hd)));
     const float inv_h0 = (float)kp.inv_h0;
     const float sig_w0 = (float)(kp.inv_h0 * kp.inv_h0 * kp.inv_h0 * kInvPi);
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double acc = 0.0;
-    for_neighbours(L, i, [&](uint32_t e) {
-        const uint32_t j = e & kIdxMask;
+    walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[1]) {
         float ox, oy, oz;
         entry_offset(e, wx, wy, wz, ox, oy, oz);
-        const float dx = (X.xr[j] - xi) + ox;
-        const float dy = (X.yr[j] - yi) + oy;
-        const float dz = (X.zr[j] - zi) + oz;
+        const float dx = (q[0].x - xi) + ox;
+        const float dy = (q[0].y - yi) + oy;
+        const float dz = (q[0].z - zi) + oz;
         const float r = __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
         float inv_h = inv_h0, sig = sig_w0;
         if (!UNI) {
-            inv_h = __builtin_amdgcn_rcpf(0.5f * (hi + X.h[j]));
+            inv_h = __builtin_amdgcn_rcpf(0.5f * (hi + q[0].w));
             sig = (inv_h * inv_h) * inv_h * (float)kInvPi;
         }
         const float w = sig * w_shape_f(r * inv_h);
-        acc += (double)(UNI ? w : X.m[j] * w);
+        acc += (double)(UNI ? w : X.m[e & kIdxMask] * w);
     });
     rho = UNI ? fma(kp.m0, acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
@@ -322,6 +327,25 @@ __global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D
         r[2] = make_double2(vxi, vyi);
         r[3] = make_double2(vzi, pr2);
     }
+}
+
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
+                                                           NbrList L, KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    density_mixed_particle<UNI>(P, D, X, g, L, kp, i, GlobalFetch<float4, 1>{X.p4});
+}
+
+template <bool UNI, int W, bool PF>
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_density_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, Bricks B,
+                       const uint32_t* __restrict__ start, uint32_t cap) {
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    brick_pass<float4, 1, W, PF>(B, g, start, cap, X.p4, H, sh_dyn,
+                                 [&](uint32_t i, const BrickFetch<float4, 1, W, PF>& f) {
+                                     density_mixed_particle<UNI>(P, D, X, g, L, kp, i, f);
+                                 });
 }
 
 // Staged launch with `per_cu` workgroups per CU and records of rec_bytes:
@@ -351,12 +375,24 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
         const StagedLaunch sl = staged_launch(B, 2, 32);
 // walk widths measured on the 1M dam break: four entries with the next four
         // loaded ahead for the uniform density (0.27 -> 0.24 ms)
+#define SPHX_DENS_STAGED(U, PER, W, PF)                                                                  \
+        hipLaunchKernelGGL((k_density_staged<U, PER, W, PF>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B, \
+                           cell_start, sl.cap)
+        if (uni && g.periodic) SPHX_DENS_STAGED(true, 1, 4, true);
+        else if (uni) SPHX_DENS_STAGED(true, 0, 4, true);
+        else if (g.periodic) SPHX_DENS_STAGED(false, 1, 2, false);
+        else SPHX_DENS_STAGED(false, 0, 2, false);
+#undef SPHX_DENS_STAGED
+        return;
+    }
+    if (mixed && B.on) {
+        const StagedLaunch sl = staged_launch(B, 2, 16);
         if (uni)
-            hipLaunchKernelGGL((k_density_staged<true, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B,
-                               cell_start, sl.cap);
+            hipLaunchKernelGGL((k_density_mixed_staged<true, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, X, g, L,
+                               kp, B, cell_start, sl.cap);
         else
-            hipLaunchKernelGGL((k_density_staged<false, 2, false>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B,
-                               cell_start, sl.cap);
+            hipLaunchKernelGGL((k_density_mixed_staged<false, 2, false>), sl.grid, sl.block, sl.dyn, s, P, D, X, g,
+                               L, kp, B, cell_start, sl.cap);
         return;
     }
     if (mixed && uni)
@@ -410,7 +446,7 @@ __device__ __forceinline__ void force_epilogue(const Particles& P, const Derived
     D.az[p] = F[2] / mi;
 }
 
-template <bool VISC_ONLY, bool STRESS, bool UNI, class Fetch>
+template <bool VISC_ONLY, bool STRESS, bool UNI, int PER, class Fetch>
 __device__ __forceinline__ void force_particle(const Particles& P, const Derived& D, const Grid& g,
                                                const NbrList& L, const KernelParams& kp, int charge, uint32_t i,
                                                const Fetch& fetch) {
@@ -423,26 +459,35 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         irho2i = 1.0 / (rhoi * rhoi);
         hb2i = 0.5 * ((bxi * bxi + byi * byi) + bzi * bzi);
     }
-    const double ac = -kp.alpha * kp.speed_of_sound;
+    // Pi = mu (beta mu - alpha c) / rho_bar with mu = h s / den, den = r^2 + 0.01 h^2
+    // (sph.rs:281-300), formed with one reciprocal:
+    // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s.
+    const double ac2 = -2.0 * kp.alpha * kp.speed_of_sound, beta2 = 2.0 * kp.beta;
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
+    // s = (v_i - v_j).(x_j - x_i) in the REFERENCE convention, its negative in MONAGHAN's
+    const double ssign = ref_conv ? 1.0 : -1.0;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     // The pair body is branch-free: a pair without a term adds +-0, which
     // leaves a sum unchanged (the sums start at +0 and never become -0).
+    // With uniform m and h, m_i m_j sigma_g is folded out of the sums.
     walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[4]) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = q[0], zr = q[1], vxy = q[2], vzp = q[3];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
+        pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const bool ok = r2 > 1e-20;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
         double r, inv_r;
         sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
-        const double dwr = ok ? (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r : 0.0;  // (dW/dr) / r
-        const double mm = UNI ? kp.m0 * kp.m0 : mi * P.m[j];
+        // (dW/dr) / r (without sigma_g m_i m_j when uniform); a factor 1 or 0
+        // rather than a select, which the compiler would turn back into a
+        // branch around the arithmetic
+        const double dwr0 = (dw_shape(r * ph.inv_h) * inv_r) * (ok ? 1.0 : 0.0);
+        const double dwr = UNI ? dwr0 : (mi * P.m[j]) * (ph.sig_g * dwr0);
         if (!VISC_ONLY) {  // sph.rs:208-221
-            const double sc = (mm * (pr2i + vzp.y)) * dwr;
+            const double sc = (pr2i + vzp.y) * dwr;
             fp[0] = fma(dx, sc, fp[0]);
             fp[1] = fma(dy, sc, fp[1]);
             fp[2] = fma(dz, sc, fp[2]);
@@ -452,28 +497,37 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         const double rhoj = zr.y;
         {
             const bool act = ref_conv ? (sdot < 0.0) : (sdot > 0.0);
-            const double rho_bar = 0.5 * (rhoi + rhoj);
-            const double ss = ref_conv ? sdot : -sdot;
-            const double mu = (ph.h_avg * ss) * rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
-            const double pi_ij = fma(kp.beta * mu, mu, ac * mu) * rcp_nr(rho_bar);
-            const double sc = act ? (mm * pi_ij) * dwr : 0.0;
+            const double hs = (ssign * ph.h_avg) * sdot;
+            const double den = fma(0.01 * ph.h_avg, ph.h_avg, r2);
+            const double num = hs * fma(beta2, hs, ac2 * den);
+            const double pi_ij = num * rcp_nr((den * den) * (rhoi + rhoj));
+            const double sc = (pi_ij * dwr) * (act ? 1.0 : 0.0);
             fv[0] = fma(dx, sc, fv[0]);
             fv[1] = fma(dy, sc, fv[1]);
             fv[2] = fma(dz, sc, fv[2]);
         }
         if (STRESS) {
-            const double gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;  // grad_i W
+            const double gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;  // grad_i W (times m_i m_j)
             const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
             const double bxj = bxy.x, byj = bxy.y, bzj = bzp.x;
             const double irho2j = rcp_nr(rhoj * rhoj);
             const double hb2j = 0.5 * ((bxj * bxj + byj * byj) + bzj * bzj);
             const double big = (bxi * gx + byi * gy) + bzi * gz;
             const double bjg = (bxj * gx + byj * gy) + bzj * gz;
-            fm[0] += mm * ((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j);
-            fm[1] += mm * ((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j);
-            fm[2] += mm * ((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j);
+            fm[0] += (bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j;
+            fm[1] += (byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j;
+            fm[2] += (bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j;
         }
     });
+    if (UNI) {
+        const double ih2 = kp.inv_h0 * kp.inv_h0;
+        const double k = (kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi);
+        for (int a = 0; a < 3; ++a) {
+            fp[a] *= k;
+            fv[a] *= k;
+            fm[a] *= k;
+        }
+    }
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
@@ -481,11 +535,11 @@ template <bool VISC_ONLY, bool STRESS, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                     KernelParams kp, int charge) {
     SPHX_PASS_PROLOGUE();
-    force_particle<VISC_ONLY, STRESS, UNI>(P, D, g, L, kp, charge, i,
+    force_particle<VISC_ONLY, STRESS, UNI, -1>(P, D, g, L, kp, charge, i,
                                            GlobalFetch<double2, 4>{reinterpret_cast<const double2*>(D.r64)});
 }
 
-template <bool VISC_ONLY, bool UNI>
+template <bool VISC_ONLY, bool UNI, int PER>
 __global__ void __launch_bounds__(kStagedBlock) k_forces_staged(Particles P, Derived D, Grid g, NbrList L,
                                                                 KernelParams kp, int charge, Bricks B,
                                                                 const uint32_t* __restrict__ start, uint32_t cap) {
@@ -493,7 +547,7 @@ __global__ void __launch_bounds__(kStagedBlock) k_forces_staged(Particles P, Der
     extern __shared__ uint4 sh_dyn[];
     brick_pass<double2, 4, 4, true>(B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
                                [&](uint32_t i, const BrickFetch<double2, 4, 4, true>& f) {
-                               force_particle<VISC_ONLY, false, UNI>(P, D, g, L, kp, charge, i, f);
+                               force_particle<VISC_ONLY, false, UNI, PER>(P, D, g, L, kp, charge, i, f);
                            });
 }
 
@@ -507,7 +561,7 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     const float4 own0 = R[2 * i], own1 = R[2 * i + 1];
     const float xi = own0.x, yi = own0.y, zi = own0.z, rhoi = own0.w;
     const float vxi = own1.x, vyi = own1.y, vzi = own1.z, pr2i = own1.w;
-    const float hi = X.h[i], mi = UNI ? 0.0f : X.m[i];
+    const float hi = X.p4[i].w, mi = UNI ? 0.0f : X.m[i];
     float bxi = 0, byi = 0, bzi = 0, irho2i = 0, hb2i = 0;
     if (STRESS) {
         bxi = (float)P.bx[i]; byi = (float)P.by[i]; bzi = (float)P.bz[i];
@@ -537,11 +591,12 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         const float r = r2 * inv_r;
         float inv_h = inv_h0, sig = sig_g0, h_avg = h0;
         if (!UNI) {
-            h_avg = 0.5f * (hi + X.h[j]);
+            h_avg = 0.5f * (hi + X.p4[j].w);
             inv_h = __builtin_amdgcn_rcpf(h_avg);
             const float ih2 = inv_h * inv_h;
             sig = (ih2 * ih2) * (float)kInvPi;
         }
+        // selects here (measured faster than the factor form of the f64 pass)
         const float dwr = ok ? (sig * dw_shape_f(r * inv_h)) * inv_r : 0.0f;
         const float mm = UNI ? mm0 : mi * X.m[j];
         if (!VISC_ONLY) {
@@ -616,8 +671,12 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
                                    g, L, kp, ch, B, cell_start, sl.cap);                                    \
             } else {                                                                                        \
                 const StagedLaunch sl = staged_launch(B, 1, 64);                                            \
-                hipLaunchKernelGGL((k_forces_staged<V, U>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, ch, \
-                                   B, cell_start, sl.cap);                                                  \
+                if (g.periodic)                                                                             \
+                    hipLaunchKernelGGL((k_forces_staged<V, U, 1>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, \
+                                       ch, B, cell_start, sl.cap);                                          \
+                else                                                                                        \
+                    hipLaunchKernelGGL((k_forces_staged<V, U, 0>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, \
+                                       ch, B, cell_start, sl.cap);                                          \
             }                                                                                               \
             return;                                                                                         \
         }

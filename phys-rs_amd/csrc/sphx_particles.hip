@@ -26,9 +26,7 @@ __device__ __forceinline__ bool track_position(const Particles& P, const Derived
                        x[2] - (g.lo[2] + cz * g.w[2])};
         if (g.periodic)
             for (int k = 0; k < 3; ++k) min_image(d[k], g.L[k], g.halfL[k]);
-        X.xr[i] = (float)d[0];
-        X.yr[i] = (float)d[1];
-        X.zr[i] = (float)d[2];
+        X.p4[i] = make_float4((float)d[0], (float)d[1], (float)d[2], (float)P.h[i]);
     }
     return (dx * dx + dy * dy) + dz * dz > kp.half_skin_sq;
 }

@@ -733,9 +733,7 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
         return bail(SPHX_E_OOM);
     {
         Mixed& X = c->X;
-        float** ff[] = {&X.xr, &X.yr, &X.zr, &X.m, &X.h};
-        for (float** f : ff)
-            if (dalloc(c, f, N)) return bail(SPHX_E_OOM);
+        if (dalloc(c, &X.p4, N) || dalloc(c, &X.m, N)) return bail(SPHX_E_OOM);
     }
     if (dalloc(c, &c->S.key, N) || dalloc(c, &c->S.rank, N) || dalloc(c, &c->S.perm, N) ||
         dalloc(c, &c->S.skey, N) || dalloc(c, &c->S.tmp, N))

@@ -221,10 +221,8 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
     // both forms, and the pair-side data of the mixed form
     int cx, cy, cz;
     decode_cell(c, g, cx, cy, cz);
-    X.xr[k] = (float)(x - (g.lo[0] + cx * g.w[0]));
-    X.yr[k] = (float)(y - (g.lo[1] + cy * g.w[1]));
-    X.zr[k] = (float)(z - (g.lo[2] + cz * g.w[2]));
-    X.h[k] = (float)h;
+    X.p4[k] = make_float4((float)(x - (g.lo[0] + cx * g.w[0])), (float)(y - (g.lo[1] + cy * g.w[1])),
+                          (float)(z - (g.lo[2] + cz * g.w[2])), (float)h);
     if (mixed) X.m[k] = (float)m;
 }
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n, bool mhd,
