@@ -433,20 +433,6 @@ struct BrickHdr {
     uint32_t own_cum[kMaxOwnRows + 1];  // exclusive prefix of the brick rows' particle counts
 };
 
-// The active bricks are split into 8 contiguous ranges, one per XCD
-// (workgroups are dealt to the XCDs round-robin; the grid is a multiple of 8),
-// and the workgroups of an XCD take its range's bricks in turn.
-struct BrickIter {
-    uint32_t b, end, step;
-};
-__device__ __forceinline__ BrickIter brick_iter(const Bricks& B) {
-    const uint32_t na = B.pos[B.count];
-    const uint32_t xcd = blockIdx.x & 7u, k = blockIdx.x >> 3, K = gridDim.x >> 3;
-    const uint32_t base = na >> 3, rem = na & 7u;
-    const uint32_t lo = xcd * base + min(xcd, rem);
-    return {lo + k, lo + base + (xcd < rem ? 1u : 0u), K};
-}
-
 // Brick `id`'s halo rows (cells x0 - 1 .. x0 + b[0] of rows y0 - 1 .. y0 + b[1],
 // z0 - 1 .. z0 + b[2], wrapped in a periodic box, clamped in a closed one;
 // the first `cap` records are staged) and its own rows.  A periodic row that
@@ -649,8 +635,9 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
     }
 }
 
-// for every brick of this workgroup: set up, stage the records of `src`
-// (NQ quads each), then body(i, fetch) for each of the brick's particles
+// The workgroup's brick (one per active brick, dealt so that each XCD takes a
+// contiguous range, see xcd_block): set up, stage the records of `src` (NQ
+// quads each), then body(i, fetch) for each of the brick's particles.
 template <class Q, int NQ, int W, bool PF, class Body>
 __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const uint32_t* __restrict__ start,
                                            uint32_t cap, const Q* __restrict__ src, BrickHdr& H,
@@ -658,20 +645,17 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
     const uint32_t hy = (uint32_t)B.b[1] + 2u, nh = hy * ((uint32_t)B.b[2] + 2u);
     const uint32_t BY = (uint32_t)B.b[1], no = BY * (uint32_t)B.b[2];
     const SPHX_LDS BrickHdr* HL = (const SPHX_LDS BrickHdr*)&H;
-    for (BrickIter it = brick_iter(B); it.b < it.end; it.b += it.step) {
-        __syncthreads();  // the previous brick's walks are done with H and the records
-        brick_setup(B.active[it.b], B, g, start, cap, H);
-        brick_stage<NQ>(H, (int)nh, reinterpret_cast<const uint4*>(src), sh, cap);
-        const BrickFetch<Q, NQ, W, PF> base_fetch{(const SPHX_LDS lds_quad_t*)sh, (const SPHX_LDS lds_quad_t*)HL->halo,
-                                           src, 0u, hy, cap};
-        const uint32_t nown = HL->own_cum[no];
-        uint32_t k = 0;
-        for (uint32_t u = threadIdx.x; u < nown; u += blockDim.x) {
-            while (u >= HL->own_cum[k + 1]) ++k;
-            BrickFetch<Q, NQ, W, PF> f = base_fetch;
-            f.base = k % BY + hy * (k / BY);
-            body(HL->own_start[k] + (u - HL->own_cum[k]), f);
-        }
+    brick_setup(B.active[xcd_block()], B, g, start, cap, H);
+    brick_stage<NQ>(H, (int)nh, reinterpret_cast<const uint4*>(src), sh, cap);
+    const BrickFetch<Q, NQ, W, PF> base_fetch{(const SPHX_LDS lds_quad_t*)sh, (const SPHX_LDS lds_quad_t*)HL->halo,
+                                              src, 0u, hy, cap};
+    const uint32_t nown = HL->own_cum[no];
+    uint32_t k = 0;
+    for (uint32_t u = threadIdx.x; u < nown; u += blockDim.x) {
+        while (u >= HL->own_cum[k + 1]) ++k;
+        BrickFetch<Q, NQ, W, PF> f = base_fetch;
+        f.base = k % BY + hy * (k / BY);
+        body(HL->own_start[k] + (u - HL->own_cum[k]), f);
     }
 }
 

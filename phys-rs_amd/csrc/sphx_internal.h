@@ -135,6 +135,9 @@ struct Bricks {
     uint32_t* active;          // ids of the bricks that hold particles, in brick order [cap]
     int cus;                   // compute units: the staged passes run cus * k workgroups
     int on;                    // staged passes enabled
+    int threads;               // workgroup size of the staged passes (256 or 512)
+    uint32_t nactive;          // active bricks of the current sort (host copy): one workgroup each
+    double ppc;                // mean particles per non-empty cell at upload (LDS sizing)
 };
 
 // f32 copies in sorted order: p4 = (xr, yr, zr, h), the position as an offset

@@ -22,6 +22,8 @@
 // like the reference's sqrt((dx*dx + dy*dy) + dz*dz) < 2*h_avg
 // (src/physics/sph.rs:102-108).  Fused multiply-adds elsewhere are fma().
 
+#include <algorithm>
+
 #include "sphx_device.h"
 
 namespace sphx {
@@ -348,20 +350,26 @@ k_density_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, Kerne
                                  });
 }
 
-// Staged launch with `per_cu` workgroups per CU and records of rec_bytes:
-// grid size (a multiple of 8, see brick_iter), dynamic LDS and record capacity.
+// Staged launch of a pass with records of rec_bytes that runs at most
+// max_waves waves per CU: LDS for as many workgroups per CU as the waves
+// allow and the mean halo fits (at least one), one workgroup per active
+// brick, and the record capacity.
 struct StagedLaunch {
     dim3 grid, block;
     size_t dyn;
     uint32_t cap;
 };
-static StagedLaunch staged_launch(const Bricks& B, int per_cu, size_t rec_bytes) {
+static StagedLaunch staged_launch(const Bricks& B, int max_waves, size_t rec_bytes) {
     StagedLaunch l;
-    const size_t per_block = kLdsPerCU / (size_t)per_cu - sizeof(BrickHdr) - 512;
+    const double halo = B.ppc * (B.b[0] + 2) * (B.b[1] + 2) * (B.b[2] + 2);
+    const size_t fixed = sizeof(BrickHdr) + 512;
+    int per_cu = std::min(max_waves / (B.threads / 64), (int)((double)kLdsPerCU / (halo * rec_bytes + fixed)));
+    per_cu = std::max(per_cu, 1);
+    const size_t per_block = kLdsPerCU / (size_t)per_cu - fixed;
     l.cap = (uint32_t)(per_block / rec_bytes);
     l.dyn = (size_t)l.cap * rec_bytes;
-    l.grid = dim3((uint32_t)((B.cus * per_cu + 7) / 8 * 8));
-    l.block = dim3(kStagedBlock);
+    l.grid = dim3(B.nactive);
+    l.block = dim3((uint32_t)B.threads);
     return l;
 }
 
@@ -371,8 +379,8 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
     if (!n) return;
     const dim3 grid(nblocks(n)), block(kBlock);
     const bool uni = kp.uniform != 0;
-    if (!mixed && B.on) {
-        const StagedLaunch sl = staged_launch(B, 2, 32);
+    if (!mixed && B.on && B.nactive) {
+        const StagedLaunch sl = staged_launch(B, 16, 32);
 // walk widths measured on the 1M dam break: four entries with the next four
         // loaded ahead for the uniform density (0.27 -> 0.24 ms)
 #define SPHX_DENS_STAGED(U, PER, W, PF)                                                                  \
@@ -385,8 +393,8 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
 #undef SPHX_DENS_STAGED
         return;
     }
-    if (mixed && B.on) {
-        const StagedLaunch sl = staged_launch(B, 2, 16);
+    if (mixed && B.on && B.nactive) {
+        const StagedLaunch sl = staged_launch(B, 16, 16);
         if (uni)
             hipLaunchKernelGGL((k_density_mixed_staged<true, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, X, g, L,
                                kp, B, cell_start, sl.cap);
@@ -662,15 +670,15 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
     const dim3 grid(nblocks(n)), block(kBlock);
     const int ch = charge ? 1 : 0;
     const bool uni = kp.uniform != 0;
-    if (B.on && !stress) {
+    if (B.on && B.nactive && !stress) {
 #define SPHX_FORCES_STAGED(V, U)                                                                            \
         if (visc_only == V && uni == U) {                                                                   \
             if (mixed) {                                                                                    \
-                const StagedLaunch sl = staged_launch(B, 2, 32);                                            \
+                const StagedLaunch sl = staged_launch(B, 16, 32);                                            \
                 hipLaunchKernelGGL((k_forces_mixed_staged<V, U, 4, false>), sl.grid, sl.block, sl.dyn, s, P, D, X, \
                                    g, L, kp, ch, B, cell_start, sl.cap);                                    \
             } else {                                                                                        \
-                const StagedLaunch sl = staged_launch(B, 1, 64);                                            \
+                const StagedLaunch sl = staged_launch(B, 8, 64);                                            \
                 if (g.periodic)                                                                             \
                     hipLaunchKernelGGL((k_forces_staged<V, U, 1>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, \
                                        ch, B, cell_start, sl.cap);                                          \

@@ -74,6 +74,9 @@ struct SphxCtx {
     Sort S{};
     Tiles T{};
     Bricks B{};
+    uint32_t* bricks_host = nullptr;       // pinned: active brick count of the last sort
+    hipEvent_t bricks_ready = nullptr;     // recorded after that count's copy
+    bool bricks_pending = false;           // count not read yet
     NbrList L{};
     Mixed X{};
     double* energy_partial = nullptr;
@@ -200,22 +203,78 @@ void make_kernel_params(SphxCtx* c) {
     k.inv_h0 = c->uniform ? 1.0 / c->h0 : 0.0;
 }
 
-// Brick shape of the staged passes: 3 x 3 x 3 cells (the 5 x 5 x 5-cell halo of
-// a dense fluid at ~19 particles per cell fills the 160 KiB LDS of a CU with
-// 64-byte records).  SPHX_BRICKS=off runs the unstaged passes; SPHX_BRICKS=bx,by,bz
-// sets another shape (by, bz <= 4).
 void brick_defaults(SphxCtx* c) {
     Bricks& B = c->B;
     B.b[0] = B.b[1] = B.b[2] = 3;
-    B.on = 1;
-    const char* e = getenv("SPHX_BRICKS");
-    if (!e) return;
-    int bx, by, bz;
-    if (!strcmp(e, "off")) B.on = 0;
-    else if (sscanf(e, "%d,%d,%d", &bx, &by, &bz) == 3 && bx >= 1 && by >= 1 && bz >= 1 && by <= kMaxBrickYZ &&
-             bz <= kMaxBrickYZ) {
-        B.b[0] = bx; B.b[1] = by; B.b[2] = bz;
+    B.on = 0;
+    B.threads = 512;
+    B.ppc = 1.0;
+}
+
+// Brick shape and workgroup size of the staged passes from ppc, the mean
+// number of particles in a non-empty cell (counted here from the uploaded
+// positions).  Among the shapes whose ppc * cells particles fill 50-100 % of a
+// 256- or 512-thread workgroup, whose f64 force halo (64-byte records) fits
+// the LDS at 8 waves per CU, and which leave at least 8 bricks per CU, the
+// one with the most brick particles per staged record, weighted by the lane
+// fill (the 1M dam break at ~19 per cell: 3 x 3 x 3 with 512 threads).  When
+// none fits (few particles) the passes run unstaged.  SPHX_BRICKS=off or
+// SPHX_BRICKS=bx,by,bz (by, bz <= 4) overrides the choice.
+void choose_bricks(SphxCtx* c, const SphxState* host, const Grid& g) {
+    Bricks& B = c->B;
+    B.on = 0;
+    const uint32_t n = host->n;
+    if (!n) return;
+    std::vector<uint64_t> seen(((size_t)g.ncell + 63) / 64, 0);
+    uint64_t cells = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double xs[3] = {host->x[i], host->y[i], host->z[i]};
+        uint64_t key = 0;
+        for (int k = 2; k >= 0; --k) {
+            double u = (xs[k] - g.lo[k]) * g.inv_w[k];
+            if (!(u >= 0.0)) u = 0.0;
+            u = std::min(u, (double)(g.dims[k] - 1));
+            key = key * (uint64_t)g.dims[k] + (uint64_t)u;
+        }
+        const uint64_t bit = 1ull << (key & 63);
+        if (!(seen[key >> 6] & bit)) {
+            seen[key >> 6] |= bit;
+            ++cells;
+        }
     }
+    const double ppc = (double)n / (double)std::max<uint64_t>(cells, 1);
+    B.ppc = ppc;
+    const char* e = getenv("SPHX_BRICKS");
+    int bx, by, bz;
+    if (e && !strcmp(e, "off")) return;
+    if (e && sscanf(e, "%d,%d,%d", &bx, &by, &bz) == 3 && bx >= 1 && by >= 1 && bz >= 1 && by <= kMaxBrickYZ &&
+        bz <= kMaxBrickYZ) {
+        B.b[0] = bx; B.b[1] = by; B.b[2] = bz;
+        B.threads = ppc * bx * by * bz <= 256 ? 256 : 512;
+        B.on = 1;
+        return;
+    }
+    const double lds_records = (double)(160 * 1024 - 2048) / 64.0;  // f64 force records, one 512-thread group
+    // at least 8 bricks per CU: a brick is one workgroup's work, and with
+    // fewer the last round of workgroups leaves much of the GPU idle
+    double best = 0.0;
+    for (int z = 1; z <= kMaxBrickYZ; ++z)
+        for (int y = 1; y <= kMaxBrickYZ; ++y)
+            for (int x = 1; x <= 8; ++x) {
+                const double own = ppc * x * y * z, halo = ppc * (x + 2) * (y + 2) * (z + 2);
+                if ((double)n / own < 8.0 * B.cus) continue;
+                for (int T : {256, 512}) {
+                    if (own > T || own < 0.5 * T) continue;
+                    if (halo > lds_records * (T / 512.0)) continue;  // 256 threads: two groups per CU
+                    const double score = (own / halo) * std::min(1.0, own / (0.85 * T));
+                    if (score > best) {
+                        best = score;
+                        B.b[0] = x; B.b[1] = y; B.b[2] = z;
+                        B.threads = T;
+                        B.on = 1;
+                    }
+                }
+            }
 }
 
 // Grid from the parameters and (for open boxes) the uploaded positions.
@@ -320,6 +379,7 @@ int setup_grid(SphxCtx* c, const SphxState* host) {
         c->T.nrows_cap = nrows;
     }
     c->T.nrows = nrows;
+    choose_bricks(c, host, g);
     {
         Bricks& B = c->B;
         uint64_t count = 1;
@@ -407,7 +467,15 @@ int do_neighbours(SphxCtx* c) {
                        c->D.xref, c->stream);
         c->cur = 1 - c->cur;
         launch_tiles(c->grid, c->S, c->T, c->stream);
-        launch_bricks(c->grid, c->S, c->B, c->stream);
+        if (c->B.on) {
+            // the staged passes launch one workgroup per active brick: the
+            // count travels to the host while the neighbour build runs
+            launch_bricks(c->grid, c->S, c->B, c->stream);
+            HIP_TRY(c, hipMemcpyAsync(c->bricks_host, c->B.pos + c->B.count, sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipEventRecord(c->bricks_ready, c->stream));
+            c->bricks_pending = true;
+        }
     }
     {
         Timer t(c, kNbr);
@@ -421,8 +489,14 @@ int do_neighbours(SphxCtx* c) {
 }
 
 int ensure_list(SphxCtx* c) {
-    if (c->listed) return SPHX_OK;
-    return do_neighbours(c);
+    int rc;
+    if (!c->listed && (rc = do_neighbours(c))) return rc;
+    if (c->bricks_pending) {
+        HIP_TRY(c, hipEventSynchronize(c->bricks_ready));
+        c->B.nactive = *c->bricks_host;
+        c->bricks_pending = false;
+    }
+    return SPHX_OK;
 }
 
 int do_density(SphxCtx* c) {
@@ -714,6 +788,16 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     if (hipDeviceGetAttribute(&c->B.cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return bail(SPHX_E_HIP);
     brick_defaults(c);
+    if (hipHostMalloc((void**)&c->bricks_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+        c->bricks_host = nullptr;
+        return bail(SPHX_E_OOM);
+    }
+    if (hipEventCreateWithFlags(&c->bricks_ready, hipEventDisableTiming) != hipSuccess) {
+        c->bricks_ready = nullptr;
+        (void)hipHostFree(c->bricks_host);
+        c->bricks_host = nullptr;
+        return bail(SPHX_E_HIP);
+    }
     const size_t N = c->cap;
     for (int b = 0; b < 2; ++b) {
         Particles& P = c->A[b];
@@ -759,6 +843,8 @@ void sphx_destroy(SphxCtx* c) {
         (void)hipEventDestroy(t.b);
     }
     for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
+    if (c->bricks_ready) (void)hipEventDestroy(c->bricks_ready);
+    if (c->bricks_host) (void)hipHostFree(c->bricks_host);
     c->adj.release();
     free_all(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
