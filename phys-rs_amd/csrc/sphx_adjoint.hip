@@ -177,6 +177,9 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
         const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
         const double sc = ref_conv ? 1.0 : -1.0;
         double gx = 0, gy = 0, gz = 0, gvx = 0, gvy = 0, gvz = 0, grho = 0;
+        // Branch-free pair body with two refined reciprocals (1/Dn, 1/rho_bar)
+        // in place of divisions; a pair outside the kernel support, at r <= 1e-10
+        // or (for the viscosity terms) not approaching/separating contributes 0.
         for_neighbours(L, i, [&](uint32_t e) {
             const uint32_t j = e & kIdxMask;
             const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
@@ -184,44 +187,53 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
             double dx, dy, dz;
             pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            if (!(r2 > 1e-20)) return;  // r <= 1e-10: no pressure force (sph.rs:52)
+            const bool ok = r2 > 1e-20;  // r <= 1e-10: no pressure force (sph.rs:52)
             double r, inv_r;
-            sqrt_rsqrt(r2, r, inv_r);
+            sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
             const double h = UNI ? kp.h0 : 0.5 * (hi + P.h[j]);
             const double inv_h = UNI ? kp.inv_h0 : rcp_nr(h);
-            const double G = kernel_dw(r, inv_h) * inv_r;
-            const double Gp = (kernel_d2w(r, inv_h) - G) * inv_r;  // dG/dr
+            const double ih2 = inv_h * inv_h, sig_g = (ih2 * ih2) * kInvPi;
+            const double qk = r * inv_h;
+            const double tk = fmax(2.0 - qk, 0.0);
+            // dW/dr and d2W/dr2 of the cubic spline, zero from q = 2 on
+            const double dw = sig_g * (qk < 1.0 ? qk * fma(2.25, qk, -3.0) : -0.75 * (tk * tk));
+            const double d2w = (sig_g * inv_h) * (qk < 1.0 ? fma(4.5, qk, -3.0) : 1.5 * tk);
+            const double okf = ok ? 1.0 : 0.0;
+            const double G = (dw * inv_r) * okf;
+            const double Gp = ((d2w - dw * inv_r) * inv_r) * okf;  // dG/dr
             const double mj = UNI ? kp.m0 : P.m[j], mm = mi * mj;
-            const double inv_mj = UNI ? inv_mi : 1.0 / mj;
+            const double inv_mj = UNI ? inv_mi : rcp_nr(mj);
             const double ux = uxi - a01.x * inv_mj, uy = uyi - a01.y * inv_mj, uz = uzi - a2.x * inv_mj;
             const double c = fma(uz, dz, fma(uy, dy, ux * dx));
             const double rhoj = zr.y;
             const double T = pr2i + vzp.y;
             const double vx = vxi - vxy.x, vy = vyi - vxy.y, vz = vzi - vzp.x;
             const double sdot = fma(vz, dz, fma(vy, dy, vx * dx));
-            const bool act = ref_conv ? (sdot < 0.0) : (sdot > 0.0);
-            double Pi = 0.0, Pi_mu = 0.0, mu = 0.0, Dn = 1.0, rho_bar = 1.0;
-            if (act) {
-                Dn = fma(0.01 * h, h, r2);
-                mu = (h * (sc * sdot)) / Dn;
-                rho_bar = 0.5 * (rhoi + rhoj);
-                Pi = (-alpha * cs * mu + beta * (mu * mu)) / rho_bar;
-                Pi_mu = (-alpha * cs + 2.0 * beta * mu) / rho_bar;
-            }
-            const double phi = mm * (T + Pi) * G;
-            const double q = c * mm * (T + Pi) * Gp * inv_r;
-            const double cg = c * mm * G;
-            const double w = act ? cg * Pi_mu / Dn : 0.0;  // d(c phi)/d(mu) * dmu/d(.) prefactor
-            // dL/dd = phi u + q d + w (h sc v - 2 mu d); particle i is the pair's "a": xbar_i -= dL/dd
-            gx -= phi * ux + q * dx + w * (h * sc * vx - 2.0 * mu * dx);
-            gy -= phi * uy + q * dy + w * (h * sc * vy - 2.0 * mu * dy);
-            gz -= phi * uz + q * dz + w * (h * sc * vz - 2.0 * mu * dz);
+            const double actf = (ref_conv ? (sdot < 0.0) : (sdot > 0.0)) ? 1.0 : 0.0;
+            const double Dn = fma(0.01 * h, h, r2);
+            const double inv_Dn = rcp_nr(Dn);
+            const double inv_rb = 2.0 * rcp_nr(rhoi + rhoj);  // 1 / rho_bar
+            const double hsc = h * sc;
+            const double mu = ((hsc * sdot) * inv_Dn) * actf;
+            const double Pi = fma(beta * mu, mu, -alpha * cs * mu) * inv_rb;
+            const double Pi_mu = (fma(2.0 * beta, mu, -alpha * cs) * inv_rb) * actf;
+            const double TP = mm * (T + Pi);
+            const double phi = TP * G;
+            const double qd = ((c * TP) * Gp) * inv_r;
+            const double cg = (c * mm) * G;
+            const double w = (cg * Pi_mu) * inv_Dn;  // d(c phi)/d(mu) times dmu/d(.) / (...)
+            const double tmu = -2.0 * mu;
+            // dL/dd = phi u + qd d + w (h sc v - 2 mu d); particle i is the pair's "a": xbar_i -= dL/dd
+            gx -= fma(w, fma(hsc, vx, tmu * dx), fma(phi, ux, qd * dx));
+            gy -= fma(w, fma(hsc, vy, tmu * dy), fma(phi, uy, qd * dy));
+            gz -= fma(w, fma(hsc, vz, tmu * dz), fma(phi, uz, qd * dz));
             // dL/dv_i
-            gvx += w * h * sc * dx;
-            gvy += w * h * sc * dy;
-            gvz += w * h * sc * dz;
-            grho += cg * (dTi + (act ? -Pi / (2.0 * rho_bar) : 0.0));
-            if (act) da += 0.5 * cg * (-cs * mu / rho_bar);  // each pair is visited from both ends
+            const double wh = w * hsc;
+            gvx = fma(wh, dx, gvx);
+            gvy = fma(wh, dy, gvy);
+            gvz = fma(wh, dz, gvz);
+            grho = fma(cg, fma(-0.5 * Pi, inv_rb, dTi), grho);
+            da = fma(0.5 * cg, (-cs * mu) * inv_rb, da);  // each pair is visited from both ends
         });
         const uint32_t u = P.orig[i];
         xb.x[u] += gx; xb.y[u] += gy; xb.z[u] += gz;
@@ -248,12 +260,13 @@ __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n,
         double dx, dy, dz;
         pair_vec(g, xi, yi, zi, xy.x, xy.y, zh.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        if (!(r2 > 0.0)) return;
+        const bool ok = r2 > 0.0;
         double r, inv_r;
-        sqrt_rsqrt(r2, r, inv_r);
+        sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
         const double inv_h = UNI ? kp.inv_h0 : rcp_nr(0.5 * (hi + zh.y));
-        const double coef = UNI ? (rbi + rhob[j]) * mi * (kernel_dw(r, inv_h) * inv_r)
-                                : (rbi * P.m[j] + rhob[j] * mi) * (kernel_dw(r, inv_h) * inv_r);
+        const double ih2 = inv_h * inv_h;
+        const double dwr = ((((ih2 * ih2) * kInvPi) * dw_shape(r * inv_h)) * inv_r) * (ok ? 1.0 : 0.0);
+        const double coef = UNI ? ((rbi + rhob[j]) * mi) * dwr : fma(rbi, P.m[j], rhob[j] * mi) * dwr;
         gx = fma(-coef, dx, gx);
         gy = fma(-coef, dy, gy);
         gz = fma(-coef, dz, gz);
