@@ -557,8 +557,8 @@ struct BrickFetch {
     // LDS record slot of entry e; false if it is not staged
     __device__ __forceinline__ bool locate(uint32_t e, uint32_t& slot) const {
         const uint32_t j = e & kIdxMask, c = e >> kIdxBits;
-        const uint32_t s = (c * 11u) >> 5, sz = (s * 11u) >> 5;
-        const lds_quad_t h = halo[base + (s - 3u * sz) + hy * sz];
+        const uint32_t s = __umul24(c, 11u) >> 5, sz = __umul24(s, 11u) >> 5;  // stage, its z row
+        const lds_quad_t h = halo[base + (s - 3u * sz) + __umul24(hy, sz)];
         const uint32_t d = j - h.x;
         slot = h.z + d;
         return d < h.y;
