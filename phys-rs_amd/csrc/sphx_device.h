@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "sphx_internal.h"
 
 namespace sphx {
@@ -372,6 +374,35 @@ __device__ __forceinline__ void for_neighbours(const NbrList& L, uint32_t i, F&&
     if (k < cnt) f(e[(size_t)k << 6]);
 }
 
+// f(e) for every entry, and flush() after each group of four consecutive
+// entries (0-3, 4-7, ...) and after the last one: the FP32-mixed passes sum a
+// group in f32 and add it to their f64 sums at the flush.
+template <typename F, typename G>
+__device__ __forceinline__ void for_neighbour_groups(const NbrList& L, uint32_t i, F&& f, G&& flush) {
+    const uint32_t* __restrict__ e = nbr_base(L, i);
+    const uint32_t cnt = L.ntot[i];
+    uint32_t k = 0;
+    for (; k + 3 < cnt; k += 4) {
+        const uint32_t e0 = e[(size_t)k << 6], e1 = e[(size_t)(k + 1) << 6];
+        const uint32_t e2 = e[(size_t)(k + 2) << 6], e3 = e[(size_t)(k + 3) << 6];
+        f(e0);
+        f(e1);
+        f(e2);
+        f(e3);
+        flush();
+    }
+    if (k < cnt) {
+        f(e[(size_t)k << 6]);
+        if (k + 1 < cnt) f(e[(size_t)(k + 1) << 6]);
+        if (k + 2 < cnt) f(e[(size_t)(k + 2) << 6]);
+        flush();
+    }
+}
+
+struct NoFlush {
+    __device__ __forceinline__ void operator()() const {}
+};
+
 // Sorted particle i's exact neighbour count r < 2*h_avg (its list may also
 // hold pairs out to 2*h_avg + skin).
 __device__ __forceinline__ uint32_t exact_count(const Particles& P, const Grid& g, const NbrList& L, uint32_t i) {
@@ -406,6 +437,10 @@ __device__ __forceinline__ uint32_t particle_index() { return xcd_block() * kBlo
 template <class Q, int NQ>
 struct GlobalFetch {
     const Q* __restrict__ glob;
+    __device__ __forceinline__ void offset(uint32_t e, float wx, float wy, float wz, float& ox, float& oy,
+                                          float& oz) const {
+        entry_offset(e, wx, wy, wz, ox, oy, oz);
+    }
     __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NQ]) const {
         const Q* p = glob + (size_t)(e & kIdxMask) * NQ;
 #pragma unroll
@@ -415,13 +450,18 @@ struct GlobalFetch {
 
 // body(e, record) for every list entry of sorted particle i, records from
 // global memory.
-template <class Q, int NQ, class Body>
-__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalFetch<Q, NQ>& F, Body&& body) {
-    for_neighbours(L, i, [&](uint32_t e) {
+template <class Q, int NQ, class Body, class Flush = NoFlush>
+__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalFetch<Q, NQ>& F, Body&& body,
+                                     Flush&& flush = Flush()) {
+    auto one = [&](uint32_t e) {
         Q q[NQ];
         F(e, q);
         body(e, q);
-    });
+    };
+    if (std::is_same<typename std::decay<Flush>::type, NoFlush>::value)
+        for_neighbours(L, i, one);
+    else
+        for_neighbour_groups(L, i, one, flush);
 }
 
 // =========================================================================
@@ -429,14 +469,17 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalF
 // =========================================================================
 struct BrickHdr {
     uint4 halo[kMaxHaloRows];           // per halo row: {first sorted index, staged count, LDS record base, 0}
+    uint4 code[27];                     // per entry cell code: {ox, oy, oz (f32 bits, times the cell widths),
+                                        // halo-row step sy + (b[1] + 2) sz}
     uint32_t own_start[kMaxOwnRows];    // per brick row: its first sorted particle
     uint32_t own_cum[kMaxOwnRows + 1];  // exclusive prefix of the brick rows' particle counts
 };
 
 // Brick `id`'s halo rows (cells x0 - 1 .. x0 + b[0] of rows y0 - 1 .. y0 + b[1],
 // z0 - 1 .. z0 + b[2], wrapped in a periodic box, clamped in a closed one;
-// the first `cap` records are staged) and its own rows.  A periodic row that
-// wraps in x stages only its unwrapped part.  Needs blockDim.x >= 128.
+// the first `cap` records are staged), its own rows and the table of entry
+// codes.  A periodic row that wraps in x stages only its unwrapped part.
+// Needs blockDim.x >= 155.
 __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const Grid& g,
                                             const uint32_t* __restrict__ start, uint32_t cap, BrickHdr& H) {
     const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
@@ -475,7 +518,15 @@ __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const 
         const uint32_t base = incl - gl;
         const uint32_t staged = base >= cap ? 0u : min(gl, cap - base);
         if (lane < nh) H.halo[lane] = make_uint4(gs, staged, base, 0u);
-    } else if (t < 128) {
+    } else if (t >= 128 && t < 128 + 27) {
+        // entry code c = stage * 3 + x offset + 1, stage = (oz + 1) * 3 + oy + 1
+        const uint32_t c = t - 128u, st = c / 3u, sy = st % 3u, sz = st / 3u;
+        const float ox = (float)((int)(c - 3u * st) - 1) * (float)g.w[0];
+        const float oy = (float)((int)sy - 1) * (float)g.w[1], oz = (float)((int)sz - 1) * (float)g.w[2];
+        H.code[c] = make_uint4(__float_as_uint(ox), __float_as_uint(oy), __float_as_uint(oz),
+                               sy + (uint32_t)(BY + 2) * sz);
+    }
+    if (t >= 64 && t < 128) {
         const int no = BY * BZ;
         uint32_t s0 = 0, cnt = 0;
         if (lane < no) {
@@ -552,16 +603,23 @@ struct BrickFetch {
     static constexpr bool PF = PF_;
     const SPHX_LDS lds_quad_t* lds;
     const SPHX_LDS lds_quad_t* halo;
+    const SPHX_LDS lds_quad_t* code;
     const Q* __restrict__ glob;
     uint32_t base, hy, cap;
     // LDS record slot of entry e; false if it is not staged
     __device__ __forceinline__ bool locate(uint32_t e, uint32_t& slot) const {
-        const uint32_t j = e & kIdxMask, c = e >> kIdxBits;
-        const uint32_t s = __umul24(c, 11u) >> 5, sz = __umul24(s, 11u) >> 5;  // stage, its z row
-        const lds_quad_t h = halo[base + (s - 3u * sz) + __umul24(hy, sz)];
+        const uint32_t j = e & kIdxMask;
+        const lds_quad_t h = halo[base + code[e >> kIdxBits].w];
         const uint32_t d = j - h.x;
         slot = h.z + d;
         return d < h.y;
+    }
+    // cell offset of entry e's neighbour times the cell widths (as entry_offset)
+    __device__ __forceinline__ void offset(uint32_t e, float, float, float, float& ox, float& oy, float& oz) const {
+        const lds_quad_t t = code[e >> kIdxBits];
+        ox = __uint_as_float(t.x);
+        oy = __uint_as_float(t.y);
+        oz = __uint_as_float(t.z);
     }
     __device__ __forceinline__ void from_lds(uint32_t slot, Q (&q)[NQ]) const {
 #pragma unroll
@@ -584,11 +642,13 @@ struct BrickFetch {
 // ahead.  When all W neighbours of every active lane are staged (a
 // wave-uniform test) the records are read with no per-lane branch, so the W
 // bodies, written branch-free, form one basic block the compiler can
-// interleave.
-template <class Q, int NQ, int W, bool PF, class Body>
+// interleave.  flush() as in for_neighbour_groups (it needs W = 4).
+template <class Q, int NQ, int W, bool PF, class Body, class Flush = NoFlush>
 __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF>& F,
-                                     Body&& body) {
+                                     Body&& body, Flush&& flush = Flush()) {
     static_assert(W == 2 || W == 4, "two or four entries per iteration");
+    static_assert(W == 4 || std::is_same<typename std::decay<Flush>::type, NoFlush>::value,
+                  "groups of four entries need four entries per iteration");
     const uint32_t* __restrict__ e = nbr_base(L, i);
     const uint32_t cnt = L.ntot[i];
     uint32_t a[W];
@@ -623,6 +683,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
         }
 #pragma unroll
         for (int u = 0; u < W; ++u) body(c[u], q[u]);
+        flush();
     }
     Q q[NQ];
 #pragma unroll
@@ -633,6 +694,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
             body(c, q);
         }
     }
+    if (k < cnt) flush();
 }
 
 // The workgroup's brick (one per active brick, dealt so that each XCD takes a
@@ -648,7 +710,7 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
     brick_setup(B.active[xcd_block()], B, g, start, cap, H);
     brick_stage<NQ>(H, (int)nh, reinterpret_cast<const uint4*>(src), sh, cap);
     const BrickFetch<Q, NQ, W, PF> base_fetch{(const SPHX_LDS lds_quad_t*)sh, (const SPHX_LDS lds_quad_t*)HL->halo,
-                                              src, 0u, hy, cap};
+                                              (const SPHX_LDS lds_quad_t*)HL->code, src, 0u, hy, cap};
     const uint32_t nown = HL->own_cum[no];
     uint32_t k = 0;
     for (uint32_t u = threadIdx.x; u < nown; u += blockDim.x) {

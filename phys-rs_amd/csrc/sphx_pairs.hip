@@ -297,9 +297,10 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     const float sig_w0 = (float)(kp.inv_h0 * kp.inv_h0 * kp.inv_h0 * kInvPi);
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double acc = 0.0;
+    float gacc = 0.0f;  // f32 sum over each group of four list entries
     walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[1]) {
         float ox, oy, oz;
-        entry_offset(e, wx, wy, wz, ox, oy, oz);
+        fetch.offset(e, wx, wy, wz, ox, oy, oz);
         const float dx = (q[0].x - xi) + ox;
         const float dy = (q[0].y - yi) + oy;
         const float dz = (q[0].z - zi) + oz;
@@ -310,7 +311,10 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
             sig = (inv_h * inv_h) * inv_h * (float)kInvPi;
         }
         const float w = sig * w_shape_f(r * inv_h);
-        acc += (double)(UNI ? w : X.m[e & kIdxMask] * w);
+        gacc += UNI ? w : X.m[e & kIdxMask] * w;
+    }, [&]() {
+        acc += (double)gacc;
+        gacc = 0.0f;
     });
     rho = UNI ? fma(kp.m0, acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
@@ -399,7 +403,7 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
             hipLaunchKernelGGL((k_density_mixed_staged<true, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, X, g, L,
                                kp, B, cell_start, sl.cap);
         else
-            hipLaunchKernelGGL((k_density_mixed_staged<false, 2, false>), sl.grid, sl.block, sl.dyn, s, P, D, X, g,
+            hipLaunchKernelGGL((k_density_mixed_staged<false, 4, false>), sl.grid, sl.block, sl.dyn, s, P, D, X, g,
                                L, kp, B, cell_start, sl.cap);
         return;
     }
@@ -584,12 +588,15 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
+    // f32 sums over each group of four list entries, added to the f64 sums
+    // once per group
+    float gp[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
     // branch-free pair body, as in the f64 pass
     walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[2]) {
         const uint32_t j = e & kIdxMask;
         const float4 a = q[0], b = q[1];
         float ox, oy, oz;
-        entry_offset(e, wx, wy, wz, ox, oy, oz);
+        fetch.offset(e, wx, wy, wz, ox, oy, oz);
         const float dx = (a.x - xi) + ox;
         const float dy = (a.y - yi) + oy;
         const float dz = (a.z - zi) + oz;
@@ -609,9 +616,9 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         const float mm = UNI ? mm0 : mi * X.m[j];
         if (!VISC_ONLY) {
             const float sc = (mm * (pr2i + b.w)) * dwr;
-            fp[0] += (double)(dx * sc);
-            fp[1] += (double)(dy * sc);
-            fp[2] += (double)(dz * sc);
+            gp[0] = fmaf(dx, sc, gp[0]);
+            gp[1] = fmaf(dy, sc, gp[1]);
+            gp[2] = fmaf(dz, sc, gp[2]);
         }
         const float sdot = fmaf(vzi - b.z, dz, fmaf(vyi - b.y, dy, (vxi - b.x) * dx));
         const float rhoj = a.w;
@@ -622,9 +629,9 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             const float mu = (h_avg * ss) * __builtin_amdgcn_rcpf(fmaf(0.01f * h_avg, h_avg, r2));
             const float pi_ij = fmaf(beta * mu, mu, ac * mu) * __builtin_amdgcn_rcpf(rho_bar);
             const float sc = act ? (mm * pi_ij) * dwr : 0.0f;
-            fv[0] += (double)(dx * sc);
-            fv[1] += (double)(dy * sc);
-            fv[2] += (double)(dz * sc);
+            gv[0] = fmaf(dx, sc, gv[0]);
+            gv[1] = fmaf(dy, sc, gv[1]);
+            gv[2] = fmaf(dz, sc, gv[2]);
         }
         if (STRESS) {
             const float gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;
@@ -637,6 +644,12 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             fm[0] += (double)(mm * ((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j));
             fm[1] += (double)(mm * ((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j));
             fm[2] += (double)(mm * ((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j));
+        }
+    }, [&]() {
+        for (int a = 0; a < 3; ++a) {
+            if (!VISC_ONLY) fp[a] += (double)gp[a];
+            fv[a] += (double)gv[a];
+            gp[a] = gv[a] = 0.0f;
         }
     });
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
