@@ -108,7 +108,8 @@ typedef struct SphxParams {
     double grid_lo[3];         /* grid box; grid_hi <= grid_lo on any axis: taken from the      */
     double grid_hi[3];         /* uploaded positions (padded), or from box_lo/hi when periodic   */
     int32_t precision;         /* enum SphxPrecision                                             */
-    int32_t max_neighbours;    /* per-particle neighbour-list capacity; <= 0: 192                */
+    int32_t max_neighbours;    /* per-particle neighbour-list capacity, rounded up to a multiple
+                                  of 4; <= 0: 192                                                */
     double neighbour_skin;     /* list radius 2*h_avg + skin, rebuilt only once some particle has
                                   moved skin/2 since the last build; < 0: 0.04 * max(h) (default);
                                   0: rebuilt every tick.  Every pass still uses exactly the pairs

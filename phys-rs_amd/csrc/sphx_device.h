@@ -354,24 +354,37 @@ __device__ __forceinline__ void cell_slots(const Grid& g, const uint32_t* __rest
 // =========================================================================
 // Neighbour lists (built per tile, walked per particle).
 // =========================================================================
+// Entry k of sorted particle i (layout in NbrList): group k / 4 of the
+// particle's list at nbr_base + (k / 4) * 256, entry k % 4 within it.
 __device__ __forceinline__ uint32_t* nbr_base(const NbrList& L, uint32_t i) {
-    return L.idx + (((size_t)(i >> 6) * (uint32_t)L.ks) << 6) + (i & 63u);
+    return L.idx + ((((size_t)(i >> 6) * (uint32_t)(L.ks >> 2)) << 6) + (i & 63u)) * 4u;
+}
+__device__ __forceinline__ size_t nbr_off(uint32_t k) { return ((size_t)(k >> 2) << 8) + (k & 3u); }
+// the four entries of the group holding entry k (k a multiple of 4)
+__device__ __forceinline__ uint4 nbr_group(const uint32_t* __restrict__ e, uint32_t k) {
+    return *reinterpret_cast<const uint4*>(e + ((size_t)(k >> 2) << 8));
 }
 
-// f(e) for every entry of sorted particle i, two entries per iteration so
-// that the second's gathers overlap the first's arithmetic.
+// f(e) for every entry of sorted particle i, a group of four per iteration
+// (one 16-byte load) so that the bodies overlap each other's gathers.
 template <typename F>
 __device__ __forceinline__ void for_neighbours(const NbrList& L, uint32_t i, F&& f) {
     const uint32_t* __restrict__ e = nbr_base(L, i);
     const uint32_t cnt = L.ntot[i];
     uint32_t k = 0;
-    for (; k + 1 < cnt; k += 2) {
-        const uint32_t e0 = e[(size_t)k << 6];
-        const uint32_t e1 = e[(size_t)(k + 1) << 6];
-        f(e0);
-        f(e1);
+    for (; k + 3 < cnt; k += 4) {
+        const uint4 q = nbr_group(e, k);
+        f(q.x);
+        f(q.y);
+        f(q.z);
+        f(q.w);
     }
-    if (k < cnt) f(e[(size_t)k << 6]);
+    if (k < cnt) {
+        const uint4 q = nbr_group(e, k);
+        f(q.x);
+        if (k + 1 < cnt) f(q.y);
+        if (k + 2 < cnt) f(q.z);
+    }
 }
 
 // f(e) for every entry, and flush() after each group of four consecutive
@@ -383,18 +396,18 @@ __device__ __forceinline__ void for_neighbour_groups(const NbrList& L, uint32_t 
     const uint32_t cnt = L.ntot[i];
     uint32_t k = 0;
     for (; k + 3 < cnt; k += 4) {
-        const uint32_t e0 = e[(size_t)k << 6], e1 = e[(size_t)(k + 1) << 6];
-        const uint32_t e2 = e[(size_t)(k + 2) << 6], e3 = e[(size_t)(k + 3) << 6];
-        f(e0);
-        f(e1);
-        f(e2);
-        f(e3);
+        const uint4 q = nbr_group(e, k);
+        f(q.x);
+        f(q.y);
+        f(q.z);
+        f(q.w);
         flush();
     }
     if (k < cnt) {
-        f(e[(size_t)k << 6]);
-        if (k + 1 < cnt) f(e[(size_t)(k + 1) << 6]);
-        if (k + 2 < cnt) f(e[(size_t)(k + 2) << 6]);
+        const uint4 q = nbr_group(e, k);
+        f(q.x);
+        if (k + 1 < cnt) f(q.y);
+        if (k + 2 < cnt) f(q.z);
         flush();
     }
 }
@@ -651,25 +664,32 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
                   "groups of four entries need four entries per iteration");
     const uint32_t* __restrict__ e = nbr_base(L, i);
     const uint32_t cnt = L.ntot[i];
+    // W entries of the list from position k (k a multiple of W): one 16- or
+    // 8-byte load
+    auto load = [&](uint32_t k, uint32_t (&c)[W]) {
+        if constexpr (W == 4) {
+            const uint4 q = nbr_group(e, k);
+            c[0] = q.x; c[1] = q.y; c[2] = q.z; c[3] = q.w;
+        } else {
+            const uint2 q = *reinterpret_cast<const uint2*>(e + nbr_off(k));
+            c[0] = q.x; c[W - 1] = q.y;
+        }
+    };
     uint32_t a[W];
-    if (PF) {
-#pragma unroll
-        for (int u = 0; u < W; ++u) a[u] = (uint32_t)u < cnt ? e[(size_t)u << 6] : 0u;
-    }
+    if (PF && cnt > 0) load(0, a);
     uint32_t k = 0;
     for (; k + (W - 1) < cnt; k += W) {
         uint32_t c[W], slot[W];
         bool staged = true;
+        if (PF) {
+            // the next W entries are loaded unconditionally (the group past the
+            // end is the last one again) so that the load has its own registers
+            // and nothing waits for it this iteration
 #pragma unroll
-        for (int u = 0; u < W; ++u) {
-            if (PF) {
-                // unconditional (clamped to the last entry) so that the load
-                // has its own registers and nothing waits for it this iteration
-                c[u] = a[u];
-                a[u] = e[(size_t)min(k + W + u, cnt - 1u) << 6];
-            } else {
-                c[u] = e[(size_t)(k + u) << 6];
-            }
+            for (int u = 0; u < W; ++u) c[u] = a[u];
+            load(min(k + W, (cnt - 1u) & ~(uint32_t)(W - 1)), a);
+        } else {
+            load(k, c);
         }
 #pragma unroll
         for (int u = 0; u < W; ++u) staged = F.locate(c[u], slot[u]) && staged;
@@ -686,15 +706,23 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
         flush();
     }
     Q q[NQ];
+    if (k < cnt) {
+        uint32_t c[W];
+        if (PF) {
 #pragma unroll
-    for (int u = 0; u < W - 1; ++u) {
-        if (k + u < cnt) {
-            const uint32_t c = PF ? a[u] : e[(size_t)(k + u) << 6];
-            F(c, q);
-            body(c, q);
+            for (int u = 0; u < W; ++u) c[u] = a[u];
+        } else {
+            load(k, c);
         }
+#pragma unroll
+        for (int u = 0; u < W - 1; ++u) {
+            if (k + u < cnt) {
+                F(c[u], q);
+                body(c[u], q);
+            }
+        }
+        flush();
     }
-    if (k < cnt) flush();
 }
 
 // The workgroup's brick (one per active brick, dealt so that each XCD takes a

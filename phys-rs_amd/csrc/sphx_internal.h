@@ -97,9 +97,11 @@ struct Tiles {
     uint32_t nrows_cap;        // rows the row arrays can hold
 };
 
-// Neighbour list of sorted particle i: ntot[i] entries at
-// idx[((i / 64) * ks + k) * 64 + i % 64], k < ntot[i], so the 64 lanes of a
-// wave at equal k read 256 contiguous bytes.  An entry is the neighbour's
+// Neighbour list of sorted particle i: ntot[i] entries, entry k at
+// idx[(((i / 64) * (ks / 4) + k / 4) * 64 + i % 64) * 4 + k % 4] (ks a multiple
+// of 4): each lane reads or writes four consecutive entries as one 16-byte
+// word, and the 64 lanes of a wave at equal k / 4 read 1 KiB contiguously.
+// An entry is the neighbour's
 // sorted index (bits 0..26) and its cell's offset from particle i's cell
 // (bits 27..31: stage * 3 + x offset + 1, stage = (oz + 1) * 3 + oy + 1).
 static constexpr uint32_t kIdxBits = 27;

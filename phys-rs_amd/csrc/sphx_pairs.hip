@@ -87,6 +87,9 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
     const float wx = (float)g.w[0];
     uint32_t total = 0;
     uint32_t* __restrict__ out = nbr_base(L, p);
+    // the last four accepted entries (b3 newest): the list is written a group
+    // of four (16 bytes) at a time
+    uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
     for (int s = 0; s < kStages; ++s) {
         Stage st;
         stage_segments(g, start, tc, s, st);
@@ -116,8 +119,14 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                 in = (ddx * ddx + ddy * ddy) + ddz * ddz < (t * t) * (1.0 + 1e-12);
             }
             if (in) {
-                if (total < (uint32_t)L.ks)
-                    out[(size_t)total << 6] = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
+                if (total < (uint32_t)L.ks) {
+                    b0 = b1;
+                    b1 = b2;
+                    b2 = b3;
+                    b3 = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
+                    if ((total & 3u) == 3u)
+                        *reinterpret_cast<uint4*>(out + nbr_off(total - 3u)) = make_uint4(b0, b1, b2, b3);
+                }
                 ++total;
             }
         };
@@ -183,7 +192,16 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             }
         }
     }
-    if (valid) L.ntot[p] = (uint16_t)min(total, (uint32_t)L.ks);
+    if (valid) {
+        // the last, partial group (ks is a multiple of 4, so a full list has none)
+        const uint32_t kept = min(total, (uint32_t)L.ks), r = kept & 3u;
+        if (r) {
+            const uint4 v = r == 1u ? make_uint4(b3, 0u, 0u, 0u)
+                                    : (r == 2u ? make_uint4(b2, b3, 0u, 0u) : make_uint4(b1, b2, b3, 0u));
+            *reinterpret_cast<uint4*>(out + nbr_off(kept - r)) = v;
+        }
+        L.ntot[p] = (uint16_t)kept;
+    }
     const uint32_t wmax = wave_max_u32(total);
     if ((t & 63) == 0 && wmax) {
         atomicMax(&L.overflow[0], wmax);

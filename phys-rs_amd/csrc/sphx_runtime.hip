@@ -772,7 +772,8 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     c->cap = capacity ? capacity : 1;
     c->prm = *params;
     make_kernel_params(c);
-    c->L.ks = params->max_neighbours > 0 ? params->max_neighbours : kDefaultNeighbours;
+    // a multiple of 4: lists are read and written in groups of four entries
+    c->L.ks = params->max_neighbours > 0 ? (params->max_neighbours + 3) / 4 * 4 : kDefaultNeighbours;
     if (c->L.ks > kMaxNeighbours) {
         delete c;
         return SPHX_E_INVALID;
