@@ -312,7 +312,6 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     const double hd = P.h[i];
     double rho = P.m[i] * (1.0 / (kPi * ((hd * hd) * hd)));
     const float inv_h0 = (float)kp.inv_h0;
-    const float sig_w0 = (float)(kp.inv_h0 * kp.inv_h0 * kp.inv_h0 * kInvPi);
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double acc = 0.0;
     float gacc = 0.0f;  // f32 sum over each group of four list entries
@@ -323,18 +322,18 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
         const float dy = (q[0].y - yi) + oy;
         const float dz = (q[0].z - zi) + oz;
         const float r = __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-        float inv_h = inv_h0, sig = sig_w0;
-        if (!UNI) {
-            inv_h = __builtin_amdgcn_rcpf(0.5f * (hi + q[0].w));
-            sig = (inv_h * inv_h) * inv_h * (float)kInvPi;
+        if (UNI) {  // m and sigma_w are folded out of the sum
+            gacc += w_shape_f(r * inv_h0);
+        } else {
+            const float inv_h = __builtin_amdgcn_rcpf(0.5f * (hi + q[0].w));
+            const float w = ((inv_h * inv_h) * inv_h * (float)kInvPi) * w_shape_f(r * inv_h);
+            gacc += X.m[e & kIdxMask] * w;
         }
-        const float w = sig * w_shape_f(r * inv_h);
-        gacc += UNI ? w : X.m[e & kIdxMask] * w;
     }, [&]() {
         acc += (double)gacc;
         gacc = 0.0f;
     });
-    rho = UNI ? fma(kp.m0, acc, rho) : rho + acc;
+    rho = UNI ? fma(kp.m0 * (((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi), acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
@@ -599,17 +598,17 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         hb2i = 0.5f * ((bxi * bxi + byi * byi) + bzi * bzi);
     }
     const float inv_h0 = (float)kp.inv_h0, h0 = (float)kp.h0;
-    const float sig_g0 = (float)(kp.inv_h0 * kp.inv_h0 * kp.inv_h0 * kp.inv_h0 * kInvPi);
-    const float mm0 = (float)(kp.m0 * kp.m0);
-    const float ac = (float)(-kp.alpha * kp.speed_of_sound);
-    const float beta = (float)kp.beta;
+    // viscosity with one reciprocal, as in the f64 pass:
+    // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s
+    const float ac2 = (float)(-2.0 * kp.alpha * kp.speed_of_sound), beta2 = (float)(2.0 * kp.beta);
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
+    const float ssign = ref_conv ? 1.0f : -1.0f;
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     // f32 sums over each group of four list entries, added to the f64 sums
-    // once per group
+    // once per group; with uniform m and h, m_i m_j sigma_g is folded out
     float gp[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
-    // branch-free pair body, as in the f64 pass
+    // branch-free pair body, as in the f64 pass (factors 1 or 0, not selects)
     walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[2]) {
         const uint32_t j = e & kIdxMask;
         const float4 a = q[0], b = q[1];
@@ -622,18 +621,19 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         const bool ok = r2 > 1e-20f;
         const float inv_r = __builtin_amdgcn_rsqf(ok ? r2 : 1.0f);
         const float r = r2 * inv_r;
-        float inv_h = inv_h0, sig = sig_g0, h_avg = h0;
+        float inv_h = inv_h0, h_avg = h0;
         if (!UNI) {
             h_avg = 0.5f * (hi + X.p4[j].w);
             inv_h = __builtin_amdgcn_rcpf(h_avg);
-            const float ih2 = inv_h * inv_h;
-            sig = (ih2 * ih2) * (float)kInvPi;
         }
-        // selects here (measured faster than the factor form of the f64 pass)
-        const float dwr = ok ? (sig * dw_shape_f(r * inv_h)) * inv_r : 0.0f;
-        const float mm = UNI ? mm0 : mi * X.m[j];
+        const float dwr0 = (dw_shape_f(r * inv_h) * inv_r) * (ok ? 1.0f : 0.0f);
+        float dwr = dwr0;
+        if (!UNI) {
+            const float ih2 = inv_h * inv_h;
+            dwr = (mi * X.m[j]) * (((ih2 * ih2) * (float)kInvPi) * dwr0);
+        }
         if (!VISC_ONLY) {
-            const float sc = (mm * (pr2i + b.w)) * dwr;
+            const float sc = (pr2i + b.w) * dwr;
             gp[0] = fmaf(dx, sc, gp[0]);
             gp[1] = fmaf(dy, sc, gp[1]);
             gp[2] = fmaf(dz, sc, gp[2]);
@@ -641,12 +641,12 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         const float sdot = fmaf(vzi - b.z, dz, fmaf(vyi - b.y, dy, (vxi - b.x) * dx));
         const float rhoj = a.w;
         {
-            const bool act = ref_conv ? (sdot < 0.0f) : (sdot > 0.0f);
-            const float rho_bar = 0.5f * (rhoi + rhoj);
-            const float ss = ref_conv ? sdot : -sdot;
-            const float mu = (h_avg * ss) * __builtin_amdgcn_rcpf(fmaf(0.01f * h_avg, h_avg, r2));
-            const float pi_ij = fmaf(beta * mu, mu, ac * mu) * __builtin_amdgcn_rcpf(rho_bar);
-            const float sc = act ? (mm * pi_ij) * dwr : 0.0f;
+            const float act = (ref_conv ? (sdot < 0.0f) : (sdot > 0.0f)) ? 1.0f : 0.0f;
+            const float hs = (ssign * h_avg) * sdot;
+            const float den = fmaf(0.01f * h_avg, h_avg, r2);
+            const float num = hs * fmaf(beta2, hs, ac2 * den);
+            const float pi_ij = num * __builtin_amdgcn_rcpf((den * den) * (rhoi + rhoj));
+            const float sc = (pi_ij * dwr) * act;
             gv[0] = fmaf(dx, sc, gv[0]);
             gv[1] = fmaf(dy, sc, gv[1]);
             gv[2] = fmaf(dz, sc, gv[2]);
@@ -659,9 +659,9 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             const float hb2j = 0.5f * ((bxj * bxj + byj * byj) + bzj * bzj);
             const float big = (bxi * gx + byi * gy) + bzi * gz;
             const float bjg = (bxj * gx + byj * gy) + bzj * gz;
-            fm[0] += (double)(mm * ((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j));
-            fm[1] += (double)(mm * ((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j));
-            fm[2] += (double)(mm * ((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j));
+            fm[0] += (double)((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j);
+            fm[1] += (double)((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j);
+            fm[2] += (double)((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j);
         }
     }, [&]() {
         for (int a = 0; a < 3; ++a) {
@@ -670,6 +670,15 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             gp[a] = gv[a] = 0.0f;
         }
     });
+    if (UNI) {
+        const double ih2 = kp.inv_h0 * kp.inv_h0;
+        const double k = (kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi);
+        for (int a = 0; a < 3; ++a) {
+            fp[a] *= k;
+            fv[a] *= k;
+            fm[a] *= k;
+        }
+    }
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
