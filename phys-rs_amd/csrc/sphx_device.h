@@ -365,13 +365,15 @@ __device__ __forceinline__ uint4 nbr_group(const uint32_t* __restrict__ e, uint3
     return *reinterpret_cast<const uint4*>(e + ((size_t)(k >> 2) << 8));
 }
 
-// f(e) for every entry of sorted particle i, a group of four per iteration
-// (one 16-byte load) so that the bodies overlap each other's gathers.
+// f(e) for every entry of sorted particle i (entries kbeg .. kend - 1, kbeg a
+// multiple of 4), a group of four per iteration (one 16-byte load) so that
+// the bodies overlap each other's gathers.
 template <typename F>
-__device__ __forceinline__ void for_neighbours(const NbrList& L, uint32_t i, F&& f) {
+__device__ __forceinline__ void for_neighbours(const NbrList& L, uint32_t i, F&& f, uint32_t kbeg = 0,
+                                               uint32_t kend = 0xffffffffu) {
     const uint32_t* __restrict__ e = nbr_base(L, i);
-    const uint32_t cnt = L.ntot[i];
-    uint32_t k = 0;
+    const uint32_t cnt = min((uint32_t)L.ntot[i], kend);
+    uint32_t k = kbeg;
     for (; k + 3 < cnt; k += 4) {
         const uint4 q = nbr_group(e, k);
         f(q.x);
@@ -450,6 +452,7 @@ __device__ __forceinline__ uint32_t particle_index() { return xcd_block() * kBlo
 template <class Q, int NQ>
 struct GlobalFetch {
     const Q* __restrict__ glob;
+    uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries
     __device__ __forceinline__ void offset(uint32_t e, float wx, float wy, float wz, float& ox, float& oy,
                                           float& oz) const {
         entry_offset(e, wx, wy, wz, ox, oy, oz);
@@ -472,7 +475,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalF
         body(e, q);
     };
     if (std::is_same<typename std::decay<Flush>::type, NoFlush>::value)
-        for_neighbours(L, i, one);
+        for_neighbours(L, i, one, F.kbeg, F.kend);
     else
         for_neighbour_groups(L, i, one, flush);
 }
@@ -619,6 +622,7 @@ struct BrickFetch {
     const SPHX_LDS lds_quad_t* code;
     const Q* __restrict__ glob;
     uint32_t base, hy, cap;
+    uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries (kbeg a multiple of 4)
     // LDS record slot of entry e; false if it is not staged
     __device__ __forceinline__ bool locate(uint32_t e, uint32_t& slot) const {
         const uint32_t j = e & kIdxMask;
@@ -663,7 +667,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
     static_assert(W == 4 || std::is_same<typename std::decay<Flush>::type, NoFlush>::value,
                   "groups of four entries need four entries per iteration");
     const uint32_t* __restrict__ e = nbr_base(L, i);
-    const uint32_t cnt = L.ntot[i];
+    const uint32_t cnt = min((uint32_t)L.ntot[i], F.kend);
     // W entries of the list from position k (k a multiple of W): one 16- or
     // 8-byte load
     auto load = [&](uint32_t k, uint32_t (&c)[W]) {
@@ -676,8 +680,8 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
         }
     };
     uint32_t a[W];
-    if (PF && cnt > 0) load(0, a);
-    uint32_t k = 0;
+    uint32_t k = F.kbeg;
+    if (PF && cnt > k) load(k, a);
     for (; k + (W - 1) < cnt; k += W) {
         uint32_t c[W], slot[W];
         bool staged = true;
@@ -728,7 +732,8 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
 // The workgroup's brick (one per active brick, dealt so that each XCD takes a
 // contiguous range, see xcd_block): set up, stage the records of `src` (NQ
 // quads each), then body(i, fetch) for each of the brick's particles.
-template <class Q, int NQ, int W, bool PF, class Body>
+// LANES = 2: two neighbouring lanes share each particle (body sees both).
+template <class Q, int NQ, int W, bool PF, int LANES = 1, class Body>
 __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const uint32_t* __restrict__ start,
                                            uint32_t cap, const Q* __restrict__ src, BrickHdr& H,
                                            uint4* __restrict__ sh, Body&& body) {
@@ -741,7 +746,7 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
                                               (const SPHX_LDS lds_quad_t*)HL->code, src, 0u, hy, cap};
     const uint32_t nown = HL->own_cum[no];
     uint32_t k = 0;
-    for (uint32_t u = threadIdx.x; u < nown; u += blockDim.x) {
+    for (uint32_t u = threadIdx.x / LANES; u < nown; u += blockDim.x / LANES) {
         while (u >= HL->own_cum[k + 1]) ++k;
         BrickFetch<Q, NQ, W, PF> f = base_fetch;
         f.base = k % BY + hy * (k / BY);
