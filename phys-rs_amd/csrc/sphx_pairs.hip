@@ -84,6 +84,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
     }
     const float skin_f = (float)skin;
     const bool trim = !exact_all;
+    const bool any_exact = __any(exact_all) != 0;  // wave-uniform
     const float wx = (float)g.w[0];
     uint32_t total = 0;
     uint32_t* __restrict__ out = nbr_base(L, p);
@@ -101,23 +102,25 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         const bool merged = (e0 || rb[0] == ra[1]) && (e2 || rb[1] == ra[2]);
         const bool single = st.total <= (uint32_t)kStageCap;
         // candidate test of slot (cell x offset o - 1); u = slot - cb
-        auto test = [&](uint32_t slot, uint32_t u, uint32_t o) {
-            if (slot == self) return;
+        // EX: some lane of the wave tests in f64 (exact_all), decided once per
+        // wave so that the usual f32-only loop has no branch for it
+        auto test = [&](uint32_t slot, uint32_t u, uint32_t o, auto ex) {
             // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
             const float dx = (lds[0][u] - xf) + (float)((int)o - 1) * wx, dy = (lds[1][u] - yf) + st.oy,
                         dz = (lds[2][u] - zf) + st.oz;
             const float tt = (UNI ? hf + hf : hf + lds[3][u]) + skin_f;  // 2 h_avg + skin
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            bool in;
-            if (!exact_all) {
-                in = r2 < (tt * tt) * (1.0f + 1e-5f);
-            } else {
-                const uint32_t j = slot_particle(st, slot);
-                double ddx, ddy, ddz;
-                pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], ddx, ddy, ddz);
-                const double t = (hi + P.h[j]) + skin;
-                in = (ddx * ddx + ddy * ddy) + ddz * ddz < (t * t) * (1.0 + 1e-12);
+            bool in = r2 < (tt * tt) * (1.0f + 1e-5f);
+            if constexpr (decltype(ex)::value) {
+                if (exact_all) {
+                    const uint32_t j = slot_particle(st, slot);
+                    double ddx, ddy, ddz;
+                    pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], ddx, ddy, ddz);
+                    const double t = (hi + P.h[j]) + skin;
+                    in = (ddx * ddx + ddy * ddy) + ddz * ddz < (t * t) * (1.0 + 1e-12);
+                }
             }
+            in = in && slot != self;
             if (in) {
                 if (total < (uint32_t)L.ks) {
                     b0 = b1;
@@ -167,29 +170,33 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             });
             __syncthreads();
             if (!valid) continue;
-            if (merged) {
-                uint32_t A = e0 ? ra[1] : ra[0], E = e2 ? rb[1] : rb[2];
-                if (trim && single) {
-                    if (!e0) A = cut(ra[0], rb[0], -wx, true);
-                    if (!e2) E = cut(ra[2], rb[2], wx, false);
-                }
-                A = max(A, cb);
-                E = min(E, ce);
-                for (uint32_t slot = A; slot < E; ++slot)
-                    test(slot, slot - cb, slot < ra[1] ? 0u : (slot < rb[1] ? 1u : 2u));
-            } else {
-                for (int q = 0; q < 3; ++q) {
-                    uint32_t a = qa[q], e = qb[q];
-                    const uint32_t o = qo[q];
-                    if (trim && single && o != 1u && a < e) {
-                        if (o == 0u) a = cut(a, e, -wx, true);
-                        else e = cut(a, e, wx, false);
+            auto run = [&](auto ex) {
+                if (merged) {
+                    uint32_t A = e0 ? ra[1] : ra[0], E = e2 ? rb[1] : rb[2];
+                    if (trim && single) {
+                        if (!e0) A = cut(ra[0], rb[0], -wx, true);
+                        if (!e2) E = cut(ra[2], rb[2], wx, false);
                     }
-                    a = max(a, cb);
-                    e = min(e, ce);
-                    for (uint32_t slot = a; slot < e; ++slot) test(slot, slot - cb, o);
+                    A = max(A, cb);
+                    E = min(E, ce);
+                    for (uint32_t slot = A; slot < E; ++slot)
+                        test(slot, slot - cb, slot < ra[1] ? 0u : (slot < rb[1] ? 1u : 2u), ex);
+                } else {
+                    for (int q = 0; q < 3; ++q) {
+                        uint32_t a = qa[q], e = qb[q];
+                        const uint32_t o = qo[q];
+                        if (trim && single && o != 1u && a < e) {
+                            if (o == 0u) a = cut(a, e, -wx, true);
+                            else e = cut(a, e, wx, false);
+                        }
+                        a = max(a, cb);
+                        e = min(e, ce);
+                        for (uint32_t slot = a; slot < e; ++slot) test(slot, slot - cb, o, ex);
+                    }
                 }
-            }
+            };
+            if (any_exact) run(std::true_type{});
+            else run(std::false_type{});
         }
     }
     if (valid) {
