@@ -159,6 +159,13 @@ __device__ __forceinline__ double w_shape(double q) {
     const double b = 0.25 * ((t * t) * t);
     return q < 1.0 ? a : b;
 }
+// 4 w_shape(q), one multiply fewer; the scaling by 4 is exact, so a sum of
+// these times 0.25 is bitwise the sum of w_shape
+__device__ __forceinline__ double w4_shape(double q) {
+    const double t = fmax(2.0 - q, 0.0);
+    const double a = fma(q * q, fma(3.0, q, -6.0), 4.0);
+    return q < 1.0 ? a : (t * t) * t;
+}
 __device__ __forceinline__ double dw_shape(double q) {
     const double t = fmax(2.0 - q, 0.0);
     const double a = q * fma(2.25, q, -3.0);
@@ -170,6 +177,17 @@ __device__ __forceinline__ float w_shape_f(float q) {
     const float a = fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
     const float b = 0.25f * ((t * t) * t);
     return q < 1.0f ? a : b;
+}
+__device__ __forceinline__ float w4_shape_f(float q) {
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float a = fmaf(q * q, fmaf(3.0f, q, -6.0f), 4.0f);
+    return q < 1.0f ? a : (t * t) * t;
+}
+// dw_shape(q) / 0.75 (the negation of t^2 is free)
+__device__ __forceinline__ float dw43_shape_f(float q) {
+    const float t = fmaxf(2.0f - q, 0.0f);
+    const float a = q * fmaf(3.0f, q, -4.0f);
+    return q < 1.0f ? a : -(t * t);
 }
 __device__ __forceinline__ float dw_shape_f(float q) {
     const float t = fmaxf(2.0f - q, 0.0f);

@@ -269,13 +269,13 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         sqrt_rsqrt(r2 > 0.0 ? r2 : 1.0, r, inv_r);
         r = r2 > 0.0 ? r : 0.0;
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : zh.y);
-        if (UNI) {  // m and sigma_w are folded out of the sum
-            acc += w_shape(r * ph.inv_h);
+        if (UNI) {  // m, sigma_w and the 1/4 of w4_shape are folded out of the sum
+            acc += w4_shape(r * ph.inv_h);
         } else {
             acc = fma(P.m[e & kIdxMask], ph.sig_w * w_shape(r * ph.inv_h), acc);
         }
     });
-    rho = UNI ? fma(kp.m0 * (((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi), acc, rho) : rho + acc;
+    rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 0.25), acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
@@ -329,18 +329,18 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
         const float dy = (q[0].y - yi) + oy;
         const float dz = (q[0].z - zi) + oz;
         const float r = __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-        if (UNI) {  // m and sigma_w are folded out of the sum
-            gacc += w_shape_f(r * inv_h0);
+        if (UNI) {  // m, sigma_w and the 1/4 of w4_shape are folded out of the sum
+            gacc += w4_shape_f(r * inv_h0);
         } else {
             const float inv_h = __builtin_amdgcn_rcpf(0.5f * (hi + q[0].w));
-            const float w = ((inv_h * inv_h) * inv_h * (float)kInvPi) * w_shape_f(r * inv_h);
+            const float w = ((inv_h * inv_h) * inv_h * (float)(0.25 * kInvPi)) * w4_shape_f(r * inv_h);
             gacc += X.m[e & kIdxMask] * w;
         }
     }, [&]() {
         acc += (double)gacc;
         gacc = 0.0f;
     });
-    rho = UNI ? fma(kp.m0 * (((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi), acc, rho) : rho + acc;
+    rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 0.25), acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
@@ -503,6 +503,9 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
     // s = (v_i - v_j).(x_j - x_i) in the REFERENCE convention, its negative in MONAGHAN's
     const double ssign = ref_conv ? 1.0 : -1.0;
+    // s clamped to [s_lo, s_hi] is 0 when the pair has no viscosity term
+    // (REFERENCE: active for s < 0; MONAGHAN: for s > 0), so hs = 0 and Pi = 0
+    const double s_lo = ref_conv ? -INFINITY : 0.0, s_hi = ref_conv ? 0.0 : INFINITY;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     // The pair body is branch-free: a pair without a term adds +-0, which
@@ -533,12 +536,11 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         const double sdot = fma(vzi - vzp.x, dz, fma(vyi - vxy.y, dy, (vxi - vxy.x) * dx));
         const double rhoj = zr.y;
         {
-            const bool act = ref_conv ? (sdot < 0.0) : (sdot > 0.0);
-            const double hs = (ssign * ph.h_avg) * sdot;
+            const double hs = (ssign * ph.h_avg) * fmin(fmax(sdot, s_lo), s_hi);
             const double den = fma(0.01 * ph.h_avg, ph.h_avg, r2);
             const double num = hs * fma(beta2, hs, ac2 * den);
             const double pi_ij = num * rcp_nr((den * den) * (rhoi + rhoj));
-            const double sc = (pi_ij * dwr) * (act ? 1.0 : 0.0);
+            const double sc = pi_ij * dwr;
             fv[0] = fma(dx, sc, fv[0]);
             fv[1] = fma(dy, sc, fv[1]);
             fv[2] = fma(dz, sc, fv[2]);
@@ -647,6 +649,8 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     const float ac2 = (float)(-2.0 * kp.alpha * kp.speed_of_sound), beta2 = (float)(2.0 * kp.beta);
     const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
     const float ssign = ref_conv ? 1.0f : -1.0f;
+    // s clamped to [s_lo, s_hi] (one med3) is 0 for a pair without a viscosity term
+    const float s_lo = ref_conv ? -INFINITY : 0.0f, s_hi = ref_conv ? 0.0f : INFINITY;
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     // f32 sums over each group of four list entries, added to the f64 sums
@@ -662,19 +666,19 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         const float dy = (a.y - yi) + oy;
         const float dz = (a.z - zi) + oz;
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const bool ok = r2 > 1e-20f;
-        const float inv_r = __builtin_amdgcn_rsqf(ok ? r2 : 1.0f);
+        // r <= 1e-10: no force; 1/r = 0 gives r = 0, q = 0 and dW/dq(0) = 0
+        const float inv_r = r2 > 1e-20f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
         const float r = r2 * inv_r;
         float inv_h = inv_h0, h_avg = h0;
         if (!UNI) {
             h_avg = 0.5f * (hi + X.p4[j].w);
             inv_h = __builtin_amdgcn_rcpf(h_avg);
         }
-        const float dwr0 = (dw_shape_f(r * inv_h) * inv_r) * (ok ? 1.0f : 0.0f);
+        const float dwr0 = dw43_shape_f(r * inv_h) * inv_r;  // its 0.75 is folded in below
         float dwr = dwr0;
         if (!UNI) {
             const float ih2 = inv_h * inv_h;
-            dwr = (mi * X.m[j]) * (((ih2 * ih2) * (float)kInvPi) * dwr0);
+            dwr = (mi * X.m[j]) * (((ih2 * ih2) * (float)(0.75 * kInvPi)) * dwr0);
         }
         if (!VISC_ONLY) {
             const float sc = (pr2i + b.w) * dwr;
@@ -685,12 +689,11 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         const float sdot = fmaf(vzi - b.z, dz, fmaf(vyi - b.y, dy, (vxi - b.x) * dx));
         const float rhoj = a.w;
         {
-            const float act = (ref_conv ? (sdot < 0.0f) : (sdot > 0.0f)) ? 1.0f : 0.0f;
-            const float hs = (ssign * h_avg) * sdot;
+            const float hs = (ssign * h_avg) * __builtin_amdgcn_fmed3f(sdot, s_lo, s_hi);
             const float den = fmaf(0.01f * h_avg, h_avg, r2);
             const float num = hs * fmaf(beta2, hs, ac2 * den);
             const float pi_ij = num * __builtin_amdgcn_rcpf((den * den) * (rhoi + rhoj));
-            const float sc = (pi_ij * dwr) * act;
+            const float sc = pi_ij * dwr;
             gv[0] = fmaf(dx, sc, gv[0]);
             gv[1] = fmaf(dy, sc, gv[1]);
             gv[2] = fmaf(dz, sc, gv[2]);
@@ -716,7 +719,7 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     });
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
-        const double k = (kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi);
+        const double k = ((kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi)) * 0.75;
         for (int a = 0; a < 3; ++a) {
             fp[a] *= k;
             fv[a] *= k;
