@@ -503,8 +503,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalF
 // =========================================================================
 struct BrickHdr {
     uint4 halo[kMaxHaloRows];           // per halo row: {first sorted index, staged count, LDS record base, 0}
-    uint4 code[27];                     // per entry cell code: {ox, oy, oz (f32 bits, times the cell widths),
-                                        // halo-row step sy + (b[1] + 2) sz}
+    uint4 code[27];                     // per entry cell code: {ox, oy, oz (f32 bits, times the cell widths), 0}
     uint32_t own_start[kMaxOwnRows];    // per brick row: its first sorted particle
     uint32_t own_cum[kMaxOwnRows + 1];  // exclusive prefix of the brick rows' particle counts
 };
@@ -557,8 +556,7 @@ __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const 
         const uint32_t c = t - 128u, st = c / 3u, sy = st % 3u, sz = st / 3u;
         const float ox = (float)((int)(c - 3u * st) - 1) * (float)g.w[0];
         const float oy = (float)((int)sy - 1) * (float)g.w[1], oz = (float)((int)sz - 1) * (float)g.w[2];
-        H.code[c] = make_uint4(__float_as_uint(ox), __float_as_uint(oy), __float_as_uint(oz),
-                               sy + (uint32_t)(BY + 2) * sz);
+        H.code[c] = make_uint4(__float_as_uint(ox), __float_as_uint(oy), __float_as_uint(oz), 0u);
     }
     if (t >= 64 && t < 128) {
         const int no = BY * BZ;
@@ -585,14 +583,49 @@ __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const 
     __syncthreads();
 }
 
+// LDS pointers carry their address space so that reads through them are
+// ds_read (a generic pointer would turn them into flat loads).
+#define SPHX_LDS __attribute__((address_space(3)))
+// 16-byte LDS word, read as a builtin vector (HIP's vector classes cannot be
+// copied out of an address-space-qualified object)
+typedef uint32_t lds_quad_t __attribute__((ext_vector_type(4)));
+template <class Q>
+__device__ __forceinline__ Q lds_quad(const SPHX_LDS lds_quad_t* p) {
+    const lds_quad_t v = *p;
+    Q q;
+    __builtin_memcpy(&q, &v, sizeof q);
+    return q;
+}
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(size_t)(const SPHX_LDS void*)p;
+}
+
+// Row-code table: one 16-byte entry per (brick row k, entry code c), at
+// rc[k * 27 + c], for list entries e = j | c << kIdxBits of the particles of
+// brick row k.  Entry {gs | c << kIdxBits, staged, adj, 0} of the halo row the
+// code points to (first sorted index gs, `staged` records from LDS slot lb):
+// e is staged when e - (gs | c << kIdxBits) = j - gs < staged, and quad 0 of
+// its record is at LDS byte address e * 16 + adj = rec + (lb + j - gs) * 16
+// (adj = rec + (lb - gs) * 16 - (c << 31), all modulo 2^32).
+static constexpr int kRowCodes = 27;
+
 // Copy the staged records (NQ 16-byte quads each) into LDS, quad-major: quad
 // k of staged record r at dst[k * cap + r], so that one quad of 64 random
 // records spreads over all LDS banks (record-major, a 64-byte record stride
 // would put quad k of every record in the same 16 of the 64 banks).  Wave w
-// copies halo rows w, w + waves, ..., each one contiguous range.
+// copies halo rows w, w + waves, ..., each one contiguous range.  The
+// threads also fill the row-code table rc of the `no` brick rows.
 template <int NQ>
 __device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uint4* __restrict__ src,
-                                            uint4* __restrict__ dst, uint32_t cap) {
+                                            uint4* __restrict__ dst, uint32_t cap, uint4* __restrict__ rc,
+                                            uint32_t no, uint32_t by, uint32_t hy) {
+    for (uint32_t t = threadIdx.x; t < no * (uint32_t)kRowCodes; t += blockDim.x) {
+        const uint32_t k = t / (uint32_t)kRowCodes, c = t - k * (uint32_t)kRowCodes;
+        const uint32_t st = c / 3u, sy = st % 3u, sz = st / 3u;
+        const uint4 h = H.halo[(k % by + sy) + hy * (k / by + sz)];
+        const uint32_t adj = lds_addr(dst) + (h.z - h.x) * 16u - (c << 31);
+        rc[t] = make_uint4(h.x | (c << kIdxBits), h.y, adj, 0u);
+    }
     const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
     for (int r = w; r < nh; r += nw) {
@@ -613,41 +646,23 @@ __device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uin
     __syncthreads();
 }
 
-// LDS pointers carry their address space so that reads through them are
-// ds_read (a generic pointer would turn them into flat loads).
-#define SPHX_LDS __attribute__((address_space(3)))
-// 16-byte LDS word, read as a builtin vector (HIP's vector classes cannot be
-// copied out of an address-space-qualified object)
-typedef uint32_t lds_quad_t __attribute__((ext_vector_type(4)));
-template <class Q>
-__device__ __forceinline__ Q lds_quad(const SPHX_LDS lds_quad_t* p) {
-    const lds_quad_t v = *p;
-    Q q;
-    __builtin_memcpy(&q, &v, sizeof q);
-    return q;
-}
-
-// Record of list entry e of a particle in brick row (ly, lz), base = ly + hy * lz
-// with hy = b[1] + 2: the neighbour lies in halo row base + sy + hy * sz, where
-// (sy, sz) = (oy + 1, oz + 1) is the entry's stage; from LDS when staged.
+// Record of list entry e of a particle in brick row k: the row-code table of
+// row k (rc) gives its LDS address when it is staged.
 template <class Q, int NQ, int W_ = 2, bool PF_ = false>
 struct BrickFetch {
     static_assert(sizeof(Q) == 16, "records are made of 16-byte quads");
     static constexpr int W = W_;
     static constexpr bool PF = PF_;
-    const SPHX_LDS lds_quad_t* lds;
-    const SPHX_LDS lds_quad_t* halo;
-    const SPHX_LDS lds_quad_t* code;
+    const SPHX_LDS lds_quad_t* rc;    // the row-code table of the particle's brick row
+    const SPHX_LDS lds_quad_t* code;  // per entry code: the cell offset
     const Q* __restrict__ glob;
-    uint32_t base, hy, cap;
+    uint32_t qstride;                 // bytes from quad k to quad k + 1 of a record (16 cap)
     uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries (kbeg a multiple of 4)
-    // LDS record slot of entry e; false if it is not staged
-    __device__ __forceinline__ bool locate(uint32_t e, uint32_t& slot) const {
-        const uint32_t j = e & kIdxMask;
-        const lds_quad_t h = halo[base + code[e >> kIdxBits].w];
-        const uint32_t d = j - h.x;
-        slot = h.z + d;
-        return d < h.y;
+    // LDS byte address of entry e's record; false if it is not staged
+    __device__ __forceinline__ bool locate(uint32_t e, uint32_t& addr) const {
+        const lds_quad_t h = rc[e >> kIdxBits];
+        addr = e * 16u + h.z;
+        return e - h.x < h.y;
     }
     // cell offset of entry e's neighbour times the cell widths (as entry_offset)
     __device__ __forceinline__ void offset(uint32_t e, float, float, float, float& ox, float& oy, float& oz) const {
@@ -656,14 +671,15 @@ struct BrickFetch {
         oy = __uint_as_float(t.y);
         oz = __uint_as_float(t.z);
     }
-    __device__ __forceinline__ void from_lds(uint32_t slot, Q (&q)[NQ]) const {
+    __device__ __forceinline__ void from_lds(uint32_t addr, Q (&q)[NQ]) const {
 #pragma unroll
-        for (int k = 0; k < NQ; ++k) q[k] = lds_quad<Q>(lds + slot + k * cap);
+        for (int k = 0; k < NQ; ++k)
+            q[k] = lds_quad<Q>((const SPHX_LDS lds_quad_t*)(size_t)(addr + (uint32_t)k * qstride));
     }
     __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NQ]) const {
-        uint32_t slot;
-        if (locate(e, slot)) {
-            from_lds(slot, q);
+        uint32_t addr;
+        if (locate(e, addr)) {
+            from_lds(addr, q);
         } else {
             const Q* p = glob + (size_t)(e & kIdxMask) * NQ;
 #pragma unroll
@@ -758,16 +774,19 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
     const uint32_t hy = (uint32_t)B.b[1] + 2u, nh = hy * ((uint32_t)B.b[2] + 2u);
     const uint32_t BY = (uint32_t)B.b[1], no = BY * (uint32_t)B.b[2];
     const SPHX_LDS BrickHdr* HL = (const SPHX_LDS BrickHdr*)&H;
+    // dynamic LDS: the row-code table (brick_rc_quads), then the records
+    uint4* rc = sh;
+    uint4* rec = sh + brick_rc_quads(B);
     brick_setup(B.active[xcd_block()], B, g, start, cap, H);
-    brick_stage<NQ>(H, (int)nh, reinterpret_cast<const uint4*>(src), sh, cap);
-    const BrickFetch<Q, NQ, W, PF> base_fetch{(const SPHX_LDS lds_quad_t*)sh, (const SPHX_LDS lds_quad_t*)HL->halo,
-                                              (const SPHX_LDS lds_quad_t*)HL->code, src, 0u, hy, cap};
+    brick_stage<NQ>(H, (int)nh, reinterpret_cast<const uint4*>(src), rec, cap, rc, no, BY, hy);
+    const BrickFetch<Q, NQ, W, PF> base_fetch{(const SPHX_LDS lds_quad_t*)rc, (const SPHX_LDS lds_quad_t*)HL->code,
+                                              src, cap * 16u};
     const uint32_t nown = HL->own_cum[no];
     uint32_t k = 0;
     for (uint32_t u = threadIdx.x / LANES; u < nown; u += blockDim.x / LANES) {
         while (u >= HL->own_cum[k + 1]) ++k;
         BrickFetch<Q, NQ, W, PF> f = base_fetch;
-        f.base = k % BY + hy * (k / BY);
+        f.rc += k * (uint32_t)kRowCodes;
         body(HL->own_start[k] + (u - HL->own_cum[k]), f);
     }
 }

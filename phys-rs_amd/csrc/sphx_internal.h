@@ -141,6 +141,11 @@ struct Bricks {
     uint32_t nactive;          // active bricks of the current sort (host copy): one workgroup each
     double ppc;                // mean particles per non-empty cell at upload (LDS sizing)
 };
+// 16-byte words of dynamic LDS the staged passes keep before the records: the
+// row-code table, 27 entries per brick row
+__host__ __device__ inline uint32_t brick_rc_quads(const Bricks& B) {
+    return (uint32_t)(B.b[1] * B.b[2]) * 27u;
+}
 
 // f32 copies in sorted order: p4 = (xr, yr, zr, h), the position as an offset
 // from the particle's own cell origin (computed in f64, rounded once), so a

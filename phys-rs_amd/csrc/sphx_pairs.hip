@@ -390,12 +390,13 @@ struct StagedLaunch {
 static StagedLaunch staged_launch(const Bricks& B, int max_waves, size_t rec_bytes) {
     StagedLaunch l;
     const double halo = B.ppc * (B.b[0] + 2) * (B.b[1] + 2) * (B.b[2] + 2);
-    const size_t fixed = sizeof(BrickHdr) + 512;
+    const size_t rc = (size_t)brick_rc_quads(B) * 16u;  // the row-code table (dynamic LDS)
+    const size_t fixed = sizeof(BrickHdr) + 512 + rc;
     int per_cu = std::min(max_waves / (B.threads / 64), (int)((double)kLdsPerCU / (halo * rec_bytes + fixed)));
     per_cu = std::max(per_cu, 1);
     const size_t per_block = kLdsPerCU / (size_t)per_cu - fixed;
     l.cap = (uint32_t)(per_block / rec_bytes);
-    l.dyn = (size_t)l.cap * rec_bytes;
+    l.dyn = (size_t)l.cap * rec_bytes + rc;
     l.grid = dim3(B.nactive);
     l.block = dim3((uint32_t)B.threads);
     return l;
