@@ -74,9 +74,12 @@ struct SphxCtx {
     Sort S{};
     Tiles T{};
     Bricks B{};
-    uint32_t* bricks_host = nullptr;       // pinned: active brick count of the last sort
+    uint32_t* bricks_host = nullptr;       // pinned: [0] active brick count of the last sort,
+                                           // [1] the moved-skin/2 flag of the last position update
     hipEvent_t bricks_ready = nullptr;     // recorded after that count's copy
     bool bricks_pending = false;           // count not read yet
+    hipEvent_t moved_ready = nullptr;      // recorded after the moved flag's copy
+    bool moved_pending = false;            // flag not read yet (passes enqueued on the kept list)
     NbrList L{};
     Mixed X{};
     double* energy_partial = nullptr;
@@ -530,7 +533,9 @@ int do_forces(SphxCtx* c, bool visc_only) {
 
 // Position update; the neighbour list stays valid while no particle has
 // moved more than skin/2 since it was built (the kernel raises a flag).
-int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr) {
+// defer: the flag travels to pinned memory and is read by settle_moved, so
+// that passes can be enqueued on the kept list before the host knows.
+int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr, bool defer = false) {
     if (!c->acc) return fail(c, SPHX_E_STATE, "vv_position needs accelerations (call sphx_forces)");
     {
         Timer t(c, kIntegrate);
@@ -542,12 +547,32 @@ int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr) {
     c->dens = c->sp = c->acc = false;
     bool rebuild = c->kp.skin <= 0.0 || !c->listed;
     if (!rebuild) {
-        uint32_t moved = 0;
-        HIP_TRY(c, hipMemcpyAsync(&moved, c->L.overflow + 2, sizeof moved, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        rebuild = moved != 0;
+        HIP_TRY(c, hipMemcpyAsync(c->bricks_host + 1, c->L.overflow + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIP_TRY(c, hipEventRecord(c->moved_ready, c->stream));
+        if (defer) {
+            c->moved_pending = true;
+            return SPHX_OK;
+        }
+        HIP_TRY(c, hipEventSynchronize(c->moved_ready));
+        rebuild = c->bricks_host[1] != 0;
     }
     if (rebuild) c->sorted = c->listed = false;
+    return SPHX_OK;
+}
+
+// Reads the flag of a deferred position update.  If a particle moved more
+// than skin/2, what was computed on the kept list since is discarded (redo).
+int settle_moved(SphxCtx* c, bool& redo) {
+    redo = false;
+    if (!c->moved_pending) return SPHX_OK;
+    c->moved_pending = false;
+    HIP_TRY(c, hipEventSynchronize(c->moved_ready));
+    if (c->bricks_host[1] != 0) {
+        c->sorted = c->listed = false;
+        c->dens = c->sp = c->acc = false;
+        redo = true;
+    }
     return SPHX_OK;
 }
 
@@ -780,6 +805,9 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     }
     auto bail = [&](int code) {
         free_all(c);
+        if (c->bricks_ready) (void)hipEventDestroy(c->bricks_ready);
+        if (c->moved_ready) (void)hipEventDestroy(c->moved_ready);
+        if (c->bricks_host) (void)hipHostFree(c->bricks_host);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return code;
@@ -789,14 +817,16 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     if (hipDeviceGetAttribute(&c->B.cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return bail(SPHX_E_HIP);
     brick_defaults(c);
-    if (hipHostMalloc((void**)&c->bricks_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&c->bricks_host, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         c->bricks_host = nullptr;
         return bail(SPHX_E_OOM);
     }
     if (hipEventCreateWithFlags(&c->bricks_ready, hipEventDisableTiming) != hipSuccess) {
         c->bricks_ready = nullptr;
-        (void)hipHostFree(c->bricks_host);
-        c->bricks_host = nullptr;
+        return bail(SPHX_E_HIP);
+    }
+    if (hipEventCreateWithFlags(&c->moved_ready, hipEventDisableTiming) != hipSuccess) {
+        c->moved_ready = nullptr;
         return bail(SPHX_E_HIP);
     }
     const size_t N = c->cap;
@@ -845,6 +875,7 @@ void sphx_destroy(SphxCtx* c) {
     }
     for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
     if (c->bricks_ready) (void)hipEventDestroy(c->bricks_ready);
+    if (c->moved_ready) (void)hipEventDestroy(c->moved_ready);
     if (c->bricks_host) (void)hipHostFree(c->bricks_host);
     c->adj.release();
     free_all(c);
@@ -1010,10 +1041,21 @@ int sphx_step(SphxCtx* c, uint32_t nsteps) {
         // a_n = A(x_n, v_n): the pressure sum at x_n is still valid from the
         // previous tick's second evaluation unless positions were replaced.
         if ((rc = do_forces(c, c->sp && c->dens))) return rc;
-        if ((rc = do_vv_position(c))) return rc;
-        if ((rc = ensure_list(c))) return rc;
-        if ((rc = do_density(c))) return rc;
-        if ((rc = do_forces(c, false))) return rc;  // a* = A(x_{n+1}, v_n)
+        // While the list is kept, density and the second force pass are
+        // enqueued before the host has read the moved flag, so the GPU does
+        // not wait for the host; they only write derived arrays, and if a
+        // particle did move skin/2 they are redone on the rebuilt list.
+        if ((rc = do_vv_position(c, nullptr, true))) return rc;
+        for (bool redo = true; redo;) {
+            if ((rc = ensure_list(c)) || (rc = do_density(c)) ||
+                (rc = do_forces(c, false))) {  // a* = A(x_{n+1}, v_n)
+                c->moved_pending = false;
+                c->sorted = c->listed = false;
+                c->dens = c->sp = c->acc = false;
+                return rc;
+            }
+            if ((rc = settle_moved(c, redo))) return rc;
+        }
         if ((rc = do_vv_velocity(c))) return rc;
         if (c->prm.xsph_epsilon != 0.0 && (rc = do_xsph(c))) return rc;
         if (c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_INDUCTION) && (rc = do_induction(c))) return rc;
