@@ -62,9 +62,9 @@ STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
 # 256 MiB copy) of the FP64 kernels on this workload, from the rocprofv3
 # counter runs summarised in profiles/dam_break_1M_fp64_mixed_kernels.md.
 MEASURED_TRAFFIC = {
-    "force": (562.1 + 50.8) * 1e6,
-    "visc_only": (578.7 + 25.4) * 1e6,
-    "density": (481.1 + 92.9) * 1e6,
+    "force": (559.7 + 50.8) * 1e6,
+    "visc_only": (577.0 + 25.4) * 1e6,
+    "density": (472.0 + 92.8) * 1e6,
 }
 STEP_FLOPS_PER_PAIR = 26.0 + 58.0 + 45.0
 
