@@ -466,29 +466,38 @@ __device__ __forceinline__ void entry_offset(uint32_t e, float wx, float wy, flo
 // Sorted particle index of the calling thread in a one-thread-per-particle pass.
 __device__ __forceinline__ uint32_t particle_index() { return xcd_block() * kBlock + threadIdx.x; }
 
-// Gather record of list entry e: NQ 16-byte quads of type Q at record j.
-template <class Q, int NQ>
+// Gather record of list entry e: NQ 16-byte quads of type Q at record j of
+// glob, then (NQ2 > 0) NQ2 quads at record j of glob2.
+template <class Q, int NQ, int NQ2 = 0>
 struct GlobalFetch {
+    static constexpr int NR = NQ + NQ2;  // quads per record: NQ from glob, then NQ2 from glob2
     const Q* __restrict__ glob;
+    const Q* __restrict__ glob2 = nullptr;
     uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries
     __device__ __forceinline__ void offset(uint32_t e, float wx, float wy, float wz, float& ox, float& oy,
                                           float& oz) const {
         entry_offset(e, wx, wy, wz, ox, oy, oz);
     }
-    __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NQ]) const {
-        const Q* p = glob + (size_t)(e & kIdxMask) * NQ;
+    __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NR]) const {
+        const size_t j = e & kIdxMask;
+        const Q* p = glob + j * NQ;
 #pragma unroll
         for (int k = 0; k < NQ; ++k) q[k] = p[k];
+        if constexpr (NQ2 > 0) {
+            const Q* p2 = glob2 + j * NQ2;
+#pragma unroll
+            for (int k = 0; k < NQ2; ++k) q[NQ + k] = p2[k];
+        }
     }
 };
 
 // body(e, record) for every list entry of sorted particle i, records from
 // global memory.
-template <class Q, int NQ, class Body, class Flush = NoFlush>
-__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalFetch<Q, NQ>& F, Body&& body,
+template <class Q, int NQ, int NQ2, class Body, class Flush = NoFlush>
+__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalFetch<Q, NQ, NQ2>& F, Body&& body,
                                      Flush&& flush = Flush()) {
     auto one = [&](uint32_t e) {
-        Q q[NQ];
+        Q q[NQ + NQ2];
         F(e, q);
         body(e, q);
     };
@@ -614,18 +623,12 @@ static constexpr int kRowCodes = 27;
 // records spreads over all LDS banks (record-major, a 64-byte record stride
 // would put quad k of every record in the same 16 of the 64 banks).  Wave w
 // copies halo rows w, w + waves, ..., each one contiguous range.  The
-// threads also fill the row-code table rc of the `no` brick rows.
+// threads also fill the row-code table rc of the `no` brick rows.  With NQ2
+// > 0 a second array of NQ2 quads per record (src2) is staged as quads NQ ..
+// NQ + NQ2 - 1 of the same records.
 template <int NQ>
-__device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uint4* __restrict__ src,
-                                            uint4* __restrict__ dst, uint32_t cap, uint4* __restrict__ rc,
-                                            uint32_t no, uint32_t by, uint32_t hy) {
-    for (uint32_t t = threadIdx.x; t < no * (uint32_t)kRowCodes; t += blockDim.x) {
-        const uint32_t k = t / (uint32_t)kRowCodes, c = t - k * (uint32_t)kRowCodes;
-        const uint32_t st = c / 3u, sy = st % 3u, sz = st / 3u;
-        const uint4 h = H.halo[(k % by + sy) + hy * (k / by + sz)];
-        const uint32_t adj = lds_addr(dst) + (h.z - h.x) * 16u - (c << 31);
-        rc[t] = make_uint4(h.x | (c << kIdxBits), h.y, adj, 0u);
-    }
+__device__ __forceinline__ void brick_copy(const BrickHdr& H, int nh, const uint4* __restrict__ src,
+                                           uint4* __restrict__ dst, uint32_t cap) {
     const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
     for (int r = w; r < nh; r += nw) {
@@ -643,19 +646,35 @@ __device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uin
         }
         for (; q < nq; q += 64u) d[(q % NQ) * cap + q / NQ] = s[q];
     }
+}
+template <int NQ, int NQ2 = 0>
+__device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uint4* __restrict__ src,
+                                            const uint4* __restrict__ src2, uint4* __restrict__ dst, uint32_t cap,
+                                            uint4* __restrict__ rc, uint32_t no, uint32_t by, uint32_t hy) {
+    for (uint32_t t = threadIdx.x; t < no * (uint32_t)kRowCodes; t += blockDim.x) {
+        const uint32_t k = t / (uint32_t)kRowCodes, c = t - k * (uint32_t)kRowCodes;
+        const uint32_t st = c / 3u, sy = st % 3u, sz = st / 3u;
+        const uint4 h = H.halo[(k % by + sy) + hy * (k / by + sz)];
+        const uint32_t adj = lds_addr(dst) + (h.z - h.x) * 16u - (c << 31);
+        rc[t] = make_uint4(h.x | (c << kIdxBits), h.y, adj, 0u);
+    }
+    brick_copy<NQ>(H, nh, src, dst, cap);
+    if constexpr (NQ2 > 0) brick_copy<NQ2>(H, nh, src2, dst + (size_t)NQ * cap, cap);
     __syncthreads();
 }
 
 // Record of list entry e of a particle in brick row k: the row-code table of
 // row k (rc) gives its LDS address when it is staged.
-template <class Q, int NQ, int W_ = 2, bool PF_ = false>
+template <class Q, int NQ, int W_ = 2, bool PF_ = false, int NQ2 = 0>
 struct BrickFetch {
     static_assert(sizeof(Q) == 16, "records are made of 16-byte quads");
     static constexpr int W = W_;
     static constexpr bool PF = PF_;
+    static constexpr int NR = NQ + NQ2;  // quads per record: NQ from glob, then NQ2 from glob2
     const SPHX_LDS lds_quad_t* rc;    // the row-code table of the particle's brick row
     const SPHX_LDS lds_quad_t* code;  // per entry code: the cell offset
     const Q* __restrict__ glob;
+    const Q* __restrict__ glob2;
     uint32_t qstride;                 // bytes from quad k to quad k + 1 of a record (16 cap)
     uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries (kbeg a multiple of 4)
     // LDS byte address of entry e's record; false if it is not staged
@@ -671,19 +690,25 @@ struct BrickFetch {
         oy = __uint_as_float(t.y);
         oz = __uint_as_float(t.z);
     }
-    __device__ __forceinline__ void from_lds(uint32_t addr, Q (&q)[NQ]) const {
+    __device__ __forceinline__ void from_lds(uint32_t addr, Q (&q)[NR]) const {
 #pragma unroll
-        for (int k = 0; k < NQ; ++k)
+        for (int k = 0; k < NR; ++k)
             q[k] = lds_quad<Q>((const SPHX_LDS lds_quad_t*)(size_t)(addr + (uint32_t)k * qstride));
     }
-    __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NQ]) const {
+    __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NR]) const {
         uint32_t addr;
         if (locate(e, addr)) {
             from_lds(addr, q);
         } else {
-            const Q* p = glob + (size_t)(e & kIdxMask) * NQ;
+            const size_t j = e & kIdxMask;
+            const Q* p = glob + j * NQ;
 #pragma unroll
             for (int k = 0; k < NQ; ++k) q[k] = p[k];
+            if constexpr (NQ2 > 0) {
+                const Q* p2 = glob2 + j * NQ2;
+#pragma unroll
+                for (int k = 0; k < NQ2; ++k) q[NQ + k] = p2[k];
+            }
         }
     }
 };
@@ -694,9 +719,10 @@ struct BrickFetch {
 // wave-uniform test) the records are read with no per-lane branch, so the W
 // bodies, written branch-free, form one basic block the compiler can
 // interleave.  flush() as in for_neighbour_groups (it needs W = 4).
-template <class Q, int NQ, int W, bool PF, class Body, class Flush = NoFlush>
-__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF>& F,
+template <class Q, int NQ, int W, bool PF, int NQ2, class Body, class Flush = NoFlush>
+__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF, NQ2>& F,
                                      Body&& body, Flush&& flush = Flush()) {
+    constexpr int NR = NQ + NQ2;
     static_assert(W == 2 || W == 4, "two or four entries per iteration");
     static_assert(W == 4 || std::is_same<typename std::decay<Flush>::type, NoFlush>::value,
                   "groups of four entries need four entries per iteration");
@@ -731,7 +757,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
         }
 #pragma unroll
         for (int u = 0; u < W; ++u) staged = F.locate(c[u], slot[u]) && staged;
-        Q q[W][NQ];
+        Q q[W][NR];
         if (__all(staged)) {
 #pragma unroll
             for (int u = 0; u < W; ++u) F.from_lds(slot[u], q[u]);
@@ -743,7 +769,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
         for (int u = 0; u < W; ++u) body(c[u], q[u]);
         flush();
     }
-    Q q[NQ];
+    Q q[NR];
     if (k < cnt) {
         uint32_t c[W];
         if (PF) {
@@ -765,12 +791,13 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
 
 // The workgroup's brick (one per active brick, dealt so that each XCD takes a
 // contiguous range, see xcd_block): set up, stage the records of `src` (NQ
-// quads each), then body(i, fetch) for each of the brick's particles.
+// quads each) and, with NQ2 > 0, of `src2` (NQ2 quads each, read as quads NQ
+// .. NQ + NQ2 - 1), then body(i, fetch) for each of the brick's particles.
 // LANES = 2: two neighbouring lanes share each particle (body sees both).
-template <class Q, int NQ, int W, bool PF, int LANES = 1, class Body>
+template <class Q, int NQ, int W, bool PF, int LANES = 1, int NQ2 = 0, class Body>
 __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const uint32_t* __restrict__ start,
                                            uint32_t cap, const Q* __restrict__ src, BrickHdr& H,
-                                           uint4* __restrict__ sh, Body&& body) {
+                                           uint4* __restrict__ sh, Body&& body, const Q* __restrict__ src2 = nullptr) {
     const uint32_t hy = (uint32_t)B.b[1] + 2u, nh = hy * ((uint32_t)B.b[2] + 2u);
     const uint32_t BY = (uint32_t)B.b[1], no = BY * (uint32_t)B.b[2];
     const SPHX_LDS BrickHdr* HL = (const SPHX_LDS BrickHdr*)&H;
@@ -778,14 +805,15 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
     uint4* rc = sh;
     uint4* rec = sh + brick_rc_quads(B);
     brick_setup(B.active[xcd_block()], B, g, start, cap, H);
-    brick_stage<NQ>(H, (int)nh, reinterpret_cast<const uint4*>(src), rec, cap, rc, no, BY, hy);
-    const BrickFetch<Q, NQ, W, PF> base_fetch{(const SPHX_LDS lds_quad_t*)rc, (const SPHX_LDS lds_quad_t*)HL->code,
-                                              src, cap * 16u};
+    brick_stage<NQ, NQ2>(H, (int)nh, reinterpret_cast<const uint4*>(src), reinterpret_cast<const uint4*>(src2), rec,
+                         cap, rc, no, BY, hy);
+    const BrickFetch<Q, NQ, W, PF, NQ2> base_fetch{(const SPHX_LDS lds_quad_t*)rc,
+                                                   (const SPHX_LDS lds_quad_t*)HL->code, src, src2, cap * 16u};
     const uint32_t nown = HL->own_cum[no];
     uint32_t k = 0;
     for (uint32_t u = threadIdx.x / LANES; u < nown; u += blockDim.x / LANES) {
         while (u >= HL->own_cum[k + 1]) ++k;
-        BrickFetch<Q, NQ, W, PF> f = base_fetch;
+        BrickFetch<Q, NQ, W, PF, NQ2> f = base_fetch;
         f.rc += k * (uint32_t)kRowCodes;
         body(HL->own_start[k] + (u - HL->own_cum[k]), f);
     }

@@ -512,7 +512,9 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     // The pair body is branch-free: a pair without a term adds +-0, which
     // leaves a sum unchanged (the sums start at +0 and never become -0).
     // With uniform m and h, m_i m_j sigma_g is folded out of the sums.
-    auto pair = [&](uint32_t e, const double2 (&q)[4]) {
+    // q: the f64 record (x, y | z, rho | vx, vy | vz, P/rho^2), then, when the
+    // fetch stages it, the B record (bx, by | bz, psi)
+    auto pair = [&](uint32_t e, const double2 (&q)[Fetch::NR]) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = q[0], zr = q[1], vxy = q[2], vzp = q[3];
         double dx, dy, dz;
@@ -548,7 +550,14 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         }
         if (STRESS) {
             const double gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;  // grad_i W (times m_i m_j)
-            const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
+            double2 bxy, bzp;
+            if constexpr (Fetch::NR >= 6) {
+                bxy = q[4];
+                bzp = q[5];
+            } else {
+                bxy = RB[2 * j];
+                bzp = RB[2 * j + 1];
+            }
             const double bxj = bxy.x, byj = bxy.y, bzj = bzp.x;
             const double irho2j = rcp_nr(rhoj * rhoj);
             const double hb2j = 0.5 * ((bxj * bxj + byj * byj) + bzj * bzj);
@@ -611,20 +620,24 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
 
 // Two lanes per particle (each walks half of its list): twice the waves per CU
 // with the one halo that fits the LDS (measured: force 0.41 -> 0.39 ms,
-// visc_only 0.40 -> 0.36 ms on the 1M dam break).
-template <bool VISC_ONLY, bool UNI, int PER>
+// visc_only 0.40 -> 0.36 ms on the 1M dam break).  With the Maxwell stress
+// the B records are staged beside the f64 records (96 bytes a record).
+template <bool VISC_ONLY, bool STRESS, bool UNI, int PER>
 __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P, Derived D, Grid g, NbrList L,
                                                                     KernelParams kp, int charge, Bricks B,
                                                                     const uint32_t* __restrict__ start,
                                                                     uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    constexpr int W = VISC_ONLY ? 4 : 2;
-    brick_pass<double2, 4, W, true, 2>(B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
-                                       [&](uint32_t i, const BrickFetch<double2, 4, W, true>& f) {
-                                           force_particle<VISC_ONLY, false, UNI, PER, 2>(P, D, g, L, kp, charge, i,
-                                                                                         f);
-                                       });
+    constexpr int W = (VISC_ONLY && !STRESS) ? 4 : 2;
+    constexpr int NQ2 = STRESS ? 2 : 0;
+    constexpr bool PF = !STRESS;  // the stress terms leave no registers for the list prefetch
+    brick_pass<double2, 4, W, PF, 2, NQ2>(
+        B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
+        [&](uint32_t i, const BrickFetch<double2, 4, W, PF, NQ2>& f) {
+            force_particle<VISC_ONLY, STRESS, UNI, PER, 2>(P, D, g, L, kp, charge, i, f);
+        },
+        reinterpret_cast<const double2*>(D.recb));
 }
 
 // FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue.
@@ -758,29 +771,35 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
     const dim3 grid(nblocks(n)), block(kBlock);
     const int ch = charge ? 1 : 0;
     const bool uni = kp.uniform != 0;
-    if (B.on && B.nactive && !stress) {
-#define SPHX_FORCES_STAGED(V, U)                                                                            \
-        if (visc_only == V && uni == U) {                                                                   \
+    if (B.on && B.nactive && !(stress && mixed)) {
+#define SPHX_FORCES_STAGED(V, M, U)                                                                         \
+        if (visc_only == V && stress == M && uni == U) {                                                    \
             if (mixed) {                                                                                    \
                 const StagedLaunch sl = staged_launch(B, 16, 32);                                            \
                 hipLaunchKernelGGL((k_forces_mixed_staged<V, U, 4, false>), sl.grid, sl.block, sl.dyn, s, P, D, X, \
                                    g, L, kp, ch, B, cell_start, sl.cap);                                    \
             } else {                                                                                        \
-                StagedLaunch sl = staged_launch(B, 8, 64);                                                  \
+                StagedLaunch sl = staged_launch(B, 8, M ? 96 : 64);                                         \
                 sl.block = dim3(2u * (uint32_t)B.threads); /* two lanes per particle */                      \
                 if (g.periodic)                                                                             \
-                    hipLaunchKernelGGL((k_forces_staged<V, U, 1>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, \
-                                       ch, B, cell_start, sl.cap);                                          \
+                    hipLaunchKernelGGL((k_forces_staged<V, M, U, 1>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, \
+                                       kp, ch, B, cell_start, sl.cap);                                      \
                 else                                                                                        \
-                    hipLaunchKernelGGL((k_forces_staged<V, U, 0>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, \
-                                       ch, B, cell_start, sl.cap);                                          \
+                    hipLaunchKernelGGL((k_forces_staged<V, M, U, 0>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, \
+                                       kp, ch, B, cell_start, sl.cap);                                      \
             }                                                                                               \
             return;                                                                                         \
         }
-        SPHX_FORCES_STAGED(false, false)
-        SPHX_FORCES_STAGED(true, false)
-        SPHX_FORCES_STAGED(false, true)
-        SPHX_FORCES_STAGED(true, true)
+        SPHX_FORCES_STAGED(false, false, false)
+        SPHX_FORCES_STAGED(true, false, false)
+        SPHX_FORCES_STAGED(false, false, true)
+        SPHX_FORCES_STAGED(true, false, true)
+        if (!mixed) {
+            SPHX_FORCES_STAGED(false, true, false)
+            SPHX_FORCES_STAGED(true, true, false)
+            SPHX_FORCES_STAGED(false, true, true)
+            SPHX_FORCES_STAGED(true, true, true)
+        }
 #undef SPHX_FORCES_STAGED
     }
 #define SPHX_FORCES(V, M, U)                                                                              \
@@ -842,26 +861,23 @@ void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g
 // mhd_induction_system (mhd.rs:68-152) in gather form.  The reference adds
 // the pair term to entity_a and its negation to entity_b (mhd.rs:121-122,
 // 130-131), so particle i's term carries +1 when id_i < id_j, -1 otherwise.
-// Neighbours come from the f64 record (x, rho, v) and the B record; the new
-// B goes to tmp3 and, with psi, to recb_next (Jacobi).
-template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                       KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
+// Neighbours come from the f64 record (x, rho, v) and the B record (one
+// 6-quad record); the new B goes to tmp3 and, with psi, to recb_next (Jacobi).
+template <bool UNI, int PER, class Fetch>
+__device__ __forceinline__ void induction_particle(const Particles& P, const Derived& D, const Grid& g,
+                                                   const NbrList& L, const KernelParams& kp, uint32_t i,
+                                                   const Fetch& fetch) {
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i];
     const uint32_t idi = P.id[i];
-    const double2* __restrict__ R = reinterpret_cast<const double2*>(D.r64);
-    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     const double inv_rhoi = rcp_nr(rhoi);
     double ux = 0, uy = 0, uz = 0;
-    for_neighbours(L, i, [&](uint32_t e) {
+    walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[6]) {
         const uint32_t j = e & kIdxMask;
-        const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
-        const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
+        const double2 xy = q[0], zr = q[1], vxy = q[2], vzp = q[3], bxy = q[4], bzp = q[5];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
+        pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 > 1e-20)) return;
         double r, inv_r;
@@ -890,6 +906,19 @@ __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, ui
     rb[0] = make_double2(bx, by);
     rb[1] = make_double2(bz, P.psi[i]);
 }
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                       KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    induction_particle<UNI, -1>(P, D, g, L, kp, i,
+                                GlobalFetch<double2, 4, 2>{reinterpret_cast<const double2*>(D.r64),
+                                                           reinterpret_cast<const double2*>(D.recb)});
+}
+
+
+// Unstaged: brick-staged forms of these two passes (96-byte records, one
+// workgroup per CU) measured slower on the 4M periodic MHD box (5.1 against
+// 4.7 ms per step for the pair) than these gathers at three workgroups per CU.
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                       const KernelParams& kp, hipStream_t s) {
     if (!n) return;
@@ -903,21 +932,18 @@ void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Gr
 // ---------------------------------------------- divergence cleaning (a12)
 // divergence_cleaning_system (mhd.rs:172-259): Jacobi over B and psi, then
 // psi *= exp(-sigma c_h dt / h) for every particle.
-template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                       KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
+template <bool UNI, int PER, class Fetch>
+__device__ __forceinline__ void div_clean_particle(const Particles& P, const Derived& D, const Grid& g,
+                                                   const NbrList& L, const KernelParams& kp, uint32_t i,
+                                                   const Fetch& fetch) {
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i], psii = P.psi[i];
-    const double2* __restrict__ R = reinterpret_cast<const double2*>(D.r64);
-    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
-    for_neighbours(L, i, [&](uint32_t e) {
+    walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[6]) {
         const uint32_t j = e & kIdxMask;
-        const double2 xy = R[4 * j], zr = R[4 * j + 1];
-        const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
+        const double2 xy = q[0], zr = q[1], bxy = q[4], bzp = q[5];
         double dx, dy, dz;
-        pair_vec(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
+        pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         if (!(r2 > 1e-20)) return;
         double r, inv_r;
@@ -943,6 +969,14 @@ __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, ui
     double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
     rb[0] = make_double2(bx, by);
     rb[1] = make_double2(bz, psi);
+}
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                       KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    div_clean_particle<UNI, -1>(P, D, g, L, kp, i,
+                                GlobalFetch<double2, 4, 2>{reinterpret_cast<const double2*>(D.r64),
+                                                           reinterpret_cast<const double2*>(D.recb)});
 }
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                       const KernelParams& kp, hipStream_t s) {

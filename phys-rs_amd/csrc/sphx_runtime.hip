@@ -92,6 +92,9 @@ struct SphxCtx {
     double hmax = 0.0;                     // largest smoothing radius (h is fixed per particle)
     // validity of derived data for the current positions and sorted order
     bool sorted = false, listed = false, dens = false, sp = false, acc = false;
+    // list lifetime in ticks: position updates since the current list's
+    // build, and the count at which the previous list was found stale (0: none yet)
+    uint32_t list_age = 0, last_life = 0;
     // timing
     bool timing = false;
     std::vector<TimedRange> pending;
@@ -257,18 +260,26 @@ void choose_bricks(SphxCtx* c, const SphxState* host, const Grid& g) {
         B.on = 1;
         return;
     }
-    const double lds_records = (double)(160 * 1024 - 2048) / 64.0;  // f64 force records, one 512-thread group
+    // the f64 force record (64 bytes); the Maxwell-stress force pass stages
+    // the B record too and reads what does not fit from global memory
+    const double rec = 64.0;
     // at least 8 bricks per CU: a brick is one workgroup's work, and with
     // fewer the last round of workgroups leaves much of the GPU idle
     double best = 0.0;
     for (int z = 1; z <= kMaxBrickYZ; ++z)
         for (int y = 1; y <= kMaxBrickYZ; ++y)
             for (int x = 1; x <= 8; ++x) {
+                // records of one 512-thread group's LDS, less the header and
+                // the row-code table; a halo that overflows it is read from
+                // global memory, which takes its waves off the LDS fast path
+                const double lds_records = (double)(160 * 1024 - 2048 - 16 * 27 * y * z) / rec;
                 const double own = ppc * x * y * z, halo = ppc * (x + 2) * (y + 2) * (z + 2);
                 if ((double)n / own < 8.0 * B.cus) continue;
                 for (int T : {256, 512}) {
                     if (own > T || own < 0.5 * T) continue;
-                    if (halo > lds_records * (T / 512.0)) continue;  // 256 threads: two groups per CU
+                    // two standard deviations of a Poisson count above the mean
+                    // halo (256 threads: two groups per CU)
+                    if (halo + 2.0 * std::sqrt(halo) > lds_records * (T / 512.0)) continue;
                     const double score = (own / halo) * std::min(1.0, own / (0.85 * T));
                     if (score > best) {
                         best = score;
@@ -487,6 +498,7 @@ int do_neighbours(SphxCtx* c) {
                          c->stream);
     }
     c->sorted = c->listed = true;
+    c->list_age = 0;
     c->dens = c->sp = c->acc = false;
     return launch_check(c);
 }
@@ -545,6 +557,7 @@ int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr, bool defer = false) {
     int rc;
     if ((rc = launch_check(c))) return rc;
     c->dens = c->sp = c->acc = false;
+    ++c->list_age;
     bool rebuild = c->kp.skin <= 0.0 || !c->listed;
     if (!rebuild) {
         HIP_TRY(c, hipMemcpyAsync(c->bricks_host + 1, c->L.overflow + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -556,9 +569,20 @@ int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr, bool defer = false) {
         }
         HIP_TRY(c, hipEventSynchronize(c->moved_ready));
         rebuild = c->bricks_host[1] != 0;
+        if (rebuild) c->last_life = c->list_age;
     }
     if (rebuild) c->sorted = c->listed = false;
     return SPHX_OK;
+}
+
+// Whether to enqueue the passes of a tick before its moved flag is read: a
+// stale list costs a discarded density and force pass, so only while the
+// list is young compared with the previous one's lifetime (and that lifetime
+// is long), or, before any list has gone stale, once it has survived 4 ticks.
+bool speculate(const SphxCtx* c) {
+    if (!c->listed || c->kp.skin <= 0.0) return false;
+    const uint32_t age = c->list_age + 1;  // after this tick's position update
+    return c->last_life ? (c->last_life >= 8 && age + 2 < c->last_life) : age >= 4;
 }
 
 // Reads the flag of a deferred position update.  If a particle moved more
@@ -569,6 +593,7 @@ int settle_moved(SphxCtx* c, bool& redo) {
     c->moved_pending = false;
     HIP_TRY(c, hipEventSynchronize(c->moved_ready));
     if (c->bricks_host[1] != 0) {
+        c->last_life = c->list_age;
         c->sorted = c->listed = false;
         c->dens = c->sp = c->acc = false;
         redo = true;
@@ -707,15 +732,31 @@ void adj_save(SphxCtx* c, double* x3, double* v3) {
     if (v3) launch_scatter3(P.orig, n, vel3(P), vec3(v3, n), c->stream);
 }
 
+// Density and the second force evaluation of a tick at the new positions,
+// a* = A(x_{n+1}, v_n).  After a deferred position update they run on the
+// kept list before the moved flag is read, and again on the rebuilt list if
+// a particle did move skin/2 (they write only derived arrays).
+int second_evaluation(SphxCtx* c) {
+    int rc;
+    for (bool redo = true; redo;) {
+        if ((rc = ensure_list(c)) || (rc = do_density(c)) || (rc = do_forces(c, false))) {
+            c->moved_pending = false;
+            c->sorted = c->listed = false;
+            c->dens = c->sp = c->acc = false;
+            return rc;
+        }
+        if ((rc = settle_moved(c, redo))) return rc;
+    }
+    return SPHX_OK;
+}
+
 // One tick of sphx_step (no XSPH / MHD); records the reflections and v' when asked.
 int adj_tick(SphxCtx* c, uint8_t* flip, double* vp3) {
     int rc;
     if ((rc = do_forces(c, c->sp && c->dens))) return rc;
-    if ((rc = do_vv_position(c, flip))) return rc;
+    if ((rc = do_vv_position(c, flip, speculate(c)))) return rc;
     if (vp3) adj_save(c, nullptr, vp3);
-    if ((rc = ensure_list(c))) return rc;
-    if ((rc = do_density(c))) return rc;
-    if ((rc = do_forces(c, false))) return rc;
+    if ((rc = second_evaluation(c))) return rc;
     return do_vv_velocity(c);
 }
 
@@ -1041,21 +1082,13 @@ int sphx_step(SphxCtx* c, uint32_t nsteps) {
         // a_n = A(x_n, v_n): the pressure sum at x_n is still valid from the
         // previous tick's second evaluation unless positions were replaced.
         if ((rc = do_forces(c, c->sp && c->dens))) return rc;
-        // While the list is kept, density and the second force pass are
-        // enqueued before the host has read the moved flag, so the GPU does
-        // not wait for the host; they only write derived arrays, and if a
-        // particle did move skin/2 they are redone on the rebuilt list.
-        if ((rc = do_vv_position(c, nullptr, true))) return rc;
-        for (bool redo = true; redo;) {
-            if ((rc = ensure_list(c)) || (rc = do_density(c)) ||
-                (rc = do_forces(c, false))) {  // a* = A(x_{n+1}, v_n)
-                c->moved_pending = false;
-                c->sorted = c->listed = false;
-                c->dens = c->sp = c->acc = false;
-                return rc;
-            }
-            if ((rc = settle_moved(c, redo))) return rc;
-        }
+        // While the list is expected to stay valid, density and the second
+        // force pass are enqueued before the host has read the moved flag,
+        // so the GPU does not wait for the host; they only write derived
+        // arrays, and if a particle did move skin/2 they are redone on the
+        // rebuilt list.
+        if ((rc = do_vv_position(c, nullptr, speculate(c)))) return rc;
+        if ((rc = second_evaluation(c))) return rc;  // a* = A(x_{n+1}, v_n)
         if ((rc = do_vv_velocity(c))) return rc;
         if (c->prm.xsph_epsilon != 0.0 && (rc = do_xsph(c))) return rc;
         if (c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_INDUCTION) && (rc = do_induction(c))) return rc;
