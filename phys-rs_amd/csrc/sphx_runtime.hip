@@ -95,6 +95,7 @@ struct SphxCtx {
     // list lifetime in ticks: position updates since the current list's
     // build, and the count at which the previous list was found stale (0: none yet)
     uint32_t list_age = 0, last_life = 0;
+    int spec_mode = 0;                     // SPHX_SPECULATE: 0 by list lifetime, 1 always, 2 never
     // timing
     bool timing = false;
     std::vector<TimedRange> pending;
@@ -580,7 +581,8 @@ int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr, bool defer = false) {
 // list is young compared with the previous one's lifetime (and that lifetime
 // is long), or, before any list has gone stale, once it has survived 4 ticks.
 bool speculate(const SphxCtx* c) {
-    if (!c->listed || c->kp.skin <= 0.0) return false;
+    if (!c->listed || c->kp.skin <= 0.0 || c->spec_mode == 2) return false;
+    if (c->spec_mode == 1) return true;
     const uint32_t age = c->list_age + 1;  // after this tick's position update
     return c->last_life ? (c->last_life >= 8 && age + 2 < c->last_life) : age >= 4;
 }
@@ -858,6 +860,9 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     if (hipDeviceGetAttribute(&c->B.cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return bail(SPHX_E_HIP);
     brick_defaults(c);
+    // SPHX_SPECULATE=always|never overrides when a tick's second evaluation
+    // is enqueued before its moved flag is read (the results do not change)
+    if (const char* e = getenv("SPHX_SPECULATE")) c->spec_mode = !strcmp(e, "always") ? 1 : !strcmp(e, "never") ? 2 : 0;
     if (hipHostMalloc((void**)&c->bricks_host, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         c->bricks_host = nullptr;
         return bail(SPHX_E_OOM);

@@ -9,6 +9,8 @@ Bars:
 """
 import math
 
+import os
+
 import numpy as np
 import pytest
 
@@ -116,6 +118,39 @@ def test_skin_lists_keep_the_exact_neighbour_set(require_gpu):
         assert rel_err(a[k], b[k]) < 1e-12, k
     for k in ("vx", "vy", "vz"):
         assert rel_err(a[k], b[k]) < 1e-9, k
+
+
+@pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
+def test_speculative_ticks_bit_identical(require_gpu, precision):
+    # A tick may enqueue density and the second force pass before the host
+    # has read the moved-skin/2 flag, and redo them on a rebuilt list when it
+    # is set.  Forced on every tick and forced off, with a small skin (many
+    # rebuilds, each one a redo when forced on), the trajectories are equal.
+    st, p = scenarios.dam_break(nx=16, ny=16, nz=8, v_sigma=0.3, seed=7)
+    p.neighbour_skin = 0.02 * 0.013
+    p.precision = precision
+    out = {}
+    old = os.environ.get("SPHX_SPECULATE")
+    try:
+        for mode in ("always", "never"):
+            os.environ["SPHX_SPECULATE"] = mode
+            with make_ctx(st, p) as ctx:
+                ctx.enable_timing(True)
+                ctx.step(30)
+                t = ctx.kernel_times()
+                ctx.forces()
+                out[mode] = (ctx.download(sphx.F_POS | sphx.F_VEL | sphx.F_ACC | sphx.F_RHO),
+                             t["neighbour"]["launches"], t["density"]["launches"])
+    finally:
+        if old is None:
+            del os.environ["SPHX_SPECULATE"]
+        else:
+            os.environ["SPHX_SPECULATE"] = old
+    (a, builds_a, dens_a), (b, builds_b, dens_b) = out["always"], out["never"]
+    assert builds_a == builds_b >= 4
+    assert dens_a == dens_b + (builds_a - 1)  # every rebuild after the first was a redo
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
 
 
 def test_pairs_exactly_at_cutoff(require_gpu):
