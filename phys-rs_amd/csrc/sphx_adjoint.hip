@@ -149,18 +149,31 @@ void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const V
     if (n) hipLaunchKernelGGL(k_adj_loss, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, target, xb, part);
 }
 
+// Two lanes per particle (the VJP passes): lane h of the pair sums list
+// entries [0, half) (h = 0) or [half, end) (h = 1), half a multiple of 4 near
+// the middle, and lane 0 adds the second half to the first.  At the sizes the
+// adjoint runs (tens of thousands of particles, one thread per particle is a
+// few waves per CU) this doubles the waves that hide the gather latency.
+__device__ __forceinline__ uint32_t list_half(const NbrList& L, uint32_t i) {
+    return (((uint32_t)L.ntot[i] + 7u) >> 3) << 2;
+}
+__device__ __forceinline__ double pair_total(double v) { return v + __shfl_xor(v, 1); }
+
+uint32_t vjp_blocks(uint32_t n) { return nblocks(2 * n); }
+
 // VJP of the pressure + viscosity acceleration at the current (sorted)
 // state.  ab4: abar records in sorted order; neighbours' x, rho, v, P/rho^2
 // come from the f64 gather record.  Adds dL/dx and dL/dv into xb, vb
 // (upload order), writes rhobar (sorted) and adds this block's share of
-// dL/dalpha to alpha_acc[block].
+// dL/dalpha to alpha_acc[block]; with `overwrite` it stores them instead of
+// adding (a pass that may be redone).
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                         KernelParams kp, const double4* __restrict__ ab4, Vec3 xb,
                                                         Vec3 vb, double* __restrict__ rhob,
-                                                        double* __restrict__ alpha_acc) {
+                                                        double* __restrict__ alpha_acc, int overwrite) {
     __shared__ double lds[kBlock];
-    const uint32_t i = particle_index();
+    const uint32_t lane = particle_index(), i = lane >> 1, h = lane & 1u;
     double da = 0.0;
     if (i < n) {
         const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i];
@@ -180,6 +193,7 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
         // Branch-free pair body with two refined reciprocals (1/Dn, 1/rho_bar)
         // in place of divisions; a pair outside the kernel support, at r <= 1e-10
         // or (for the viscosity terms) not approaching/separating contributes 0.
+        const uint32_t half = list_half(L, i);
         for_neighbours(L, i, [&](uint32_t e) {
             const uint32_t j = e & kIdxMask;
             const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
@@ -234,14 +248,27 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
             gvz = fma(wh, dz, gvz);
             grho = fma(cg, fma(-0.5 * Pi, inv_rb, dTi), grho);
             da = fma(0.5 * cg, (-cs * mu) * inv_rb, da);  // each pair is visited from both ends
-        });
-        const uint32_t u = P.orig[i];
-        xb.x[u] += gx; xb.y[u] += gy; xb.z[u] += gz;
-        vb.x[u] += gvx; vb.y[u] += gvy; vb.z[u] += gvz;
-        rhob[i] = grho;
+        }, h ? half : 0u, h ? 0xffffffffu : half);
+        gx = pair_total(gx); gy = pair_total(gy); gz = pair_total(gz);
+        gvx = pair_total(gvx); gvy = pair_total(gvy); gvz = pair_total(gvz);
+        grho = pair_total(grho);
+        if (!h) {
+            const uint32_t u = P.orig[i];
+            if (overwrite) {
+                xb.x[u] = gx; xb.y[u] = gy; xb.z[u] = gz;
+                vb.x[u] = gvx; vb.y[u] = gvy; vb.z[u] = gvz;
+            } else {
+                xb.x[u] += gx; xb.y[u] += gy; xb.z[u] += gz;
+                vb.x[u] += gvx; vb.y[u] += gvy; vb.z[u] += gvz;
+            }
+            rhob[i] = grho;
+        }
     }
     const double t = block_sum(da, lds);
-    if (threadIdx.x == 0) alpha_acc[xcd_block()] += t;
+    if (threadIdx.x == 0) {
+        if (overwrite) alpha_acc[xcd_block()] = t;
+        else alpha_acc[xcd_block()] += t;
+    }
 }
 
 // VJP of the summation density: rho_i = m_i W(0, h_i) + sum_j m_j W(r_ij, h_ij)
@@ -249,11 +276,12 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n, Grid g, NbrList L,
                                                          KernelParams kp, const double* __restrict__ rhob, Vec3 xb) {
-    const uint32_t i = particle_index();
+    const uint32_t lane = particle_index(), i = lane >> 1, h = lane & 1u;
     if (i >= n) return;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i], rbi = rhob[i];
     const double2* __restrict__ pos = reinterpret_cast<const double2*>(P.pos);
     double gx = 0, gy = 0, gz = 0;
+    const uint32_t half = list_half(L, i);
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = pos[2 * j], zh = pos[2 * j + 1];
@@ -270,27 +298,46 @@ __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n,
         gx = fma(-coef, dx, gx);
         gy = fma(-coef, dy, gy);
         gz = fma(-coef, dz, gz);
-    });
+    }, h ? half : 0u, h ? 0xffffffffu : half);
+    gx = pair_total(gx); gy = pair_total(gy); gz = pair_total(gz);
+    if (h) return;
     const uint32_t u = P.orig[i];
     xb.x[u] += gx; xb.y[u] += gy; xb.z[u] += gz;
 }
 
 void launch_vjp(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                 const KernelParams& kp, const double4* ab4, const Vec3& xb, const Vec3& vb, double* rhob,
-                double* alpha_acc, hipStream_t s) {
+                double* alpha_acc, hipStream_t s, bool overwrite) {
     if (!n) return;
-    const dim3 grid(nblocks(n)), block(kBlock);
+    const dim3 grid(vjp_blocks(n)), block(kBlock);  // two lanes per particle
+    const int ow = overwrite ? 1 : 0;
     if (kp.uniform) {
         hipLaunchKernelGGL(k_vjp_forces<true>, grid, block, occupancy_lds(3), s, P, D, n, g, L, kp, ab4, xb, vb, rhob,
-                           alpha_acc);
+                           alpha_acc, ow);
         hipLaunchKernelGGL(k_vjp_density<true>, grid, block, occupancy_lds(4), s, P, n, g, L, kp,
                            (const double*)rhob, xb);
     } else {
         hipLaunchKernelGGL(k_vjp_forces<false>, grid, block, occupancy_lds(3), s, P, D, n, g, L, kp, ab4, xb, vb,
-                           rhob, alpha_acc);
+                           rhob, alpha_acc, ow);
         hipLaunchKernelGGL(k_vjp_density<false>, grid, block, occupancy_lds(4), s, P, n, g, L, kp,
                            (const double*)rhob, xb);
     }
+}
+
+__global__ void k_adj_accumulate(uint32_t n, Vec3 xs, Vec3 vs, Vec3 xb, Vec3 vb, const double* __restrict__ as,
+                                 double* __restrict__ alpha_acc, uint32_t nb) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) {
+        xb.x[u] += xs.x[u]; xb.y[u] += xs.y[u]; xb.z[u] += xs.z[u];
+        vb.x[u] += vs.x[u]; vb.y[u] += vs.y[u]; vb.z[u] += vs.z[u];
+    }
+    if (u < nb) alpha_acc[u] += as[u];
+}
+void launch_adj_accumulate(uint32_t n, const Vec3& xs, const Vec3& vs, const Vec3& xb, const Vec3& vb,
+                           const double* as, double* alpha_acc, uint32_t nb, hipStream_t s) {
+    const uint32_t m = n > nb ? n : nb;
+    if (m) hipLaunchKernelGGL(k_adj_accumulate, dim3(nblocks(m)), dim3(kBlock), 0, s, n, xs, vs, xb, vb, as, alpha_acc,
+                              nb);
 }
 
 }  // namespace sphx

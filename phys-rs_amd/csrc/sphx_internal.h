@@ -235,8 +235,17 @@ void launch_adj_flip(uint32_t n, const uint8_t* flip, double dt, double hdt2, co
                      const Vec3& apb, const Vec3& ab, hipStream_t s);
 void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const Vec3& xb, double* part,
                      hipStream_t s);
+// workgroups of the VJP passes (two lanes per particle): the size of the
+// per-block dL/dalpha array
+uint32_t vjp_blocks(uint32_t n);
 void launch_vjp(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                 const KernelParams& kp, const double4* ab4, const Vec3& xb, const Vec3& vb, double* rhob,
-                double* alpha_acc, hipStream_t s);
+                double* alpha_acc, hipStream_t s,
+                bool overwrite = false);
+// xb += xs, vb += vs (upload order, n each) and alpha_acc[b] += as[b] for the
+// nb VJP blocks: the VJP of a tick written with overwrite = true, applied
+// once its neighbour list is known to be valid
+void launch_adj_accumulate(uint32_t n, const Vec3& xs, const Vec3& vs, const Vec3& xb, const Vec3& vb,
+                           const double* as, double* alpha_acc, uint32_t nb, hipStream_t s);
 
 }  // namespace sphx

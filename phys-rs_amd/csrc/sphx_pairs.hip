@@ -610,11 +610,14 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
+// Two lanes per particle, as the staged pass (same halves, same sums): at
+// the sizes that run unstaged, one thread per particle is few waves per CU.
 template <bool VISC_ONLY, bool STRESS, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                     KernelParams kp, int charge) {
-    SPHX_PASS_PROLOGUE();
-    force_particle<VISC_ONLY, STRESS, UNI, -1, 1>(P, D, g, L, kp, charge, i,
+    const uint32_t i = particle_index() >> 1;
+    if (i >= n) return;
+    force_particle<VISC_ONLY, STRESS, UNI, -1, 2>(P, D, g, L, kp, charge, i,
                                            GlobalFetch<double2, 4>{reinterpret_cast<const double2*>(D.r64)});
 }
 
@@ -808,7 +811,8 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
             hipLaunchKernelGGL((k_forces_mixed<V, M, U>), grid, block, occupancy_lds(4), s, P, D, X, n, g, L, \
                                kp, ch);                                                                   \
         else                                                                                              \
-            hipLaunchKernelGGL((k_forces<V, M, U>), grid, block, M ? 0 : occupancy_lds(3), s, P, D, n, g, L, kp, \
+            hipLaunchKernelGGL((k_forces<V, M, U>), dim3(nblocks(2 * n)), block, M ? 0 : occupancy_lds(3), s, P, D, \
+                               n, g, L, kp,                                                               \
                                ch);                                                                       \
         return;                                                                                           \
     }

@@ -53,6 +53,8 @@ struct AdjBuffers {
     double* target = nullptr;      // 3n
     double* part = nullptr;        // per-block loss partial sums
     double* alpha_acc = nullptr;   // per-block dL/dalpha
+    double* spec = nullptr;        // 2 x 3n: a tick's VJP (xbar, vbar shares) before it is added
+    double* alpha_spec = nullptr;  // per-block dL/dalpha share of that VJP
     void release() {
         for (void* p : allocs) (void)hipFree(p);
         allocs.clear();
@@ -692,7 +694,8 @@ int adj_alloc(SphxCtx* c, uint32_t n, uint32_t k, uint32_t m) {
         (rc = get(&A.tv, (size_t)(k + 1) * 3 * N)) || (rc = get(&A.tvp, (size_t)k * 3 * N)) ||
         (rc = get(&A.flip, (size_t)k * N)) || (rc = get(&A.fin, 6 * N)) || (rc = get(&A.adj, 12 * N)) ||
         (rc = get(&A.ab4, N)) || (rc = get(&A.rhob, N)) || (rc = get(&A.target, 3 * N)) ||
-        (rc = get(&A.part, nb)) || (rc = get(&A.alpha_acc, nb))) {
+        (rc = get(&A.part, nb)) || (rc = get(&A.alpha_acc, vjp_blocks(n))) || (rc = get(&A.spec, 6 * N)) ||
+        (rc = get(&A.alpha_spec, vjp_blocks(n)))) {
         A.release();
         return rc;
     }
@@ -705,9 +708,12 @@ int adj_alloc(SphxCtx* c, uint32_t n, uint32_t k, uint32_t m) {
 // state (upload order) -> working arrays (current sorted order); x may be
 // NULL.  The neighbour list is kept while no particle is further than skin/2
 // from where it was at the list's build.
-int adj_load(SphxCtx* c, const double* x3, const double* v3) {
+// defer: as for do_vv_position, the moved flag is read by settle_moved.
+int adj_load(SphxCtx* c, const double* x3, const double* v3, bool defer = false) {
     const uint32_t n = c->n;
     Particles& P = c->P();
+    launch_load_velocity(P, c->D, n, vec3(const_cast<double*>(v3), n), c->stream);
+    c->acc = false;
     if (x3) {
         launch_gather3(P.orig, n, vec3(const_cast<double*>(x3), n), pos3(P), c->stream);
         c->dens = c->sp = c->acc = false;
@@ -715,16 +721,19 @@ int adj_load(SphxCtx* c, const double* x3, const double* v3) {
             launch_fill_u32(c->L.overflow + 2, 0u, 1, c->stream);
             launch_refresh_positions(P, c->D, n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, c->L.overflow + 2,
                                      c->stream);
-            uint32_t moved = 0;
-            HIP_TRY(c, hipMemcpyAsync(&moved, c->L.overflow + 2, sizeof moved, hipMemcpyDeviceToHost, c->stream));
-            HIP_TRY(c, hipStreamSynchronize(c->stream));
-            if (moved) c->sorted = c->listed = false;
+            HIP_TRY(c, hipMemcpyAsync(c->bricks_host + 1, c->L.overflow + 2, sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipEventRecord(c->moved_ready, c->stream));
+            if (defer) {
+                c->moved_pending = true;
+            } else {
+                HIP_TRY(c, hipEventSynchronize(c->moved_ready));
+                if (c->bricks_host[1]) c->sorted = c->listed = false;
+            }
         } else {
             c->sorted = c->listed = false;
         }
     }
-    launch_load_velocity(P, c->D, n, vec3(const_cast<double*>(v3), n), c->stream);
-    c->acc = false;
     return launch_check(c);
 }
 void adj_save(SphxCtx* c, double* x3, double* v3) {
@@ -1127,6 +1136,7 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
     HIP_TRY(c, hipMemcpyAsync(A.target + 2 * (size_t)n, tgz, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     const Vec3 xb = vec3(A.adj, n), vb = vec3(A.adj + n3, n), ab = vec3(A.adj + 2 * n3, n),
                apb = vec3(A.adj + 3 * n3, n);
+    const Vec3 xs = vec3(A.spec, n), vs = vec3(A.spec + n3, n);
     const uint32_t last0 = (M - 1) * k;  // first tick of the last segment
     // forward: checkpoints every k ticks, the last segment's tape kept
     for (uint32_t t = 0; t < nsteps; ++t) {
@@ -1145,7 +1155,7 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
     // loss and dL/dx_N
     launch_adj_loss(c->P(), n, vec3(A.target, n), xb, A.part, c->stream);
     launch_fill_u32(reinterpret_cast<uint32_t*>(vb.x), 0u, 2 * n3, c->stream);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(A.alpha_acc), 0u, 2 * (size_t)nblocks(n), c->stream);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(A.alpha_acc), 0u, 2 * (size_t)vjp_blocks(n), c->stream);
     const double dt = c->kp.dt, hdt = c->kp.half_dt, hdt2 = c->kp.half_dt_sq;
     // reverse, one segment at a time
     for (uint32_t m = M; m-- > 0;) {
@@ -1163,20 +1173,29 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
             launch_adj_seed(c->P().orig, n, hdt, vb, apb, A.ab4, c->stream);
             launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, A.ab4, xb, vb, A.rhob, A.alpha_acc, c->stream);
             launch_adj_flip(n, A.flip + (size_t)s * n, dt, hdt2, xb, vb, apb, ab, c->stream);
-            // at (x_t, v_t)
-            if ((rc = adj_load(c, A.tx + s * n3, A.tv + s * n3))) return rc;
-            if ((rc = ensure_list(c)) || (rc = do_density(c))) return rc;
-            launch_gather_ab(c->P().orig, n, ab, A.ab4, c->stream);
-            launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, A.ab4, xb, vb, A.rhob, A.alpha_acc, c->stream);
-            if ((rc = launch_check(c))) return rc;
+            // at (x_t, v_t): enqueued before the moved flag of x_t is read,
+            // into A.spec (stored, not added), and redone on a rebuilt list
+            // if a particle moved skin/2; then added to xbar, vbar, alpha
+            if ((rc = adj_load(c, A.tx + s * n3, A.tv + s * n3, true))) return rc;
+            for (bool redo = true; redo;) {
+                if ((rc = ensure_list(c)) || (rc = do_density(c))) {
+                    c->moved_pending = false;
+                    return rc;
+                }
+                launch_gather_ab(c->P().orig, n, ab, A.ab4, c->stream);
+                launch_vjp(c->P(), c->D, n, c->grid, c->L, c->kp, A.ab4, xs, vs, A.rhob, A.alpha_spec, c->stream,
+                           true);
+                if ((rc = launch_check(c)) || (rc = settle_moved(c, redo))) return rc;
+            }
+            launch_adj_accumulate(n, xs, vs, xb, vb, A.alpha_spec, A.alpha_acc, vjp_blocks(n), c->stream);
         }
     }
     if ((rc = check_overflow(c))) return rc;
     // results, then the state after the last tick back in place
-    const uint32_t nb = nblocks(n);
-    std::vector<double> part(nb), acc(nb);
+    const uint32_t nb = nblocks(n), nv = vjp_blocks(n);
+    std::vector<double> part(nb), acc(nv);
     HIP_TRY(c, hipMemcpyAsync(part.data(), A.part, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(acc.data(), A.alpha_acc, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(acc.data(), A.alpha_acc, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
     double* outs[6] = {gvx, gvy, gvz, gx, gy, gz};
     const double* srcs[6] = {vb.x, vb.y, vb.z, xb.x, xb.y, xb.z};
     for (int q = 0; q < 6; ++q)
@@ -1184,10 +1203,8 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
     if ((rc = adj_load(c, A.fin, A.fin + n3))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     double l = 0.0, ga = 0.0;
-    for (uint32_t b = 0; b < nb; ++b) {
-        l += part[b];
-        ga += acc[b];
-    }
+    for (uint32_t b = 0; b < nb; ++b) l += part[b];
+    for (uint32_t b = 0; b < nv; ++b) ga += acc[b];
     *loss = l / (3.0 * (double)n);
     *galpha = ga;
     return launch_check(c);
