@@ -253,14 +253,17 @@ __device__ __forceinline__ PairH pair_h(const KernelParams& kp, double hi, doubl
     const uint32_t i = particle_index();    \
     if (i >= n) return
 
-template <bool UNI, int PER, class Fetch>
+// The list is summed in two halves, entries 0 .. half - 1 and half .. end
+// (half a multiple of 4 near the middle), added in that order; with LANES = 2
+// the calling lane and its neighbour (lane ^ 1) take one half each.
+template <bool UNI, int PER, int LANES, class Fetch>
 __device__ __forceinline__ void density_particle(const Particles& P, const Derived& D, const Grid& g,
                                                  const NbrList& L, const KernelParams& kp, uint32_t i,
                                                  const Fetch& fetch) {
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     double rho = P.m[i] * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
     double acc = 0.0;
-    walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[2]) {
+    auto pair = [&](uint32_t e, const double2 (&q)[2]) {
         const double2 xy = q[0], zh = q[1];
         double dx, dy, dz;
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zh.x, dx, dy, dz);
@@ -274,7 +277,26 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         } else {
             acc = fma(P.m[e & kIdxMask], ph.sig_w * w_shape(r * ph.inv_h), acc);
         }
-    });
+    };
+    const uint32_t half = (((uint32_t)L.ntot[i] + 7u) >> 3) << 2;
+    auto sum_range = [&](uint32_t kb, uint32_t ke) {
+        Fetch f = fetch;
+        f.kbeg = kb;
+        f.kend = ke;
+        walk(L, i, f, pair);
+    };
+    if (LANES == 2) {
+        const uint32_t h = threadIdx.x & 1u;
+        sum_range(h ? half : 0u, h ? 0xffffffffu : half);
+        acc += __shfl_xor(acc, 1);  // lane 0: first half + second half
+        if (h) return;
+    } else {
+        sum_range(0u, half);
+        const double a0 = acc;
+        acc = 0.0;
+        sum_range(half, 0xffffffffu);
+        acc = a0 + acc;
+    }
     rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 0.25), acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho);
     const double pr2 = pr / (rho * rho);
@@ -291,22 +313,27 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                      KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
-    density_particle<UNI, -1>(P, D, g, L, kp, i, GlobalFetch<double2, 2>{reinterpret_cast<const double2*>(P.pos)});
+    const uint32_t i = particle_index() >> 1;  // two lanes per particle
+    if (i >= n) return;
+    density_particle<UNI, -1, 2>(P, D, g, L, kp, i,
+                                 GlobalFetch<double2, 2>{reinterpret_cast<const double2*>(P.pos)});
 }
 
 // Brick-staged forms: the neighbours' records come from LDS (bit-identical
 // results: same records, same order, same arithmetic).
+// One lane per particle summing the two halves in turn (two lanes per
+// particle measured slower here: one workgroup per CU instead of two, 0.21 ->
+// 0.25 ms on the 1M dam break).
 template <bool UNI, int PER, int W, bool PF>
-__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4))) k_density_staged(Particles P, Derived D, Grid g, NbrList L,
-                                                                 KernelParams kp, Bricks B,
-                                                                 const uint32_t* __restrict__ start, uint32_t cap) {
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4))) k_density_staged(
+    Particles P, Derived D, Grid g, NbrList L, KernelParams kp, Bricks B, const uint32_t* __restrict__ start,
+    uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
     brick_pass<double2, 2, W, PF>(B, g, start, cap, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
-                               [&](uint32_t i, const BrickFetch<double2, 2, W, PF>& f) {
-                               density_particle<UNI, PER>(P, D, g, L, kp, i, f);
-                           });
+                                  [&](uint32_t i, const BrickFetch<double2, 2, W, PF>& f) {
+                                      density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f);
+                                  });
 }
 
 // FP32-mixed density: f32 pair terms on cell-relative coordinates, f64 sum.
@@ -410,8 +437,8 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
     const bool uni = kp.uniform != 0;
     if (!mixed && B.on && B.nactive) {
         const StagedLaunch sl = staged_launch(B, 16, 32);
-// walk widths measured on the 1M dam break: four entries with the next four
-        // loaded ahead for the uniform density (0.27 -> 0.24 ms)
+        // walk widths measured on the 1M dam break: four entries with the next
+        // four loaded ahead for the uniform density (0.27 -> 0.24 ms)
 #define SPHX_DENS_STAGED(U, PER, W, PF)                                                                  \
         hipLaunchKernelGGL((k_density_staged<U, PER, W, PF>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B, \
                            cell_start, sl.cap)
@@ -437,9 +464,9 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
     else if (mixed)
         hipLaunchKernelGGL(k_density_mixed<false>, grid, block, 0, s, P, D, X, n, g, L, kp);
     else if (uni)
-        hipLaunchKernelGGL(k_density<true>, grid, block, occupancy_lds(4), s, P, D, n, g, L, kp);
+        hipLaunchKernelGGL(k_density<true>, dim3(nblocks(2 * n)), block, occupancy_lds(4), s, P, D, n, g, L, kp);
     else
-        hipLaunchKernelGGL(k_density<false>, grid, block, occupancy_lds(4), s, P, D, n, g, L, kp);
+        hipLaunchKernelGGL(k_density<false>, dim3(nblocks(2 * n)), block, occupancy_lds(4), s, P, D, n, g, L, kp);
 }
 
 // ------------------------- pressure + viscosity + Maxwell stress + walls (a6-a9, a13)
