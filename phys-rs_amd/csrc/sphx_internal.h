@@ -203,11 +203,11 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
                    const KernelParams& kp, bool visc_only, bool mhd_stress, bool charge, bool mixed,
                    const Mixed& X, const Bricks& B, const uint32_t* cell_start, hipStream_t s);
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
-                        const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint8_t* flip,
-                        hipStream_t s);
+                        const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint32_t* moved_host,
+                        uint8_t* flip, hipStream_t s);
 void launch_refresh_positions(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                               const Grid& g, const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved,
-                              hipStream_t s);
+                              uint32_t* moved_host, hipStream_t s);
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         bool mixed, const Mixed& X, hipStream_t s);
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,

@@ -31,23 +31,33 @@ __device__ __forceinline__ bool track_position(const Particles& P, const Derived
     return (dx * dx + dy * dy) + dz * dz > kp.half_skin_sq;
 }
 
+// The moved-skin/2 flag: the device word (sticky until it is cleared) and its
+// copy in host memory (pinned, fine-grained), which the host reads after an
+// event instead of copying the device word back.  Only the wave that finds
+// the device word clear writes the host word (one store across the bus per
+// epoch, however many waves raise the flag).
+__device__ __forceinline__ void raise_moved(uint32_t* moved, uint32_t* moved_host) {
+    if (atomicOr(moved, 1u) == 0u) *reinterpret_cast<volatile uint32_t*>(moved_host) = 1u;
+}
+
 // Positions replaced from outside (the adjoint's tape): same bookkeeping.
 __global__ void k_refresh_positions(Particles P, Derived D, uint32_t n, KernelParams kp, Grid g,
-                                    const uint32_t* __restrict__ skey, int mixed, Mixed X, uint32_t* moved) {
+                                    const uint32_t* __restrict__ skey, int mixed, Mixed X, uint32_t* moved,
+                                    uint32_t* moved_host) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool far = false;
     if (i < n) {
         const double x[3] = {P.x[i], P.y[i], P.z[i]};
         far = track_position(P, D, kp, g, skey, mixed, X, i, x);
     }
-    if (__ballot(far) && (threadIdx.x & 63u) == 0u) atomicOr(moved, 1u);
+    if (__ballot(far) && (threadIdx.x & 63u) == 0u) raise_moved(moved, moved_host);
 }
 void launch_refresh_positions(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                               const Grid& g, const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved,
-                              hipStream_t s) {
+                              uint32_t* moved_host, hipStream_t s) {
     if (n)
         hipLaunchKernelGGL(k_refresh_positions, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, g, skey,
-                           (int)mixed, X, moved);
+                           (int)mixed, X, moved, moved_host);
 }
 
 // velocity_verlet_position_system (integrators.rs:63-83), then the box:
@@ -58,7 +68,7 @@ void launch_refresh_positions(const Particles& P, const Derived& D, uint32_t n, 
 // *moved (see track_position).
 __global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams kp, Grid g,
                               const uint32_t* __restrict__ skey, int mixed, Mixed X, uint32_t* moved,
-                              uint8_t* flip) {
+                              uint32_t* moved_host, uint8_t* flip) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool far = false;
     if (i < n) {
@@ -71,34 +81,38 @@ __global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams k
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const double a = as[k][i];
-            double v = vs[k][i];
+            const double v = vs[k][i];
             double ap = a;
             double x = xs[k][i] + (v * kp.dt + a * kp.half_dt_sq);
             if (kp.bnd & SPHX_BND_REFLECT) {
                 const double lo = kp.box_lo[k], hi = kp.box_hi[k];
-                if (x < lo) { x = lo + (lo - x); v = -v; ap = -ap; bits |= 1u << k; }
-                else if (x > hi) { x = hi - (x - hi); v = -v; ap = -ap; bits |= 1u << k; }
+                const bool below = x < lo, above = x > hi;
+                if (below || above) {  // v changes only here: no store otherwise
+                    x = below ? lo + (lo - x) : hi - (x - hi);
+                    vs[k][i] = -v;
+                    ap = -ap;
+                    bits |= 1u << k;
+                }
             } else if (kp.bnd & SPHX_BND_PERIODIC) {
                 const double lo = kp.box_lo[k], hi = kp.box_hi[k], L = hi - lo;
                 if (x >= hi) x -= L;
                 else if (x < lo) x += L;
             }
             xs[k][i] = x;
-            vs[k][i] = v;
             aps[k][i] = ap;
             nx[k] = x;
         }
         if (flip) flip[P.orig[i]] = (uint8_t)bits;
         far = track_position(P, D, kp, g, skey, mixed, X, i, nx);
     }
-    if (__ballot(far) && (threadIdx.x & 63u) == 0u) atomicOr(moved, 1u);
+    if (__ballot(far) && (threadIdx.x & 63u) == 0u) raise_moved(moved, moved_host);
 }
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
-                        const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint8_t* flip,
-                        hipStream_t s) {
+                        const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint32_t* moved_host,
+                        uint8_t* flip, hipStream_t s) {
     if (n)
         hipLaunchKernelGGL(k_vv_position, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, g, skey, (int)mixed, X,
-                           moved, flip);
+                           moved, moved_host, flip);
 }
 
 // velocity_verlet_velocity_system (integrators.rs:92-104)

@@ -76,11 +76,11 @@ struct SphxCtx {
     Sort S{};
     Tiles T{};
     Bricks B{};
-    uint32_t* bricks_host = nullptr;       // pinned: [0] active brick count of the last sort,
-                                           // [1] the moved-skin/2 flag of the last position update
+    uint32_t* bricks_host = nullptr;       // pinned: active brick count of the last sort
     hipEvent_t bricks_ready = nullptr;     // recorded after that count's copy
     bool bricks_pending = false;           // count not read yet
-    hipEvent_t moved_ready = nullptr;      // recorded after the moved flag's copy
+    hipEvent_t moved_ready = nullptr;      // recorded after the kernel that raises the moved flag
+    uint32_t* moved_host = nullptr;        // pinned, fine-grained: the moved flag as the kernels leave it
     bool moved_pending = false;            // flag not read yet (passes enqueued on the kept list)
     NbrList L{};
     Mixed X{};
@@ -497,6 +497,7 @@ int do_neighbours(SphxCtx* c) {
     {
         Timer t(c, kNbr);
         launch_fill_u32(c->L.overflow + 2, 0u, 1, c->stream);  // displacements count from this build
+        launch_fill_u32(c->moved_host, 0u, 1, c->stream);
         launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->kp.uniform != 0, c->X, c->hmax, c->kp.skin,
                          c->stream);
     }
@@ -550,12 +551,16 @@ int do_forces(SphxCtx* c, bool visc_only) {
 // moved more than skin/2 since it was built (the kernel raises a flag).
 // defer: the flag travels to pinned memory and is read by settle_moved, so
 // that passes can be enqueued on the kept list before the host knows.
+// The moved flag as the last kernel that raises it left it (read after
+// moved_ready has completed).
+bool moved_flag(const SphxCtx* c) { return *reinterpret_cast<volatile const uint32_t*>(c->moved_host) != 0u; }
+
 int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr, bool defer = false) {
     if (!c->acc) return fail(c, SPHX_E_STATE, "vv_position needs accelerations (call sphx_forces)");
     {
         Timer t(c, kIntegrate);
-        launch_vv_position(c->P(), c->D, c->n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, c->L.overflow + 2, flip,
-                           c->stream);
+        launch_vv_position(c->P(), c->D, c->n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, c->L.overflow + 2,
+                           c->moved_host, flip, c->stream);
     }
     int rc;
     if ((rc = launch_check(c))) return rc;
@@ -563,15 +568,13 @@ int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr, bool defer = false) {
     ++c->list_age;
     bool rebuild = c->kp.skin <= 0.0 || !c->listed;
     if (!rebuild) {
-        HIP_TRY(c, hipMemcpyAsync(c->bricks_host + 1, c->L.overflow + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                  c->stream));
         HIP_TRY(c, hipEventRecord(c->moved_ready, c->stream));
         if (defer) {
             c->moved_pending = true;
             return SPHX_OK;
         }
         HIP_TRY(c, hipEventSynchronize(c->moved_ready));
-        rebuild = c->bricks_host[1] != 0;
+        rebuild = moved_flag(c);
         if (rebuild) c->last_life = c->list_age;
     }
     if (rebuild) c->sorted = c->listed = false;
@@ -596,7 +599,7 @@ int settle_moved(SphxCtx* c, bool& redo) {
     if (!c->moved_pending) return SPHX_OK;
     c->moved_pending = false;
     HIP_TRY(c, hipEventSynchronize(c->moved_ready));
-    if (c->bricks_host[1] != 0) {
+    if (moved_flag(c)) {
         c->last_life = c->list_age;
         c->sorted = c->listed = false;
         c->dens = c->sp = c->acc = false;
@@ -719,16 +722,15 @@ int adj_load(SphxCtx* c, const double* x3, const double* v3, bool defer = false)
         c->dens = c->sp = c->acc = false;
         if (c->listed && c->kp.skin > 0.0) {
             launch_fill_u32(c->L.overflow + 2, 0u, 1, c->stream);
+            launch_fill_u32(c->moved_host, 0u, 1, c->stream);
             launch_refresh_positions(P, c->D, n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, c->L.overflow + 2,
-                                     c->stream);
-            HIP_TRY(c, hipMemcpyAsync(c->bricks_host + 1, c->L.overflow + 2, sizeof(uint32_t),
-                                      hipMemcpyDeviceToHost, c->stream));
+                                     c->moved_host, c->stream);
             HIP_TRY(c, hipEventRecord(c->moved_ready, c->stream));
             if (defer) {
                 c->moved_pending = true;
             } else {
                 HIP_TRY(c, hipEventSynchronize(c->moved_ready));
-                if (c->bricks_host[1]) c->sorted = c->listed = false;
+                if (moved_flag(c)) c->sorted = c->listed = false;
             }
         } else {
             c->sorted = c->listed = false;
@@ -860,6 +862,7 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
         if (c->bricks_ready) (void)hipEventDestroy(c->bricks_ready);
         if (c->moved_ready) (void)hipEventDestroy(c->moved_ready);
         if (c->bricks_host) (void)hipHostFree(c->bricks_host);
+        if (c->moved_host) (void)hipHostFree(c->moved_host);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return code;
@@ -872,7 +875,7 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     // SPHX_SPECULATE=always|never overrides when a tick's second evaluation
     // is enqueued before its moved flag is read (the results do not change)
     if (const char* e = getenv("SPHX_SPECULATE")) c->spec_mode = !strcmp(e, "always") ? 1 : !strcmp(e, "never") ? 2 : 0;
-    if (hipHostMalloc((void**)&c->bricks_host, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&c->bricks_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         c->bricks_host = nullptr;
         return bail(SPHX_E_OOM);
     }
@@ -884,6 +887,11 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
         c->moved_ready = nullptr;
         return bail(SPHX_E_HIP);
     }
+    if (hipHostMalloc((void**)&c->moved_host, sizeof(uint32_t), hipHostMallocCoherent) != hipSuccess) {
+        c->moved_host = nullptr;
+        return bail(SPHX_E_OOM);
+    }
+    *c->moved_host = 0u;
     const size_t N = c->cap;
     for (int b = 0; b < 2; ++b) {
         Particles& P = c->A[b];
@@ -932,6 +940,7 @@ void sphx_destroy(SphxCtx* c) {
     if (c->bricks_ready) (void)hipEventDestroy(c->bricks_ready);
     if (c->moved_ready) (void)hipEventDestroy(c->moved_ready);
     if (c->bricks_host) (void)hipHostFree(c->bricks_host);
+    if (c->moved_host) (void)hipHostFree(c->moved_host);
     c->adj.release();
     free_all(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
