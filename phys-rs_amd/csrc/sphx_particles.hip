@@ -15,10 +15,6 @@ __device__ __forceinline__ bool track_position(const Particles& P, const Derived
     double2* pos = reinterpret_cast<double2*>(P.pos + i);
     pos[0] = make_double2(x[0], x[1]);
     pos[1] = make_double2(x[2], P.h[i]);
-    const double2* ref = reinterpret_cast<const double2*>(D.xref + i);
-    const double2 r01 = ref[0], r2_ = ref[1];
-    double dx, dy, dz;
-    pair_vec(g, r01.x, r01.y, r2_.x, x[0], x[1], x[2], dx, dy, dz);
     if (mixed) {
         int cx, cy, cz;
         decode_cell(skey[i], g, cx, cy, cz);
@@ -28,6 +24,11 @@ __device__ __forceinline__ bool track_position(const Particles& P, const Derived
             for (int k = 0; k < 3; ++k) min_image(d[k], g.L[k], g.halfL[k]);
         X.p4[i] = make_float4((float)d[0], (float)d[1], (float)d[2], (float)P.h[i]);
     }
+    if (!(kp.skin > 0.0)) return false;  // no skin: the list is rebuilt every tick anyway
+    const double2* ref = reinterpret_cast<const double2*>(D.xref + i);
+    const double2 r01 = ref[0], r2_ = ref[1];
+    double dx, dy, dz;
+    pair_vec(g, r01.x, r01.y, r2_.x, x[0], x[1], x[2], dx, dy, dz);
     return (dx * dx + dy * dy) + dz * dz > kp.half_skin_sq;
 }
 
@@ -36,8 +37,12 @@ __device__ __forceinline__ bool track_position(const Particles& P, const Derived
 // event instead of copying the device word back.  Only the wave that finds
 // the device word clear writes the host word (one store across the bus per
 // epoch, however many waves raise the flag).
+// The word is read first: once it is set (with a list rebuilt every tick,
+// every wave finds a particle beyond skin/2), waves skip the atomic instead
+// of queueing on one address.
 __device__ __forceinline__ void raise_moved(uint32_t* moved, uint32_t* moved_host) {
-    if (atomicOr(moved, 1u) == 0u) *reinterpret_cast<volatile uint32_t*>(moved_host) = 1u;
+    if (*reinterpret_cast<volatile const uint32_t*>(moved) == 0u && atomicOr(moved, 1u) == 0u)
+        *reinterpret_cast<volatile uint32_t*>(moved_host) = 1u;
 }
 
 // Positions replaced from outside (the adjoint's tape): same bookkeeping.
