@@ -63,8 +63,8 @@ STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
 # counter runs summarised in profiles/dam_break_1M_fp64_mixed_kernels.md.
 MEASURED_TRAFFIC = {
     "force": (559.7 + 50.8) * 1e6,
-    "visc_only": (577.0 + 25.4) * 1e6,
-    "density": (472.0 + 92.8) * 1e6,
+    "visc_only": (576.8 + 25.4) * 1e6,
+    "density": (472.1 + 92.8) * 1e6,
 }
 STEP_FLOPS_PER_PAIR = 26.0 + 58.0 + 45.0
 
