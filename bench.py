@@ -54,7 +54,9 @@ ALG = {
     "force":     {"bytes": 128, "flops_pair": 58.0},  # x, v, m, h, rho, P -> F_p, a*
     "visc_only": {"bytes": 120, "flops_pair": 45.0},  # x, v, m, h, rho, F_p -> a_n
     "integrate": {"bytes": 96, "flops_pair": 0.0},    # vv_position / vv_velocity, each 96
-    "mhd":       {"bytes": 200, "flops_pair": 0.0},
+    # induction (73 flops a pair: mhd.rs:68-152) and cleaning (38: mhd.rs:172-259),
+    # one launch each, so 55.5 per launch on average
+    "mhd":       {"bytes": 200, "flops_pair": 55.5},
 }
 STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
 
@@ -216,10 +218,16 @@ def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
             if v["launches"] == 0:
                 continue
             avg_s = v["ms"] / v["launches"] * 1e-3
+            gbs = ALG[k]["bytes"] * n / avg_s / 1e9
+            tflops = ALG[k]["flops_pair"] * pairs / avg_s / 1e12
+            # the mixed form's pair math is f32 in the density and force passes
+            # (against the f32 vector peak); the MHD passes are f64 in both
+            f32 = prec == "mixed" and k in ("density", "force", "visc_only")
             kern[k] = {"ms_per_step": round(v["ms"] / steps, 4), "avg_ms": round(avg_s * 1e3, 4),
-                       "launches": v["launches"],
-                       "alg_GBs": round(ALG[k]["bytes"] * n / avg_s / 1e9, 1),
-                       "alg_TFLOPs": round(ALG[k]["flops_pair"] * pairs / avg_s / 1e12, 3)}
+                       "launches": v["launches"], "alg_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                       "alg_TFLOPs": round(tflops, 3),
+                       "flop_frac": round(tflops / (FP32_PEAK_TFLOPS if f32 else FP64_PEAK_TFLOPS), 4),
+                       "flop_peak": "fp32 157.3" if f32 else "fp64 78.6"}
         out[prec] = {"ms_per_step": round(secs / steps * 1e3, 4), "particle_steps_per_s": round(n * steps / secs, 1),
                      "directed_neighbour_pairs": pairs, "max_neighbours": e["max_neighbours"], "kernels": kern}
     out["mixed_speedup"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
