@@ -176,7 +176,7 @@ struct KernelParams {          // what the kernels need from SphxParams, by valu
     // the pair passes then use these instead of staging m and h
     int uniform;
     double m0, h0, inv_h0;
-    int rec64;                 // keep the f64 gather record current (FP64 form, or MHD in either form)
+    int rec64;                 // keep the f64 gather record current (the FP64 form)
     double skin;               // neighbour-list skin (list radius 2 h_avg + skin)
     double half_skin_sq;       // (skin / 2)^2: displacement since the last build that forces a rebuild
 };
@@ -213,9 +213,9 @@ void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const 
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                  const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                      const KernelParams& kp, hipStream_t s);
+                      const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                      const KernelParams& kp, hipStream_t s);
+                      const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_export_pairs(const Particles& P, uint32_t n, const Grid& g, const NbrList& L, uint32_t* out,
                          unsigned long long* counter, uint64_t out_cap, hipStream_t s);
 void launch_energy(const Particles& P, const Derived& D, uint32_t n, const Grid& g,

@@ -377,13 +377,6 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
     rec[0] = make_float4(xi, yi, zi, (float)rho);
     rec[1] = make_float4((float)vxi, (float)vyi, (float)vzi, (float)pr2);
-    if (kp.rec64) {  // the f64 MHD passes gather the f64 record
-        double2* r = reinterpret_cast<double2*>(D.r64 + i);
-        r[0] = make_double2(P.x[i], P.y[i]);
-        r[1] = make_double2(P.z[i], rho);
-        r[2] = make_double2(vxi, vyi);
-        r[3] = make_double2(vzi, pr2);
-    }
 }
 
 template <bool UNI>
@@ -947,12 +940,108 @@ __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, ui
 }
 
 
+// FP32-mixed MHD passes: f32 pair terms on the cell-relative f32 record (as
+// the mixed force pass), the neighbour's B converted to f32, each group of
+// four list entries summed in f32 and added to f64 sums; B, psi and the
+// update stay f64.  dW/dr / r with its 0.75 folded into sigma_g.
+struct MixedPairG {
+    float dx, dy, dz, r2, gx, gy, gz, h_avg;
+};
+template <bool UNI>
+__device__ __forceinline__ MixedPairG mixed_pair_g(const Mixed& X, float hi, float inv_h0, float h0, float sig0,
+                                                   uint32_t e, const float4& a, float xi, float yi, float zi,
+                                                   float wx, float wy, float wz) {
+    MixedPairG p;
+    float ox, oy, oz;
+    entry_offset(e, wx, wy, wz, ox, oy, oz);
+    p.dx = (a.x - xi) + ox;
+    p.dy = (a.y - yi) + oy;
+    p.dz = (a.z - zi) + oz;
+    p.r2 = fmaf(p.dz, p.dz, fmaf(p.dy, p.dy, p.dx * p.dx));
+    // r <= 1e-10: no term (1/r = 0 gives q = 0 and dW/dq(0) = 0)
+    const float inv_r = p.r2 > 1e-20f ? __builtin_amdgcn_rsqf(p.r2) : 0.0f;
+    float inv_h = inv_h0, sig = sig0;
+    p.h_avg = h0;
+    if (!UNI) {
+        p.h_avg = 0.5f * (hi + X.p4[e & kIdxMask].w);
+        inv_h = __builtin_amdgcn_rcpf(p.h_avg);
+        const float ih2 = inv_h * inv_h;
+        sig = (ih2 * ih2) * (float)(0.75 * kInvPi);
+    }
+    const float dwr = (sig * dw43_shape_f((p.r2 * inv_r) * inv_h)) * inv_r;
+    p.gx = p.dx * dwr;
+    p.gy = p.dy * dwr;
+    p.gz = p.dz * dwr;
+    return p;
+}
+
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_induction_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
+                                                             NbrList L, KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
+    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
+    const float4 own0 = R[2 * i], own1 = R[2 * i + 1];
+    const float xi = own0.x, yi = own0.y, zi = own0.z, rhoi = own0.w;
+    const float vxi = own1.x, vyi = own1.y, vzi = own1.z;
+    const double bxd = P.bx[i], byd = P.by[i], bzd = P.bz[i];
+    const float bxi = (float)bxd, byi = (float)byd, bzi = (float)bzd;
+    const uint32_t idi = P.id[i];
+    const float hi = X.p4[i].w, inv_rhoi = __builtin_amdgcn_rcpf(rhoi);
+    const float inv_h0 = (float)kp.inv_h0, h0 = (float)kp.h0, m0 = (float)kp.m0;
+    const float sig0 = (float)((kp.inv_h0 * kp.inv_h0) * (kp.inv_h0 * kp.inv_h0) * (0.75 * kInvPi));
+    const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
+    const float eta2 = (float)(2.0 * kp.eta), dt = (float)kp.dt;
+    double u[3] = {0, 0, 0};
+    float gu[3] = {0, 0, 0};
+    walk(L, i, GlobalFetch<float4, 2>{R}, [&](uint32_t e, const float4 (&q)[2]) {
+        const uint32_t j = e & kIdxMask;
+        const MixedPairG p = mixed_pair_g<UNI>(X, hi, inv_h0, h0, sig0, e, q[0], xi, yi, zi, wx, wy, wz);
+        const float vx = vxi - q[1].x, vy = vyi - q[1].y, vz = vzi - q[1].z;
+        const float rhoj = q[0].w, mj = UNI ? m0 : X.m[j];
+        const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
+        const float bxj = (float)bxy.x, byj = (float)bxy.y, bzj = (float)bzp.x;
+        const float v_dot_g = (vx * p.gx + vy * p.gy) + vz * p.gz;
+        const float b_dot_g = (bxi * p.gx + byi * p.gy) + bzi * p.gz;
+        const float inv_rhoj = __builtin_amdgcn_rcpf(rhoj);
+        const float f = mj * (inv_rhoi * inv_rhoj);
+        const float r_dot_g = (p.dx * p.gx + p.dy * p.gy) + p.dz * p.gz;
+        const float res = ((eta2 * (mj * inv_rhoj)) * r_dot_g) *
+                          __builtin_amdgcn_rcpf(fmaf(0.01f * p.h_avg, p.h_avg, p.r2));
+        const float sg = (idi < P.id[j]) ? dt : -dt;
+        gu[0] += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (bxj - bxi));
+        gu[1] += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (byj - byi));
+        gu[2] += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (bzj - bzi));
+    }, [&]() {
+        for (int a = 0; a < 3; ++a) {
+            u[a] += (double)gu[a];
+            gu[a] = 0.0f;
+        }
+    });
+    const double bx = bxd + u[0], by = byd + u[1], bz = bzd + u[2];
+    D.tmp3[0][i] = bx;
+    D.tmp3[1][i] = by;
+    D.tmp3[2][i] = bz;
+    double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
+    rb[0] = make_double2(bx, by);
+    rb[1] = make_double2(bz, P.psi[i]);
+}
+
 // Unstaged: brick-staged forms of these two passes (96-byte records, one
 // workgroup per CU) measured slower on the 4M periodic MHD box (5.1 against
 // 4.7 ms per step for the pair) than these gathers at three workgroups per CU.
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                      const KernelParams& kp, hipStream_t s) {
+                      const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
     if (!n) return;
+    if (mixed) {
+        if (kp.uniform)
+            hipLaunchKernelGGL(k_induction_mixed<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, X,
+                               n, g, L, kp);
+        else
+            hipLaunchKernelGGL(k_induction_mixed<false>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, X,
+                               n, g, L, kp);
+        return;
+    }
     if (kp.uniform)
         hipLaunchKernelGGL(k_induction<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L, kp);
     else
@@ -1009,9 +1098,66 @@ __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, ui
                                 GlobalFetch<double2, 4, 2>{reinterpret_cast<const double2*>(D.r64),
                                                            reinterpret_cast<const double2*>(D.recb)});
 }
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
+                                                             NbrList L, KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
+    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
+    const float4 own0 = R[2 * i];
+    const float xi = own0.x, yi = own0.y, zi = own0.z;
+    const double bxd = P.bx[i], byd = P.by[i], bzd = P.bz[i], psid = P.psi[i];
+    const float bxi = (float)bxd, byi = (float)byd, bzi = (float)bzd, psii = (float)psid;
+    const float hi = X.p4[i].w;
+    const float inv_h0 = (float)kp.inv_h0, h0 = (float)kp.h0, m0 = (float)kp.m0;
+    const float sig0 = (float)((kp.inv_h0 * kp.inv_h0) * (kp.inv_h0 * kp.inv_h0) * (0.75 * kInvPi));
+    const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
+    double div = 0, gp[3] = {0, 0, 0};
+    float gdiv = 0, ggp[3] = {0, 0, 0};
+    // the second quad of the record (velocity) is not used: its load is dropped
+    walk(L, i, GlobalFetch<float4, 2>{R}, [&](uint32_t e, const float4 (&q)[2]) {
+        const uint32_t j = e & kIdxMask;
+        const MixedPairG p = mixed_pair_g<UNI>(X, hi, inv_h0, h0, sig0, e, q[0], xi, yi, zi, wx, wy, wz);
+        const float vol = (UNI ? m0 : X.m[j]) * __builtin_amdgcn_rcpf(q[0].w);
+        const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
+        gdiv += vol * ((((float)bxy.x - bxi) * p.gx + ((float)bxy.y - byi) * p.gy) + ((float)bzp.x - bzi) * p.gz);
+        const float dp = vol * ((float)bzp.y - psii);
+        ggp[0] = fmaf(dp, p.gx, ggp[0]);
+        ggp[1] = fmaf(dp, p.gy, ggp[1]);
+        ggp[2] = fmaf(dp, p.gz, ggp[2]);
+    }, [&]() {
+        div += (double)gdiv;
+        gdiv = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            gp[a] += (double)ggp[a];
+            ggp[a] = 0.0f;
+        }
+    });
+    const double dt = kp.dt;
+    const double bx = bxd - gp[0] * dt, by = byd - gp[1] * dt, bz = bzd - gp[2] * dt;
+    D.tmp3[0][i] = bx;
+    D.tmp3[1][i] = by;
+    D.tmp3[2][i] = bz;
+    const double decay = exp(-kp.sigma_c * kp.ch * dt / P.h[i]);
+    const double psi = (psid - (kp.ch * kp.ch) * div * dt) * decay;
+    D.tmp1[i] = psi;
+    double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
+    rb[0] = make_double2(bx, by);
+    rb[1] = make_double2(bz, psi);
+}
+
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                      const KernelParams& kp, hipStream_t s) {
+                      const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
     if (!n) return;
+    if (mixed) {
+        if (kp.uniform)
+            hipLaunchKernelGGL(k_div_clean_mixed<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, X,
+                               n, g, L, kp);
+        else
+            hipLaunchKernelGGL(k_div_clean_mixed<false>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, X,
+                               n, g, L, kp);
+        return;
+    }
     if (kp.uniform)
         hipLaunchKernelGGL(k_div_clean<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L, kp);
     else

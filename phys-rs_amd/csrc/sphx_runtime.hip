@@ -204,7 +204,7 @@ void make_kernel_params(SphxCtx* c) {
     k.bnd = p.boundary_flags;
     k.mhd = p.mhd_flags;
     k.uniform = c->uniform ? 1 : 0;
-    k.rec64 = (!c->mixed() || c->has_mhd) ? 1 : 0;
+    k.rec64 = c->mixed() ? 0 : 1;
     k.skin = p.neighbour_skin < 0.0 ? 0.04 * c->hmax : p.neighbour_skin;
     k.half_skin_sq = 0.25 * k.skin * k.skin;
     k.m0 = c->m0;
@@ -646,7 +646,7 @@ int do_induction(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_induction(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->stream);
+        launch_induction(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
     }
     Particles& P = c->P();
     swap3(P.bx, P.by, P.bz, c->D.tmp3);
@@ -661,7 +661,7 @@ int do_div_clean(SphxCtx* c) {
     if (!c->dens && (rc = do_density(c))) return rc;
     {
         Timer t(c, kMhd);
-        launch_div_clean(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->stream);
+        launch_div_clean(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->mixed(), c->X, c->stream);
     }
     Particles& P = c->P();
     swap3(P.bx, P.by, P.bz, c->D.tmp3);

@@ -258,6 +258,39 @@ def test_mhd_passes_parity(require_gpu):
     assert np.abs(b2["psi"] - s["psi"]).max() < 1e-12 * np.abs(s["psi"] - psi0).max()
 
 
+def test_mixed_mhd_passes_error(require_gpu):
+    # FP32-mixed induction and cleaning: f32 pair terms, f64 sums and update.
+    # The change of B (and psi) is compared with the oracle's f64 change at
+    # the same state, relative to its largest component.
+    st, p = scenarios.mhd_periodic(16)
+    p.resistivity = 1e-3
+    p.precision = sphx.PREC_FP32_MIXED
+    with make_ctx(st, p) as ctx:
+        ctx.forces()
+        ctx.mhd_induction()
+        b1 = ctx.download(sphx.F_B)
+        ctx.div_clean()
+        b2 = ctx.download(sphx.F_B | sphx.F_PSI)
+    p.precision = sphx.PREC_FP64
+    s = O.State(st)
+    pr = O.pairs(s, p)
+    O.evaluate(s, p, pr)
+    b0 = {k: st[k].copy() for k in ("bx", "by", "bz")}
+    O.mhd_induction(s, p, pr)
+    for k in ("bx", "by", "bz"):
+        d_ref = s[k] - b0[k]
+        assert np.abs(b1[k] - s[k]).max() < 2e-5 * np.abs(d_ref).max(), k
+    psi0 = s["psi"].copy()
+    bb = {k: s[k].copy() for k in ("bx", "by", "bz")}
+    O.div_clean(s, p, pr)
+    # cleaning's own change of B (zero here: psi starts at 0) and of psi
+    for k in ("bx", "by", "bz"):
+        d_ref = s[k] - bb[k]
+        assert np.abs((b2[k] - b1[k]) - d_ref).max() <= 2e-5 * np.abs(d_ref).max(), k
+    d_psi = s["psi"] - psi0
+    assert np.abs(b2["psi"] - s["psi"]).max() < 1e-4 * np.abs(d_psi).max()
+
+
 def test_xsph_parity(require_gpu):
     st, p = scenarios.dam_break(nx=16, ny=16, nz=16)
     st = with_velocity(st, 0.4)
