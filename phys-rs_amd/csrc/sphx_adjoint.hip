@@ -56,30 +56,6 @@ void launch_gather_ab(const uint32_t* orig, uint32_t n, const Vec3& src, double4
     if (n) hipLaunchKernelGGL(k_gather_ab, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, ab4);
 }
 
-// velocities (upload order) -> sorted state and the f64 gather record
-__global__ void k_load_velocity(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, Particles P, Derived D) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t u = orig[i];
-    const double vx = src.x[u], vy = src.y[u], vz = src.z[u];
-    P.vx[i] = vx; P.vy[i] = vy; P.vz[i] = vz;
-    D.r64[i].vx = vx; D.r64[i].vy = vy; D.r64[i].vz = vz;
-}
-void launch_load_velocity(const Particles& P, const Derived& D, uint32_t n, const Vec3& src, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_load_velocity, dim3(nblocks(n)), dim3(kBlock), 0, s, P.orig, n, src, P, D);
-}
-
-// dst (sorted) = src (upload order) through orig
-__global__ void k_gather3(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, Vec3 dst) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t u = orig[i];
-    dst.x[i] = src.x[u]; dst.y[i] = src.y[u]; dst.z[i] = src.z[u];
-}
-void launch_gather3(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_gather3, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, dst);
-}
-
 // dst (upload order) = src (sorted) through orig
 __global__ void k_scatter3(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, Vec3 dst) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -202,12 +202,24 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                    const KernelParams& kp, bool visc_only, bool mhd_stress, bool charge, bool mixed,
                    const Mixed& X, const Bricks& B, const uint32_t* cell_start, hipStream_t s);
+// The moved-skin/2 flag of the position updates: the device word and its
+// copy in pinned host memory, both holding the epoch in which a particle was
+// last found beyond skin/2.  The host starts a new epoch at each list build
+// and each replacement of the positions, so nothing needs clearing.
+struct MovedFlag {
+    uint32_t* dev;
+    uint32_t* host;
+    uint32_t epoch;
+};
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
-                        const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint32_t* moved_host,
-                        uint8_t* flip, hipStream_t s);
-void launch_refresh_positions(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
-                              const Grid& g, const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved,
-                              uint32_t* moved_host, hipStream_t s);
+                        const uint32_t* skey, bool mixed, const Mixed& X, const MovedFlag& mf, uint8_t* flip,
+                        hipStream_t s);
+// State (upload order, through orig) -> sorted arrays: velocities and their
+// record copies, and, with x3, positions with the position bookkeeping of a
+// position update (the moved flag raised in epoch mf.epoch when mf.dev).
+void launch_load_state(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
+                       const uint32_t* skey, bool mixed, const Mixed& X, const double* x3, const double* v3,
+                       const MovedFlag& mf, hipStream_t s);
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
                         bool mixed, const Mixed& X, hipStream_t s);
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
@@ -228,8 +240,6 @@ uint32_t energy_blocks(uint32_t n);
 void launch_adj_seed(const uint32_t* orig, uint32_t n, double hdt, const Vec3& vb, const Vec3& apb,
                      double4* ab4, hipStream_t s);
 void launch_gather_ab(const uint32_t* orig, uint32_t n, const Vec3& src, double4* ab4, hipStream_t s);
-void launch_load_velocity(const Particles& P, const Derived& D, uint32_t n, const Vec3& src, hipStream_t s);
-void launch_gather3(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst_s, hipStream_t s);
 void launch_scatter3(const uint32_t* orig, uint32_t n, const Vec3& src_s, const Vec3& dst, hipStream_t s);
 void launch_adj_flip(uint32_t n, const uint8_t* flip, double dt, double hdt2, const Vec3& xb, const Vec3& vb,
                      const Vec3& apb, const Vec3& ab, hipStream_t s);

@@ -32,37 +32,51 @@ __device__ __forceinline__ bool track_position(const Particles& P, const Derived
     return (dx * dx + dy * dy) + dz * dz > kp.half_skin_sq;
 }
 
-// The moved-skin/2 flag: the device word (sticky until it is cleared) and its
-// copy in host memory (pinned, fine-grained), which the host reads after an
-// event instead of copying the device word back.  Only the wave that finds
-// the device word clear writes the host word (one store across the bus per
-// epoch, however many waves raise the flag).
-// The word is read first: once it is set (with a list rebuilt every tick,
-// every wave finds a particle beyond skin/2), waves skip the atomic instead
-// of queueing on one address.
-__device__ __forceinline__ void raise_moved(uint32_t* moved, uint32_t* moved_host) {
-    if (*reinterpret_cast<volatile const uint32_t*>(moved) == 0u && atomicOr(moved, 1u) == 0u)
-        *reinterpret_cast<volatile uint32_t*>(moved_host) = 1u;
+// Raise the moved flag in epoch mf.epoch.  The device word is read first:
+// once it holds the epoch (with a list rebuilt every tick every wave finds a
+// particle beyond skin/2), waves skip the atomic instead of queueing on one
+// address, and only the wave that moves it to the epoch writes the host word
+// (one store across the bus per epoch).
+__device__ __forceinline__ void raise_moved(const MovedFlag& mf) {
+    if (*reinterpret_cast<volatile const uint32_t*>(mf.dev) != mf.epoch && atomicMax(mf.dev, mf.epoch) < mf.epoch)
+        *reinterpret_cast<volatile uint32_t*>(mf.host) = mf.epoch;
 }
 
-// Positions replaced from outside (the adjoint's tape): same bookkeeping.
-__global__ void k_refresh_positions(Particles P, Derived D, uint32_t n, KernelParams kp, Grid g,
-                                    const uint32_t* __restrict__ skey, int mixed, Mixed X, uint32_t* moved,
-                                    uint32_t* moved_host) {
+// State replaced from outside (the adjoint's tape and checkpoints), in one
+// pass: velocities (state and record), and positions with the bookkeeping of
+// a position update.
+__global__ void k_load_state(Particles P, Derived D, uint32_t n, KernelParams kp, Grid g,
+                             const uint32_t* __restrict__ skey, int mixed, Mixed X, Vec3 xs, Vec3 vs, MovedFlag mf) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool far = false;
     if (i < n) {
-        const double x[3] = {P.x[i], P.y[i], P.z[i]};
-        far = track_position(P, D, kp, g, skey, mixed, X, i, x);
+        const uint32_t u = P.orig[i];
+        const double vx = vs.x[u], vy = vs.y[u], vz = vs.z[u];
+        P.vx[i] = vx;
+        P.vy[i] = vy;
+        P.vz[i] = vz;
+        set_record_velocity(D, kp, i, mixed != 0, vx, vy, vz);
+        if (xs.x) {
+            const double x[3] = {xs.x[u], xs.y[u], xs.z[u]};
+            P.x[i] = x[0];
+            P.y[i] = x[1];
+            P.z[i] = x[2];
+            far = track_position(P, D, kp, g, skey, mixed, X, i, x) && mf.dev;
+        }
     }
-    if (__ballot(far) && (threadIdx.x & 63u) == 0u) raise_moved(moved, moved_host);
+    if (__ballot(far) && (threadIdx.x & 63u) == 0u) raise_moved(mf);
 }
-void launch_refresh_positions(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp,
-                              const Grid& g, const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved,
-                              uint32_t* moved_host, hipStream_t s) {
+void launch_load_state(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
+                       const uint32_t* skey, bool mixed, const Mixed& X, const double* x3, const double* v3,
+                       const MovedFlag& mf, hipStream_t s) {
+    const size_t N = n;
+    double* xp = const_cast<double*>(x3);
+    double* vp = const_cast<double*>(v3);
+    const Vec3 xs = x3 ? Vec3{xp, xp + N, xp + 2 * N} : Vec3{nullptr, nullptr, nullptr};
+    const Vec3 vs{vp, vp + N, vp + 2 * N};
     if (n)
-        hipLaunchKernelGGL(k_refresh_positions, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, g, skey,
-                           (int)mixed, X, moved, moved_host);
+        hipLaunchKernelGGL(k_load_state, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, g, skey, (int)mixed, X,
+                           xs, vs, mf);
 }
 
 // velocity_verlet_position_system (integrators.rs:63-83), then the box:
@@ -72,8 +86,8 @@ void launch_refresh_positions(const Particles& P, const Derived& D, uint32_t n, 
 // that has moved more than skin/2 since the last neighbour build raises
 // *moved (see track_position).
 __global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams kp, Grid g,
-                              const uint32_t* __restrict__ skey, int mixed, Mixed X, uint32_t* moved,
-                              uint32_t* moved_host, uint8_t* flip) {
+                              const uint32_t* __restrict__ skey, int mixed, Mixed X, MovedFlag mf,
+                              uint8_t* flip) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool far = false;
     if (i < n) {
@@ -110,14 +124,14 @@ __global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams k
         if (flip) flip[P.orig[i]] = (uint8_t)bits;
         far = track_position(P, D, kp, g, skey, mixed, X, i, nx);
     }
-    if (__ballot(far) && (threadIdx.x & 63u) == 0u) raise_moved(moved, moved_host);
+    if (__ballot(far) && (threadIdx.x & 63u) == 0u) raise_moved(mf);
 }
 void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, const Grid& g,
-                        const uint32_t* skey, bool mixed, const Mixed& X, uint32_t* moved, uint32_t* moved_host,
-                        uint8_t* flip, hipStream_t s) {
+                        const uint32_t* skey, bool mixed, const Mixed& X, const MovedFlag& mf, uint8_t* flip,
+                        hipStream_t s) {
     if (n)
         hipLaunchKernelGGL(k_vv_position, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, g, skey, (int)mixed, X,
-                           moved, moved_host, flip);
+                           mf, flip);
 }
 
 // velocity_verlet_velocity_system (integrators.rs:92-104)

@@ -81,6 +81,7 @@ struct SphxCtx {
     bool bricks_pending = false;           // count not read yet
     hipEvent_t moved_ready = nullptr;      // recorded after the kernel that raises the moved flag
     uint32_t* moved_host = nullptr;        // pinned, fine-grained: the moved flag as the kernels leave it
+    uint32_t epoch = 1;                    // epoch of the moved flag (see MovedFlag); the words start at 0
     bool moved_pending = false;            // flag not read yet (passes enqueued on the kept list)
     NbrList L{};
     Mixed X{};
@@ -496,8 +497,7 @@ int do_neighbours(SphxCtx* c) {
     }
     {
         Timer t(c, kNbr);
-        launch_fill_u32(c->L.overflow + 2, 0u, 1, c->stream);  // displacements count from this build
-        launch_fill_u32(c->moved_host, 0u, 1, c->stream);
+        ++c->epoch;  // displacements count from this build
         launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->kp.uniform != 0, c->X, c->hmax, c->kp.skin,
                          c->stream);
     }
@@ -553,14 +553,17 @@ int do_forces(SphxCtx* c, bool visc_only) {
 // that passes can be enqueued on the kept list before the host knows.
 // The moved flag as the last kernel that raises it left it (read after
 // moved_ready has completed).
-bool moved_flag(const SphxCtx* c) { return *reinterpret_cast<volatile const uint32_t*>(c->moved_host) != 0u; }
+bool moved_flag(const SphxCtx* c) {
+    return *reinterpret_cast<volatile const uint32_t*>(c->moved_host) == c->epoch;
+}
+MovedFlag moved_flag_args(SphxCtx* c) { return MovedFlag{c->L.overflow + 2, c->moved_host, c->epoch}; }
 
 int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr, bool defer = false) {
     if (!c->acc) return fail(c, SPHX_E_STATE, "vv_position needs accelerations (call sphx_forces)");
     {
         Timer t(c, kIntegrate);
-        launch_vv_position(c->P(), c->D, c->n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, c->L.overflow + 2,
-                           c->moved_host, flip, c->stream);
+        launch_vv_position(c->P(), c->D, c->n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, moved_flag_args(c), flip,
+                           c->stream);
     }
     int rc;
     if ((rc = launch_check(c))) return rc;
@@ -715,16 +718,17 @@ int adj_alloc(SphxCtx* c, uint32_t n, uint32_t k, uint32_t m) {
 int adj_load(SphxCtx* c, const double* x3, const double* v3, bool defer = false) {
     const uint32_t n = c->n;
     Particles& P = c->P();
-    launch_load_velocity(P, c->D, n, vec3(const_cast<double*>(v3), n), c->stream);
     c->acc = false;
+    // the moved flag is tested against the list's build positions in a new
+    // epoch (the replaced positions start it afresh)
+    const bool track = x3 && c->listed && c->kp.skin > 0.0;
+    if (track) ++c->epoch;
+    MovedFlag mf = moved_flag_args(c);
+    if (!track) mf.dev = nullptr;
+    launch_load_state(P, c->D, n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, x3, v3, mf, c->stream);
     if (x3) {
-        launch_gather3(P.orig, n, vec3(const_cast<double*>(x3), n), pos3(P), c->stream);
         c->dens = c->sp = c->acc = false;
-        if (c->listed && c->kp.skin > 0.0) {
-            launch_fill_u32(c->L.overflow + 2, 0u, 1, c->stream);
-            launch_fill_u32(c->moved_host, 0u, 1, c->stream);
-            launch_refresh_positions(P, c->D, n, c->kp, c->grid, c->S.skey, c->mixed(), c->X, c->L.overflow + 2,
-                                     c->moved_host, c->stream);
+        if (track) {
             HIP_TRY(c, hipEventRecord(c->moved_ready, c->stream));
             if (defer) {
                 c->moved_pending = true;
