@@ -56,15 +56,25 @@ void launch_gather_ab(const uint32_t* orig, uint32_t n, const Vec3& src, double4
     if (n) hipLaunchKernelGGL(k_gather_ab, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, ab4);
 }
 
-// dst (upload order) = src (sorted) through orig
-__global__ void k_scatter3(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, Vec3 dst) {
+// dst (upload order) = src (sorted) through orig; with src2.x set, also
+// dst2 = src2 (positions and velocities of the tape in one pass)
+__global__ void k_scatter3(const uint32_t* __restrict__ orig, uint32_t n, Vec3 src, Vec3 dst, Vec3 src2,
+                           Vec3 dst2) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t u = orig[i];
     dst.x[u] = src.x[i]; dst.y[u] = src.y[i]; dst.z[u] = src.z[i];
+    if (src2.x) {
+        dst2.x[u] = src2.x[i]; dst2.y[u] = src2.y[i]; dst2.z[u] = src2.z[i];
+    }
 }
 void launch_scatter3(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_scatter3, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, dst);
+    const Vec3 none{nullptr, nullptr, nullptr};
+    if (n) hipLaunchKernelGGL(k_scatter3, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, dst, none, none);
+}
+void launch_scatter6(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst, const Vec3& src2,
+                     const Vec3& dst2, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_scatter3, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, src, dst, src2, dst2);
 }
 
 // Reverse of the position update and wall reflection (integrators.rs:63-83):

@@ -241,6 +241,8 @@ void launch_adj_seed(const uint32_t* orig, uint32_t n, double hdt, const Vec3& v
                      double4* ab4, hipStream_t s);
 void launch_gather_ab(const uint32_t* orig, uint32_t n, const Vec3& src, double4* ab4, hipStream_t s);
 void launch_scatter3(const uint32_t* orig, uint32_t n, const Vec3& src_s, const Vec3& dst, hipStream_t s);
+void launch_scatter6(const uint32_t* orig, uint32_t n, const Vec3& src, const Vec3& dst, const Vec3& src2,
+                     const Vec3& dst2, hipStream_t s);
 void launch_adj_flip(uint32_t n, const uint8_t* flip, double dt, double hdt2, const Vec3& xb, const Vec3& vb,
                      const Vec3& apb, const Vec3& ab, hipStream_t s);
 void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const Vec3& xb, double* part,

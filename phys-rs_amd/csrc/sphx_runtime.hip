@@ -745,8 +745,9 @@ int adj_load(SphxCtx* c, const double* x3, const double* v3, bool defer = false)
 void adj_save(SphxCtx* c, double* x3, double* v3) {
     const uint32_t n = c->n;
     Particles& P = c->P();
-    if (x3) launch_scatter3(P.orig, n, pos3(P), vec3(x3, n), c->stream);
-    if (v3) launch_scatter3(P.orig, n, vel3(P), vec3(v3, n), c->stream);
+    if (x3 && v3) launch_scatter6(P.orig, n, pos3(P), vec3(x3, n), vel3(P), vec3(v3, n), c->stream);
+    else if (x3) launch_scatter3(P.orig, n, pos3(P), vec3(x3, n), c->stream);
+    else if (v3) launch_scatter3(P.orig, n, vel3(P), vec3(v3, n), c->stream);
 }
 
 // Density and the second force evaluation of a tick at the new positions,
