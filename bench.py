@@ -220,9 +220,8 @@ def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
             avg_s = v["ms"] / v["launches"] * 1e-3
             gbs = ALG[k]["bytes"] * n / avg_s / 1e9
             tflops = ALG[k]["flops_pair"] * pairs / avg_s / 1e12
-            # the mixed form's pair math is f32 in the density and force passes
-            # (against the f32 vector peak); the MHD passes are f64 in both
-            f32 = prec == "mixed" and k in ("density", "force", "visc_only")
+            # the mixed form's pair math is f32 (against the f32 vector peak)
+            f32 = prec == "mixed" and k in ("density", "force", "visc_only", "mhd")
             kern[k] = {"ms_per_step": round(v["ms"] / steps, 4), "avg_ms": round(avg_s * 1e3, 4),
                        "launches": v["launches"], "alg_GBs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                        "alg_TFLOPs": round(tflops, 3),
