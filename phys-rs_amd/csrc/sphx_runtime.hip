@@ -586,13 +586,14 @@ int do_vv_position(SphxCtx* c, uint8_t* flip = nullptr, bool defer = false) {
 
 // Whether to enqueue the passes of a tick before its moved flag is read: a
 // stale list costs a discarded density and force pass, so only while the
-// list is young compared with the previous one's lifetime (and that lifetime
-// is long), or, before any list has gone stale, once it has survived 4 ticks.
+// list is young compared with the previous one's lifetime (younger than it
+// by more than two ticks), or, before any list has gone stale, once it has
+// survived 4 ticks.
 bool speculate(const SphxCtx* c) {
     if (!c->listed || c->kp.skin <= 0.0 || c->spec_mode == 2) return false;
     if (c->spec_mode == 1) return true;
     const uint32_t age = c->list_age + 1;  // after this tick's position update
-    return c->last_life ? (c->last_life >= 8 && age + 2 < c->last_life) : age >= 4;
+    return c->last_life ? age + 2 < c->last_life : age >= 4;
 }
 
 // Reads the flag of a deferred position update.  If a particle moved more
