@@ -799,7 +799,7 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
         if (visc_only == V && stress == M && uni == U) {                                                    \
             if (mixed) {                                                                                    \
                 const StagedLaunch sl = staged_launch(B, 16, 32);                                            \
-                hipLaunchKernelGGL((k_forces_mixed_staged<V, U, 4, false>), sl.grid, sl.block, sl.dyn, s, P, D, X, \
+                hipLaunchKernelGGL((k_forces_mixed_staged<V, U, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, X, \
                                    g, L, kp, ch, B, cell_start, sl.cap);                                    \
             } else {                                                                                        \
                 StagedLaunch sl = staged_launch(B, 8, M ? 96 : 64);                                         \
