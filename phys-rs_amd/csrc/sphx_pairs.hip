@@ -694,7 +694,7 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     // once per group; with uniform m and h, m_i m_j sigma_g is folded out
     float gp[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
     // branch-free pair body, as in the f64 pass (factors 1 or 0, not selects)
-    walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[2]) {
+    walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[Fetch::NR]) {
         const uint32_t j = e & kIdxMask;
         const float4 a = q[0], b = q[1];
         float ox, oy, oz;
@@ -737,7 +737,18 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         }
         if (STRESS) {
             const float gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;
-            const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
+            // B_j: quads 2 and 3 of a staged record hold the bits of (bx, by)
+            // and (bz, psi); unstaged it is read from the B records
+            double2 bxy, bzp;
+            if constexpr (Fetch::NR >= 4) {
+                const float4 u = q[2], v = q[3];
+                bxy = make_double2(__hiloint2double(__float_as_int(u.y), __float_as_int(u.x)),
+                                   __hiloint2double(__float_as_int(u.w), __float_as_int(u.z)));
+                bzp = make_double2(__hiloint2double(__float_as_int(v.y), __float_as_int(v.x)), 0.0);
+            } else {
+                bxy = RB[2 * j];
+                bzp = RB[2 * j + 1];
+            }
             const float bxj = (float)bxy.x, byj = (float)bxy.y, bzj = (float)bzp.x;
             const float irho2j = __builtin_amdgcn_rcpf(rhoj * rhoj);
             const float hb2j = 0.5f * ((bxj * bxj + byj * byj) + bzj * bzj);
@@ -774,17 +785,25 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
                                                  GlobalFetch<float4, 2>{reinterpret_cast<const float4*>(D.r32)});
 }
 
-template <bool VISC_ONLY, bool UNI, int W, bool PF>
-__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4))) k_forces_mixed_staged(Particles P, Derived D, Mixed X, Grid g,
-                                                                      NbrList L, KernelParams kp, int charge,
-                                                                      Bricks B, const uint32_t* __restrict__ start,
-                                                                      uint32_t cap) {
+// With STRESS the B records (bx, by, bz, psi) in f64 are staged beside the
+// f32 records as quads 2 and 3 (64 bytes a record); its halo fills one CU's
+// LDS, so the registers are sized for one workgroup per CU (two waves per
+// SIMD), which keeps the sixteen record quads of a four-entry step out of
+// scratch memory.
+template <bool VISC_ONLY, bool STRESS, bool UNI, int W, bool PF>
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(STRESS ? 2 : 4)))
+k_forces_mixed_staged(
+    Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, int charge, Bricks B,
+    const uint32_t* __restrict__ start, uint32_t cap) {
+    constexpr int NQ2 = STRESS ? 2 : 0;
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    brick_pass<float4, 2, W, PF>(B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
-                              [&](uint32_t i, const BrickFetch<float4, 2, W, PF>& f) {
-                              force_mixed_particle<VISC_ONLY, false, UNI>(P, D, X, g, L, kp, charge, i, f);
-                          });
+    brick_pass<float4, 2, W, PF, 1, NQ2>(
+        B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+        [&](uint32_t i, const BrickFetch<float4, 2, W, PF, NQ2>& f) {
+            force_mixed_particle<VISC_ONLY, STRESS, UNI>(P, D, X, g, L, kp, charge, i, f);
+        },
+        reinterpret_cast<const float4*>(D.recb));
 }
 
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
@@ -794,13 +813,13 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
     const dim3 grid(nblocks(n)), block(kBlock);
     const int ch = charge ? 1 : 0;
     const bool uni = kp.uniform != 0;
-    if (B.on && B.nactive && !(stress && mixed)) {
+    if (B.on && B.nactive) {
 #define SPHX_FORCES_STAGED(V, M, U)                                                                         \
         if (visc_only == V && stress == M && uni == U) {                                                    \
             if (mixed) {                                                                                    \
-                const StagedLaunch sl = staged_launch(B, 16, 32);                                            \
-                hipLaunchKernelGGL((k_forces_mixed_staged<V, U, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, X, \
-                                   g, L, kp, ch, B, cell_start, sl.cap);                                    \
+                const StagedLaunch sl = staged_launch(B, 16, M ? 64 : 32);                                  \
+                hipLaunchKernelGGL((k_forces_mixed_staged<V, M, U, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, \
+                                   X, g, L, kp, ch, B, cell_start, sl.cap);                                 \
             } else {                                                                                        \
                 StagedLaunch sl = staged_launch(B, 8, M ? 96 : 64);                                         \
                 sl.block = dim3(2u * (uint32_t)B.threads); /* two lanes per particle */                      \
@@ -817,12 +836,10 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
         SPHX_FORCES_STAGED(true, false, false)
         SPHX_FORCES_STAGED(false, false, true)
         SPHX_FORCES_STAGED(true, false, true)
-        if (!mixed) {
-            SPHX_FORCES_STAGED(false, true, false)
-            SPHX_FORCES_STAGED(true, true, false)
-            SPHX_FORCES_STAGED(false, true, true)
-            SPHX_FORCES_STAGED(true, true, true)
-        }
+        SPHX_FORCES_STAGED(false, true, false)
+        SPHX_FORCES_STAGED(true, true, false)
+        SPHX_FORCES_STAGED(false, true, true)
+        SPHX_FORCES_STAGED(true, true, true)
 #undef SPHX_FORCES_STAGED
     }
 #define SPHX_FORCES(V, M, U)                                                                              \
