@@ -1046,7 +1046,8 @@ __global__ void __launch_bounds__(kBlock) k_induction_mixed(Particles P, Derived
 
 // Unstaged: brick-staged forms of these two passes (96-byte records, one
 // workgroup per CU) measured slower on the 4M periodic MHD box (5.1 against
-// 4.7 ms per step for the pair) than these gathers at three workgroups per CU.
+// 4.7 ms per step for the pair) than these gathers at three workgroups per CU,
+// and so did staged FP32-mixed forms (64-byte records: 2.87 against 2.57 ms).
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                       const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
     if (!n) return;
