@@ -515,6 +515,7 @@ struct BrickHdr {
     uint4 code[27];                     // per entry cell code: {ox, oy, oz (f32 bits, times the cell widths), 0}
     uint32_t own_start[kMaxOwnRows];    // per brick row: its first sorted particle
     uint32_t own_cum[kMaxOwnRows + 1];  // exclusive prefix of the brick rows' particle counts
+    uint32_t whole;                     // 1: every halo row is staged whole (no list entry is read from global)
 };
 
 // Brick `id`'s halo rows (cells x0 - 1 .. x0 + b[0] of rows y0 - 1 .. y0 + b[1],
@@ -533,6 +534,7 @@ __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const 
     if (t < 64) {
         const int hy = BY + 2, nh = hy * (BZ + 2);
         uint32_t gs = 0, gl = 0;
+        bool cut = false;
         if (lane < nh) {
             int y = y0 - 1 + lane % hy, z = z0 - 1 + lane / hy;
             bool ok = true;
@@ -545,6 +547,8 @@ __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const 
             if (ok) {
                 int xa = x0 - 1, xe = x0 + BX;
                 if (g.periodic && xe - xa + 1 >= nx) { xa = 0; xe = nx - 1; }
+                // a periodic row cut at x = 0 or nx - 1 leaves its wrapped cells unstaged
+                cut = g.periodic && (xa < 0 || xe > nx - 1);
                 xa = max(xa, 0);
                 xe = min(xe, nx - 1);
                 const uint32_t row = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
@@ -560,6 +564,8 @@ __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const 
         const uint32_t base = incl - gl;
         const uint32_t staged = base >= cap ? 0u : min(gl, cap - base);
         if (lane < nh) H.halo[lane] = make_uint4(gs, staged, base, 0u);
+        const bool whole = __all(staged == gl && !cut) != 0;
+        if (lane == 0) H.whole = whole ? 1u : 0u;
     } else if (t >= 128 && t < 128 + 27) {
         // entry code c = stage * 3 + x offset + 1, stage = (oz + 1) * 3 + oy + 1
         const uint32_t c = t - 128u, st = c / 3u, sy = st % 3u, sz = st / 3u;
@@ -676,6 +682,7 @@ struct BrickFetch {
     const Q* __restrict__ glob;
     const Q* __restrict__ glob2;
     uint32_t qstride;                 // bytes from quad k to quad k + 1 of a record (16 cap)
+    uint32_t whole;                   // every list entry of the brick's particles is staged (uniform)
     uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries (kbeg a multiple of 4)
     // LDS byte address of entry e's record; false if it is not staged
     __device__ __forceinline__ bool locate(uint32_t e, uint32_t& addr) const {
@@ -683,6 +690,8 @@ struct BrickFetch {
         addr = e * 16u + h.z;
         return e - h.x < h.y;
     }
+    // LDS byte address of a staged entry's record
+    __device__ __forceinline__ uint32_t address(uint32_t e) const { return e * 16u + rc[e >> kIdxBits].z; }
     // cell offset of entry e's neighbour times the cell widths (as entry_offset)
     __device__ __forceinline__ void offset(uint32_t e, float, float, float, float& ox, float& oy, float& oz) const {
         const lds_quad_t t = code[e >> kIdxBits];
@@ -718,7 +727,8 @@ struct BrickFetch {
 // ahead.  When all W neighbours of every active lane are staged (a
 // wave-uniform test) the records are read with no per-lane branch, so the W
 // bodies, written branch-free, form one basic block the compiler can
-// interleave.  flush() as in for_neighbour_groups (it needs W = 4).
+// interleave.  In a brick whose halo rows are all staged whole (F.whole)
+// every entry is staged and the test is skipped.  flush() as in for_neighbour_groups (it needs W = 4).
 template <class Q, int NQ, int W, bool PF, int NQ2, class Body, class Flush = NoFlush>
 __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF, NQ2>& F,
                                      Body&& body, Flush&& flush = Flush()) {
@@ -742,33 +752,44 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
     uint32_t a[W];
     uint32_t k = F.kbeg;
     if (PF && cnt > k) load(k, a);
-    for (; k + (W - 1) < cnt; k += W) {
-        uint32_t c[W], slot[W];
-        bool staged = true;
-        if (PF) {
-            // the next W entries are loaded unconditionally (the group past the
-            // end is the last one again) so that the load has its own registers
-            // and nothing waits for it this iteration
+    // WHOLE (the brick's halo is staged whole): no staged test per entry
+    auto steps = [&](auto whole) {
+        for (; k + (W - 1) < cnt; k += W) {
+            uint32_t c[W];
+            if (PF) {
+                // the next W entries are loaded unconditionally (the group past
+                // the end is the last one again) so that the load has its own
+                // registers and nothing waits for it this iteration
 #pragma unroll
-            for (int u = 0; u < W; ++u) c[u] = a[u];
-            load(min(k + W, (cnt - 1u) & ~(uint32_t)(W - 1)), a);
-        } else {
-            load(k, c);
+                for (int u = 0; u < W; ++u) c[u] = a[u];
+                load(min(k + W, (cnt - 1u) & ~(uint32_t)(W - 1)), a);
+            } else {
+                load(k, c);
+            }
+            Q q[W][NR];
+            if constexpr (decltype(whole)::value) {
+#pragma unroll
+                for (int u = 0; u < W; ++u) F.from_lds(F.address(c[u]), q[u]);
+            } else {
+                uint32_t slot[W];
+                bool staged = true;
+#pragma unroll
+                for (int u = 0; u < W; ++u) staged = F.locate(c[u], slot[u]) && staged;
+                if (__all(staged)) {
+#pragma unroll
+                    for (int u = 0; u < W; ++u) F.from_lds(slot[u], q[u]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < W; ++u) F(c[u], q[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < W; ++u) body(c[u], q[u]);
+            flush();
         }
-#pragma unroll
-        for (int u = 0; u < W; ++u) staged = F.locate(c[u], slot[u]) && staged;
-        Q q[W][NR];
-        if (__all(staged)) {
-#pragma unroll
-            for (int u = 0; u < W; ++u) F.from_lds(slot[u], q[u]);
-        } else {
-#pragma unroll
-            for (int u = 0; u < W; ++u) F(c[u], q[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < W; ++u) body(c[u], q[u]);
-        flush();
-    }
+    };
+    if (F.whole) steps(std::true_type{});
+    else steps(std::false_type{});
     Q q[NR];
     if (k < cnt) {
         uint32_t c[W];
@@ -808,7 +829,8 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
     brick_stage<NQ, NQ2>(H, (int)nh, reinterpret_cast<const uint4*>(src), reinterpret_cast<const uint4*>(src2), rec,
                          cap, rc, no, BY, hy);
     const BrickFetch<Q, NQ, W, PF, NQ2> base_fetch{(const SPHX_LDS lds_quad_t*)rc,
-                                                   (const SPHX_LDS lds_quad_t*)HL->code, src, src2, cap * 16u};
+                                                   (const SPHX_LDS lds_quad_t*)HL->code, src, src2, cap * 16u,
+                                                   (uint32_t)__builtin_amdgcn_readfirstlane(HL->whole)};
     const uint32_t nown = HL->own_cum[no];
     uint32_t k = 0;
     for (uint32_t u = threadIdx.x / LANES; u < nown; u += blockDim.x / LANES) {

@@ -652,7 +652,9 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
                                                                     uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    constexpr int W = (VISC_ONLY && !STRESS) ? 4 : 2;
+    // two entries per step (four measured slower once bricks with a whole
+    // staged halo skip the staged test: VISC_ONLY 0.322 against 0.315 ms)
+    constexpr int W = 2;
     constexpr int NQ2 = STRESS ? 2 : 0;
     constexpr bool PF = !STRESS;  // the stress terms leave no registers for the list prefetch
     brick_pass<double2, 4, W, PF, 2, NQ2>(
