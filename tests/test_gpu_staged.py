@@ -98,3 +98,20 @@ def test_staged_mhd_periodic_bit_identical(require_gpu, precision):
     got = run(st, p, "3,3,3", 4)
     assert_identical(got[0], ref[0])
     assert_identical(got[1], ref[1])
+
+
+@pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
+def test_staged_mhd_variable_h_bit_identical(require_gpu, precision):
+    # per-particle h and m: the non-uniform Maxwell-stress force passes, whose
+    # staged records carry the B records beside the gather records
+    st, p = scenarios.mhd_periodic(16)
+    rng = scenarios.SplitMix64(7)
+    n = len(st["x"])
+    st["h"] = st["h"] * (1.0 + 0.1 * rng.uniform(n))
+    st["m"] = st["m"] * (1.0 + 0.2 * rng.uniform(n))
+    p.precision = precision
+    ref = run(st, p, "off", 4)
+    for shape in ("3,3,3", "2,2,2"):
+        got = run(st, p, shape, 4)
+        assert_identical(got[0], ref[0])
+        assert_identical(got[1], ref[1])
