@@ -138,6 +138,39 @@ typedef struct SphxState {
     uint32_t* id;                         /* ECS order; NULL -> 0..n-1     */
 } SphxState;
 
+/* ---- inputs of the reference's WGSL path (SphMode::Gpu callers) ---------- */
+/* The uniform block SphUniformData (src/gpu/buffers.rs:25-38), laid out as the
+ * WGSL struct SphParams (src/gpu/sph.wgsl:1-12): 40 bytes, f32, one mass and
+ * one smoothing radius for every particle. */
+typedef struct SphxUniformData {
+    uint32_t num_particles;
+    float mass;
+    float h;
+    float dt;
+    float rest_density;
+    float speed_of_sound;
+    float gamma;
+    float alpha_visc;
+    float beta_visc;
+    uint32_t _padding;
+} SphxUniformData;
+
+/* Replaces the uniform upload of create_buffers_and_bind_group
+ * (src/gpu/pipeline.rs:168-181): dt, the Tait EOS (rest_density,
+ * speed_of_sound, gamma) and the viscosity (alpha_visc, beta_visc) are copied
+ * into *p, widened to f64; every other field of *p is kept.  SPHX_E_INVALID
+ * for a NULL pointer or a non-positive dt, rest_density, speed_of_sound or
+ * gamma (p is then unchanged). */
+int sphx_params_from_uniform(const SphxUniformData* u, SphxParams* p);
+
+/* Replaces the vec4 particle buffers of the same call (SphParticleVec4,
+ * buffers.rs:5-22: x, y, z, w, w unused): fills the caller's SphxState arrays
+ * x, y, z, vx, vy, vz from pos4 and vel4 (4 floats a particle, widened to f64),
+ * m = u->mass and h = u->h for all u->num_particles particles, id = 0 .. n-1
+ * (buffer order) when st->id is not NULL, and sets st->n.  SPHX_E_INVALID for
+ * a NULL pointer or array or a non-positive mass or h. */
+int sphx_state_from_vec4(const SphxUniformData* u, const float* pos4, const float* vel4, SphxState* st);
+
 /* fields for sphx_download */
 enum SphxFields {
     SPHX_F_POS = 1, SPHX_F_VEL = 2, SPHX_F_RHO = 4, SPHX_F_P = 8, SPHX_F_ACC = 16,

@@ -821,6 +821,38 @@ void sphx_default_params(SphxParams* p) {
     p->neighbour_skin = -1.0;
 }
 
+int sphx_params_from_uniform(const SphxUniformData* u, SphxParams* p) {
+    if (!u || !p) return SPHX_E_INVALID;
+    if (!(u->dt > 0.0f) || !(u->rest_density > 0.0f) || !(u->speed_of_sound > 0.0f) || !(u->gamma > 0.0f))
+        return SPHX_E_INVALID;
+    p->dt = (double)u->dt;
+    p->rest_density = (double)u->rest_density;
+    p->speed_of_sound = (double)u->speed_of_sound;
+    p->gamma = (double)u->gamma;
+    p->visc_alpha = (double)u->alpha_visc;
+    p->visc_beta = (double)u->beta_visc;
+    return SPHX_OK;
+}
+
+int sphx_state_from_vec4(const SphxUniformData* u, const float* pos4, const float* vel4, SphxState* st) {
+    if (!u || !st) return SPHX_E_INVALID;
+    const uint32_t n = u->num_particles;
+    if (!(u->mass > 0.0f) || !(u->h > 0.0f)) return SPHX_E_INVALID;
+    if (n && (!pos4 || !vel4 || !st->x || !st->y || !st->z || !st->vx || !st->vy || !st->vz || !st->m || !st->h))
+        return SPHX_E_INVALID;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* a = pos4 + 4 * (size_t)i;
+        const float* b = vel4 + 4 * (size_t)i;
+        st->x[i] = a[0]; st->y[i] = a[1]; st->z[i] = a[2];
+        st->vx[i] = b[0]; st->vy[i] = b[1]; st->vz[i] = b[2];
+        st->m[i] = u->mass;
+        st->h[i] = u->h;
+        if (st->id) st->id[i] = i;
+    }
+    st->n = n;
+    return SPHX_OK;
+}
+
 int sphx_device_count(int* count) {
     if (!count) return SPHX_E_INVALID;
     *count = 0;

@@ -57,6 +57,15 @@ class SphxState(C.Structure):
     ]
 
 
+class SphxUniformData(C.Structure):
+    """SphUniformData of the reference's WGSL path (src/gpu/buffers.rs:25-38)."""
+    _fields_ = [
+        ("num_particles", C.c_uint32), ("mass", C.c_float), ("h", C.c_float), ("dt", C.c_float),
+        ("rest_density", C.c_float), ("speed_of_sound", C.c_float), ("gamma", C.c_float),
+        ("alpha_visc", C.c_float), ("beta_visc", C.c_float), ("_padding", C.c_uint32),
+    ]
+
+
 class SphxTimes(C.Structure):
     _fields_ = [
         ("sort_ms", C.c_double), ("neighbour_ms", C.c_double), ("density_ms", C.c_double),
@@ -76,6 +85,9 @@ class SphxEnergy(C.Structure):
 # name -> (restype, argtypes); every symbol include/sphx.h declares.
 SIGNATURES = {
     "sphx_default_params": (None, [C.POINTER(SphxParams)]),
+    "sphx_params_from_uniform": (C.c_int, [C.POINTER(SphxUniformData), C.POINTER(SphxParams)]),
+    "sphx_state_from_vec4": (C.c_int, [C.POINTER(SphxUniformData), C.c_void_p, C.c_void_p,
+                                       C.POINTER(SphxState)]),
     "sphx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "sphx_create": (C.c_int, [C.c_int, C.c_uint32, C.POINTER(SphxParams), C.POINTER(C.c_void_p)]),
     "sphx_destroy": (None, [C.c_void_p]),

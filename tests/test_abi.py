@@ -16,7 +16,7 @@ ROOT = Path(__file__).resolve().parent.parent
 def declared_symbols():
     text = (ROOT / "include" / "sphx.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(sphx_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sphx_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_and_binding_agree():
@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layout_matches_header(tmp_path):
     # offsetof/sizeof as the C compiler lays out include/sphx.h, against ctypes
-    structs = [_abi.SphxParams, _abi.SphxState, _abi.SphxTimes, _abi.SphxEnergy]
+    structs = [_abi.SphxParams, _abi.SphxState, _abi.SphxTimes, _abi.SphxEnergy, _abi.SphxUniformData]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sphx.h"', "int main(void) {"]
     for s in structs:
         lines.append(f'printf("{s.__name__} sizeof %zu\\n", sizeof({s.__name__}));')
@@ -114,3 +114,68 @@ def test_scenarios_are_deterministic_and_shaped():
     assert pm.boundary_flags == sphx.BND_PERIODIC
     # the SplitMix64 stream: first output for seed 0 (published test vector)
     assert int(scenarios.SplitMix64(0).next_u64(1)[0]) == 0xE220A8397B1DCDAF
+
+
+def test_uniform_block_is_the_wgsl_layout():
+    # SphParams in src/gpu/sph.wgsl:1-12: ten 4-byte fields, no padding between
+    assert C.sizeof(_abi.SphxUniformData) == 40
+    assert [f for f, _ in _abi.SphxUniformData._fields_] == [
+        "num_particles", "mass", "h", "dt", "rest_density", "speed_of_sound", "gamma", "alpha_visc",
+        "beta_visc", "_padding"]
+
+
+SPHX_E_INVALID = 4
+
+
+def uniform(n=5, **kw):
+    vals = dict(num_particles=n, mass=0.125, h=0.0265, dt=1e-4, rest_density=998.5, speed_of_sound=88.3,
+                gamma=7.0, alpha_visc=0.35, beta_visc=0.7)
+    vals.update(kw)
+    return _abi.SphxUniformData(**vals)
+
+
+def test_params_from_uniform_widens_and_keeps_the_rest():
+    lib = _abi.load()
+    u = uniform()
+    p = sphx.default_params()
+    p.stiffness, p.neighbour_skin = 123.0, 0.5
+    assert lib.sphx_params_from_uniform(C.byref(u), C.byref(p)) == _abi.SPHX_OK
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    assert (p.dt, p.rest_density, p.speed_of_sound, p.gamma, p.visc_alpha, p.visc_beta) == (
+        f32(1e-4), f32(998.5), f32(88.3), 7.0, f32(0.35), f32(0.7))
+    assert (p.stiffness, p.neighbour_skin, p.eos) == (123.0, 0.5, _abi.EOS_TAIT)
+    before = bytes(p)
+    for bad in (dict(dt=0.0), dict(rest_density=-1.0), dict(speed_of_sound=float("nan")), dict(gamma=0.0)):
+        assert lib.sphx_params_from_uniform(C.byref(uniform(**bad)), C.byref(p)) == SPHX_E_INVALID
+        assert bytes(p) == before
+    assert lib.sphx_params_from_uniform(None, C.byref(p)) == SPHX_E_INVALID
+
+
+def test_state_from_vec4_fills_the_soa_arrays():
+    lib = _abi.load()
+    n = 7
+    rng = np.random.default_rng(3)
+    pos4 = rng.normal(size=(n, 4)).astype(np.float32)
+    vel4 = rng.normal(size=(n, 4)).astype(np.float32)
+    arrs = {k: np.full(n, -1.0) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "h")}
+    ids = np.full(n, 99, dtype=np.uint32)
+    st = _abi.SphxState()
+    for k, a in arrs.items():
+        setattr(st, k, a.ctypes.data_as(_abi.PD))
+    st.id = ids.ctypes.data_as(_abi.PU)
+    u = uniform(n)
+    rc = lib.sphx_state_from_vec4(C.byref(u), pos4.ctypes.data, vel4.ctypes.data, C.byref(st))
+    assert rc == _abi.SPHX_OK and st.n == n
+    for c, k in enumerate(("x", "y", "z")):
+        assert np.array_equal(arrs[k], pos4[:, c].astype(np.float64))
+        assert np.array_equal(arrs["v" + k], vel4[:, c].astype(np.float64))
+    assert np.all(arrs["m"] == np.float32(0.125)) and np.all(arrs["h"] == np.float32(0.0265))
+    assert np.array_equal(ids, np.arange(n, dtype=np.uint32))
+    # without id the ids are left to the caller; bad inputs are status codes
+    st.id = None
+    assert lib.sphx_state_from_vec4(C.byref(u), pos4.ctypes.data, vel4.ctypes.data, C.byref(st)) == _abi.SPHX_OK
+    assert lib.sphx_state_from_vec4(C.byref(u), None, vel4.ctypes.data, C.byref(st)) == SPHX_E_INVALID
+    assert lib.sphx_state_from_vec4(C.byref(uniform(n, h=0.0)), pos4.ctypes.data, vel4.ctypes.data,
+                                    C.byref(st)) == SPHX_E_INVALID
+    st.m = None
+    assert lib.sphx_state_from_vec4(C.byref(u), pos4.ctypes.data, vel4.ctypes.data, C.byref(st)) == SPHX_E_INVALID
