@@ -191,6 +191,17 @@ const char* sphx_last_error(const SphxCtx* ctx);
  *      and read_densities / read_forces (gpu/buffers.rs:163-213) --------- */
 int sphx_upload(SphxCtx* ctx, const SphxState* host);
 int sphx_download(SphxCtx* ctx, SphxState* host, uint32_t fields);
+/* Residency: replaces the per-tick re-upload of buffers in the reference's
+ * GPU path (create_buffers_and_bind_group, gpu/pipeline.rs:168-206, runs for
+ * every dispatch) when only some components changed.  Writes the fields
+ * SPHX_F_VEL, SPHX_F_B and SPHX_F_PSI of host (upload order, host->n equal to
+ * the context's n) into the resident state, keeping the sort, the grid and the
+ * neighbour list; density and forces are recomputed at the next use.
+ * Positions, masses and radii change through sphx_upload.  Blocks until the
+ * copies are done.  SPHX_E_INVALID for other fields or a missing array,
+ * SPHX_E_NONFINITE for a non-finite value, SPHX_E_STATE for B / psi without an
+ * uploaded field. */
+int sphx_update(SphxCtx* ctx, const SphxState* host, uint32_t fields);
 
 /* ---- passes (each replaces one ECS system; async on the ctx stream) ---- */
 /* build_broadphase_system + broadphase_query_system (broadphase.rs:33-96):

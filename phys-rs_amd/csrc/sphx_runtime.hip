@@ -1097,6 +1097,66 @@ int sphx_download(SphxCtx* c, SphxState* h, uint32_t fields) {
     return SPHX_OK;
 }
 
+int sphx_update(SphxCtx* c, const SphxState* h, uint32_t fields) {
+    if (!c || !h) return SPHX_E_INVALID;
+    if (!c->grid_set) return fail(c, SPHX_E_STATE, "upload particles first");
+    if (h->n != c->n) return fail(c, SPHX_E_INVALID, "state.n = %u but the context holds %u", h->n, c->n);
+    if (fields & ~(uint32_t)(SPHX_F_VEL | SPHX_F_B | SPHX_F_PSI))
+        return fail(c, SPHX_E_INVALID, "sphx_update takes SPHX_F_VEL, SPHX_F_B and SPHX_F_PSI (positions, masses "
+                                       "and radii change through sphx_upload)");
+    const bool vel = fields & SPHX_F_VEL, b = fields & SPHX_F_B, psi = fields & SPHX_F_PSI;
+    if ((b || psi) && !c->has_mhd) return fail(c, SPHX_E_STATE, "no magnetic field was uploaded");
+    if ((vel && (!h->vx || !h->vy || !h->vz)) || (b && (!h->bx || !h->by || !h->bz)) || (psi && !h->psi))
+        return fail(c, SPHX_E_INVALID, "an array of the requested fields is missing");
+    const uint32_t n = c->n;
+    for (uint32_t i = 0; i < n; ++i) {
+        const bool ok = (!vel || (std::isfinite(h->vx[i]) && std::isfinite(h->vy[i]) && std::isfinite(h->vz[i]))) &&
+                        (!b || (std::isfinite(h->bx[i]) && std::isfinite(h->by[i]) && std::isfinite(h->bz[i]))) &&
+                        (!psi || std::isfinite(h->psi[i]));
+        if (!ok) return fail(c, SPHX_E_NONFINITE, "non-finite input at particle %u", i);
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!n || !fields) return SPHX_OK;
+    Particles& P = c->P();
+    std::vector<uint32_t> orig(n);
+    HIP_TRY(c, hipMemcpyAsync(orig.data(), P.orig, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // sorted slot k holds the particle uploaded at orig[k]
+    std::vector<double> buf(n);
+    auto put = [&](const double* src, double* dev) -> int {
+        for (uint32_t k = 0; k < n; ++k) buf[k] = src[orig[k]];
+        HIP_TRY(c, hipMemcpyAsync(dev, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return SPHX_OK;
+    };
+    int rc = SPHX_OK;
+    if (vel && ((rc = put(h->vx, P.vx)) || (rc = put(h->vy, P.vy)) || (rc = put(h->vz, P.vz)))) return rc;
+    if (b && ((rc = put(h->bx, P.bx)) || (rc = put(h->by, P.by)) || (rc = put(h->bz, P.bz)))) return rc;
+    if (psi && (rc = put(h->psi, P.psi))) return rc;
+    if (b || psi) {
+        // the passes read B and psi from the (bx, by, bz, psi) records
+        std::vector<double> bx(n), by(n), bz(n), ps(n);
+        HIP_TRY(c, hipMemcpyAsync(bx.data(), P.bx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(by.data(), P.by, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(bz.data(), P.bz, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(ps.data(), P.psi, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        std::vector<double> rec(4 * (size_t)n);
+        for (uint32_t k = 0; k < n; ++k) {
+            rec[4 * (size_t)k] = bx[k];
+            rec[4 * (size_t)k + 1] = by[k];
+            rec[4 * (size_t)k + 2] = bz[k];
+            rec[4 * (size_t)k + 3] = ps[k];
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->D.recb, rec.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    // the gather records hold velocities, and a, the pressure sums and the
+    // MHD terms depend on v and B: all recomputed at the next use
+    c->dens = c->sp = c->acc = false;
+    return SPHX_OK;
+}
+
 #define SPHX_ENTER(c)                                        \
     if (!(c)) return SPHX_E_INVALID;                         \
     if (!(c)->grid_set) return fail(c, SPHX_E_STATE, "upload particles first"); \

@@ -210,6 +210,20 @@ class Context:
         self._check(self._lib.sphx_download(self._h, C.byref(st), fields))
         return out
 
+    def update(self, state: Dict[str, np.ndarray], fields: int) -> None:
+        """sphx_update: new velocities (F_VEL) and / or B (F_B) and psi (F_PSI), upload order."""
+        st = _abi.SphxState()
+        st.n = self.n
+        keep = []
+        groups = [(F_VEL, ("vx", "vy", "vz")), (F_B, ("bx", "by", "bz")), (F_PSI, ("psi",))]
+        for bit, names in groups:
+            if fields & bit:
+                for k in names:
+                    a = np.ascontiguousarray(state[k], dtype=np.float64)
+                    keep.append(a)
+                    setattr(st, k, _ptr(a))
+        self._check(self._lib.sphx_update(self._h, C.byref(st), fields))
+
     # -- systems
     def neighbours(self) -> None:
         self._check(self._lib.sphx_neighbours(self._h))

@@ -85,6 +85,7 @@ class SphxEnergy(C.Structure):
 # name -> (restype, argtypes); every symbol include/sphx.h declares.
 SIGNATURES = {
     "sphx_default_params": (None, [C.POINTER(SphxParams)]),
+    "sphx_update": (C.c_int, [C.c_void_p, C.POINTER(SphxState), C.c_uint32]),
     "sphx_params_from_uniform": (C.c_int, [C.POINTER(SphxUniformData), C.POINTER(SphxParams)]),
     "sphx_state_from_vec4": (C.c_int, [C.POINTER(SphxUniformData), C.c_void_p, C.c_void_p,
                                        C.POINTER(SphxState)]),

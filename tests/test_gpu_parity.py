@@ -498,3 +498,42 @@ def test_mixed_mhd_periodic_step(require_gpu):
     O.step(s, p, 5)
     for k in ("vx", "vy", "vz", "bx", "by", "bz"):
         assert np.abs(g[k] - s[k]).max() < 1e-5 * np.abs(s[k]).max(), k
+
+
+def test_update_keeps_residency_and_matches_upload(require_gpu):
+    # sphx_update writes new velocities and B / psi into the resident sorted
+    # state.  Writing back the values the context already holds changes no
+    # bit of the trajectory; new values agree with a fresh upload of the same
+    # state (a different sort and list order: rounding-level differences).
+    st, p = scenarios.mhd_periodic(16)
+    fields = sphx.F_POS | sphx.F_VEL | sphx.F_B | sphx.F_PSI
+    with sphx.Context(len(st["x"]), p) as a, sphx.Context(len(st["x"]), p) as b:
+        a.upload(st)
+        b.upload(st)
+        a.step(3)
+        b.step(3)
+        now = a.download(fields)
+        a.update(now, sphx.F_VEL | sphx.F_B | sphx.F_PSI)
+        a.step(3)
+        b.step(3)
+        ra, rb = a.download(fields), b.download(fields)
+        for k in ra:
+            assert np.array_equal(ra[k], rb[k]), k
+        # new velocities and field: against a fresh upload of the same state
+        new = dict(ra)
+        rng = scenarios.SplitMix64(5)
+        n = len(st["x"])
+        for k in ("vx", "vy", "vz"):
+            new[k] = ra[k] + 0.01 * rng.normal(n)
+        new["bz"] = ra["bz"] + 1e-3
+        a.update(new, sphx.F_VEL | sphx.F_B)
+        fresh = dict(st)
+        fresh.update({k: new[k] for k in ("x", "y", "z", "vx", "vy", "vz", "bx", "by", "bz", "psi")})
+        with sphx.Context(n, p) as c:
+            c.upload(fresh)
+            a.step(2)
+            c.step(2)
+            ga, gc = a.download(fields), c.download(fields)
+    for k in ga:
+        scale = max(np.abs(gc[k]).max(), 1e-30)
+        assert np.abs(ga[k] - gc[k]).max() <= 1e-9 * scale, k
