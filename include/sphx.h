@@ -258,6 +258,18 @@ int sphx_synchronize(SphxCtx* ctx);
  * This is the set CandidatePairs holds after the SPH systems' r < 2h filter. */
 int sphx_export_pairs(SphxCtx* ctx, uint32_t* pairs, uint64_t cap_pairs, uint64_t* count);
 
+/* Broadphase export for the systems beside SPH that read CandidatePairs
+ * (heat_conduction_system, thermodynamics.rs:110-171; collisions): replaces
+ * broadphase_query_system (src/physics/broadphase.rs:62-96).  radius[i] is
+ * BoundingRadius of particle i (upload order).  Writes every (id_a, id_b) with
+ * id_a < id_b and |x_a - x_b|^2 < (4 R_a)^2, R_a the radius of the smaller id
+ * (the reference's query radius 4 R around the querying entity), as pairs
+ * sorted ascending; *count receives the total (even when > cap_pairs).  Needs
+ * 4 max(R) + skin <= the cell width (SphxParams.cell_size), else
+ * SPHX_E_INVALID.  Exact equality d^2 = (4 R)^2 is not recorded. */
+int sphx_export_candidates(SphxCtx* ctx, const double* radius, uint32_t* pairs, uint64_t cap_pairs,
+                           uint64_t* count);
+
 typedef struct SphxTimes {       /* device milliseconds, accumulated since last reset */
     double sort_ms;              /* hash + counting sort + reorder                  */
     double neighbour_ms;         /* neighbour-list build (exact predicate)          */

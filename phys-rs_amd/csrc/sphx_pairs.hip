@@ -1213,4 +1213,71 @@ void launch_export_pairs(const Particles& P, uint32_t n, const Grid& g, const Nb
                            out_cap);
 }
 
+// ------------------------------------- broadphase candidate export (non-SPH consumers)
+// broadphase_query_system (broadphase.rs:62-96): entity a queries the tree
+// within 4 R_a (squared Euclidean, summed x, y, z as kiddo's SquaredEuclidean
+// does) and records (a, b) when a < b.  Here one thread per sorted particle
+// walks the 27 cells around its own (each distinct cell once when a periodic
+// axis has fewer than three) and records (id_i, id_j) for id_i < id_j and
+// d^2 < (4 R_i)^2, R in sorted order.  The host checks 4 R_max + skin <= the
+// cell width, so the stencil holds every such pair of the binned particles.
+__global__ void __launch_bounds__(kBlock) k_export_candidates(Particles P, uint32_t n, Grid g,
+                                                               const uint32_t* __restrict__ skey,
+                                                               const uint32_t* __restrict__ start,
+                                                               const double* __restrict__ R, uint32_t* __restrict__ out,
+                                                               unsigned long long* counter, uint64_t out_cap) {
+    SPHX_PASS_PROLOGUE();
+    const uint32_t idi = P.id[i];
+    const double xi = P.x[i], yi = P.y[i], zi = P.z[i];
+    const double r4 = 4.0 * R[i], r2max = r4 * r4;
+    const uint32_t key = skey[i];
+    const int nx = g.dims[0], ny = g.dims[1];
+    const int c[3] = {(int)(key % (uint32_t)nx), (int)((key / (uint32_t)nx) % (uint32_t)ny),
+                      (int)(key / ((uint32_t)nx * (uint32_t)ny))};
+    // distinct neighbour coordinates per axis
+    int nb[3][3], cnt[3];
+    for (int a = 0; a < 3; ++a) {
+        const int d = g.dims[a];
+        cnt[a] = 0;
+        for (int o = -1; o <= 1; ++o) {
+            int v = c[a] + o;
+            if (g.periodic) {
+                v = (v % d + d) % d;
+            } else if (v < 0 || v >= d) {
+                continue;
+            }
+            bool seen = false;
+            for (int q = 0; q < cnt[a]; ++q) seen = seen || nb[a][q] == v;
+            if (!seen) nb[a][cnt[a]++] = v;
+        }
+    }
+    for (int iz = 0; iz < cnt[2]; ++iz)
+        for (int iy = 0; iy < cnt[1]; ++iy)
+            for (int ix = 0; ix < cnt[0]; ++ix) {
+                const uint32_t cell = ((uint32_t)nb[2][iz] * (uint32_t)ny + (uint32_t)nb[1][iy]) * (uint32_t)nx +
+                                      (uint32_t)nb[0][ix];
+                for (uint32_t j = start[cell], e = start[cell + 1]; j < e; ++j) {
+                    const uint32_t idj = P.id[j];
+                    if (!(idi < idj)) continue;
+                    double dx, dy, dz;
+                    pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+                    const double d2 = (dx * dx + dy * dy) + dz * dz;
+                    if (d2 < r2max) {
+                        const unsigned long long pos = atomicAdd(counter, 1ull);
+                        if (pos < out_cap) {
+                            out[2 * pos] = idi;
+                            out[2 * pos + 1] = idj;
+                        }
+                    }
+                }
+            }
+}
+void launch_export_candidates(const Particles& P, uint32_t n, const Grid& g, const uint32_t* skey,
+                              const uint32_t* cell_start, const double* R, uint32_t* out,
+                              unsigned long long* counter, uint64_t out_cap, hipStream_t s) {
+    if (n)
+        hipLaunchKernelGGL(k_export_candidates, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, g, skey, cell_start, R,
+                           out, counter, out_cap);
+}
+
 }  // namespace sphx

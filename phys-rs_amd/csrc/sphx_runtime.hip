@@ -89,6 +89,8 @@ struct SphxCtx {
     uint32_t* pair_buf = nullptr;
     unsigned long long* pair_counter = nullptr;
     uint64_t pair_buf_cap = 0;
+    double* cand_r = nullptr;  // bounding radii in sorted order (sphx_export_candidates)
+    uint32_t cand_r_cap = 0;
     bool has_mhd = false, has_charge = false;
     bool uniform = false;                  // all m equal and all h equal
     double m0 = 0.0, h0 = 0.0;
@@ -1356,6 +1358,77 @@ int sphx_export_pairs(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t*
     std::vector<uint64_t> packed(ng);
     std::vector<uint32_t> raw(2 * ng);
     HIP_TRY(c, hipMemcpy(raw.data(), c->pair_buf, sizeof(uint32_t) * 2 * ng, hipMemcpyDeviceToHost));
+    for (uint64_t k = 0; k < ng; ++k) packed[k] = ((uint64_t)raw[2 * k] << 32) | raw[2 * k + 1];
+    std::sort(packed.begin(), packed.end());
+    for (uint64_t k = 0; k < ng; ++k) {
+        pairs[2 * k] = (uint32_t)(packed[k] >> 32);
+        pairs[2 * k + 1] = (uint32_t)packed[k];
+    }
+    return SPHX_OK;
+}
+
+int sphx_export_candidates(SphxCtx* c, const double* radius, uint32_t* pairs, uint64_t cap_pairs,
+                           uint64_t* count) {
+    SPHX_ENTER(c);
+    if (!count) return fail(c, SPHX_E_INVALID, "count is NULL");
+    const uint32_t n = c->n;
+    if (n && !radius) return fail(c, SPHX_E_INVALID, "bounding radii are required");
+    double rmax = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!std::isfinite(radius[i]) || radius[i] < 0.0)
+            return fail(c, SPHX_E_INVALID, "bounding radius %u is negative or not finite", i);
+        rmax = std::max(rmax, radius[i]);
+    }
+    // the particles were binned at the last list build and have moved less than
+    // skin/2 since: pairs within 4 R lie in neighbouring cells when
+    // 4 R_max + skin fits in a cell
+    const Grid& g = c->grid;
+    for (int a = 0; a < 3; ++a) {
+        if (4.0 * rmax + c->kp.skin > g.w[a])
+            return fail(c, SPHX_E_INVALID, "4 * bounding radius (%g) + skin exceeds the cell width %g; raise "
+                        "SphxParams.cell_size", 4.0 * rmax, g.w[a]);
+    }
+    int rc;
+    if ((rc = ensure_list(c))) return rc;
+    *count = 0;
+    if (!n) return SPHX_OK;
+    Particles& P = c->P();
+    if (n > c->cand_r_cap) {
+        if ((rc = dalloc(c, &c->cand_r, n))) return rc;
+        c->cand_r_cap = n;
+    }
+    {
+        std::vector<uint32_t> orig(n);
+        HIP_TRY(c, hipMemcpyAsync(orig.data(), P.orig, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        std::vector<double> rs(n);
+        for (uint32_t k = 0; k < n; ++k) rs[k] = radius[orig[k]];
+        HIP_TRY(c, hipMemcpyAsync(c->cand_r, rs.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    unsigned long long got = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        launch_fill_u32(reinterpret_cast<uint32_t*>(c->pair_counter), 0u, 2, c->stream);
+        launch_export_candidates(P, n, g, c->S.skey, c->S.cell_start, c->cand_r, c->pair_buf, c->pair_counter,
+                                 c->pair_buf_cap, c->stream);
+        if ((rc = launch_check(c))) return rc;
+        HIP_TRY(c, hipMemcpyAsync(&got, c->pair_counter, sizeof got, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (got <= c->pair_buf_cap) break;
+        // the first pass counted them: grow the buffer and write them all
+        if ((rc = dalloc(c, &c->pair_buf, 2 * (size_t)got))) return rc;
+        c->pair_buf_cap = got;
+    }
+    const uint64_t ng = got;
+    *count = ng;
+    if (ng > cap_pairs)
+        return fail(c, SPHX_E_CAPACITY, "%llu pairs do not fit in %llu", (unsigned long long)ng,
+                    (unsigned long long)cap_pairs);
+    if (!ng) return SPHX_OK;
+    if (!pairs) return fail(c, SPHX_E_INVALID, "pairs is NULL");
+    std::vector<uint32_t> raw(2 * ng);
+    HIP_TRY(c, hipMemcpy(raw.data(), c->pair_buf, sizeof(uint32_t) * 2 * ng, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> packed(ng);
     for (uint64_t k = 0; k < ng; ++k) packed[k] = ((uint64_t)raw[2 * k] << 32) | raw[2 * k + 1];
     std::sort(packed.begin(), packed.end());
     for (uint64_t k = 0; k < ng; ++k) {

@@ -286,6 +286,18 @@ class Context:
         self._check(self._lib.sphx_export_pairs(self._h, _ptr(out, C.c_uint32), count.value, C.byref(count)))
         return out[: count.value]
 
+    def export_candidates(self, radius: np.ndarray) -> np.ndarray:
+        """sphx_export_candidates: broadphase pairs within 4 R of the smaller id (upload-order radii)."""
+        r = np.ascontiguousarray(radius, dtype=np.float64)
+        count = C.c_uint64(0)
+        rc = self._lib.sphx_export_candidates(self._h, _ptr(r), None, 0, C.byref(count))
+        if rc not in (0, 3):  # SPHX_E_CAPACITY is expected with cap 0
+            self._check(rc)
+        out = np.empty((max(count.value, 1), 2), dtype=np.uint32)
+        self._check(self._lib.sphx_export_candidates(self._h, _ptr(r), _ptr(out, C.c_uint32), count.value,
+                                                     C.byref(count)))
+        return out[: count.value]
+
     def enable_timing(self, on: bool = True) -> None:
         self._check(self._lib.sphx_enable_timing(self._h, 1 if on else 0))
 

@@ -537,3 +537,51 @@ def test_update_keeps_residency_and_matches_upload(require_gpu):
     for k in ga:
         scale = max(np.abs(gc[k]).max(), 1e-30)
         assert np.abs(ga[k] - gc[k]).max() <= 1e-9 * scale, k
+
+
+def brute_candidates(st, radius, periodic, L=1.0):
+    # broadphase_query_system (broadphase.rs:62-96): (a, b) with a < b and
+    # ((dx*dx + dy*dy) + dz*dz) < (4 R_a)^2, R_a the querying (smaller) id's radius
+    ids = st["id"].astype(np.int64)
+    x = np.stack([st["x"], st["y"], st["z"]], axis=1)
+    out = []
+    for i in range(len(ids)):
+        d = x - x[i]
+        if periodic:  # the minimum image as the kernels form it
+            d = np.where(d > 0.5 * L, d - L, np.where(d < -0.5 * L, d + L, d))
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        r4 = 4.0 * radius[i]
+        sel = (ids[i] < ids) & (d2 < r4 * r4)
+        out.extend((int(ids[i]), int(b)) for b in ids[sel])
+    return np.array(sorted(out), dtype=np.uint32).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_export_candidates_match_brute_force(require_gpu, periodic):
+    # CandidatePairs for non-SPH consumers: per-particle bounding radii, the
+    # query radius 4 R of the smaller id; before and after ticks that moved the
+    # particles without a list rebuild
+    rng = scenarios.SplitMix64(31)
+    n = 3000
+    st = {k: rng.uniform(n) for k in ("x", "y", "z")}
+    st.update(vx=0.3 * rng.normal(n), vy=0.3 * rng.normal(n), vz=0.3 * rng.normal(n), m=np.full(n, 1.0 / n),
+              h=0.03 + 0.02 * rng.uniform(n), id=np.argsort(rng.uniform(n)).astype(np.uint32))
+    radius = 0.45 * st["h"].max() * (0.5 + 0.5 * rng.uniform(n))
+    kw = dict(boundary_flags=sphx.BND_PERIODIC, box_lo=(0, 0, 0), box_hi=(1, 1, 1)) if periodic else {}
+    p = sphx.make_params(2e-5, tait=sphx.TaitEquationConfig(1.0, 5.0, 7.0),
+                         visc=sphx.SphViscosityConfig(0.5, 0.3), **kw)
+    with make_ctx(st, p) as ctx:
+        got0 = ctx.export_candidates(radius)
+        ctx.step(5)
+        cur = ctx.download(sphx.F_POS)
+        got1 = ctx.export_candidates(radius)
+        # a radius beyond the cell is refused, not truncated
+        with pytest.raises(sphx.SphxError):
+            ctx.export_candidates(radius * 3.0)
+    ref0 = brute_candidates(st, radius, periodic)
+    moved = dict(st)
+    moved.update(cur)
+    ref1 = brute_candidates(moved, radius, periodic)
+    assert len(ref0) > 3000
+    assert np.array_equal(got0, ref0)
+    assert np.array_equal(got1, ref1)
