@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                                                    const uint32_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ start, NbrList L,
                                                    float reach, double skin) {
-    __shared__ float lds[4][kStageCap];
+    __shared__ float4 lds[kStageCap];  // staged candidates: cell-relative x, y, z and h
     SPHX_TILE_PROLOGUE();
     const int cx = valid ? (int)(skey[p] % (uint32_t)g.dims[0]) : 0;
     float xf = 0, yf = 0, zf = 0, hf = 0;
@@ -106,9 +106,10 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         // wave so that the usual f32-only loop has no branch for it
         auto test = [&](uint32_t slot, uint32_t u, uint32_t o, auto ex) {
             // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
-            const float dx = (lds[0][u] - xf) + (float)((int)o - 1) * wx, dy = (lds[1][u] - yf) + st.oy,
-                        dz = (lds[2][u] - zf) + st.oz;
-            const float tt = (UNI ? hf + hf : hf + lds[3][u]) + skin_f;  // 2 h_avg + skin
+            const float4 q = lds[u];
+            const float dx = (q.x - xf) + (float)((int)o - 1) * wx, dy = (q.y - yf) + st.oy,
+                        dz = (q.z - zf) + st.oz;
+            const float tt = (UNI ? hf + hf : hf + q.w) + skin_f;  // 2 h_avg + skin
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
             bool in = r2 < (tt * tt) * (1.0f + 1e-5f);
             if constexpr (decltype(ex)::value) {
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         auto cut = [&](uint32_t a, uint32_t e, float off, bool left) {
             while (a < e) {
                 const uint32_t mid = (a + e) >> 1;
-                const float d = (lds[0][mid] - xf) + off;
+                const float d = (lds[mid].x - xf) + off;
                 if (left ? d <= -reach : d < reach) a = mid + 1;
                 else e = mid;
             }
@@ -165,8 +166,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             __syncthreads();
             SPHX_STAGE_CHUNK({
                 const float4 v = X.p4[j];
-                lds[0][u] = v.x; lds[1][u] = v.y; lds[2][u] = v.z;
-                if (!UNI) lds[3][u] = v.w;
+                lds[u] = v;
             });
             __syncthreads();
             if (!valid) continue;
