@@ -179,3 +179,16 @@ def test_state_from_vec4_fills_the_soa_arrays():
                                     C.byref(st)) == SPHX_E_INVALID
     st.m = None
     assert lib.sphx_state_from_vec4(C.byref(u), pos4.ctypes.data, vel4.ctypes.data, C.byref(st)) == SPHX_E_INVALID
+
+
+def test_null_context_is_an_error_code():
+    # every entry point taking a context answers a NULL one with SPHX_E_INVALID
+    lib = _abi.load()
+    st = _abi.SphxState()
+    count = C.c_uint64(7)
+    r = np.ones(4)
+    assert lib.sphx_update(None, C.byref(st), 2) == SPHX_E_INVALID
+    assert lib.sphx_export_candidates(None, r.ctypes.data_as(_abi.PD), None, 0, C.byref(count)) == SPHX_E_INVALID
+    assert lib.sphx_export_pairs(None, None, 0, C.byref(count)) == SPHX_E_INVALID
+    assert lib.sphx_download(None, C.byref(st), 1) == SPHX_E_INVALID
+    assert lib.sphx_upload(None, C.byref(st)) == SPHX_E_INVALID
