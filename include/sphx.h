@@ -13,10 +13,11 @@
  *    sphx_last_error() returns a message for the last failing call on a ctx.
  *  - Particle arrays are SoA, f64 (the reference's DVec3 / f64 components),
  *    in the caller's order ("upload order").  Results are returned in the
- *    same order.  `id` is the ECS iteration order of the entity: it decides
- *    pair orientation exactly like Entity ordering does for CandidatePairs
- *    (src/physics/broadphase.rs:87) and is required for MHD induction,
- *    whose per-pair sign depends on it (src/physics/mhd.rs:121-122).
+ *    same order.  `id` is the entity's rank under Entity's own ordering
+ *    (bevy_ecs compares Entity::to_bits()), not the query iteration index:
+ *    it decides pair orientation exactly like entity_a < entity_b does for
+ *    CandidatePairs (src/physics/broadphase.rs:87) and is required for MHD
+ *    induction, whose per-pair sign depends on it (src/physics/mhd.rs:121-131).
  *  - A ctx is not thread safe; all passes are enqueued on the ctx stream and
  *    sphx_download / sphx_export_pairs / sphx_synchronize block on it.
  *  - All fluid particles must share one Sector (src/core/coordinates.rs:51-65
@@ -135,7 +136,7 @@ typedef struct SphxState {
     double* bx; double* by; double* bz;   /* MagneticField  (optional)     */
     double* psi;                          /* DivCleaningPsi (optional)     */
     double* q;                            /* Charge         (optional)     */
-    uint32_t* id;                         /* ECS order; NULL -> 0..n-1     */
+    uint32_t* id;                         /* rank of Entity::to_bits(); NULL -> 0..n-1 */
 } SphxState;
 
 /* ---- inputs of the reference's WGSL path (SphMode::Gpu callers) ---------- */
@@ -184,6 +185,11 @@ typedef struct SphxCtx SphxCtx;
 int sphx_device_count(int* count);
 int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx** out);
 void sphx_destroy(SphxCtx* ctx);
+/* New parameters for the next passes.  max_neighbours <= 0 keeps the list
+ * capacity; any other value must round up (to a multiple of 4) to the
+ * capacity chosen at sphx_create.  A failing call (SPHX_E_INVALID) leaves the
+ * context unchanged.  Changing the box, grid, cell size or skin needs a new
+ * sphx_upload. */
 int sphx_set_params(SphxCtx* ctx, const SphxParams* params);
 const char* sphx_last_error(const SphxCtx* ctx);
 

@@ -991,6 +991,12 @@ int sphx_set_params(SphxCtx* c, const SphxParams* p) {
     if (!c) return SPHX_E_INVALID;
     int rc = check_params(c, p);
     if (rc) return rc;
+    // the list capacity is fixed at sphx_create, where max_neighbours was
+    // rounded up to a multiple of 4 (0 or less here keeps it); a failed call
+    // leaves the context as it was
+    if (p->max_neighbours > 0 && (p->max_neighbours + 3) / 4 * 4 != c->L.ks)
+        return fail(c, SPHX_E_INVALID, "max_neighbours is fixed at sphx_create (capacity %d, asked %d)", c->L.ks,
+                    p->max_neighbours);
     const SphxParams old = c->prm;
     c->prm = *p;
     make_kernel_params(c);
@@ -1001,8 +1007,6 @@ int sphx_set_params(SphxCtx* c, const SphxParams* p) {
         grid_change |= old.box_lo[k] != p->box_lo[k] || old.box_hi[k] != p->box_hi[k] ||
                        old.grid_lo[k] != p->grid_lo[k] || old.grid_hi[k] != p->grid_hi[k];
     if (grid_change && c->n) c->grid_set = false;
-    if (p->max_neighbours > 0 && p->max_neighbours != c->L.ks)
-        return fail(c, SPHX_E_INVALID, "max_neighbours is fixed at sphx_create");
     // derived quantities depend on the EOS, viscosity and walls; the
     // precision form also changes the neighbour predicate and f32 copies
     c->dens = c->sp = c->acc = false;

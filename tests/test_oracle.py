@@ -10,6 +10,7 @@
 | orc_density                | sph.rs:72-116             | lattice sum m*sum W = 0.99726183 m/dx^3 at h = 1.3 dx;     |
 |                            |                           | two-particle closed form                                   |
 | orc_eos                    | sph.rs:149-169            | P(rho0) = 0, P(1100) closed form, clamp (sph.rs:518-526)   |
+|   polytropic / isothermal  | new / thermodyn.rs:232-238| P(rho) closed forms at three densities; P(2r)/P(r) = 2^g   |
 | pressure / viscosity       | sph.rs:177-322            | momentum conservation; hydrostatic column (sph.rs:528-570)|
 |                            |                           | mirror symmetry; two-particle viscosity both conventions   |
 | boundary penalty           | sph.rs:443-484            | one-particle closed form                                   |
@@ -17,9 +18,14 @@
 | vv_position / vv_velocity  | integrators.rs:63-104     | free fall is an exact parabola; reflection keeps speed     |
 | orc_step                   | main.rs:103-157           | dam-break spreading (sph.rs:575-679); energy conservation  |
 | orc_mhd_induction          | mhd.rs:68-152             | uniform B unchanged (mhd.rs:314-368); two-particle shear   |
-|                            |                           | case by hand; swapping ids flips entity_a's term           |
+|                            |                           | case by hand; swapping ids flips entity_a's term;          |
+|                            |                           | resistive term (eta != 0) by hand, both ends, id swap      |
 | orc_div_clean              | mhd.rs:172-259            | psi decay exp(-sigma c_h dt / h)^n (mhd.rs:274-310);       |
-|                            |                           | uniform B and psi are fixed points                         |
+|                            |                           | uniform B and psi are fixed points; div B and grad psi     |
+|                            |                           | pair terms by hand (unequal m, rho; sign of dpsi)          |
+| orc_maxwell (new)          | SPMHD, M = BB - B^2 I / 2 | two particles by hand: tension along B, pressure across,   |
+|                            |                           | transverse for oblique B, unequal rho; sum m a = 0 in a    |
+|                            |                           | jittered periodic box; uniform B on a lattice: a = 0       |
 | orc_xsph                   | sph.rs:349-405            | uniform v fixed point; two-particle closed form            |
 | orc_energy                 | new                       | conservation over an inviscid run                          |
 """
@@ -414,3 +420,194 @@ def test_xsph_closed_form_and_fixed_point():
     O.density(s, p, pr)
     O.xsph(s, p, pr)
     assert np.array_equal(s["vx"], [2.0, 2.0])
+
+
+# ---------------------------------------------------------------- MHD force, cleaning, resistivity, EOS
+def _two(h=0.1, r=None, ma=1.0, mb=1.0, axis=0, **extra):
+    """Particles a (id 0) at the origin and b (id 1) at distance r (default h) along `axis`."""
+    r = h if r is None else r
+    pos = np.zeros((3, 2))
+    pos[axis, 1] = r
+    st = dict(x=pos[0], y=pos[1], z=pos[2], vx=np.zeros(2), vy=np.zeros(2), vz=np.zeros(2),
+              m=np.array([ma, mb]), h=np.full(2, h), id=np.array([0, 1], dtype=np.uint32))
+    st.update(extra)
+    return st
+
+
+def _stress_params(**kw):
+    # Tait with rho0 far above the pair's density: P clamps to 0; no viscosity
+    return params(tait=sphx.TaitEquationConfig(1e12, 20.0, 7.0), visc=sphx.SphViscosityConfig(0.0, 0.0),
+                  mhd_flags=sphx.MHD_MAXWELL_STRESS, **kw)
+
+
+@pytest.mark.parametrize("case", ["parallel", "perpendicular", "oblique"])
+def test_maxwell_stress_two_particles(case):
+    # M = B B - |B|^2 I / 2 (mu0 = 1); F_a = m_a m_b (M_a/rho_a^2 + M_b/rho_b^2) . grad_a W,
+    # grad_a W = G x_hat at q = 1 with G = 0.75 / (pi h^4).  A field along the
+    # separation pulls the pair together (tension), a field across it pushes
+    # it apart (magnetic pressure), an oblique one gives a transverse force.
+    h, m, B0 = 0.1, 2.0, 0.6
+    B = {"parallel": (B0, 0.0, 0.0), "perpendicular": (0.0, 0.0, B0),
+         "oblique": (B0 / math.sqrt(2), B0 / math.sqrt(2), 0.0)}[case]
+    st = _two(h, ma=m, mb=m, bx=np.full(2, B[0]), by=np.full(2, B[1]), bz=np.full(2, B[2]))
+    s = O.State(st)
+    O.evaluate(s, _stress_params())
+    rho = 1.25 * m / (PI * h ** 3)
+    assert s["rho"] == pytest.approx([rho, rho], rel=1e-14)
+    assert s["p"][0] == 0.0
+    G = 0.75 / (PI * h ** 4)
+    F = {"parallel": (1.0, 0.0, 0.0), "perpendicular": (-1.0, 0.0, 0.0),
+         "oblique": (0.0, 1.0, 0.0)}[case]
+    mag = m * m * G * B0 * B0 / (rho * rho)
+    Fa = m * np.array([s["ax"][0], s["ay"][0], s["az"][0]])
+    Fb = m * np.array([s["ax"][1], s["ay"][1], s["az"][1]])
+    assert Fa == pytest.approx(mag * np.array(F), rel=1e-13, abs=1e-13 * mag)
+    assert Fb == pytest.approx(-mag * np.array(F), rel=1e-13, abs=1e-13 * mag)
+
+
+def test_maxwell_stress_unequal_densities():
+    # B_a along the separation, B_b across it, m_a != m_b: the two stresses
+    # cancel only if each is divided by its own particle's rho^2
+    h, ma, mb, B0 = 0.1, 1.0, 3.0, 0.5
+    st = _two(h, ma=ma, mb=mb, bx=np.array([B0, 0.0]), by=np.array([0.0, B0]), bz=np.zeros(2))
+    s = O.State(st)
+    O.evaluate(s, _stress_params())
+    sig = 1.0 / (PI * h ** 3)
+    ra, rb = (ma + 0.25 * mb) * sig, (mb + 0.25 * ma) * sig
+    assert s["rho"] == pytest.approx([ra, rb], rel=1e-14)
+    G = 0.75 / (PI * h ** 4)
+    fa = ma * mb * G * (0.5 * B0 * B0 / ra ** 2 - 0.5 * B0 * B0 / rb ** 2)
+    assert ma * s["ax"][0] == pytest.approx(fa, rel=1e-12)
+    assert mb * s["ax"][1] == pytest.approx(-fa, rel=1e-12)
+    assert s["ay"][0] == 0.0 and s["az"][0] == 0.0
+
+
+def _periodic_box(n_side=8, jitter=0.15, seed=21, random_b=True):
+    st, p = scenarios.mhd_periodic(n_side, jitter=jitter, seed=seed, h_factor=1.2)
+    if random_b:
+        rng = scenarios.SplitMix64(seed + 100)
+        n = len(st["x"])
+        st["bx"], st["by"], st["bz"] = 0.3 * rng.normal(n), 0.3 * rng.normal(n), 0.3 * rng.normal(n)
+    return st, p
+
+
+def test_maxwell_stress_conserves_momentum():
+    # antisymmetric pair force: sum m a = 0 with pressure, viscosity and stress
+    st, p = _periodic_box()
+    s = O.State(st)
+    O.evaluate(s, p)
+    m = st["m"]
+    F = np.stack([m * s["ax"], m * s["ay"], m * s["az"]], 1)
+    scale = np.abs(F).max()
+    assert scale > 0
+    assert np.abs(F.sum(0)).max() < 1e-13 * scale * len(m)
+    # the stress is a sizeable part of the force here
+    p0 = sphx.make_params(p.dt, tait=sphx.TaitEquationConfig(1.0, p.speed_of_sound, p.gamma),
+                          visc=sphx.SphViscosityConfig(p.visc_alpha, 0.0), eos=p.eos, poly_k=p.poly_k,
+                          boundary_flags=p.boundary_flags, box_lo=(0, 0, 0), box_hi=(1, 1, 1))
+    s0 = O.State(st)
+    O.evaluate(s0, p0)
+    assert np.abs(s0["ax"] - s["ax"]).max() > 0.05 * np.abs(s["ax"]).max()
+
+
+def test_uniform_field_on_periodic_lattice_is_force_free():
+    # perfect periodic lattice: sum_j grad W = 0 by symmetry, rho and P are
+    # uniform, so pressure and a uniform B give no force (round-off only)
+    st, p = scenarios.mhd_periodic(8, jitter=0.0, h_factor=1.2)
+    n = len(st["x"])
+    st["vx"], st["vy"], st["vz"] = np.zeros(n), np.zeros(n), np.zeros(n)
+    st["bx"], st["by"], st["bz"] = np.full(n, 0.4), np.full(n, -0.2), np.full(n, 0.1)
+    s = O.State(st)
+    O.evaluate(s, p)
+    # acceleration one pair at q = 1 would give, times the neighbour count
+    a_pair = st["m"][0] * (0.75 / (PI * st["h"][0] ** 4)) * 0.21 / s["rho"][0] ** 2
+    for k in ("ax", "ay", "az"):
+        assert np.abs(s[k]).max() < 1e-12 * 60 * a_pair, k
+    # a non-uniform field on the same lattice is not force free
+    st["by"] = 0.3 * np.sin(2 * PI * st["x"])
+    s = O.State(st)
+    O.evaluate(s, p)
+    assert np.abs(s["ax"]).max() > 1e-2 * a_pair
+
+
+def test_div_clean_pair_terms_by_hand():
+    # mhd.rs:205-236 with r_vec = x_b - x_a, grad_w = r_hat dW (dW < 0):
+    #   div_a = (m_b/rho_b)(B_b - B_a).grad_w,  div_b = (m_a/rho_a)(B_a - B_b).(-grad_w)
+    #   gpsi_a = (m_b/rho_b)(psi_b - psi_a) grad_w, gpsi_b = (m_a/rho_a)(psi_a - psi_b)(-grad_w)
+    #   B -= gpsi dt; psi -= c_h^2 div dt; then psi *= exp(-sigma c_h dt / h)
+    h, ma, mb, dt, ch, sig = 0.1, 1.0, 2.0, 1e-3, 3.0, 0.5
+    Ba, Bb, psa, psb = 0.2, 0.7, 0.05, -0.4
+    ra, rb = 1.5, 2.5
+    st = _two(h, ma=ma, mb=mb, bx=np.array([Ba, Bb]), by=np.array([0.0, 0.3]), bz=np.zeros(2),
+              psi=np.array([psa, psb]), rho=np.array([ra, rb]))
+    p = params(mhd=sphx.MhdConfig(0.0, ch, sig), dt=dt)
+    s = O.State(st)
+    O.div_clean(s, p, O.pairs(s, p))
+    G = 0.75 / (PI * h ** 4)           # grad_w = -G x_hat
+    div_a = (mb / rb) * (Bb - Ba) * (-G)
+    div_b = (ma / ra) * (Bb - Ba) * (-G)
+    decay = math.exp(-sig * ch * dt / h)
+    assert s["psi"][0] == pytest.approx((psa - ch * ch * div_a * dt) * decay, rel=1e-13)
+    assert s["psi"][1] == pytest.approx((psb - ch * ch * div_b * dt) * decay, rel=1e-13)
+    # psi_b < psi_a: grad psi along x is negative, so B_x of both rises by |..| dt
+    assert s["bx"][0] == pytest.approx(Ba + (mb / rb) * (psb - psa) * G * dt, rel=1e-13)
+    assert s["bx"][1] == pytest.approx(Bb + (ma / ra) * (psb - psa) * G * dt, rel=1e-13)
+    assert s["bx"][0] < Ba and s["bx"][1] < Bb
+    assert s["by"] == pytest.approx([0.0, 0.3], abs=1e-300)
+    assert s["psi"][0] > psa * decay  # B rises along +x towards b: div_a < 0 raises psi_a
+
+
+def test_resistive_induction_by_hand():
+    # mhd.rs:124-131 with v = 0 (no ideal term), eta != 0, r = h:
+    #   dB_a = 2 eta (m_b/rho_b)(B_b - B_a)(r . grad_w) / (r^2 + 0.01 h^2) dt
+    #   dB_b = 2 eta (m_a/rho_a)(B_a - B_b)(-(r . grad_w)) / (...) dt
+    # with r . grad_w = r dW < 0 (the reference's signs, kept as written).
+    h, ma, mb, dt, eta = 0.1, 1.0, 2.0, 1e-3, 0.05
+    ra, rb = 1.5, 2.5
+    Ba = np.array([0.1, -0.2, 0.3])
+    Bb = np.array([0.4, 0.1, -0.1])
+    st = _two(h, ma=ma, mb=mb, axis=1, bx=np.array([Ba[0], Bb[0]]), by=np.array([Ba[1], Bb[1]]),
+              bz=np.array([Ba[2], Bb[2]]), psi=np.zeros(2), rho=np.array([ra, rb]))
+    p = params(mhd=sphx.MhdConfig(eta, 1.0, 0.5), dt=dt)
+    s = O.State(st)
+    pr = O.pairs(s, p)
+    O.mhd_induction(s, p, pr)
+    rdw = h * (-0.75 / (PI * h ** 4))
+    den = h * h + 0.01 * h * h
+    dA = 2 * eta * (mb / rb) * (Bb - Ba) * rdw / den * dt
+    dB = 2 * eta * (ma / ra) * (Bb - Ba) * rdw / den * dt
+    got_a = np.array([s["bx"][0], s["by"][0], s["bz"][0]])
+    got_b = np.array([s["bx"][1], s["by"][1], s["bz"][1]])
+    assert got_a == pytest.approx(Ba + dA, rel=1e-13)
+    assert got_b == pytest.approx(Bb + dB, rel=1e-13)
+    # swapping which particle is entity_a flips both terms
+    st["id"] = np.array([1, 0], dtype=np.uint32)
+    s = O.State(st)
+    O.mhd_induction(s, p, O.pairs(s, p))
+    assert np.array([s["bx"][0], s["by"][0], s["bz"][0]]) == pytest.approx(Ba - dA, rel=1e-13)
+    assert np.array([s["bx"][1], s["by"][1], s["bz"][1]]) == pytest.approx(Bb - dB, rel=1e-13)
+
+
+@pytest.mark.parametrize("eos", ["polytropic", "isothermal"])
+def test_eos_closed_forms(eos):
+    # polytropic (new): P = K rho^gamma; isothermal: P = rho k_B T / m
+    # (ideal_gas_pressure, thermodynamics.rs:232-238) with k_B T / m = iso_cs2
+    K, gam, cs2 = 1.3, 5.0 / 3.0, 2.7
+    kind = sphx.EOS_POLYTROPIC if eos == "polytropic" else sphx.EOS_ISOTHERMAL
+    p = params(eos=kind, poly_k=K, tait=sphx.TaitEquationConfig(1.0, 1.0, gam), iso_cs2=cs2)
+    for rho in (1.0, 2.0, 0.37):
+        st = dict(x=np.zeros(1), y=np.zeros(1), z=np.zeros(1), vx=np.zeros(1), vy=np.zeros(1),
+                  vz=np.zeros(1), m=np.ones(1), h=np.ones(1), rho=np.array([rho]))
+        s = O.State(st)
+        O.lib().orc_eos(s.ref(), O._pp(p))
+        expect = K * rho ** gam if eos == "polytropic" else cs2 * rho
+        assert s["p"][0] == pytest.approx(expect, rel=1e-14)
+    # the polytrope's exponent: P(2 rho) / P(rho) = 2^gamma
+    if eos == "polytropic":
+        st["rho"] = np.array([0.8])
+        s1 = O.State(st)
+        O.lib().orc_eos(s1.ref(), O._pp(p))
+        st["rho"] = np.array([1.6])
+        s2 = O.State(st)
+        O.lib().orc_eos(s2.ref(), O._pp(p))
+        assert s2["p"][0] / s1["p"][0] == pytest.approx(2 ** gam, rel=1e-14)
