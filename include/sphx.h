@@ -50,7 +50,11 @@ enum {
 enum SphxEos {
     SPHX_EOS_TAIT = 0,       /* P = B((rho/rho0)^gamma - 1), B = rho0 c^2/gamma, clamp >= 0 (sph.rs:157-167) */
     SPHX_EOS_POLYTROPIC = 1, /* adiabatic ideal gas at fixed entropy: P = K rho^gamma                       */
-    SPHX_EOS_ISOTHERMAL = 2  /* P = rho * (k_B T / m) = rho * cs2 (thermodynamics.rs:232-238)              */
+    SPHX_EOS_ISOTHERMAL = 2, /* P = rho * (k_B T / m) = rho * cs2 (thermodynamics.rs:232-238)              */
+    SPHX_EOS_ADIABATIC = 3   /* ideal gas P = (gamma - 1) rho u with the specific internal energy u
+                                (InternalEnergy / Mass, components/thermal.rs:9) evolved by
+                                du_i/dt = sum_j m_j (P_i/rho_i^2 + Pi_ij/2) v_ij . grad_i W_ij,
+                                the energy equation of the pressure and viscosity forces above */
 };
 enum SphxViscConvention {
     SPHX_VISC_REFERENCE = 0, /* active iff (v_i - v_j).(x_j - x_i) < 0, as sph.rs:292-295 evaluates it */
@@ -137,6 +141,7 @@ typedef struct SphxState {
     double* psi;                          /* DivCleaningPsi (optional)     */
     double* q;                            /* Charge         (optional)     */
     uint32_t* id;                         /* rank of Entity::to_bits(); NULL -> 0..n-1 */
+    double* u;                            /* InternalEnergy / Mass (J/kg); SPHX_EOS_ADIABATIC */
 } SphxState;
 
 /* ---- inputs of the reference's WGSL path (SphMode::Gpu callers) ---------- */
@@ -175,7 +180,7 @@ int sphx_state_from_vec4(const SphxUniformData* u, const float* pos4, const floa
 /* fields for sphx_download */
 enum SphxFields {
     SPHX_F_POS = 1, SPHX_F_VEL = 2, SPHX_F_RHO = 4, SPHX_F_P = 8, SPHX_F_ACC = 16,
-    SPHX_F_B = 32, SPHX_F_PSI = 64, SPHX_F_ALL = 127
+    SPHX_F_B = 32, SPHX_F_PSI = 64, SPHX_F_U = 128, SPHX_F_ALL = 255
 };
 
 typedef struct SphxCtx SphxCtx;

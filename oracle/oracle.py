@@ -20,7 +20,8 @@ PU = C.POINTER(C.c_uint32)
 class OrcState(C.Structure):
     _fields_ = [("n", C.c_uint32)] + [(k, PD) for k in (
         "x", "y", "z", "vx", "vy", "vz", "m", "h", "rho", "p", "ax", "ay", "az",
-        "apx", "apy", "apz", "bx", "by", "bz", "psi", "q")] + [("id", PU)]
+        "apx", "apy", "apz", "bx", "by", "bz", "psi", "q")] + [("id", PU)] + [
+        (k, PD) for k in ("u", "du", "dup")]
 
 
 _lib = None
@@ -87,6 +88,10 @@ class State:
             self.a["psi"] = np.array(src.get("psi", np.zeros(n)), dtype=np.float64, copy=True)
         if "q" in src:
             self.a["q"] = np.array(src["q"], dtype=np.float64, copy=True)
+        if "u" in src:  # adiabatic EOS: specific internal energy and its rates
+            self.a["u"] = np.array(src["u"], dtype=np.float64, copy=True)
+            for k in ("du", "dup"):
+                self.a[k] = np.array(src[k], dtype=np.float64, copy=True) if k in src else np.zeros(n)
         self.id = np.array(src.get("id", np.arange(n)), dtype=np.uint32, copy=True)
         self.c = OrcState()
         self.c.n = n

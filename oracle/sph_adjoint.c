@@ -20,7 +20,8 @@
  * density sum.  The neighbour sets and the viscosity switch are held fixed
  * (the map is piecewise smooth; the gradient is that of the active piece).
  *
- * Unsupported (returns -1): penalty walls, MHD, charge, XSPH, periodic boxes.
+ * Unsupported (returns -1): penalty walls, MHD, charge, XSPH, periodic boxes,
+ * the adiabatic EOS.
  */
 #include <math.h>
 #include <stdint.h>
@@ -161,7 +162,7 @@ static void snap_free(Snap* S) {
 int orc_adjoint(OrcState* s, const SphxParams* p, uint32_t nsteps, const double* target, double* loss,
                 double* gx, double* gv, double* galpha) {
     if ((p->boundary_flags & (SPHX_BND_PENALTY_CUBE | SPHX_BND_PERIODIC)) || s->bx || s->q ||
-        p->xsph_epsilon != 0.0)
+        p->xsph_epsilon != 0.0 || p->eos == SPHX_EOS_ADIABATIC)
         return -1;
     const uint32_t n = s->n;
     const double dt = p->dt, hdt = 0.5 * p->dt, hdt2 = 0.5 * p->dt * p->dt;

@@ -239,7 +239,9 @@ void orc_density(OrcState* s, const SphxParams* p, const uint32_t* pairs, uint64
 }
 
 /* ---- EOS: sph_eos_system, sph.rs:149-169 (Tait); polytropic is new;
- *      isothermal follows ideal_gas_pressure, thermodynamics.rs:232-238 --- */
+ *      isothermal follows ideal_gas_pressure, thermodynamics.rs:232-238;
+ *      adiabatic (new) is the ideal gas at the particle's specific internal
+ *      energy u = InternalEnergy / Mass (components/thermal.rs:9) ---------- */
 void orc_eos(OrcState* s, const SphxParams* p) {
     for (uint32_t i = 0; i < s->n; ++i) {
         double P;
@@ -250,6 +252,8 @@ void orc_eos(OrcState* s, const SphxParams* p) {
             if (P < 0.0) P = 0.0;
         } else if (p->eos == SPHX_EOS_POLYTROPIC) {
             P = p->poly_k * pow(s->rho[i], p->gamma);
+        } else if (p->eos == SPHX_EOS_ADIABATIC) {
+            P = (p->gamma - 1.0) * s->rho[i] * s->u[i];
         } else {
             P = s->rho[i] * p->iso_cs2;
         }
@@ -283,6 +287,15 @@ static void orc_lorentz(const OrcState* s, const SphxParams* p, double* F) {
     }
 }
 
+/* The energy equation of the adiabatic EOS (new; the reference has no SPH
+ * energy equation): du_a/dt = sum_b m_b (P_a/rho_a^2 + Pi_ab/2) v_ab . grad_a W_ab
+ * (Monaghan 1992, Annu. Rev. Astron. Astrophys. 30:543, eq. 2.23 with the
+ * viscous term), the rate that makes sum (m v^2/2 + m u) conserved under the
+ * pressure and viscosity forces below.  Accumulated by those two pair loops. */
+static inline int energy_on(const OrcState* s, const SphxParams* p) {
+    return p->eos == SPHX_EOS_ADIABATIC && s->u && s->du;
+}
+
 /* sph_pressure_force_system, sph.rs:177-228 */
 static void orc_pressure(const OrcState* s, const SphxParams* p, const uint32_t* pairs, uint64_t np,
                          double* F) {
@@ -300,6 +313,13 @@ static void orc_pressure(const OrcState* s, const SphxParams* p, const uint32_t*
             double fx = rx * mag, fy = ry * mag, fz = rz * mag;
             F[3 * a + 0] += fx; F[3 * a + 1] += fy; F[3 * a + 2] += fz;
             F[3 * b + 0] -= fx; F[3 * b + 1] -= fy; F[3 * b + 2] -= fz;
+            if (energy_on(s, p)) {
+                /* v_ab . grad_a W = -((v_a - v_b) . (x_b - x_a)) dW / r, the same for both ends */
+                double vg = -vdot(s->vx[a] - s->vx[b], s->vy[a] - s->vy[b], s->vz[a] - s->vz[b], dx, dy, dz) *
+                            (dw / r);
+                s->du[a] += s->m[b] * (s->p[a] / (s->rho[a] * s->rho[a])) * vg;
+                s->du[b] += s->m[a] * (s->p[b] / (s->rho[b] * s->rho[b])) * vg;
+            }
         }
     }
 }
@@ -333,6 +353,11 @@ static void orc_viscosity(const OrcState* s, const SphxParams* p, const uint32_t
             double fx = rx * mag, fy = ry * mag, fz = rz * mag;
             F[3 * a + 0] += fx; F[3 * a + 1] += fy; F[3 * a + 2] += fz;
             F[3 * b + 0] -= fx; F[3 * b + 1] -= fy; F[3 * b + 2] -= fz;
+            if (energy_on(s, p)) {
+                double vg = -sdot * (dw / r); /* v_ab . grad_a W */
+                s->du[a] += s->m[b] * (0.5 * pi_ij) * vg;
+                s->du[b] += s->m[a] * (0.5 * pi_ij) * vg;
+            }
         }
     }
 }
@@ -390,6 +415,7 @@ static void orc_boundary_penalty(const OrcState* s, const SphxParams* p, double*
 void orc_eval(OrcState* s, const SphxParams* p, const uint32_t* pairs, uint64_t np) {
     uint32_t n = s->n;
     double* F = (double*)calloc((size_t)3 * (n ? n : 1), sizeof(double));
+    if (energy_on(s, p)) memset(s->du, 0, sizeof(double) * n);
     orc_gravity(s, p, F);
     if (p->mhd_flags & SPHX_MHD_LORENTZ_CHARGE) orc_lorentz(s, p, F);
     orc_density(s, p, pairs, np);
@@ -421,6 +447,7 @@ void orc_vv_position(OrcState* s, const SphxParams* p) {
             *ap[k] = *a[k];
             *x[k] += *v[k] * dt + *a[k] * half_dt_sq;
         }
+        if (energy_on(s, p)) s->dup[i] = s->du[i];
         for (int k = 0; k < 3; ++k) {
             double lo = p->box_lo[k], hi = p->box_hi[k];
             if (p->boundary_flags & SPHX_BND_REFLECT) {
@@ -442,6 +469,8 @@ void orc_vv_velocity(OrcState* s, const SphxParams* p) {
         s->vx[i] += (s->apx[i] + s->ax[i]) * half_dt;
         s->vy[i] += (s->apy[i] + s->ay[i]) * half_dt;
         s->vz[i] += (s->apz[i] + s->az[i]) * half_dt;
+        /* the internal energy is advanced like the velocity (new) */
+        if (energy_on(s, p)) s->u[i] += (s->dup[i] + s->du[i]) * half_dt;
     }
 }
 
@@ -615,7 +644,7 @@ void orc_energy(const OrcState* s, const SphxParams* p, double out[8]) {
         double v2 = vdot(s->vx[i], s->vy[i], s->vz[i], s->vx[i], s->vy[i], s->vz[i]);
         ke += 0.5 * m * v2;
         pe -= m * ((p->gravity[0] * s->x[i] + p->gravity[1] * s->y[i]) + p->gravity[2] * s->z[i]);
-        ie += m * specific_internal(p, s->rho[i]);
+        ie += m * ((p->eos == SPHX_EOS_ADIABATIC && s->u) ? s->u[i] : specific_internal(p, s->rho[i]));
         if (s->bx) {
             double b2 = vdot(s->bx[i], s->by[i], s->bz[i], s->bx[i], s->by[i], s->bz[i]);
             me += 0.5 * (m / s->rho[i]) * b2;

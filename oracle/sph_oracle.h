@@ -17,6 +17,7 @@ typedef struct OrcState {
     double *apx, *apy, *apz; /* PreviousAcceleration        */
     double *bx, *by, *bz, *psi, *q; /* optional (NULL)      */
     uint32_t* id;
+    double *u, *du, *dup;    /* SPHX_EOS_ADIABATIC: specific internal energy, du/dt, previous du/dt */
 } OrcState;
 
 double orc_kernel_w(double r, double h);

@@ -18,8 +18,8 @@ import numpy as np
 
 from . import _abi
 from ._abi import (  # noqa: F401  (re-exported constants)
-    BND_PENALTY_CUBE, BND_PERIODIC, BND_REFLECT, EOS_ISOTHERMAL, EOS_POLYTROPIC, EOS_TAIT,
-    F_ACC, F_ALL, F_B, F_P, F_POS, F_PSI, F_RHO, F_VEL, FORCES_FULL, FORCES_VISC_ONLY,
+    BND_PENALTY_CUBE, BND_PERIODIC, BND_REFLECT, EOS_ADIABATIC, EOS_ISOTHERMAL, EOS_POLYTROPIC, EOS_TAIT,
+    F_ACC, F_ALL, F_B, F_P, F_POS, F_PSI, F_RHO, F_U, F_VEL, FORCES_FULL, FORCES_VISC_ONLY,
     MHD_CLEANING, MHD_INDUCTION, MHD_LORENTZ_CHARGE, MHD_MAXWELL_STRESS, PREC_FP32_MIXED,
     PREC_FP64, VISC_MONAGHAN, VISC_REFERENCE, SphxParams,
 )
@@ -191,6 +191,10 @@ class Context:
             ida = np.ascontiguousarray(state["id"], dtype=np.uint32)
             st.id = _ptr(ida, C.c_uint32)
             keep.append(ida)
+        if "u" in state:
+            ua = np.ascontiguousarray(state["u"], dtype=np.float64)
+            st.u = _ptr(ua)
+            keep.append(ua)
         self._check(self._lib.sphx_upload(self._h, C.byref(st)))
         self.n = n
 
@@ -201,7 +205,7 @@ class Context:
         st.n = n
         groups = [(F_POS, ("x", "y", "z")), (F_VEL, ("vx", "vy", "vz")), (F_RHO, ("rho",)),
                   (F_P, ("p",)), (F_ACC, ("ax", "ay", "az")), (F_B, ("bx", "by", "bz")),
-                  (F_PSI, ("psi",))]
+                  (F_PSI, ("psi",)), (F_U, ("u",))]
         for bit, names in groups:
             if fields & bit:
                 for k in names:
@@ -211,11 +215,11 @@ class Context:
         return out
 
     def update(self, state: Dict[str, np.ndarray], fields: int) -> None:
-        """sphx_update: new velocities (F_VEL) and / or B (F_B) and psi (F_PSI), upload order."""
+        """sphx_update: new velocities (F_VEL), B (F_B), psi (F_PSI) and / or internal energy (F_U), upload order."""
         st = _abi.SphxState()
         st.n = self.n
         keep = []
-        groups = [(F_VEL, ("vx", "vy", "vz")), (F_B, ("bx", "by", "bz")), (F_PSI, ("psi",))]
+        groups = [(F_VEL, ("vx", "vy", "vz")), (F_B, ("bx", "by", "bz")), (F_PSI, ("psi",)), (F_U, ("u",))]
         for bit, names in groups:
             if fields & bit:
                 for k in names:

@@ -16,12 +16,12 @@ ERRORS = {
     1: "SPHX_E_NODEVICE", 2: "SPHX_E_OOM", 3: "SPHX_E_CAPACITY", 4: "SPHX_E_INVALID",
     5: "SPHX_E_MULTISECTOR", 6: "SPHX_E_NONFINITE", 7: "SPHX_E_HIP", 8: "SPHX_E_STATE",
 }
-EOS_TAIT, EOS_POLYTROPIC, EOS_ISOTHERMAL = 0, 1, 2
+EOS_TAIT, EOS_POLYTROPIC, EOS_ISOTHERMAL, EOS_ADIABATIC = 0, 1, 2, 3
 VISC_REFERENCE, VISC_MONAGHAN = 0, 1
 BND_PENALTY_CUBE, BND_REFLECT, BND_PERIODIC = 1, 2, 4
 MHD_INDUCTION, MHD_CLEANING, MHD_MAXWELL_STRESS, MHD_LORENTZ_CHARGE = 1, 2, 4, 8
 PREC_FP64, PREC_FP32_MIXED = 0, 1
-F_POS, F_VEL, F_RHO, F_P, F_ACC, F_B, F_PSI, F_ALL = 1, 2, 4, 8, 16, 32, 64, 127
+F_POS, F_VEL, F_RHO, F_P, F_ACC, F_B, F_PSI, F_U, F_ALL = 1, 2, 4, 8, 16, 32, 64, 128, 255
 FORCES_FULL, FORCES_VISC_ONLY = 0, 1
 
 D3 = C.c_double * 3
@@ -53,7 +53,7 @@ class SphxState(C.Structure):
         ("n", C.c_uint32),
         ("x", PD), ("y", PD), ("z", PD), ("vx", PD), ("vy", PD), ("vz", PD),
         ("m", PD), ("h", PD), ("rho", PD), ("p", PD), ("ax", PD), ("ay", PD), ("az", PD),
-        ("bx", PD), ("by", PD), ("bz", PD), ("psi", PD), ("q", PD), ("id", PU),
+        ("bx", PD), ("by", PD), ("bz", PD), ("psi", PD), ("q", PD), ("id", PU), ("u", PD),
     ]
 
 

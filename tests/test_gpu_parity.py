@@ -40,6 +40,15 @@ def oracle_pairs_ids(state, params, brute=False):
     return O.id_pairs(s, O.pairs(s, params, brute=brute))
 
 
+def shuffle_ids(state, seed=31):
+    """ids in a seeded random order: the ECS rank then differs from the upload slot,
+    so a pass that orients pairs by the slot instead of the id fails."""
+    s = dict(state)
+    s["id"] = np.argsort(scenarios.SplitMix64(seed).uniform(len(state["x"]))).astype(np.uint32)
+    assert (s["id"] != np.arange(len(s["id"]))).mean() > 0.9
+    return s
+
+
 def rel_err(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
@@ -231,6 +240,7 @@ def test_penalty_walls_lorentz_parity(require_gpu):
 # ------------------------------------------------------------- MHD
 def test_mhd_passes_parity(require_gpu):
     st, p = scenarios.mhd_periodic(16)
+    st = shuffle_ids(st)
     p.resistivity = 1e-3  # exercise the resistive term too
     with make_ctx(st, p) as ctx:
         ctx.forces()
@@ -346,6 +356,7 @@ def test_c0_dam_break_200_steps(require_gpu):
 
 def test_step_parity_mhd(require_gpu):
     st, p = scenarios.mhd_periodic(16)
+    st = shuffle_ids(st, seed=32)
     nsteps = 5
     with make_ctx(st, p) as ctx:
         ctx.step(nsteps)
@@ -415,6 +426,32 @@ def test_error_paths(require_gpu):
         with pytest.raises(sphx.SphxError) as e:
             ctx.step(1)
         assert e.value.code == 3
+
+
+def test_set_params_failure_leaves_context_unchanged(require_gpu):
+    # max_neighbours is rounded up to a multiple of 4 at create (190 -> 192):
+    # passing the create-time value back is accepted; a different capacity is
+    # SPHX_E_INVALID and the context keeps its old parameters and state
+    st, p = scenarios.dam_break(nx=12, ny=12, nz=12)
+    st = with_velocity(st, 0.3)
+    p.max_neighbours = 190
+    with make_ctx(st, p) as ctx:
+        ctx.set_params(p)
+        ctx.forces()
+        a0 = ctx.download(sphx.F_ACC)
+        bad = sphx.make_params(p.dt, visc=sphx.SphViscosityConfig(2.0, 0.0), max_neighbours=300)
+        with pytest.raises(sphx.SphxError) as e:
+            ctx.set_params(bad)
+        assert e.value.code == 4
+        a1 = ctx.download(sphx.F_ACC)  # derived data still valid: nothing was reset
+        ctx.forces()
+        a2 = ctx.download(sphx.F_ACC)  # recomputed with the old parameters
+        p.max_neighbours = 0           # keeps the capacity
+        ctx.set_params(p)
+        ctx.forces()
+        a3 = ctx.download(sphx.F_ACC)
+    for k in ("ax", "ay", "az"):
+        assert np.array_equal(a0[k], a1[k]) and np.array_equal(a0[k], a2[k]) and np.array_equal(a0[k], a3[k]), k
 
 
 def test_determinism(require_gpu):

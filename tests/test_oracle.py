@@ -11,6 +11,10 @@
 |                            |                           | two-particle closed form                                   |
 | orc_eos                    | sph.rs:149-169            | P(rho0) = 0, P(1100) closed form, clamp (sph.rs:518-526)   |
 |   polytropic / isothermal  | new / thermodyn.rs:232-238| P(rho) closed forms at three densities; P(2r)/P(r) = 2^g   |
+|   adiabatic (new) + du/dt  | ideal gas P=(g-1) rho u;  | P closed form; two-particle du by hand (both conventions,  |
+|                            | thermal.rs:9 InternalEn.  | expansion cools, compression heats, pair power balance);   |
+|                            |                           | inviscid box keeps each particle's entropy and the total   |
+|                            |                           | energy, drift halving with dt; viscous heating conserves E |
 | pressure / viscosity       | sph.rs:177-322            | momentum conservation; hydrostatic column (sph.rs:528-570)|
 |                            |                           | mirror symmetry; two-particle viscosity both conventions   |
 | boundary penalty           | sph.rs:443-484            | one-particle closed form                                   |
@@ -611,3 +615,116 @@ def test_eos_closed_forms(eos):
         s2 = O.State(st)
         O.lib().orc_eos(s2.ref(), O._pp(p))
         assert s2["p"][0] / s1["p"][0] == pytest.approx(2 ** gam, rel=1e-14)
+
+
+# ---------------------------------------------------------------- adiabatic ideal gas (new)
+def _adiabatic(**kw):
+    base = dict(tait=sphx.TaitEquationConfig(1.0, 1.3, 5.0 / 3.0), visc=sphx.SphViscosityConfig(0.0, 0.0),
+                eos=sphx.EOS_ADIABATIC)
+    base.update(kw)
+    return params(**base)
+
+
+def test_adiabatic_eos_closed_form():
+    gam = 5.0 / 3.0
+    p = _adiabatic()
+    for rho, u in ((1.0, 1.5), (2.0, 0.4), (0.3, 7.0)):
+        st = dict(x=np.zeros(1), y=np.zeros(1), z=np.zeros(1), vx=np.zeros(1), vy=np.zeros(1),
+                  vz=np.zeros(1), m=np.ones(1), h=np.ones(1), rho=np.array([rho]), u=np.array([u]))
+        s = O.State(st)
+        O.lib().orc_eos(s.ref(), O._pp(p))
+        assert s["p"][0] == pytest.approx((gam - 1.0) * rho * u, rel=1e-15)
+
+
+@pytest.mark.parametrize("conv", [sphx.VISC_REFERENCE, sphx.VISC_MONAGHAN])
+@pytest.mark.parametrize("moving", ["apart", "together"])
+def test_adiabatic_energy_rate_two_particles(conv, moving):
+    # a at x = 0, b at x = h (q = 1), v_a = -+w, v_b = +-w, unequal u:
+    #   du_a = m_b (P_a/rho_a^2 + Pi/2) v_ab . grad_a W, grad_a W = G x_hat, G = 0.75/(pi h^4)
+    # v_ab . grad_a W = -2 w G apart (expansion cools), +2 w G together (compression heats)
+    h, m, w, alpha, c, gam = 0.1, 0.002, 0.3, 0.8, 1.3, 5.0 / 3.0
+    sgn = -1.0 if moving == "apart" else 1.0
+    ua, ub = 1.5, 2.5
+    st = dict(x=np.array([0.0, h]), y=np.zeros(2), z=np.zeros(2), vx=np.array([sgn * w, -sgn * w]),
+              vy=np.zeros(2), vz=np.zeros(2), m=np.full(2, m), h=np.full(2, h), u=np.array([ua, ub]))
+    p = _adiabatic(visc=sphx.SphViscosityConfig(alpha, 0.0), visc_convention=conv,
+                   tait=sphx.TaitEquationConfig(1.0, c, gam))
+    s = O.State(st)
+    O.evaluate(s, p)
+    rho = m * 1.25 / (PI * h ** 3)
+    assert s["rho"] == pytest.approx([rho, rho], rel=1e-14)
+    Pa, Pb = (gam - 1) * rho * ua, (gam - 1) * rho * ub
+    G = 0.75 / (PI * h ** 4)
+    vg = 2 * w * G * (1.0 if moving == "together" else -1.0)
+    active = (moving == "apart") == (conv == sphx.VISC_REFERENCE)
+    # |mu| = h (2 w h) / (1.01 h^2); Pi = alpha c |mu| / rho when the switch is on
+    pi_ab = alpha * c * (2 * w / 1.01) / rho if active else 0.0
+    assert s["du"][0] == pytest.approx(m * (Pa / rho ** 2 + 0.5 * pi_ab) * vg, rel=1e-12)
+    assert s["du"][1] == pytest.approx(m * (Pb / rho ** 2 + 0.5 * pi_ab) * vg, rel=1e-12)
+    # the pair's power balance: m_a v_a.a_a + m_b v_b.a_b + m_a du_a + m_b du_b = 0
+    power = m * (s["vx"][0] * s["ax"][0] + s["vx"][1] * s["ax"][1]) + m * (s["du"][0] + s["du"][1])
+    assert abs(power) < 1e-12 * m * abs(s["du"][0])
+
+
+def _gas_box(n_side=6, jitter=0.2, seed=41, v=0.3):
+    rng = scenarios.SplitMix64(seed)
+    sp = 1.0 / n_side
+    x, y, z = scenarios.lattice(n_side, n_side, n_side, sp)
+    n = x.size
+    x = np.mod(x + (2 * rng.uniform(n) - 1) * jitter * sp, 1.0)
+    y = np.mod(y + (2 * rng.uniform(n) - 1) * jitter * sp, 1.0)
+    z = np.mod(z + (2 * rng.uniform(n) - 1) * jitter * sp, 1.0)
+    st = dict(x=x, y=y, z=z, vx=v * rng.normal(n), vy=v * rng.normal(n), vz=v * rng.normal(n),
+              m=np.full(n, 1.0 / n), h=np.full(n, 1.3 * sp), u=1.5 + 0.3 * rng.uniform(n))
+    kw = dict(boundary_flags=sphx.BND_PERIODIC, box_lo=(0, 0, 0), box_hi=(1, 1, 1))
+    return st, kw
+
+
+def _adiabatic_drift(st, p, steps):
+    """Largest entropy change |A1/A0 - 1| of a particle, A = (gamma - 1) u / rho^(gamma - 1),
+    the total-energy change and the two energy budgets over `steps` ticks."""
+    gam = p.gamma
+    s = O.State(st)
+    O.evaluate(s, p)
+    A0 = (gam - 1) * s["u"] / s["rho"] ** (gam - 1)
+    e0 = O.energy(s, p)
+    O.step(s, p, steps)
+    O.evaluate(s, p)
+    A1 = (gam - 1) * s["u"] / s["rho"] ** (gam - 1)
+    e1 = O.energy(s, p)
+    tot0, tot1 = e0["kinetic"] + e0["internal"], e1["kinetic"] + e1["internal"]
+    return A1 / A0 - 1, (tot1 - tot0) / tot0, e0, e1, s
+
+
+def test_adiabatic_inviscid_run_conserves_entropy_and_energy():
+    # With summation density, sum_b m_b v_ab . grad_a W = d rho_a / dt, so the
+    # pressure part of the energy equation is du = (P / rho^2) d rho: without
+    # viscosity each particle keeps its entropy A = (gamma - 1) u / rho^(gamma - 1)
+    # and the total energy sum (m v^2/2 + m u) is conserved, up to the error of
+    # the tick.  That error is first order in dt (the schedule's second
+    # evaluation uses v_n, main.rs:103-157), so over a fixed time both drifts
+    # halve with dt; a wrong energy equation leaves a drift that does not.
+    st, kw = _gas_box()
+    dA1, dE1, e0, e1, s = _adiabatic_drift(st, _adiabatic(dt=2e-4, **kw), 200)
+    dA2, dE2, _, _, _ = _adiabatic_drift(st, _adiabatic(dt=1e-4, **kw), 400)
+    assert np.max(np.abs(s["u"] - st["u"])) > 0.05         # the gas was compressed / expanded ...
+    assert abs(e1["kinetic"] - e0["kinetic"]) > 1e-3 * e0["internal"]
+    assert np.abs(dA1).max() < 1e-4                         # ... adiabatically
+    assert np.abs(dA2).max() == pytest.approx(0.5 * np.abs(dA1).max(), rel=0.05)
+    assert abs(dE1) < 1e-5
+    assert dE2 == pytest.approx(0.5 * dE1, rel=0.05)
+
+
+def test_adiabatic_viscous_heating_conserves_total_energy():
+    # Monaghan's switch (approaching pairs) with alpha = 1: kinetic energy goes
+    # into heat (sum m u rises, the mean entropy rises) and the total is
+    # conserved up to the tick's first-order error, as without viscosity
+    st, kw = _gas_box(v=0.6)
+    vk = dict(visc=sphx.SphViscosityConfig(1.0, 0.0), visc_convention=sphx.VISC_MONAGHAN)
+    dA1, dE1, e0, e1, _ = _adiabatic_drift(st, _adiabatic(dt=2e-4, **vk, **kw), 200)
+    dA2, dE2, _, _, _ = _adiabatic_drift(st, _adiabatic(dt=1e-4, **vk, **kw), 400)
+    assert e1["internal"] - e0["internal"] > 0.05 * e0["kinetic"]
+    assert e1["kinetic"] < 0.95 * e0["kinetic"]
+    assert np.mean(dA1) > 1e-3 and np.mean(dA2) == pytest.approx(np.mean(dA1), rel=0.05)
+    assert abs(dE1) < 1e-4 * (e0["kinetic"] / (e0["kinetic"] + e0["internal"]))
+    assert dE2 == pytest.approx(0.5 * dE1, rel=0.1)
