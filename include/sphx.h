@@ -205,32 +205,35 @@ int sphx_download(SphxCtx* ctx, SphxState* host, uint32_t fields);
 /* Residency: replaces the per-tick re-upload of buffers in the reference's
  * GPU path (create_buffers_and_bind_group, gpu/pipeline.rs:168-206, runs for
  * every dispatch) when only some components changed.  Writes the fields
- * SPHX_F_VEL, SPHX_F_B and SPHX_F_PSI of host (upload order, host->n equal to
+ * SPHX_F_VEL, SPHX_F_B, SPHX_F_PSI and SPHX_F_U of host (upload order, host->n equal to
  * the context's n) into the resident state, keeping the sort, the grid and the
  * neighbour list; density and forces are recomputed at the next use.
  * Positions, masses and radii change through sphx_upload.  Blocks until the
  * copies are done.  SPHX_E_INVALID for other fields or a missing array,
- * SPHX_E_NONFINITE for a non-finite value, SPHX_E_STATE for B / psi without an
- * uploaded field. */
+ * SPHX_E_NONFINITE for a non-finite value, SPHX_E_STATE for B / psi / u without
+ * an uploaded field. */
 int sphx_update(SphxCtx* ctx, const SphxState* host, uint32_t fields);
 
 /* ---- passes (each replaces one ECS system; async on the ctx stream) ---- */
 /* build_broadphase_system + broadphase_query_system (broadphase.rs:33-96):
  * cell hash, counting sort, reorder, exact neighbour lists r < 2*h_avg.   */
 int sphx_neighbours(SphxCtx* ctx);
-/* sph_density_system (sph.rs:72-116) + sph_eos_system (sph.rs:149-169)    */
+/* sph_density_system (sph.rs:72-116) + sph_eos_system (sph.rs:149-169)
+ * (the adiabatic EOS at each particle's current u)                         */
 int sphx_density(SphxCtx* ctx);
 /* reset_forces (forces.rs:15) + sph_pressure_force_system (sph.rs:177-228)
  * + sph_viscosity_system (sph.rs:260-322) + sph_boundary_system
  * (sph.rs:443-484) + lorentz_force_system (lorentz.rs:21-39)
  * + compute_acceleration_system (integrators.rs:110-118).
  * flags = SPHX_FORCES_VISC_ONLY reuses the pressure sum stored by the last
- * full call at the same positions (bit-identical to recomputing it).       */
+ * full call at the same positions (bit-identical to recomputing it); it is
+ * SPHX_E_STATE with the adiabatic EOS, whose full pass also gives du/dt.   */
 enum { SPHX_FORCES_FULL = 0, SPHX_FORCES_VISC_ONLY = 1 };
 int sphx_forces(SphxCtx* ctx, uint32_t flags);
 /* velocity_verlet_position_system (integrators.rs:63-83)                  */
 int sphx_vv_position(SphxCtx* ctx);
-/* velocity_verlet_velocity_system (integrators.rs:92-104)                 */
+/* velocity_verlet_velocity_system (integrators.rs:92-104); with the
+ * adiabatic EOS also u += (du_n + du*) dt / 2, du_n saved by vv_position   */
 int sphx_vv_velocity(SphxCtx* ctx);
 /* sph_xsph_system (sph.rs:349-405)                                         */
 int sphx_xsph(SphxCtx* ctx);
@@ -255,7 +258,7 @@ int sphx_step(SphxCtx* ctx, uint32_t nsteps);
  * every nsteps) and recomputes each segment's tape during the reverse pass.
  * Arrays are in upload order.  On return the context holds the state after
  * nsteps ticks, as sphx_step would leave it.  FP64 form only; no XSPH, MHD,
- * penalty walls or periodic boxes (SPHX_E_INVALID).  The neighbour sets and
+ * adiabatic EOS, penalty walls or periodic boxes (SPHX_E_INVALID).  The neighbour sets and
  * the viscosity switch are those of the forward trajectory.              */
 int sphx_adjoint(SphxCtx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
                  const double* target_x, const double* target_y, const double* target_z,

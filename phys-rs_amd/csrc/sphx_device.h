@@ -223,12 +223,14 @@ __device__ __forceinline__ float inv_h_pair_f(float hi, float hj, float inv_hi) 
 }
 
 // ---- equations of state (sph.rs:157-167; thermodynamics.rs:232-238) -----
-__device__ __forceinline__ double eos_pressure(const KernelParams& kp, double rho) {
+// u: the particle's specific internal energy (read by the adiabatic EOS only)
+__device__ __forceinline__ double eos_pressure(const KernelParams& kp, double rho, double u = 0.0) {
     if (kp.eos == SPHX_EOS_TAIT) {
         double P = kp.tait_B * (pow(rho / kp.rest_density, kp.gamma) - 1.0);
         return P < 0.0 ? 0.0 : P;
     }
     if (kp.eos == SPHX_EOS_POLYTROPIC) return kp.poly_k * pow(rho, kp.gamma);
+    if (kp.eos == SPHX_EOS_ADIABATIC) return (kp.gamma - 1.0) * rho * u;
     return rho * kp.iso_cs2;
 }
 // dP/drho (0 where the Tait clamp is active)

@@ -39,7 +39,8 @@ struct Particles {
     double *m, *h;
     double *bx, *by, *bz, *psi;
     double *q;
-    uint32_t *id, *orig;       // ECS order id; upload slot
+    double *u, *dup;           // adiabatic EOS: specific internal energy, du/dt of the tick's first evaluation
+    uint32_t *id, *orig;       // rank of the entity (pair orientation); upload slot
 };
 
 // Gather records of the force pass: one particle's neighbour-side fields in
@@ -66,6 +67,7 @@ struct Derived {
     double *rho, *p, *pr2;     // density, pressure, P / rho^2
     double *ax, *ay, *az;      // acceleration
     double *spx, *spy, *spz;   // m g + sum of pressure terms (reused by VISC_ONLY)
+    double* du;                // adiabatic EOS: du/dt of the last full force pass
     double *tmp3[3];           // Jacobi scratch (xsph / induction / cleaning)
     double *tmp1;
 };
@@ -189,7 +191,7 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, 
 void launch_scatter(const Sort& S, uint32_t n, hipStream_t s);
 void launch_cell_order(const Sort& S, uint32_t n, const double* x, hipStream_t s);
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n,
-                    bool mhd, bool charge, bool mixed, const Mixed& X, const Grid& g, double4* recb,
+                    bool mhd, bool charge, bool mixed, bool energy, const Mixed& X, const Grid& g, double4* recb,
                     double4* xref, hipStream_t s);
 void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
 void launch_bricks(const Grid& g, const Sort& S, const Bricks& B, hipStream_t s);
@@ -199,9 +201,14 @@ void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort&
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                     const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
                     const uint32_t* cell_start, hipStream_t s);
+// energy: also du/dt of the adiabatic EOS (full passes only)
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                   const KernelParams& kp, bool visc_only, bool mhd_stress, bool charge, bool mixed,
+                   const KernelParams& kp, bool visc_only, bool energy, bool mhd_stress, bool charge, bool mixed,
                    const Mixed& X, const Bricks& B, const uint32_t* cell_start, hipStream_t s);
+// P, P/rho^2 and the records' P/rho^2 from the kept density and the current
+// internal energy (the adiabatic EOS after the internal energy changed)
+void launch_eos(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, bool mixed,
+                hipStream_t s);
 // The moved-skin/2 flag of the position updates: the device word and its
 // copy in pinned host memory, both holding the epoch in which a particle was
 // last found beyond skin/2.  The host starts a new epoch at each list build

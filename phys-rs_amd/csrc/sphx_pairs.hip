@@ -298,7 +298,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         acc = a0 + acc;
     }
     rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 0.25), acc, rho) : rho + acc;
-    const double pr = eos_pressure(kp, rho);
+    const double pr = eos_pressure(kp, rho, kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
     D.p[i] = pr;
@@ -368,7 +368,7 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
         gacc = 0.0f;
     });
     rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 0.25), acc, rho) : rho + acc;
-    const double pr = eos_pressure(kp, rho);
+    const double pr = eos_pressure(kp, rho, kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
     D.p[i] = pr;
@@ -504,7 +504,10 @@ __device__ __forceinline__ void force_epilogue(const Particles& P, const Derived
 }
 
 // LANES = 2: the calling lane and its neighbour (lane ^ 1) share particle i.
-template <bool VISC_ONLY, bool STRESS, bool UNI, int PER, int LANES, class Fetch>
+// ENERGY (adiabatic EOS, full pass only): du_i/dt = sum_j m_j (P_i/rho_i^2 +
+// Pi_ij/2) v_ij . grad_i W_ij, from the same pair terms (v_ij . grad_i W =
+// -s dW/r with s = (v_i - v_j).(x_j - x_i)), summed beside the force.
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, int PER, int LANES, class Fetch>
 __device__ __forceinline__ void force_particle(const Particles& P, const Derived& D, const Grid& g,
                                                const NbrList& L, const KernelParams& kp, int charge, uint32_t i,
                                                const Fetch& fetch) {
@@ -528,6 +531,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     // (REFERENCE: active for s < 0; MONAGHAN: for s > 0), so hs = 0 and Pi = 0
     const double s_lo = ref_conv ? -INFINITY : 0.0, s_hi = ref_conv ? 0.0 : INFINITY;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
+    double de = 0.0;  // ENERGY: sum of (P_i/rho_i^2 + Pi/2) s dW/r
     const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     // The pair body is branch-free: a pair without a term adds +-0, which
     // leaves a sum unchanged (the sums start at +0 and never become -0).
@@ -567,6 +571,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
             fv[0] = fma(dx, sc, fv[0]);
             fv[1] = fma(dy, sc, fv[1]);
             fv[2] = fma(dz, sc, fv[2]);
+            if (ENERGY) de = fma(fma(0.5, pi_ij, pr2i), sdot * dwr, de);
         }
         if (STRESS) {
             const double gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;  // grad_i W (times m_i m_j)
@@ -606,17 +611,21 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
             fv[a] += __shfl_xor(fv[a], 1);
             fm[a] += __shfl_xor(fm[a], 1);
         }
+        if (ENERGY) de += __shfl_xor(de, 1);
         if (h) return;
     } else {
         sum_range(0u, half);
         const double p0[3] = {fp[0], fp[1], fp[2]}, v0[3] = {fv[0], fv[1], fv[2]}, m0[3] = {fm[0], fm[1], fm[2]};
+        const double e0 = de;
         for (int a = 0; a < 3; ++a) fp[a] = fv[a] = fm[a] = 0.0;
+        de = 0.0;
         sum_range(half, 0xffffffffu);
         for (int a = 0; a < 3; ++a) {
             fp[a] = p0[a] + fp[a];
             fv[a] = v0[a] + fv[a];
             fm[a] = m0[a] + fm[a];
         }
+        de = e0 + de;
     }
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
@@ -626,18 +635,21 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
             fv[a] *= k;
             fm[a] *= k;
         }
+        if (ENERGY) D.du[i] = -de * (kp.m0 * ((ih2 * ih2) * kInvPi));
+    } else if (ENERGY) {
+        D.du[i] = -de / mi;
     }
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
 // Two lanes per particle, as the staged pass (same halves, same sums): at
 // the sizes that run unstaged, one thread per particle is few waves per CU.
-template <bool VISC_ONLY, bool STRESS, bool UNI>
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                     KernelParams kp, int charge) {
     const uint32_t i = particle_index() >> 1;
     if (i >= n) return;
-    force_particle<VISC_ONLY, STRESS, UNI, -1, 2>(P, D, g, L, kp, charge, i,
+    force_particle<VISC_ONLY, ENERGY, STRESS, UNI, -1, 2>(P, D, g, L, kp, charge, i,
                                            GlobalFetch<double2, 4>{reinterpret_cast<const double2*>(D.r64)});
 }
 
@@ -645,7 +657,7 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
 // with the one halo that fits the LDS (measured: force 0.41 -> 0.39 ms,
 // visc_only 0.40 -> 0.36 ms on the 1M dam break).  With the Maxwell stress
 // the B records are staged beside the f64 records (96 bytes a record).
-template <bool VISC_ONLY, bool STRESS, bool UNI, int PER>
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, int PER>
 __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P, Derived D, Grid g, NbrList L,
                                                                     KernelParams kp, int charge, Bricks B,
                                                                     const uint32_t* __restrict__ start,
@@ -660,13 +672,14 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
     brick_pass<double2, 4, W, PF, 2, NQ2>(
         B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<double2, 4, W, PF, NQ2>& f) {
-            force_particle<VISC_ONLY, STRESS, UNI, PER, 2>(P, D, g, L, kp, charge, i, f);
+            force_particle<VISC_ONLY, ENERGY, STRESS, UNI, PER, 2>(P, D, g, L, kp, charge, i, f);
         },
         reinterpret_cast<const double2*>(D.recb));
 }
 
-// FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue.
-template <bool VISC_ONLY, bool STRESS, bool UNI, class Fetch>
+// FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue (and the
+// ENERGY sum as in the f64 pass, f32 per group of four entries).
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, class Fetch>
 __device__ __forceinline__ void force_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
                                                      const Grid& g, const NbrList& L, const KernelParams& kp,
                                                      int charge, uint32_t i, const Fetch& fetch) {
@@ -695,6 +708,8 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     // f32 sums over each group of four list entries, added to the f64 sums
     // once per group; with uniform m and h, m_i m_j sigma_g is folded out
     float gp[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
+    double de = 0.0;
+    float ge = 0.0f;
     // branch-free pair body, as in the f64 pass (factors 1 or 0, not selects)
     walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[Fetch::NR]) {
         const uint32_t j = e & kIdxMask;
@@ -736,6 +751,7 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             gv[0] = fmaf(dx, sc, gv[0]);
             gv[1] = fmaf(dy, sc, gv[1]);
             gv[2] = fmaf(dz, sc, gv[2]);
+            if (ENERGY) ge = fmaf(fmaf(0.5f, pi_ij, pr2i), sdot * dwr, ge);
         }
         if (STRESS) {
             const float gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;
@@ -766,6 +782,10 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             fv[a] += (double)gv[a];
             gp[a] = gv[a] = 0.0f;
         }
+        if (ENERGY) {
+            de += (double)ge;
+            ge = 0.0f;
+        }
     });
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
@@ -775,15 +795,18 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             fv[a] *= k;
             fm[a] *= k;
         }
+        if (ENERGY) D.du[i] = -de * ((kp.m0 * ((ih2 * ih2) * kInvPi)) * 0.75);
+    } else if (ENERGY) {
+        D.du[i] = -de / P.m[i];
     }
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
 
-template <bool VISC_ONLY, bool STRESS, bool UNI>
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
                                                           NbrList L, KernelParams kp, int charge) {
     SPHX_PASS_PROLOGUE();
-    force_mixed_particle<VISC_ONLY, STRESS, UNI>(P, D, X, g, L, kp, charge, i,
+    force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI>(P, D, X, g, L, kp, charge, i,
                                                  GlobalFetch<float4, 2>{reinterpret_cast<const float4*>(D.r32)});
 }
 
@@ -792,7 +815,7 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
 // LDS, so the registers are sized for one workgroup per CU (two waves per
 // SIMD), which keeps the sixteen record quads of a four-entry step out of
 // scratch memory.
-template <bool VISC_ONLY, bool STRESS, bool UNI, int W, bool PF>
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, int W, bool PF>
 __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(STRESS ? 2 : 4)))
 k_forces_mixed_staged(
     Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, int charge, Bricks B,
@@ -803,66 +826,68 @@ k_forces_mixed_staged(
     brick_pass<float4, 2, W, PF, 1, NQ2>(
         B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<float4, 2, W, PF, NQ2>& f) {
-            force_mixed_particle<VISC_ONLY, STRESS, UNI>(P, D, X, g, L, kp, charge, i, f);
+            force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI>(P, D, X, g, L, kp, charge, i, f);
         },
         reinterpret_cast<const float4*>(D.recb));
 }
 
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                   const KernelParams& kp, bool visc_only, bool stress, bool charge, bool mixed, const Mixed& X,
-                   const Bricks& B, const uint32_t* cell_start, hipStream_t s) {
+                   const KernelParams& kp, bool visc_only, bool energy, bool stress, bool charge, bool mixed,
+                   const Mixed& X, const Bricks& B, const uint32_t* cell_start, hipStream_t s) {
     if (!n) return;
     const dim3 grid(nblocks(n)), block(kBlock);
     const int ch = charge ? 1 : 0;
     const bool uni = kp.uniform != 0;
+    energy = energy && !visc_only;  // the energy equation needs the pressure term of a full pass
     if (B.on && B.nactive) {
-#define SPHX_FORCES_STAGED(V, M, U)                                                                         \
-        if (visc_only == V && stress == M && uni == U) {                                                    \
+#define SPHX_FORCES_STAGED(V, E, M, U)                                                                      \
+        if (visc_only == V && energy == E && stress == M && uni == U) {                                     \
             if (mixed) {                                                                                    \
                 const StagedLaunch sl = staged_launch(B, 16, M ? 64 : 32);                                  \
-                hipLaunchKernelGGL((k_forces_mixed_staged<V, M, U, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, \
-                                   X, g, L, kp, ch, B, cell_start, sl.cap);                                 \
+                hipLaunchKernelGGL((k_forces_mixed_staged<V, E, M, U, 4, true>), sl.grid, sl.block, sl.dyn, s, P, \
+                                   D, X, g, L, kp, ch, B, cell_start, sl.cap);                              \
             } else {                                                                                        \
                 StagedLaunch sl = staged_launch(B, 8, M ? 96 : 64);                                         \
                 sl.block = dim3(2u * (uint32_t)B.threads); /* two lanes per particle */                      \
                 if (g.periodic)                                                                             \
-                    hipLaunchKernelGGL((k_forces_staged<V, M, U, 1>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, \
-                                       kp, ch, B, cell_start, sl.cap);                                      \
+                    hipLaunchKernelGGL((k_forces_staged<V, E, M, U, 1>), sl.grid, sl.block, sl.dyn, s, P, D, g, \
+                                       L, kp, ch, B, cell_start, sl.cap);                                   \
                 else                                                                                        \
-                    hipLaunchKernelGGL((k_forces_staged<V, M, U, 0>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, \
-                                       kp, ch, B, cell_start, sl.cap);                                      \
+                    hipLaunchKernelGGL((k_forces_staged<V, E, M, U, 0>), sl.grid, sl.block, sl.dyn, s, P, D, g, \
+                                       L, kp, ch, B, cell_start, sl.cap);                                   \
             }                                                                                               \
             return;                                                                                         \
         }
-        SPHX_FORCES_STAGED(false, false, false)
-        SPHX_FORCES_STAGED(true, false, false)
-        SPHX_FORCES_STAGED(false, false, true)
-        SPHX_FORCES_STAGED(true, false, true)
-        SPHX_FORCES_STAGED(false, true, false)
-        SPHX_FORCES_STAGED(true, true, false)
-        SPHX_FORCES_STAGED(false, true, true)
-        SPHX_FORCES_STAGED(true, true, true)
+#define SPHX_FORCES_STAGED_MU(M, U)        \
+        SPHX_FORCES_STAGED(false, false, M, U) \
+        SPHX_FORCES_STAGED(true, false, M, U)  \
+        SPHX_FORCES_STAGED(false, true, M, U)
+        SPHX_FORCES_STAGED_MU(false, false)
+        SPHX_FORCES_STAGED_MU(false, true)
+        SPHX_FORCES_STAGED_MU(true, false)
+        SPHX_FORCES_STAGED_MU(true, true)
+#undef SPHX_FORCES_STAGED_MU
 #undef SPHX_FORCES_STAGED
     }
-#define SPHX_FORCES(V, M, U)                                                                              \
-    if (visc_only == V && stress == M && uni == U) {                                                     \
+#define SPHX_FORCES(V, E, M, U)                                                                           \
+    if (visc_only == V && energy == E && stress == M && uni == U) {                                      \
         if (mixed)                                                                                        \
-            hipLaunchKernelGGL((k_forces_mixed<V, M, U>), grid, block, occupancy_lds(4), s, P, D, X, n, g, L, \
+            hipLaunchKernelGGL((k_forces_mixed<V, E, M, U>), grid, block, occupancy_lds(4), s, P, D, X, n, g, L, \
                                kp, ch);                                                                   \
         else                                                                                              \
-            hipLaunchKernelGGL((k_forces<V, M, U>), dim3(nblocks(2 * n)), block, M ? 0 : occupancy_lds(3), s, P, D, \
-                               n, g, L, kp,                                                               \
-                               ch);                                                                       \
+            hipLaunchKernelGGL((k_forces<V, E, M, U>), dim3(nblocks(2 * n)), block, M ? 0 : occupancy_lds(3), s, \
+                               P, D, n, g, L, kp, ch);                                                    \
         return;                                                                                           \
     }
-    SPHX_FORCES(false, false, false)
-    SPHX_FORCES(true, false, false)
-    SPHX_FORCES(false, true, false)
-    SPHX_FORCES(true, true, false)
-    SPHX_FORCES(false, false, true)
-    SPHX_FORCES(true, false, true)
-    SPHX_FORCES(false, true, true)
-    SPHX_FORCES(true, true, true)
+#define SPHX_FORCES_MU(M, U)        \
+    SPHX_FORCES(false, false, M, U) \
+    SPHX_FORCES(true, false, M, U)  \
+    SPHX_FORCES(false, true, M, U)
+    SPHX_FORCES_MU(false, false)
+    SPHX_FORCES_MU(true, false)
+    SPHX_FORCES_MU(false, true)
+    SPHX_FORCES_MU(true, true)
+#undef SPHX_FORCES_MU
 #undef SPHX_FORCES
 }
 

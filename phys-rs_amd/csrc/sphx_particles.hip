@@ -122,6 +122,7 @@ __global__ void k_vv_position(Particles P, Derived D, uint32_t n, KernelParams k
             nx[k] = x;
         }
         if (flip) flip[P.orig[i]] = (uint8_t)bits;
+        if (kp.eos == SPHX_EOS_ADIABATIC) P.dup[i] = D.du[i];  // du/dt of the tick's first evaluation
         far = track_position(P, D, kp, g, skey, mixed, X, i, nx);
     }
     if (__ballot(far) && (threadIdx.x & 63u) == 0u) raise_moved(mf);
@@ -135,7 +136,9 @@ void launch_vv_position(const Particles& P, const Derived& D, uint32_t n, const 
 }
 
 // velocity_verlet_velocity_system (integrators.rs:92-104)
-// The force pass's gather record is updated with the new velocity.
+// The force pass's gather record is updated with the new velocity.  With the
+// adiabatic EOS the internal energy is advanced the same way,
+// u += (du_n + du*) dt / 2.
 __global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams kp, int mixed) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -146,6 +149,26 @@ __global__ void k_vv_velocity(Particles P, Derived D, uint32_t n, KernelParams k
     P.vy[i] = vy;
     P.vz[i] = vz;
     set_record_velocity(D, kp, i, mixed != 0, vx, vy, vz);
+    if (kp.eos == SPHX_EOS_ADIABATIC) P.u[i] += (P.dup[i] + D.du[i]) * kp.half_dt;
+}
+
+// Adiabatic EOS after the internal energy changed at fixed positions: P,
+// P/rho^2 and the gather records' P/rho^2 from the kept density and the new u
+// (the same arithmetic as the density pass's epilogue).
+__global__ void k_eos(Particles P, Derived D, uint32_t n, KernelParams kp, int mixed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double rho = D.rho[i];
+    const double pr = eos_pressure(kp, rho, P.u[i]);
+    const double pr2 = pr / (rho * rho);
+    D.p[i] = pr;
+    D.pr2[i] = pr2;
+    if (mixed) D.r32[i].pr2 = (float)pr2;
+    if (kp.rec64) D.r64[i].pr2 = pr2;
+}
+void launch_eos(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, bool mixed,
+                hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_eos, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, (int)mixed);
 }
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, bool mixed,
                         const Mixed& X, hipStream_t s) {
@@ -177,7 +200,7 @@ __global__ void k_energy(Particles P, Derived D, uint32_t n, Grid g, NbrList L, 
         const double m = P.m[i], vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
         v[0] = 0.5 * m * ((vx * vx + vy * vy) + vz * vz);
         v[1] = -m * ((kp.g[0] * P.x[i] + kp.g[1] * P.y[i]) + kp.g[2] * P.z[i]);
-        v[2] = m * specific_internal(kp, D.rho[i]);
+        v[2] = m * (kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : specific_internal(kp, D.rho[i]));
         if (mhd) {
             const double bx = P.bx[i], by = P.by[i], bz = P.bz[i];
             v[3] = 0.5 * (m / D.rho[i]) * ((bx * bx + by * by) + bz * bz);

@@ -92,6 +92,8 @@ struct SphxCtx {
     double* cand_r = nullptr;  // bounding radii in sorted order (sphx_export_candidates)
     uint32_t cand_r_cap = 0;
     bool has_mhd = false, has_charge = false;
+    bool has_u = false;                    // an internal energy was uploaded
+    bool pres = false;                     // P (adiabatic: from the current u) matches the density
     bool uniform = false;                  // all m equal and all h equal
     double m0 = 0.0, h0 = 0.0;
     double hmax = 0.0;                     // largest smoothing radius (h is fixed per particle)
@@ -113,6 +115,8 @@ struct SphxCtx {
 
     Particles& P() { return A[cur]; }
     bool mixed() const { return prm.precision == SPHX_PREC_FP32_MIXED; }
+    // the adiabatic EOS: u is evolved with du/dt from the full force passes
+    bool energy() const { return has_u && prm.eos == SPHX_EOS_ADIABATIC; }
 };
 
 namespace {
@@ -158,7 +162,7 @@ void free_all(SphxCtx* c) {
 int check_params(SphxCtx* c, const SphxParams* p) {
     if (!p) return fail(c, SPHX_E_INVALID, "params is NULL");
     if (!(p->dt > 0.0) || !std::isfinite(p->dt)) return fail(c, SPHX_E_INVALID, "dt must be > 0");
-    if (p->eos < SPHX_EOS_TAIT || p->eos > SPHX_EOS_ISOTHERMAL)
+    if (p->eos < SPHX_EOS_TAIT || p->eos > SPHX_EOS_ADIABATIC)
         return fail(c, SPHX_E_INVALID, "unknown eos %d", p->eos);
     if (p->eos == SPHX_EOS_TAIT && (!(p->rest_density > 0.0) || !(p->gamma != 0.0)))
         return fail(c, SPHX_E_INVALID, "Tait EOS needs rest_density > 0 and gamma != 0");
@@ -483,7 +487,7 @@ int do_neighbours(SphxCtx* c) {
         launch_scan(c->S.cell_count, c->S.cell_start, c->grid.ncell, c->S.block_sums, c->stream);
         launch_scatter(c->S, n, c->stream);
         launch_cell_order(c->S, n, src.x, c->stream);
-        launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->mixed(), c->X, c->grid, c->D.recb,
+        launch_reorder(src, dst, c->S, n, c->has_mhd, c->has_charge, c->mixed(), c->energy(), c->X, c->grid, c->D.recb,
                        c->D.xref, c->stream);
         c->cur = 1 - c->cur;
         launch_tiles(c->grid, c->S, c->T, c->stream);
@@ -528,7 +532,7 @@ int do_density(SphxCtx* c) {
         launch_density(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->mixed(), c->X, c->B, c->S.cell_start,
                        c->stream);
     }
-    c->dens = true;
+    c->dens = c->pres = true;
     return launch_check(c);
 }
 
@@ -537,12 +541,20 @@ int do_forces(SphxCtx* c, bool visc_only) {
     if (!c->dens && (rc = do_density(c))) return rc;
     if (visc_only && !c->sp)
         return fail(c, SPHX_E_STATE, "SPHX_FORCES_VISC_ONLY needs a full force pass at the current positions");
+    if (visc_only && c->energy())
+        return fail(c, SPHX_E_STATE, "SPHX_FORCES_VISC_ONLY is not available with the adiabatic EOS (the energy "
+                                     "equation needs the pressure term of a full pass)");
     const bool stress = c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_MAXWELL_STRESS);
     const bool charge = c->has_charge && c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_LORENTZ_CHARGE);
+    if (!c->pres) {  // the internal energy changed at these positions: P from the kept density
+        Timer t(c, kDensity);
+        launch_eos(c->P(), c->D, c->n, c->kp, c->mixed(), c->stream);
+        c->pres = true;
+    }
     {
         Timer t(c, visc_only ? kViscOnly : kForce);
-        launch_forces(c->P(), c->D, c->n, c->grid, c->L, c->kp, visc_only, stress, charge, c->mixed(),
-                      c->X, c->B, c->S.cell_start, c->stream);
+        launch_forces(c->P(), c->D, c->n, c->grid, c->L, c->kp, visc_only, c->energy(), stress, charge,
+                      c->mixed(), c->X, c->B, c->S.cell_start, c->stream);
     }
     c->sp = true;
     c->acc = true;
@@ -621,6 +633,7 @@ int do_vv_velocity(SphxCtx* c) {
         launch_vv_velocity(c->P(), c->D, c->n, c->kp, c->mixed(), c->X, c->stream);
     }
     c->acc = false;  // a is a function of v, which just changed
+    if (c->energy()) c->pres = c->sp = false;  // and P of u, which did too
     return launch_check(c);
 }
 
@@ -774,7 +787,7 @@ int second_evaluation(SphxCtx* c) {
 // One tick of sphx_step (no XSPH / MHD); records the reflections and v' when asked.
 int adj_tick(SphxCtx* c, uint8_t* flip, double* vp3) {
     int rc;
-    if ((rc = do_forces(c, c->sp && c->dens))) return rc;
+    if ((rc = do_forces(c, c->sp && c->dens && !c->energy()))) return rc;
     if ((rc = do_vv_position(c, flip, speculate(c)))) return rc;
     if (vp3) adj_save(c, nullptr, vp3);
     if ((rc = second_evaluation(c))) return rc;
@@ -936,14 +949,14 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     for (int b = 0; b < 2; ++b) {
         Particles& P = c->A[b];
         double** dd[] = {&P.x, &P.y, &P.z, &P.vx, &P.vy, &P.vz, &P.apx, &P.apy, &P.apz,
-                         &P.m, &P.h, &P.bx, &P.by, &P.bz, &P.psi, &P.q};
+                         &P.m, &P.h, &P.bx, &P.by, &P.bz, &P.psi, &P.q, &P.u, &P.dup};
         for (double** d : dd)
             if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
         if (dalloc(c, &P.id, N) || dalloc(c, &P.orig, N) || dalloc(c, &P.pos, N)) return bail(SPHX_E_OOM);
     }
     Derived& D = c->D;
     double** dd[] = {&D.rho, &D.p, &D.pr2, &D.ax, &D.ay, &D.az, &D.spx, &D.spy, &D.spz,
-                     &D.tmp3[0], &D.tmp3[1], &D.tmp3[2], &D.tmp1};
+                     &D.tmp3[0], &D.tmp3[1], &D.tmp3[2], &D.tmp1, &D.du};
     for (double** d : dd)
         if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
     if (dalloc(c, &D.r64, N) || dalloc(c, &D.r32, N) || dalloc(c, &D.recb, N) || dalloc(c, &D.recb_next, N) ||
@@ -997,6 +1010,8 @@ int sphx_set_params(SphxCtx* c, const SphxParams* p) {
     if (p->max_neighbours > 0 && (p->max_neighbours + 3) / 4 * 4 != c->L.ks)
         return fail(c, SPHX_E_INVALID, "max_neighbours is fixed at sphx_create (capacity %d, asked %d)", c->L.ks,
                     p->max_neighbours);
+    if (p->eos == SPHX_EOS_ADIABATIC && c->n && !c->has_u)
+        return fail(c, SPHX_E_STATE, "the adiabatic EOS needs the internal energy u (upload it)");
     const SphxParams old = c->prm;
     c->prm = *p;
     make_kernel_params(c);
@@ -1026,6 +1041,11 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
         if (!(h->m[i] > 0.0)) return fail(c, SPHX_E_INVALID, "mass must be > 0 (particle %u)", i);
         if (!(h->h[i] > 0.0)) return fail(c, SPHX_E_INVALID, "smoothing radius must be > 0 (particle %u)", i);
     }
+    if (c->prm.eos == SPHX_EOS_ADIABATIC && h->n && !h->u)
+        return fail(c, SPHX_E_INVALID, "the adiabatic EOS needs the internal energy u");
+    if (h->u)
+        for (uint32_t i = 0; i < h->n; ++i)
+            if (!std::isfinite(h->u[i])) return fail(c, SPHX_E_NONFINITE, "non-finite u at particle %u", i);
     const bool mhd = h->bx && h->by && h->bz;
     if ((h->bx || h->by || h->bz) && !mhd) return fail(c, SPHX_E_INVALID, "bx, by, bz go together");
     if (h->q && !mhd) return fail(c, SPHX_E_INVALID, "charge needs a magnetic field");
@@ -1041,6 +1061,7 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
     int rc = setup_grid(c, h);
     if (rc) return rc;
     c->has_charge = h->q != nullptr;
+    c->has_u = h->u != nullptr;
     c->cur = 0;
     Particles& P = c->A[0];
     const size_t bytes = sizeof(double) * h->n;
@@ -1058,6 +1079,10 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
             HIP_TRY(c, hipMemcpyAsync(P.psi, h->psi ? h->psi : zero.data(), bytes, hipMemcpyHostToDevice, c->stream));
         }
         if (h->q) HIP_TRY(c, hipMemcpyAsync(P.q, h->q, bytes, hipMemcpyHostToDevice, c->stream));
+        if (h->u) {
+            HIP_TRY(c, hipMemcpyAsync(P.u, h->u, bytes, hipMemcpyHostToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(P.dup, zero.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        }
         if (h->id) HIP_TRY(c, hipMemcpyAsync(P.id, h->id, sizeof(uint32_t) * h->n, hipMemcpyHostToDevice, c->stream));
         else launch_iota_u32(P.id, h->n, c->stream);
         launch_iota_u32(P.orig, h->n, c->stream);
@@ -1078,9 +1103,16 @@ int sphx_download(SphxCtx* c, SphxState* h, uint32_t fields) {
         return fail(c, SPHX_E_STATE, "accelerations are not valid for the current state (call sphx_forces)");
     if ((fields & (SPHX_F_B | SPHX_F_PSI)) && !c->has_mhd)
         return fail(c, SPHX_E_STATE, "no magnetic field was uploaded");
+    if ((fields & SPHX_F_U) && !c->has_u) return fail(c, SPHX_E_STATE, "no internal energy was uploaded");
     const uint32_t n = c->n;
     if (!n) return SPHX_OK;
     Particles& P = c->P();
+    if ((fields & SPHX_F_P) && !c->pres) {  // P of the current internal energy
+        launch_eos(P, c->D, n, c->kp, c->mixed(), c->stream);
+        int rc0;
+        if ((rc0 = launch_check(c))) return rc0;
+        c->pres = true;
+    }
     std::vector<uint32_t> orig(n);
     HIP_TRY(c, hipMemcpyAsync(orig.data(), P.orig, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
     std::vector<double> buf(n);
@@ -1100,6 +1132,7 @@ int sphx_download(SphxCtx* c, SphxState* h, uint32_t fields) {
     if (fields & SPHX_F_ACC) { if ((rc = fetch(c->D.ax, h->ax)) || (rc = fetch(c->D.ay, h->ay)) || (rc = fetch(c->D.az, h->az))) return rc; }
     if (fields & SPHX_F_B) { if ((rc = fetch(P.bx, h->bx)) || (rc = fetch(P.by, h->by)) || (rc = fetch(P.bz, h->bz))) return rc; }
     if (fields & SPHX_F_PSI) { if ((rc = fetch(P.psi, h->psi))) return rc; }
+    if (fields & SPHX_F_U) { if ((rc = fetch(P.u, h->u))) return rc; }
     return SPHX_OK;
 }
 
@@ -1107,18 +1140,20 @@ int sphx_update(SphxCtx* c, const SphxState* h, uint32_t fields) {
     if (!c || !h) return SPHX_E_INVALID;
     if (!c->grid_set) return fail(c, SPHX_E_STATE, "upload particles first");
     if (h->n != c->n) return fail(c, SPHX_E_INVALID, "state.n = %u but the context holds %u", h->n, c->n);
-    if (fields & ~(uint32_t)(SPHX_F_VEL | SPHX_F_B | SPHX_F_PSI))
-        return fail(c, SPHX_E_INVALID, "sphx_update takes SPHX_F_VEL, SPHX_F_B and SPHX_F_PSI (positions, masses "
-                                       "and radii change through sphx_upload)");
-    const bool vel = fields & SPHX_F_VEL, b = fields & SPHX_F_B, psi = fields & SPHX_F_PSI;
+    if (fields & ~(uint32_t)(SPHX_F_VEL | SPHX_F_B | SPHX_F_PSI | SPHX_F_U))
+        return fail(c, SPHX_E_INVALID, "sphx_update takes SPHX_F_VEL, SPHX_F_B, SPHX_F_PSI and SPHX_F_U (positions, "
+                                       "masses and radii change through sphx_upload)");
+    const bool vel = fields & SPHX_F_VEL, b = fields & SPHX_F_B, psi = fields & SPHX_F_PSI, ue = fields & SPHX_F_U;
     if ((b || psi) && !c->has_mhd) return fail(c, SPHX_E_STATE, "no magnetic field was uploaded");
-    if ((vel && (!h->vx || !h->vy || !h->vz)) || (b && (!h->bx || !h->by || !h->bz)) || (psi && !h->psi))
+    if (ue && !c->has_u) return fail(c, SPHX_E_STATE, "no internal energy was uploaded");
+    if ((vel && (!h->vx || !h->vy || !h->vz)) || (b && (!h->bx || !h->by || !h->bz)) || (psi && !h->psi) ||
+        (ue && !h->u))
         return fail(c, SPHX_E_INVALID, "an array of the requested fields is missing");
     const uint32_t n = c->n;
     for (uint32_t i = 0; i < n; ++i) {
         const bool ok = (!vel || (std::isfinite(h->vx[i]) && std::isfinite(h->vy[i]) && std::isfinite(h->vz[i]))) &&
                         (!b || (std::isfinite(h->bx[i]) && std::isfinite(h->by[i]) && std::isfinite(h->bz[i]))) &&
-                        (!psi || std::isfinite(h->psi[i]));
+                        (!psi || std::isfinite(h->psi[i])) && (!ue || std::isfinite(h->u[i]));
         if (!ok) return fail(c, SPHX_E_NONFINITE, "non-finite input at particle %u", i);
     }
     HIP_TRY(c, hipSetDevice(c->device));
@@ -1139,6 +1174,7 @@ int sphx_update(SphxCtx* c, const SphxState* h, uint32_t fields) {
     if (vel && ((rc = put(h->vx, P.vx)) || (rc = put(h->vy, P.vy)) || (rc = put(h->vz, P.vz)))) return rc;
     if (b && ((rc = put(h->bx, P.bx)) || (rc = put(h->by, P.by)) || (rc = put(h->bz, P.bz)))) return rc;
     if (psi && (rc = put(h->psi, P.psi))) return rc;
+    if (ue && (rc = put(h->u, P.u))) return rc;
     if (b || psi) {
         // the passes read B and psi from the (bx, by, bz, psi) records
         std::vector<double> bx(n), by(n), bz(n), ps(n);
@@ -1207,8 +1243,9 @@ int sphx_step(SphxCtx* c, uint32_t nsteps) {
     int rc;
     for (uint32_t t = 0; t < nsteps; ++t) {
         // a_n = A(x_n, v_n): the pressure sum at x_n is still valid from the
-        // previous tick's second evaluation unless positions were replaced.
-        if ((rc = do_forces(c, c->sp && c->dens))) return rc;
+        // previous tick's second evaluation unless positions were replaced
+        // (never with the adiabatic EOS: P follows u, which vv_velocity moved).
+        if ((rc = do_forces(c, c->sp && c->dens && !c->energy()))) return rc;
         // While the list is expected to stay valid, density and the second
         // force pass are enqueued before the host has read the moved flag,
         // so the GPU does not wait for the host; they only write derived
@@ -1237,6 +1274,7 @@ int sphx_adjoint(SphxCtx* c, uint32_t nsteps, uint32_t every, const double* tgx,
         return fail(c, SPHX_E_INVALID, "sphx_adjoint supports open and reflective boxes only");
     if (c->has_mhd || c->prm.xsph_epsilon != 0.0)
         return fail(c, SPHX_E_INVALID, "sphx_adjoint does not differentiate XSPH or MHD");
+    if (c->energy()) return fail(c, SPHX_E_INVALID, "sphx_adjoint does not differentiate the adiabatic EOS");
     const uint32_t n = c->n;
     const uint32_t k = (every == 0 || every > nsteps) ? nsteps : every;
     const uint32_t M = (nsteps + k - 1) / k;

@@ -177,8 +177,8 @@ void launch_cell_order(const Sort& S, uint32_t n, const double* x, hipStream_t s
 // the cell origin).
 __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restrict__ perm,
                           const uint32_t* __restrict__ key, uint32_t* __restrict__ skey, uint32_t n,
-                          int mhd, int charge, int mixed, Mixed X, Grid g, double4* __restrict__ recb,
-                          double4* __restrict__ xref) {
+                          int mhd, int charge, int mixed, int energy, Mixed X, Grid g,
+                          double4* __restrict__ recb, double4* __restrict__ xref) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t i = perm[k];
@@ -217,6 +217,10 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
         rb[1] = make_double2(bz, psi);
     }
     if (charge) dst.q[k] = src.q[i];
+    if (energy) {
+        dst.u[k] = src.u[i];
+        dst.dup[k] = src.dup[i];
+    }
     // f32 cell-relative coordinates and h: the neighbour build's prefilter in
     // both forms, and the pair-side data of the mixed form
     int cx, cy, cz;
@@ -226,11 +230,11 @@ __global__ void k_reorder(Particles src, Particles dst, const uint32_t* __restri
     if (mixed) X.m[k] = (float)m;
 }
 void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, uint32_t n, bool mhd,
-                    bool charge, bool mixed, const Mixed& X, const Grid& g, double4* recb, double4* xref,
-                    hipStream_t s) {
+                    bool charge, bool mixed, bool energy, const Mixed& X, const Grid& g, double4* recb,
+                    double4* xref, hipStream_t s) {
     if (n)
         hipLaunchKernelGGL(k_reorder, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, S.perm, S.key,
-                           S.skey, n, (int)mhd, (int)charge, (int)mixed, X, g, recb, xref);
+                           S.skey, n, (int)mhd, (int)charge, (int)mixed, (int)energy, X, g, recb, xref);
 }
 
 // ---- tiles: each row cut into runs of <= kTile sorted particles ---------

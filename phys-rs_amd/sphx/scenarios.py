@@ -10,8 +10,8 @@ from typing import Dict, Tuple
 
 import numpy as np
 
-from . import (BND_PERIODIC, BND_REFLECT, EOS_POLYTROPIC, EOS_TAIT, MHD_CLEANING, MHD_INDUCTION,
-               MHD_MAXWELL_STRESS, VISC_REFERENCE, MhdConfig, SphViscosityConfig, TaitEquationConfig,
+from . import (BND_PERIODIC, BND_REFLECT, EOS_ADIABATIC, EOS_POLYTROPIC, EOS_TAIT, MHD_CLEANING, MHD_INDUCTION,
+               MHD_MAXWELL_STRESS, VISC_MONAGHAN, VISC_REFERENCE, MhdConfig, SphViscosityConfig, TaitEquationConfig,
                make_params)
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -97,12 +97,15 @@ def dam_break(nx: int = 32, ny: int = 32, nz: int = 16, spacing: float = 0.01,
 
 
 def mhd_periodic(n_side: int = 64, jitter: float = 0.02, seed: int = 1, h_factor: float = 1.06,
-                 dt: float = 1e-4, random_fill: int = 0) -> Tuple[Dict[str, np.ndarray], object]:
+                 dt: float = 1e-4, random_fill: int = 0,
+                 eos: str = "adiabatic") -> Tuple[Dict[str, np.ndarray], object]:
     """Ideal-MHD periodic unit cube (the 262,144 and 4,194,304 workloads).
 
     Lattice n_side^3 with +-jitter*spacing (or, with random_fill = N, N
-    seeded uniform positions and h from <N_nb> = 40), polytropic gamma = 5/3
-    with rho = 1, P = 1 (K = 1), v ~ N(0, 0.1^2), B = (0.1, 0, 0) plus
+    seeded uniform positions and h from <N_nb> = 40), ideal gas gamma = 5/3
+    with rho = 1, P = 1: the adiabatic EOS P = (gamma - 1) rho u with u = 1.5
+    evolved by the energy equation (eos="polytropic": P = K rho^gamma, K = 1,
+    at fixed entropy instead), v ~ N(0, 0.1^2), B = (0.1, 0, 0) plus
     0.01 * (0, sin(2 pi x + phi), cos(2 pi x + phi)) with a seeded phase phi,
     Monaghan alpha = 1, c = sqrt(gamma P / rho), ideal induction (eta = 0),
     Dedner cleaning at c_h = fast speed, Maxwell-stress force on.
@@ -134,6 +137,10 @@ def mhd_periodic(n_side: int = 64, jitter: float = 0.02, seed: int = 1, h_factor
         "psi": np.zeros(n),
         "id": np.arange(n, dtype=np.uint32),
     }
+    if eos == "adiabatic":
+        state["u"] = np.full(n, 1.0 / ((gamma - 1.0) * 1.0))  # P = (gamma - 1) rho u = 1 at rho = 1
+    elif eos != "polytropic":
+        raise ValueError(f"eos must be 'adiabatic' or 'polytropic', not {eos!r}")
     cs = math.sqrt(gamma * 1.0 / 1.0)
     c_fast = math.sqrt(cs * cs + 0.1 ** 2 + 0.01 ** 2)
     params = make_params(
@@ -141,8 +148,42 @@ def mhd_periodic(n_side: int = 64, jitter: float = 0.02, seed: int = 1, h_factor
         tait=TaitEquationConfig(1.0, cs, gamma),
         visc=SphViscosityConfig(1.0, 0.0),
         mhd=MhdConfig(0.0, c_fast, 0.5),
-        eos=EOS_POLYTROPIC, poly_k=1.0, visc_convention=VISC_REFERENCE,
+        eos=EOS_ADIABATIC if eos == "adiabatic" else EOS_POLYTROPIC, poly_k=1.0, visc_convention=VISC_REFERENCE,
         boundary_flags=BND_PERIODIC, box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, 1.0, 1.0),
         mhd_flags=MHD_INDUCTION | MHD_CLEANING | MHD_MAXWELL_STRESS,
+    )
+    return state, params
+
+
+def ideal_gas_box(n_side: int = 16, jitter: float = 0.2, seed: int = 41, v_sigma: float = 0.3,
+                  alpha: float = 1.0, dt: float = 2e-4, monaghan: bool = True,
+                  variable: bool = False) -> Tuple[Dict[str, np.ndarray], object]:
+    """Adiabatic ideal gas (gamma = 5/3, rho ~ 1, u ~ 1.5 + 0.3 U(0,1)) in the
+    periodic unit cube: jittered n_side^3 lattice, h = 1.3 spacing, seeded
+    Gaussian velocities, Monaghan viscosity alpha (approaching pairs unless
+    monaghan=False).  variable: per-particle h and m spread by 10 % / 20 %."""
+    rng = SplitMix64(seed)
+    sp = 1.0 / n_side
+    x, y, z = lattice(n_side, n_side, n_side, sp)
+    n = x.size
+    x = np.mod(x + (2.0 * rng.uniform(n) - 1.0) * jitter * sp, 1.0)
+    y = np.mod(y + (2.0 * rng.uniform(n) - 1.0) * jitter * sp, 1.0)
+    z = np.mod(z + (2.0 * rng.uniform(n) - 1.0) * jitter * sp, 1.0)
+    vx, vy, vz = (v_sigma * rng.normal(n) for _ in range(3))
+    u = 1.5 + 0.3 * rng.uniform(n)
+    h = np.full(n, 1.3 * sp)
+    m = np.full(n, 1.0 / n)
+    if variable:
+        h = h * (1.0 + 0.1 * rng.uniform(n))
+        m = m * (1.0 + 0.2 * rng.uniform(n))
+    gamma = 5.0 / 3.0
+    state = {"x": x, "y": y, "z": z, "vx": vx, "vy": vy, "vz": vz, "m": m, "h": h, "u": u,
+             "id": np.arange(n, dtype=np.uint32)}
+    params = make_params(
+        dt,
+        tait=TaitEquationConfig(1.0, math.sqrt(gamma * (gamma - 1.0) * 1.5), gamma),
+        visc=SphViscosityConfig(alpha, 0.0),
+        eos=EOS_ADIABATIC, visc_convention=VISC_MONAGHAN if monaghan else VISC_REFERENCE,
+        boundary_flags=BND_PERIODIC, box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, 1.0, 1.0),
     )
     return state, params

@@ -355,12 +355,14 @@ def test_c0_dam_break_200_steps(require_gpu):
 
 
 def test_step_parity_mhd(require_gpu):
+    # the 262,144-particle workload's physics (adiabatic ideal gas, induction,
+    # cleaning, Maxwell stress) on a 16^3 lattice, ids shuffled
     st, p = scenarios.mhd_periodic(16)
     st = shuffle_ids(st, seed=32)
     nsteps = 5
     with make_ctx(st, p) as ctx:
         ctx.step(nsteps)
-        g = ctx.download(sphx.F_POS | sphx.F_VEL | sphx.F_B | sphx.F_PSI)
+        g = ctx.download(sphx.F_POS | sphx.F_VEL | sphx.F_B | sphx.F_PSI | sphx.F_U)
     s = O.State(st)
     O.step(s, p, nsteps)
     for k in ("x", "y", "z"):
@@ -370,6 +372,65 @@ def test_step_parity_mhd(require_gpu):
     for k in ("vx", "vy", "vz", "bx", "by", "bz"):
         assert np.abs(g[k] - s[k]).max() < 1e-10 * np.abs(s[k]).max(), k
     assert np.abs(g["psi"] - s["psi"]).max() < 1e-8 * max(np.abs(s["psi"]).max(), 1e-12)
+    du = np.abs(s["u"] - st["u"]).max()
+    assert du > 0
+    assert np.abs(g["u"] - s["u"]).max() < 1e-9 * du
+
+
+# ------------------------------------------------------------- adiabatic ideal gas
+@pytest.mark.parametrize("monaghan", [True, False])
+@pytest.mark.parametrize("variable", [False, True])
+def test_adiabatic_step_parity(require_gpu, monaghan, variable):
+    # P = (gamma - 1) rho u and du/dt from the full force passes; uniform and
+    # per-particle h and m, both viscosity conventions, shuffled ids
+    st, p = scenarios.ideal_gas_box(16, monaghan=monaghan, variable=variable)
+    st = shuffle_ids(st, seed=33)
+    nsteps = 10
+    with make_ctx(st, p) as ctx:
+        ctx.forces()
+        first = ctx.download(sphx.F_RHO | sphx.F_P | sphx.F_ACC)
+        with pytest.raises(sphx.SphxError) as e:
+            ctx.forces(visc_only=True)  # the energy equation needs the full pass
+        assert e.value.code == 8
+        ctx.step(nsteps)
+        g = ctx.download(sphx.F_POS | sphx.F_VEL | sphx.F_U | sphx.F_RHO | sphx.F_P)
+        eg = ctx.energy()
+    s0 = O.State(st)
+    O.evaluate(s0, p)
+    assert rel_err(first["rho"], s0["rho"]) < 1e-13
+    assert rel_err(first["p"], s0["p"]) < 1e-13
+    for k in ("ax", "ay", "az"):
+        assert np.abs(first[k] - s0[k]).max() < 1e-11 * np.abs(s0[k]).max(), k
+    s = O.State(st)
+    O.step(s, p, nsteps)
+    O.lib().orc_eos(s.ref(), O._pp(p))  # P at the new u (the download recomputes it)
+    for k in ("x", "y", "z"):
+        d = np.abs(g[k] - s[k])
+        assert np.minimum(d, 1.0 - d).max() < 1e-12, k
+    for k in ("vx", "vy", "vz"):
+        assert np.abs(g[k] - s[k]).max() < 1e-10 * np.abs(s[k]).max(), k
+    du = np.abs(s["u"] - st["u"]).max()
+    assert du > 1e-4
+    assert np.abs(g["u"] - s["u"]).max() < 1e-9 * du
+    assert rel_err(g["rho"], s["rho"]) < 1e-12
+    assert rel_err(g["p"], s["p"]) < 1e-11
+    assert eg["internal"] == pytest.approx(float(np.sum(st["m"] * s["u"])), rel=1e-12)
+
+
+def test_mixed_adiabatic_error(require_gpu):
+    # FP32-mixed form of the energy equation: f32 pair terms, f64 sums and u
+    st, p = scenarios.ideal_gas_box(16)
+    out = {}
+    for prec in (sphx.PREC_FP64, sphx.PREC_FP32_MIXED):
+        p.precision = prec
+        with make_ctx(st, p) as ctx:
+            ctx.step(10)
+            out[prec] = ctx.download(sphx.F_U | sphx.F_VEL)
+    a, b = out[sphx.PREC_FP32_MIXED], out[sphx.PREC_FP64]
+    du = np.abs(b["u"] - st["u"]).max()
+    err = np.abs(a["u"] - b["u"]).max() / du
+    print(f"mixed adiabatic: max |u32 - u64| / max|du64| = {err:.3e}")
+    assert err < 1e-4
 
 
 def test_free_fall_matches_oracle_bitwise(require_gpu):
@@ -538,19 +599,20 @@ def test_mixed_mhd_periodic_step(require_gpu):
 
 
 def test_update_keeps_residency_and_matches_upload(require_gpu):
-    # sphx_update writes new velocities and B / psi into the resident sorted
-    # state.  Writing back the values the context already holds changes no
-    # bit of the trajectory; new values agree with a fresh upload of the same
-    # state (a different sort and list order: rounding-level differences).
+    # sphx_update writes new velocities, B / psi and internal energies into
+    # the resident sorted state.  Writing back the values the context already
+    # holds changes no bit of the trajectory; new values agree with a fresh
+    # upload of the same state (a different sort and list order:
+    # rounding-level differences).
     st, p = scenarios.mhd_periodic(16)
-    fields = sphx.F_POS | sphx.F_VEL | sphx.F_B | sphx.F_PSI
+    fields = sphx.F_POS | sphx.F_VEL | sphx.F_B | sphx.F_PSI | sphx.F_U
     with sphx.Context(len(st["x"]), p) as a, sphx.Context(len(st["x"]), p) as b:
         a.upload(st)
         b.upload(st)
         a.step(3)
         b.step(3)
         now = a.download(fields)
-        a.update(now, sphx.F_VEL | sphx.F_B | sphx.F_PSI)
+        a.update(now, sphx.F_VEL | sphx.F_B | sphx.F_PSI | sphx.F_U)
         a.step(3)
         b.step(3)
         ra, rb = a.download(fields), b.download(fields)
@@ -563,9 +625,10 @@ def test_update_keeps_residency_and_matches_upload(require_gpu):
         for k in ("vx", "vy", "vz"):
             new[k] = ra[k] + 0.01 * rng.normal(n)
         new["bz"] = ra["bz"] + 1e-3
-        a.update(new, sphx.F_VEL | sphx.F_B)
+        new["u"] = ra["u"] * (1.0 + 0.01 * rng.uniform(n))
+        a.update(new, sphx.F_VEL | sphx.F_B | sphx.F_U)
         fresh = dict(st)
-        fresh.update({k: new[k] for k in ("x", "y", "z", "vx", "vy", "vz", "bx", "by", "bz", "psi")})
+        fresh.update({k: new[k] for k in ("x", "y", "z", "vx", "vy", "vz", "bx", "by", "bz", "psi", "u")})
         with sphx.Context(n, p) as c:
             c.upload(fresh)
             a.step(2)

@@ -31,7 +31,7 @@ def run(state, params, bricks, nsteps):
             first = ctx.download(sphx.F_ACC | sphx.F_RHO)
             ctx.step(nsteps)
             ctx.forces()
-            last = ctx.download(FIELDS)
+            last = ctx.download(FIELDS | (sphx.F_U if "u" in state else 0))
     finally:
         if old is None:
             del os.environ["SPHX_BRICKS"]
@@ -88,6 +88,19 @@ def test_staged_mixed_dam_break_bit_identical(require_gpu):
     got = run(st, p, "3,3,3", 12)
     assert_identical(got[0], ref[0])
     assert_identical(got[1], ref[1])
+
+
+@pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
+@pytest.mark.parametrize("variable", [False, True])
+def test_staged_adiabatic_gas_bit_identical(require_gpu, precision, variable):
+    # the energy equation's sums in the staged force passes (u compared too)
+    st, p = scenarios.ideal_gas_box(16, variable=variable)
+    p.precision = precision
+    ref = run(st, p, "off", 6)
+    for shape in ("3,3,3", "2,2,2"):
+        got = run(st, p, shape, 6)
+        assert_identical(got[0], ref[0])
+        assert_identical(got[1], ref[1])
 
 
 @pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
