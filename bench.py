@@ -18,6 +18,11 @@ The FP64 drop-in is `value`; beside it the line reports
     (64 x 32 x 32, seeded initial velocities sigma 0.05 m/s, 100 steps,
     checkpoint every 10, target = the same start run with alpha = 0.15):
     forward and adjoint (recompute + reverse) milliseconds per step.
+  * "c2_500": the whole 500-step run of the same workload from the upload,
+    FP64 and FP32-mixed, mean ms per tick over every tick and every
+    neighbour-list build (the timed steps above are a quiet stretch of it);
+  * "cpu_baseline": the CPU path on the host's cores (all cores, gather form;
+    and one thread, the reference's pair structure).
 --precision-report runs the 500-step FP64 / FP32-mixed comparison (errors at
 the FP64 end state, total-energy drift of each form) and prints it instead.
 --mhd-report runs the 4,194,304-particle periodic MHD box (seeded uniform
@@ -184,6 +189,41 @@ def run_adjoint(steps: int = 100, every: int = 10, reps: int = 3):
                     "differentiates, subtracted here); host-timed, state resident"}
 
 
+def mhd_eval_errors(st, p):
+    """FP32-mixed against FP64 at one MHD state: density and acceleration of a
+    force evaluation, the change of B by induction and of psi by cleaning."""
+    import copy
+
+    import numpy as np
+    import sphx
+
+    res = {}
+    for prec in ("fp64", "mixed"):
+        q = copy.copy(p)
+        q.precision = sphx.PREC_FP64 if prec == "fp64" else sphx.PREC_FP32_MIXED
+        with sphx.Context(len(st["x"]), q) as ctx:
+            ctx.upload(st)
+            ctx.forces()
+            a = ctx.download(sphx.F_RHO | sphx.F_ACC)
+            ctx.mhd_induction()
+            b1 = ctx.download(sphx.F_B)
+            ctx.div_clean()
+            b2 = ctx.download(sphx.F_PSI)
+        res[prec] = (a, b1, b2)
+    (a64, b164, b264), (a32, b132, b232) = res["fp64"], res["mixed"]
+    drho = np.abs(a32["rho"] / a64["rho"] - 1)
+    acc64 = np.stack([a64[k] for k in ("ax", "ay", "az")], 1)
+    acc32 = np.stack([a32[k] for k in ("ax", "ay", "az")], 1)
+    dacc = np.linalg.norm(acc32 - acc64, axis=1) / np.linalg.norm(acc64, axis=1).max()
+    dB = max(float(np.abs(b132[k] - b164[k]).max() / np.abs(b164[k] - st[k]).max()) for k in ("bx", "by", "bz"))
+    dpsi = float(np.abs(b232["psi"] - b264["psi"]).max() / np.abs(b264["psi"] - st["psi"]).max())
+    return {"density_rel_max": float(drho.max()), "density_rel_rms": float(np.sqrt(np.mean(drho ** 2))),
+            "accel_rel_max": float(dacc.max()), "accel_rel_rms": float(np.sqrt(np.mean(dacc ** 2))),
+            "induction_dB_rel_max": dB, "cleaning_dpsi_rel_max": dpsi,
+            "note": "at the FP64 run's end state; acceleration relative to the largest |a|, dB and dpsi relative "
+                    "to the largest FP64 change"}
+
+
 def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
     """Per-kernel breakdown of the periodic MHD box in the FP64 and FP32-mixed forms
     (seeded random fill of n_fill particles, or the lattice^3 jittered lattice)."""
@@ -212,6 +252,9 @@ def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
             secs = time.perf_counter() - t0
             times = ctx.kernel_times(reset=True)
             e = ctx.energy()
+            if prec == "fp64":
+                end = ctx.download(sphx.F_POS | sphx.F_VEL | sphx.F_B | sphx.F_PSI | sphx.F_U)
+                end.update(m=st["m"], h=st["h"], id=st["id"])
         pairs = e["total_neighbours"]
         kern = {}
         for k, v in times.items():
@@ -230,6 +273,7 @@ def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
         out[prec] = {"ms_per_step": round(secs / steps * 1e3, 4), "particle_steps_per_s": round(n * steps / secs, 1),
                      "directed_neighbour_pairs": pairs, "max_neighbours": e["max_neighbours"], "kernels": kern}
     out["mixed_speedup"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    out["mixed_errors"] = mhd_eval_errors(end, p)
     return out
 
 
@@ -297,21 +341,90 @@ def roofline(n: int, times: dict, pairs: int, steps: int):
     return rf, kern
 
 
+def c2_full_run(steps: int = 500):
+    """The whole 500-step 1M dam break from the initial upload, in both forms:
+    mean ms per step over every tick, every neighbour-list build included."""
+    import sphx
+    from sphx import scenarios
+
+    st, p = scenarios.dam_break(nx=128, ny=128, nz=64, spacing=0.01, box=(3.2, 1.6, 1.6))
+    n = len(st["x"])
+    out = {"workload": "dam_break_1M", "steps": steps}
+    for prec in ("fp64", "mixed"):
+        p.precision = sphx.PREC_FP64 if prec == "fp64" else sphx.PREC_FP32_MIXED
+        with sphx.Context(n, p) as ctx:
+            ctx.upload(st)
+            ctx.enable_timing(True)
+            ctx.kernel_times(reset=True)
+            sync_all(ctx)
+            t0 = time.perf_counter()
+            ctx.step(steps)
+            sync_all(ctx)
+            secs = time.perf_counter() - t0
+            kt = ctx.kernel_times(reset=True)
+        out[prec] = {"ms_per_step": round(secs / steps * 1e3, 4), "particle_steps_per_s": round(n * steps / secs, 1),
+                     "neighbour_builds": kt["neighbour"]["launches"],
+                     "build_ms_per_step": round((kt["neighbour"]["ms"] + kt["sort"]["ms"]) / steps, 4)}
+    out["mixed_speedup"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    return out
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(steps: int):
+    """The CPU path on this host's cores, on the same 1M dam break:
+      * all cores (the top-level value): the oracle's gather form (oracle/sph_omp.c,
+        OpenMP over particles, lists rebuilt for each evaluation as CandidatePairs
+        is each tick), `steps` full velocity-Verlet ticks; beside it, shaped like
+        the reference's bench_sph_cpu (tests/gpu_bench.rs:220-254), one warm-up
+        and 20 timed force evaluations on a fixed list (particles/s = n / mean);
+      * one thread: the oracle's pair form (the reference's structure: an
+        undirected pair list, two-sided accumulation per system), `steps` ticks."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from sphx import scenarios
 
     st, p = scenarios.dam_break(nx=128, ny=128, nz=64, spacing=0.01, box=(3.2, 1.6, 1.6))
+    n = len(st["x"])
+    model, nproc = host_cpu()
+    g = O.Gather()
+    threads = O.Gather.threads()
+    s = O.State(st)
+    g.build(s, p)
+    g.evaluate(s, p)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(20):
+        g.evaluate(s, p)
+    t_eval = (time.perf_counter() - t0) / 20
+    s = O.State(st)
+    t0 = time.perf_counter()
+    g.step(s, p, steps)
+    t_all = time.perf_counter() - t0
+    g.close()
     s = O.State(st)
     t0 = time.perf_counter()
     O.step(s, p, steps)
-    dt = time.perf_counter() - t0
-    n = len(st["x"])
-    return {"value": round(n * steps / dt, 1), "unit": "particle-steps/s", "cores": 1,
+    t_one = time.perf_counter() - t0
+    return {"value": round(n * steps / t_all, 1), "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle (C restatement of the reference CPU path, pair list, 1 thread) on the "
-                      f"same 1,048,576-particle dam break, {steps} steps, {dt:.1f} s"}
+            "sample": f"oracle gather form (C, OpenMP, {threads} threads) on the same 1,048,576-particle dam "
+                      f"break, {steps} ticks, {t_all:.1f} s",
+            "host": {"cpu_model": model, "nproc": nproc, "omp_threads": threads},
+            "evaluation_like_bench_sph_cpu": {"particles_per_s": round(n / t_eval, 1), "ms_per_evaluation":
+                                              round(t_eval * 1e3, 2), "threads": threads, "evaluations": 20},
+            "single_thread": {"value": round(n * steps / t_one, 1), "unit": "particle-steps/s", "cores": 1,
+                              "sample": f"oracle pair form (the reference's structure), 1 thread, {steps} "
+                                        f"ticks, {t_one:.1f} s"}}
 
 
 def main() -> int:
@@ -402,6 +515,7 @@ def main() -> int:
                                    "value": round(n3 * args.steps / secs3, 1),
                                    "neighbour_builds": times3["neighbour"]["launches"]}
         out["adjoint"] = run_adjoint()
+        out["c2_500"] = c2_full_run()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

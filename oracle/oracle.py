@@ -201,3 +201,53 @@ def adjoint(st: State, params, nsteps: int, target: np.ndarray, with_gx: bool = 
 def loss(st: State, params, nsteps: int, target: np.ndarray) -> float:
     tgt = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
     return lib().orc_loss(st.ref(), _pp(params), nsteps, tgt.ctypes.data_as(PD))
+
+
+# ---- the all-core gather form (sph_omp.c) -----------------------------------
+def _omp_lib():
+    L = lib()
+    if not getattr(L, "_omp_ready", False):
+        L.orc_omp_new.restype = C.c_void_p
+        L.orc_omp_new.argtypes = []
+        L.orc_omp_free.restype = None
+        L.orc_omp_free.argtypes = [C.c_void_p]
+        L.orc_omp_threads.restype = C.c_int
+        L.orc_omp_threads.argtypes = []
+        L.orc_omp_build.restype = C.c_uint64
+        L.orc_omp_build.argtypes = [C.c_void_p, C.POINTER(OrcState), C.c_void_p]
+        L.orc_omp_eval.restype = None
+        L.orc_omp_eval.argtypes = [C.c_void_p, C.POINTER(OrcState), C.c_void_p]
+        L.orc_omp_step.restype = C.c_int
+        L.orc_omp_step.argtypes = [C.c_void_p, C.POINTER(OrcState), C.c_void_p, C.c_uint32]
+        L._omp_ready = True
+    return L
+
+
+class Gather:
+    """Neighbour lists and the OpenMP gather-form evaluation / tick."""
+
+    def __init__(self):
+        self.L = _omp_lib()
+        self.h = self.L.orc_omp_new()
+
+    def close(self):
+        if self.h:
+            self.L.orc_omp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def threads() -> int:
+        return _omp_lib().orc_omp_threads()
+
+    def build(self, st: State, params) -> int:
+        return int(self.L.orc_omp_build(self.h, st.ref(), _pp(params)))
+
+    def evaluate(self, st: State, params) -> None:
+        self.L.orc_omp_eval(self.h, st.ref(), _pp(params))
+
+    def step(self, st: State, params, nsteps: int) -> None:
+        if self.L.orc_omp_step(self.h, st.ref(), _pp(params), nsteps) != 0:
+            raise ValueError("orc_omp_step: XSPH and the MHD passes are not in the gather form")

@@ -56,36 +56,6 @@ double orc_kernel_dw(double r, double h) {
     return sigma * (-0.75 * (t * t));
 }
 
-/* ---- pair vector: core/coordinates.rs:51-65 (one sector: 0 + (b - a)),
- *      plus minimum image for the new periodic box ------------------------ */
-static inline void pair_vec(const OrcState* s, const SphxParams* p, uint32_t a, uint32_t b,
-                            double* dx, double* dy, double* dz) {
-    double d[3] = {s->x[b] - s->x[a], s->y[b] - s->y[a], s->z[b] - s->z[a]};
-    if (p->boundary_flags & SPHX_BND_PERIODIC) {
-        for (int k = 0; k < 3; ++k) {
-            double L = p->box_hi[k] - p->box_lo[k];
-            if (d[k] > 0.5 * L) d[k] -= L;
-            else if (d[k] < -0.5 * L) d[k] += L;
-        }
-    }
-    *dx = d[0]; *dy = d[1]; *dz = d[2];
-}
-
-/* glam DVec3::length: sqrt((x*x + y*y) + z*z) */
-static inline double vlen(double x, double y, double z) { return sqrt((x * x + y * y) + z * z); }
-static inline double vdot(double ax, double ay, double az, double bx, double by, double bz) {
-    return (ax * bx + ay * by) + az * bz;
-}
-
-/* neighbour predicate, sph.rs:102-108 */
-static inline int is_pair(const OrcState* s, const SphxParams* p, uint32_t a, uint32_t b) {
-    double dx, dy, dz;
-    pair_vec(s, p, a, b, &dx, &dy, &dz);
-    double r = vlen(dx, dy, dz);
-    double h_avg = 0.5 * (s->h[a] + s->h[b]);
-    return r < 2.0 * h_avg;
-}
-
 static int cmp_pair(const void* A, const void* B) {
     const uint32_t* a = (const uint32_t*)A;
     const uint32_t* b = (const uint32_t*)B;

@@ -728,3 +728,49 @@ def test_adiabatic_viscous_heating_conserves_total_energy():
     assert np.mean(dA1) > 1e-3 and np.mean(dA2) == pytest.approx(np.mean(dA1), rel=0.05)
     assert abs(dE1) < 1e-4 * (e0["kinetic"] / (e0["kinetic"] + e0["internal"]))
     assert dE2 == pytest.approx(0.5 * dE1, rel=0.1)
+
+
+# ---------------------------------------------------------------- all-core gather form (sph_omp.c)
+@pytest.mark.parametrize("case", ["dam_break", "mhd_adiabatic", "penalty_charge"])
+def test_gather_form_matches_pair_form(case):
+    # the bench's all-core CPU baseline: same neighbour sets (count) and the
+    # same rho, P, a, du/dt as the pair form up to summation order
+    if case == "dam_break":
+        st, p = scenarios.dam_break(12, 10, 8, v_sigma=0.2, seed=4)
+    elif case == "mhd_adiabatic":
+        st, p = _periodic_box()
+        st["u"] = 1.5 + 0.2 * scenarios.SplitMix64(8).uniform(len(st["x"]))
+        p.eos = sphx.EOS_ADIABATIC
+    else:
+        st = blob(v=0.4)
+        n = len(st["x"])
+        st.update(bx=np.full(n, 0.3), by=np.zeros(n), bz=np.full(n, -0.1), q=np.full(n, 1e-3))
+        p = params(boundary=sphx.SphBoundaryConfig(0.12, 0.05, 1e3), boundary_flags=sphx.BND_PENALTY_CUBE,
+                   mhd_flags=sphx.MHD_LORENTZ_CHARGE | sphx.MHD_MAXWELL_STRESS, e_field=(0.1, 0.0, -0.2),
+                   visc=sphx.SphViscosityConfig(0.5, 0.5), visc_convention=sphx.VISC_MONAGHAN)
+    a, b = O.State(st), O.State(st)
+    npairs = len(O.pairs(a, p))
+    O.evaluate(a, p)
+    g = O.Gather()
+    assert g.build(b, p) == 2 * npairs
+    g.evaluate(b, p)
+    assert np.abs(b["rho"] / a["rho"] - 1).max() < 1e-14
+    # Tait's B((rho/rho0)^7 - 1) cancels near rho0: P on the scale rho0 c^2
+    pscale = np.abs(a["p"]).max() + p.rest_density * p.speed_of_sound ** 2
+    assert np.abs(b["p"] - a["p"]).max() <= 1e-14 * pscale
+    for k in ("ax", "ay", "az"):
+        assert np.abs(b[k] - a[k]).max() < 1e-12 * np.abs(a[k]).max(), k
+    if "u" in st:
+        assert np.abs(b["du"] - a["du"]).max() < 1e-12 * np.abs(a["du"]).max()
+    assert O.Gather.threads() >= 1
+
+
+def test_gather_form_step_matches_pair_form():
+    st, p = scenarios.dam_break(12, 10, 8, v_sigma=0.2, seed=6)
+    a, b = O.State(st), O.State(st)
+    O.step(a, p, 10)
+    O.Gather().step(b, p, 10)
+    for k in ("x", "y", "z"):
+        assert np.abs(b[k] - a[k]).max() < 1e-13, k
+    for k in ("vx", "vy", "vz"):
+        assert np.abs(b[k] - a[k]).max() < 1e-10 * np.abs(a[k]).max(), k
