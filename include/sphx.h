@@ -272,6 +272,17 @@ int sphx_synchronize(SphxCtx* ctx);
  * This is the set CandidatePairs holds after the SPH systems' r < 2h filter. */
 int sphx_export_pairs(SphxCtx* ctx, uint32_t* pairs, uint64_t cap_pairs, uint64_t* count);
 
+/* No counterpart in the reference: sphx_export_pairs for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same result as sphx_export_pairs; the
+ * pairs are staged in a fixed device array of the library (2^21 pairs, 16 MiB
+ * per GPU, held until the process ends) instead of a buffer it allocates, and
+ * sorted in place in `pairs`.  Calls no hipMalloc / hipFree.  More than 2^21
+ * pairs: SPHX_E_CAPACITY with *count set (use sphx_export_pairs).  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (two threads, two
+ * contexts on the same device) race. */
+int sphx_export_pairs_noalloc(SphxCtx* ctx, uint32_t* pairs, uint64_t cap_pairs, uint64_t* count);
+
 /* Broadphase export for the systems beside SPH that read CandidatePairs
  * (heat_conduction_system, thermodynamics.rs:110-171; collisions): replaces
  * broadphase_query_system (src/physics/broadphase.rs:62-96).  radius[i] is

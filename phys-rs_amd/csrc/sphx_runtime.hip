@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -28,6 +29,35 @@ constexpr int kDefaultNeighbours = 192;  // list capacity per particle when max_
 constexpr int kMaxNeighbours = 4096;
 
 enum Slot { kSort = 0, kNbr, kDensity, kForce, kViscOnly, kIntegrate, kMhd, kNumSlots };
+
+// ---- scratch of the entry point that allocates nothing -----------------
+// sphx_export_pairs_noalloc stages the pairs (two u32 a pair) and their
+// 64-bit counter in this file-scope device array instead of memory the
+// library allocates: 2^21 pairs (16 MiB) and the counter, held by every GPU
+// the library runs on until the process ends.  Host code never passes the
+// name `scratch` to a kernel or copy: its device address on each device is
+// resolved once, at sphx_create with that device current, into
+// g_scratch[device].
+constexpr uint64_t kNoAllocPairs = 1ull << 21;
+constexpr size_t kScratchWords = 2 * kNoAllocPairs + 64;  // the pairs, then the counter (256-byte aligned)
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (once per device).
+int resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return SPHX_E_NODEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return SPHX_OK;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return SPHX_E_HIP;
+    }
+    g_scratch[device] = ptr;
+    return SPHX_OK;
+}
 
 struct TimedRange {
     hipEvent_t a, b;
@@ -921,6 +951,7 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
         return code;
     };
     if (hipSetDevice(device) != hipSuccess) return bail(SPHX_E_HIP);
+    if (resolve_scratch(device) != SPHX_OK) return bail(SPHX_E_HIP);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(SPHX_E_HIP);
     if (hipDeviceGetAttribute(&c->B.cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return bail(SPHX_E_HIP);
@@ -1406,6 +1437,42 @@ int sphx_export_pairs(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t*
         pairs[2 * k] = (uint32_t)(packed[k] >> 32);
         pairs[2 * k + 1] = (uint32_t)packed[k];
     }
+    return SPHX_OK;
+}
+
+int sphx_export_pairs_noalloc(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t* count) {
+    SPHX_ENTER(c);
+    if (!count) return fail(c, SPHX_E_INVALID, "count is NULL");
+    void* base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch[c->device] : nullptr;
+    if (!base) return fail(c, SPHX_E_STATE, "no scratch on device %d", c->device);
+    int rc;
+    if ((rc = ensure_list(c))) return rc;
+    if ((rc = check_overflow(c))) return rc;
+    uint32_t* out = static_cast<uint32_t*>(base);
+    unsigned long long* counter = reinterpret_cast<unsigned long long*>(out + 2 * kNoAllocPairs);
+    // the counter keeps the last call's value: cleared by a kernel first
+    launch_fill_u32(reinterpret_cast<uint32_t*>(counter), 0u, 2, c->stream);
+    launch_export_pairs(c->P(), c->n, c->grid, c->L, out, counter, kNoAllocPairs, c->stream);
+    if ((rc = launch_check(c))) return rc;
+    unsigned long long got = 0;
+    HIP_TRY(c, hipMemcpyAsync(&got, counter, sizeof got, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *count = got;
+    if (got > kNoAllocPairs)
+        return fail(c, SPHX_E_CAPACITY, "%llu pairs exceed the %llu the no-allocation export holds (use "
+                    "sphx_export_pairs)", got, (unsigned long long)kNoAllocPairs);
+    if (got > cap_pairs)
+        return fail(c, SPHX_E_CAPACITY, "%llu pairs do not fit in %llu", got, (unsigned long long)cap_pairs);
+    if (!got) return SPHX_OK;
+    if (!pairs) return fail(c, SPHX_E_INVALID, "pairs is NULL");
+    HIP_TRY(c, hipMemcpyAsync(pairs, out, sizeof(uint32_t) * 2 * got, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // ascending (id_a, id_b), sorted in place in the caller's array
+    struct IdPair {
+        uint32_t a, b;
+    };
+    IdPair* pp = reinterpret_cast<IdPair*>(pairs);
+    std::sort(pp, pp + got, [](const IdPair& x, const IdPair& y) { return x.a != y.a ? x.a < y.a : x.b < y.b; });
     return SPHX_OK;
 }
 

@@ -281,13 +281,15 @@ class Context:
         return out
 
     # -- measurement
-    def export_pairs(self) -> np.ndarray:
+    def export_pairs(self, noalloc: bool = False) -> np.ndarray:
+        """Sorted (id_a, id_b) neighbour pairs; noalloc: through sphx_export_pairs_noalloc."""
+        fn = self._lib.sphx_export_pairs_noalloc if noalloc else self._lib.sphx_export_pairs
         count = C.c_uint64(0)
-        rc = self._lib.sphx_export_pairs(self._h, None, 0, C.byref(count))
+        rc = fn(self._h, None, 0, C.byref(count))
         if rc not in (0, 3):  # SPHX_E_CAPACITY is expected with cap 0
             self._check(rc)
         out = np.empty((max(count.value, 1), 2), dtype=np.uint32)
-        self._check(self._lib.sphx_export_pairs(self._h, _ptr(out, C.c_uint32), count.value, C.byref(count)))
+        self._check(fn(self._h, _ptr(out, C.c_uint32), count.value, C.byref(count)))
         return out[: count.value]
 
     def export_candidates(self, radius: np.ndarray) -> np.ndarray:

@@ -108,6 +108,7 @@ SIGNATURES = {
     "sphx_adjoint": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32] + [C.POINTER(C.c_double)] * 11),
     "sphx_synchronize": (C.c_int, [C.c_void_p]),
     "sphx_export_pairs": (C.c_int, [C.c_void_p, PU, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "sphx_export_pairs_noalloc": (C.c_int, [C.c_void_p, PU, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sphx_export_candidates": (C.c_int, [C.c_void_p, PD, PU, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sphx_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sphx_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(SphxTimes), C.c_int]),
