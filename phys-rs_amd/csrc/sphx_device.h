@@ -37,6 +37,18 @@ __device__ __forceinline__ uint32_t xcd_block() {
     return xcd * base + min(xcd, rem) + local;
 }
 
+// The same dealing for a grid launched larger than the count of work items
+// (count known only on the device): each XCD gets an equal contiguous share
+// of the count items; blocks left over return 0xffffffff.  The grid must hold
+// at least count + 8 blocks.
+__device__ __forceinline__ uint32_t xcd_block_of(uint32_t count) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t xcd = b & 7u, local = b >> 3;
+    const uint32_t base = count >> 3, rem = count & 7u;
+    if (local >= base + (xcd < rem ? 1u : 0u)) return 0xffffffffu;
+    return xcd * base + min(xcd, rem) + local;
+}
+
 __device__ __forceinline__ void min_image(double& d, double L, double halfL) {
     if (d > halfL) d -= L;
     else if (d < -halfL) d += L;
@@ -291,6 +303,10 @@ struct Stage {
     uint32_t total;             // staged candidates
     uint32_t rowbase;           // linear index of cell (0, y, z)
     float oy, oz;               // stage cell offsets times the cell width (mixed form)
+    // logical cell x of a staged particle = physical + lshift of its segment:
+    // the stage's cells L0 .. L1 around the tile, unwrapped across a
+    // periodic boundary, ascend with the slots; lorigin = the first one
+    int lshift0, lshift1, lorigin;
 };
 
 __device__ __forceinline__ uint32_t tile_count(const Tiles& T) { return T.row_off[T.nrows]; }
@@ -318,6 +334,7 @@ __device__ __forceinline__ void stage_segments(const Grid& g, const uint32_t* __
     st.off1 = 0;
     st.ps0 = st.ps1 = 0;
     st.c00 = st.c01 = st.c10 = st.c11 = 0;
+    st.lshift0 = st.lshift1 = st.lorigin = 0;
     st.oy = (float)(oy * g.w[1]);
     st.oz = (float)(oz * g.w[2]);
     st.rowbase = 0;
@@ -336,11 +353,14 @@ __device__ __forceinline__ void stage_segments(const Grid& g, const uint32_t* __
         st.nseg = 1; st.c00 = 0; st.c01 = nx - 1;
     } else if (L0 < 0) {
         st.nseg = 2; st.c00 = L0 + nx; st.c01 = nx - 1; st.c10 = 0; st.c11 = L1;
+        st.lshift0 = -nx;
     } else if (L1 > nx - 1) {
         st.nseg = 2; st.c00 = L0; st.c01 = nx - 1; st.c10 = 0; st.c11 = L1 - nx;
+        st.lshift1 = nx;
     } else {
         st.nseg = 1; st.c00 = L0; st.c01 = L1;
     }
+    st.lorigin = st.c00 + st.lshift0;
     st.ps0 = start[st.rowbase + st.c00];
     st.total = start[st.rowbase + st.c01 + 1] - st.ps0;
     if (st.nseg == 2) {

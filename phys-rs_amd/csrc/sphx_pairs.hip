@@ -29,8 +29,8 @@
 namespace sphx {
 
 #define SPHX_TILE_PROLOGUE()                                             \
-    const uint32_t b_ = xcd_block();                                     \
-    if (b_ >= tile_count(T)) return;                                     \
+    const uint32_t b_ = xcd_block_of(tile_count(T));                     \
+    if (b_ == 0xffffffffu) return;                                       \
     const TileCtx tc = load_tile(b_, T, g, skey);                        \
     const uint32_t t = threadIdx.x;                                      \
     const bool valid = t < tc.np;                                        \
@@ -69,7 +69,9 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                                                    const uint32_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ start, NbrList L,
                                                    float reach, double skin) {
-    __shared__ float4 lds[kStageCap];  // staged candidates: cell-relative x, y, z and h
+    // staged candidates: cell-relative x, y, z and h; three spare records so
+    // that a four-slot step may read past the chunk (those slots are masked)
+    __shared__ float4 lds[kStageCap + 3];
     SPHX_TILE_PROLOGUE();
     const int cx = valid ? (int)(skey[p] % (uint32_t)g.dims[0]) : 0;
     float xf = 0, yf = 0, zf = 0, hf = 0;
@@ -83,6 +85,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         xi = P.x[p]; yi = P.y[p]; zi = P.z[p]; hi = P.h[p];
     }
     const float skin_f = (float)skin;
+    const float wy = (float)g.w[1], wz = (float)g.w[2];
     const bool trim = !exact_all;
     const bool any_exact = __any(exact_all) != 0;  // wave-uniform
     const float wx = (float)g.w[0];
@@ -101,50 +104,146 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         const bool e0 = ra[0] == rb[0], e2 = ra[2] == rb[2];
         const bool merged = (e0 || rb[0] == ra[1]) && (e2 || rb[1] == ra[2]);
         const bool single = st.total <= (uint32_t)kStageCap;
-        // candidate test of slot (cell x offset o - 1); u = slot - cb
-        // EX: some lane of the wave tests in f64 (exact_all), decided once per
-        // wave so that the usual f32-only loop has no branch for it
-        auto test = [&](uint32_t slot, uint32_t u, uint32_t o, auto ex) {
-            // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
-            const float4 q = lds[u];
-            const float dx = (q.x - xf) + (float)((int)o - 1) * wx, dy = (q.y - yf) + st.oy,
-                        dz = (q.z - zf) + st.oz;
-            const float tt = (UNI ? hf + hf : hf + q.w) + skin_f;  // 2 h_avg + skin
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            bool in = r2 < (tt * tt) * (1.0f + 1e-5f);
-            if constexpr (decltype(ex)::value) {
-                if (exact_all) {
-                    const uint32_t j = slot_particle(st, slot);
-                    double ddx, ddy, ddz;
-                    pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], ddx, ddy, ddz);
-                    const double t = (hi + P.h[j]) + skin;
-                    in = (ddx * ddx + ddy * ddy) + ddz * ddz < (t * t) * (1.0 + 1e-12);
-                }
-            }
-            in = in && slot != self;
-            if (in) {
-                if (total < (uint32_t)L.ks) {
-                    b0 = b1;
-                    b1 = b2;
-                    b2 = b3;
-                    b3 = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
-                    if ((total & 3u) == 3u)
-                        *reinterpret_cast<uint4*>(out + nbr_off(total - 3u)) = make_uint4(b0, b1, b2, b3);
-                }
-                ++total;
-            }
-        };
+        // The reach in x left for this row: a candidate of row (oy, oz) is at
+        // least dy (dz) away, the distance from i to the cell face towards
+        // that row (0 for the own row), so |dx| < sqrt(reach^2 - dy^2 - dz^2);
+        // a row with nothing in reach is skipped.  Shrunk distances keep the
+        // bound conservative in f32.
+        float rreach = reach;
+        bool row_empty = false;
+        if (trim) {
+            const int oy = s % 3 - 1, oz = s / 3 - 1;
+            const float dyl = oy == 0 ? 0.0f : fmaxf(oy < 0 ? yf : wy - yf, 0.0f) * (1.0f - 1e-5f);
+            const float dzl = oz == 0 ? 0.0f : fmaxf(oz < 0 ? zf : wz - zf, 0.0f) * (1.0f - 1e-5f);
+            const float rr2 = fmaf(-dzl, dzl, fmaf(-dyl, dyl, reach * reach));
+            row_empty = rr2 <= 0.0f;
+            rreach = __builtin_amdgcn_sqrtf(fmaxf(rr2, 0.0f)) * (1.0f + 1e-5f);
+        }
         // first slot of [a, e) (ascending x, whole stage staged) with
-        // (x_slot - x_i) + off beyond the bound: below -reach (left cell) or
-        // at least reach (right cell)
+        // (x_slot - x_i) + off beyond the row's reach: below -rreach (left
+        // cell) or at least rreach (right cell)
         auto cut = [&](uint32_t a, uint32_t e, float off, bool left) {
             while (a < e) {
                 const uint32_t mid = (a + e) >> 1;
                 const float d = (lds[mid].x - xf) + off;
-                if (left ? d <= -reach : d < reach) a = mid + 1;
+                if (left ? d <= -rreach : d < rreach) a = mid + 1;
                 else e = mid;
             }
             return a;
+        };
+        // the slots [a, e) of one cell (x offset o - 1), chunk starting at cb:
+        // the candidate tests of up to 32 slots go into a bit mask first, then
+        // each lane appends its accepted slots; EX: some lane of the wave
+        // tests in f64 (exact_all), decided once per wave so that the usual
+        // f32-only loop has no branch for it
+        auto test_cell = [&](uint32_t a, uint32_t e, uint32_t o, uint32_t cb, auto ex) {
+            const float ox = (float)((int)o - 1) * wx;
+            const float tu = (hf + hf) + skin_f;
+            const float lim_u = (tu * tu) * (1.0f + 1e-5f);  // (2 h + skin)^2 with a 1e-5 margin
+            const uint32_t code = ((uint32_t)s * 3u + o) << kIdxBits;
+            for (uint32_t base = a; base < e; base += 32u) {
+                const uint32_t cnt = min(32u, e - base);
+                uint32_t mask = 0;
+                // four slots a step from one LDS address (their reads issued
+                // together); slots past e read spare records and are masked
+                const float4* __restrict__ row = lds + (base - cb);
+                for (uint32_t k = 0; k < cnt; k += 4u) {
+                    float4 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = row[k + (uint32_t)u];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
+                        const float dx = (q[u].x - xf) + ox, dy = (q[u].y - yf) + st.oy, dz = (q[u].z - zf) + st.oz;
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        float lim = lim_u;
+                        if (!UNI) {
+                            const float tt = (hf + q[u].w) + skin_f;  // 2 h_avg + skin
+                            lim = (tt * tt) * (1.0f + 1e-5f);
+                        }
+                        bool in = r2 < lim;
+                        if constexpr (decltype(ex)::value) {
+                            if (exact_all) {
+                                const uint32_t j = slot_particle(st, min(base + k + (uint32_t)u, e - 1u));
+                                double ddx, ddy, ddz;
+                                pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], ddx, ddy, ddz);
+                                const double t = (hi + P.h[j]) + skin;
+                                in = (ddx * ddx + ddy * ddy) + ddz * ddz < (t * t) * (1.0 + 1e-12);
+                            }
+                        }
+                        mask |= in ? (1u << (k + (uint32_t)u)) : 0u;
+                    }
+                }
+                if (cnt < 32u) mask &= (1u << cnt) - 1u;
+                if (self - base < cnt) mask &= ~(1u << (self - base));
+                while (mask) {
+                    const uint32_t k = (uint32_t)__builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    if (total < (uint32_t)L.ks) {
+                        b0 = b1;
+                        b1 = b2;
+                        b2 = b3;
+                        b3 = slot_particle(st, base + k) | code;
+                        if ((total & 3u) == 3u)
+                            *reinterpret_cast<uint4*>(out + nbr_off(total - 3u)) = make_uint4(b0, b1, b2, b3);
+                    }
+                    ++total;
+                }
+            }
+        };
+        // Uniform h: the lane's whole window [a, e) over its three cells in
+        // one loop.  The staged w holds the candidate's logical cell (see
+        // Stage), so the cell offset of the pair vector is (w - own) wx with
+        // w - own in {-1, 0, 1}: fma((w - own), wx, x_j - x_i) rounds exactly
+        // as (x_j - x_i) + offset does.
+        auto test_window = [&](uint32_t a, uint32_t e, uint32_t cb, auto ex) {
+            const float own = (float)(cx - st.lorigin);
+            const float tu = (hf + hf) + skin_f;
+            const float lim = (tu * tu) * (1.0f + 1e-5f);
+            for (uint32_t base = a; base < e; base += 32u) {
+                const uint32_t cnt = min(32u, e - base);
+                uint32_t mask = 0;
+                const float4* __restrict__ row = lds + (base - cb);
+                for (uint32_t k = 0; k < cnt; k += 4u) {
+                    float4 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = row[k + (uint32_t)u];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float dx = fmaf(q[u].w - own, wx, q[u].x - xf);
+                        const float dy = (q[u].y - yf) + st.oy, dz = (q[u].z - zf) + st.oz;
+                        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        bool in = r2 < lim;
+                        if constexpr (decltype(ex)::value) {
+                            if (exact_all) {
+                                const uint32_t j = slot_particle(st, min(base + k + (uint32_t)u, e - 1u));
+                                double ddx, ddy, ddz;
+                                pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], ddx, ddy, ddz);
+                                const double t = (hi + P.h[j]) + skin;
+                                in = (ddx * ddx + ddy * ddy) + ddz * ddz < (t * t) * (1.0 + 1e-12);
+                            }
+                        }
+                        mask |= in ? (1u << (k + (uint32_t)u)) : 0u;
+                    }
+                }
+                if (cnt < 32u) mask &= (1u << cnt) - 1u;
+                if (self - base < cnt) mask &= ~(1u << (self - base));
+                while (mask) {
+                    const uint32_t k = (uint32_t)__builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    if (total < (uint32_t)L.ks) {
+                        const uint32_t slot = base + k;
+                        const uint32_t o = (slot >= ra[1] ? 1u : 0u) + (slot >= rb[1] ? 1u : 0u);
+                        b0 = b1;
+                        b1 = b2;
+                        b2 = b3;
+                        b3 = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
+                        if ((total & 3u) == 3u)
+                            *reinterpret_cast<uint4*>(out + nbr_off(total - 3u)) = make_uint4(b0, b1, b2, b3);
+                    }
+                    ++total;
+                }
+            }
         };
         // non-merged rows: the cells in slot order (a periodic row spanning
         // the whole grid stages in physical, not logical, cell order)
@@ -165,22 +264,28 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             const uint32_t ce = min(st.total, cb + kStageCap);
             __syncthreads();
             SPHX_STAGE_CHUNK({
-                const float4 v = X.p4[j];
+                float4 v = X.p4[j];
+                if (UNI)  // h is uniform: w carries the candidate's logical cell
+                    v.w = (float)((int)(skey[j] % (uint32_t)g.dims[0]) +
+                                  (st.nseg == 2 && cb + u >= st.off1 ? st.lshift1 : st.lshift0) - st.lorigin);
                 lds[u] = v;
             });
             __syncthreads();
-            if (!valid) continue;
+            if (!valid || row_empty) continue;
             auto run = [&](auto ex) {
-                if (merged) {
+                if (merged) {  // the three cells are consecutive slots: left, own, right
                     uint32_t A = e0 ? ra[1] : ra[0], E = e2 ? rb[1] : rb[2];
                     if (trim && single) {
                         if (!e0) A = cut(ra[0], rb[0], -wx, true);
                         if (!e2) E = cut(ra[2], rb[2], wx, false);
                     }
-                    A = max(A, cb);
-                    E = min(E, ce);
-                    for (uint32_t slot = A; slot < E; ++slot)
-                        test(slot, slot - cb, slot < ra[1] ? 0u : (slot < rb[1] ? 1u : 2u), ex);
+                    if (UNI) {
+                        test_window(max(A, cb), min(E, ce), cb, ex);
+                    } else {
+                        test_cell(max(A, cb), min(ra[1], ce), 0u, cb, ex);
+                        test_cell(max(ra[1], cb), min(rb[1], ce), 1u, cb, ex);
+                        test_cell(max(rb[1], cb), min(E, ce), 2u, cb, ex);
+                    }
                 } else {
                     for (int q = 0; q < 3; ++q) {
                         uint32_t a = qa[q], e = qb[q];
@@ -189,9 +294,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                             if (o == 0u) a = cut(a, e, -wx, true);
                             else e = cut(a, e, wx, false);
                         }
-                        a = max(a, cb);
-                        e = min(e, ce);
-                        for (uint32_t slot = a; slot < e; ++slot) test(slot, slot - cb, o, ex);
+                        test_cell(max(a, cb), min(e, ce), o, cb, ex);
                     }
                 }
             };

@@ -300,9 +300,11 @@ void launch_bricks(const Grid& g, const Sort& S, const Bricks& B, hipStream_t s)
 }
 
 // Upper bound of the tile count: sum over rows of ceil(n_r / kTile).
+// Blocks of a launch over the tiles: a bound on their count (a row of m
+// particles has at most m / kTile + 1 tiles) plus the 8 that the XCD dealing
+// of xcd_block_of needs; blocks past the device-side count return at once.
 uint32_t tile_grid(uint32_t n, const Tiles& T) {
-    uint64_t bound = (uint64_t)n / kTile + std::min<uint64_t>(T.nrows, n);
-    return (uint32_t)std::min<uint64_t>(bound, T.cap);
+    return (uint32_t)((uint64_t)n / kTile + std::min<uint64_t>(T.nrows, n) + 8);
 }
 
 }  // namespace sphx
