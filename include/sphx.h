@@ -266,6 +266,28 @@ int sphx_adjoint(SphxCtx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
                  double* grad_vx, double* grad_vy, double* grad_vz, double* grad_alpha);
 int sphx_synchronize(SphxCtx* ctx);
 
+/* ---- checkpoint / resume ------------------------------------------------ */
+/* The SPH state for QuickSave / QuickLoad (src/render/surface.rs:853-932),
+ * whose SavedEntity (src/core/serialization.rs:7-16) keeps name, type, mass,
+ * radius, position, sector, velocity and temperature but drops what the SPH
+ * step needs to resume: h, PreviousAcceleration, B, psi, charge, internal
+ * energy, the entity ranks and the parameters.  A checkpoint holds those, the
+ * positions at the last neighbour-list build, the grid and the list's age, so
+ * that a context loaded from it continues bit for bit as the saved one does.
+ * The caller stores the bytes (as QuickSave writes quicksave.bin); the layout
+ * is native little-endian and opens with the 8-byte magic "SPHXCKP1", then
+ * u32 version, header bytes, SphxParams bytes and the particle count n.
+ * sphx_checkpoint_size: bytes of a checkpoint of the current state.
+ * sphx_save: writes it into buf (cap bytes); *written receives the size
+ *   (also on SPHX_E_CAPACITY).  Builds the neighbour list if none is current.
+ * sphx_load: replaces the parameters and state of ctx (capacity >= the saved
+ *   n, the same max_neighbours: SPHX_E_CAPACITY / SPHX_E_INVALID) and keeps
+ *   the caller's upload order of the saved context for download / update.
+ *   SPHX_E_INVALID for a buffer that is not a checkpoint of this version. */
+int sphx_checkpoint_size(SphxCtx* ctx, uint64_t* bytes);
+int sphx_save(SphxCtx* ctx, void* buf, uint64_t cap, uint64_t* written);
+int sphx_load(SphxCtx* ctx, const void* buf, uint64_t size);
+
 /* ---- test and measurement -------------------------------------------- */
 /* Undirected neighbour pairs (id_min, id_max) with r < 2*h_avg at the current
  * positions, sorted ascending; *count receives the total (even when > cap).

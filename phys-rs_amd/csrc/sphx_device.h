@@ -49,6 +49,15 @@ __device__ __forceinline__ uint32_t xcd_block_of(uint32_t count) {
     return xcd * base + min(xcd, rem) + local;
 }
 
+// key % nx for a cell key (< 2^27) and nx (< 2^16): (key + 1/2) / nx in f64
+// lies at least 1/(2 nx) from an integer, far beyond the rounding of the
+// product, so its floor is the exact quotient (a few instructions instead of
+// an integer division).
+__device__ __forceinline__ uint32_t cell_x_of(uint32_t key, uint32_t nx, double inv_nx) {
+    const uint32_t q = (uint32_t)(((double)key + 0.5) * inv_nx);
+    return key - q * nx;
+}
+
 __device__ __forceinline__ void min_image(double& d, double L, double halfL) {
     if (d > halfL) d -= L;
     else if (d < -halfL) d += L;
@@ -383,7 +392,7 @@ __device__ __forceinline__ void cell_slots(const Grid& g, const uint32_t* __rest
     if (st.nseg == 0) return;
     const int nx = g.dims[0];
     int c = cx + ox;
-    if (g.periodic) c = (c + nx) % nx;
+    if (g.periodic) c = c < 0 ? c + nx : (c >= nx ? c - nx : c);
     else if (c < 0 || c >= nx) return;
     const bool second = st.nseg == 2 && c >= st.c10 && c <= st.c11;
     const uint32_t base = second ? st.off1 - st.ps1 : 0u - st.ps0;  // slot = particle + base (mod 2^32)

@@ -27,6 +27,7 @@ struct Grid {
     double L[3];      // periodic box length
     double halfL[3];
     uint32_t ncell;
+    double inv_nx;    // 1 / dims[0] (cell_x_of)
 };
 
 // Persistent per-particle state, in sorted (cell) order.  Two copies: the

@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             SPHX_STAGE_CHUNK({
                 float4 v = X.p4[j];
                 if (UNI)  // h is uniform: w carries the candidate's logical cell
-                    v.w = (float)((int)(skey[j] % (uint32_t)g.dims[0]) +
+                    v.w = (float)((int)cell_x_of(skey[j], (uint32_t)g.dims[0], g.inv_nx) +
                                   (st.nseg == 2 && cb + u >= st.off1 ? st.lshift1 : st.lshift0) - st.lorigin);
                 lds[u] = v;
             });

@@ -132,6 +132,7 @@ struct SphxCtx {
     // list lifetime in ticks: position updates since the current list's
     // build, and the count at which the previous list was found stale (0: none yet)
     uint32_t list_age = 0, last_life = 0;
+    uint64_t ticks = 0;                    // sphx_step ticks since the upload (kept by checkpoints)
     int spec_mode = 0;                     // SPHX_SPECULATE: 0 by list lifetime, 1 always, 2 never
     // timing
     bool timing = false;
@@ -331,87 +332,93 @@ void choose_bricks(SphxCtx* c, const SphxState* host, const Grid& g) {
             }
 }
 
-// Grid from the parameters and (for open boxes) the uploaded positions.
-int setup_grid(SphxCtx* c, const SphxState* host) {
+// Grid from the parameters and (for open boxes) the uploaded positions, or
+// the given one (a checkpoint's), then its cell, tile and brick arrays.
+int setup_grid(SphxCtx* c, const SphxState* host, const Grid* fixed = nullptr) {
     const SphxParams& p = c->prm;
     double hmax = 0.0;
     for (uint32_t i = 0; i < host->n; ++i) hmax = std::max(hmax, host->h[i]);
     if (host->n && !(hmax > 0.0)) return fail(c, SPHX_E_INVALID, "smoothing radius must be > 0");
     c->hmax = hmax;
     make_kernel_params(c);  // the skin follows h_max
-    double cs = (2.0 * hmax + c->kp.skin) * (1.0 + 1e-9);
-    if (p.cell_size > cs) cs = p.cell_size;
-    if (!(cs > 0.0)) cs = 1.0;
     Grid g{};
-    const bool periodic = (p.boundary_flags & SPHX_BND_PERIODIC) != 0;
-    double lo[3], hi[3];
-    bool explicit_grid = true;
-    for (int k = 0; k < 3; ++k)
-        if (!(p.grid_hi[k] > p.grid_lo[k])) explicit_grid = false;
-    if (periodic) {
-        for (int k = 0; k < 3; ++k) { lo[k] = p.box_lo[k]; hi[k] = p.box_hi[k]; }
-    } else if (explicit_grid) {
-        for (int k = 0; k < 3; ++k) { lo[k] = p.grid_lo[k]; hi[k] = p.grid_hi[k]; }
-    } else if (p.boundary_flags & SPHX_BND_REFLECT) {
-        for (int k = 0; k < 3; ++k) { lo[k] = p.box_lo[k]; hi[k] = p.box_hi[k]; }
+    if (fixed) {
+        g = *fixed;
     } else {
-        // bounding box of the uploaded particles, padded by a quarter on each side
-        if (host->n == 0) { for (int k = 0; k < 3; ++k) { lo[k] = 0; hi[k] = 1; } }
-        else {
-            const double* xs[3] = {host->x, host->y, host->z};
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = hi[k] = xs[k][0];
-                for (uint32_t i = 1; i < host->n; ++i) {
-                    lo[k] = std::min(lo[k], xs[k][i]);
-                    hi[k] = std::max(hi[k], xs[k][i]);
+        double cs = (2.0 * hmax + c->kp.skin) * (1.0 + 1e-9);
+        if (p.cell_size > cs) cs = p.cell_size;
+        if (!(cs > 0.0)) cs = 1.0;
+        const bool periodic = (p.boundary_flags & SPHX_BND_PERIODIC) != 0;
+        double lo[3], hi[3];
+        bool explicit_grid = true;
+        for (int k = 0; k < 3; ++k)
+            if (!(p.grid_hi[k] > p.grid_lo[k])) explicit_grid = false;
+        if (periodic) {
+            for (int k = 0; k < 3; ++k) { lo[k] = p.box_lo[k]; hi[k] = p.box_hi[k]; }
+        } else if (explicit_grid) {
+            for (int k = 0; k < 3; ++k) { lo[k] = p.grid_lo[k]; hi[k] = p.grid_hi[k]; }
+        } else if (p.boundary_flags & SPHX_BND_REFLECT) {
+            for (int k = 0; k < 3; ++k) { lo[k] = p.box_lo[k]; hi[k] = p.box_hi[k]; }
+        } else {
+            // bounding box of the uploaded particles, padded by a quarter on each side
+            if (host->n == 0) { for (int k = 0; k < 3; ++k) { lo[k] = 0; hi[k] = 1; } }
+            else {
+                const double* xs[3] = {host->x, host->y, host->z};
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = hi[k] = xs[k][0];
+                    for (uint32_t i = 1; i < host->n; ++i) {
+                        lo[k] = std::min(lo[k], xs[k][i]);
+                        hi[k] = std::max(hi[k], xs[k][i]);
+                    }
+                    double pad = 0.25 * (hi[k] - lo[k]) + cs;
+                    lo[k] -= pad;
+                    hi[k] += pad;
                 }
-                double pad = 0.25 * (hi[k] - lo[k]) + cs;
-                lo[k] -= pad;
-                hi[k] += pad;
             }
         }
-    }
-    // Cells never get narrower than 2*max(h); a grid above kMaxCells cells is
-    // made coarser (wider cells stay correct, the walk just tests more).
-    const double kMaxCells = (double)(1u << 24);
-    double ext[3];
-    for (int k = 0; k < 3; ++k) {
-        ext[k] = hi[k] - lo[k];
-        if (!std::isfinite(ext[k]) || !(ext[k] > 0.0)) return fail(c, SPHX_E_NONFINITE, "bad grid extent on axis %d", k);
-    }
-    for (int iter = 0;; ++iter) {
-        double dd[3], prod = 1.0;
+        // Cells never get narrower than 2*max(h); a grid above kMaxCells cells is
+        // made coarser (wider cells stay correct, the walk just tests more).
+        const double kMaxCells = (double)(1u << 24);
+        double ext[3];
         for (int k = 0; k < 3; ++k) {
-            dd[k] = periodic ? std::floor(ext[k] / cs) : std::ceil(ext[k] / cs);
-            dd[k] = std::max(dd[k], 1.0);
-            prod *= dd[k];
+            ext[k] = hi[k] - lo[k];
+            if (!std::isfinite(ext[k]) || !(ext[k] > 0.0)) return fail(c, SPHX_E_NONFINITE, "bad grid extent on axis %d", k);
         }
-        if (prod <= kMaxCells || iter > 64) {
-            for (int k = 0; k < 3; ++k) g.dims[k] = (int)std::min(dd[k], 65536.0);
-            break;
+        for (int iter = 0;; ++iter) {
+            double dd[3], prod = 1.0;
+            for (int k = 0; k < 3; ++k) {
+                dd[k] = periodic ? std::floor(ext[k] / cs) : std::ceil(ext[k] / cs);
+                dd[k] = std::max(dd[k], 1.0);
+                prod *= dd[k];
+            }
+            if (prod <= kMaxCells || iter > 64) {
+                for (int k = 0; k < 3; ++k) g.dims[k] = (int)std::min(dd[k], 65536.0);
+                break;
+            }
+            cs *= std::max(1.01, std::cbrt(prod / kMaxCells));
         }
-        cs *= std::max(1.01, std::cbrt(prod / kMaxCells));
+        uint64_t ncell = 1;
+        for (int k = 0; k < 3; ++k) {
+            const int d = g.dims[k];
+            if (periodic) {
+                if (d < 3)
+                    return fail(c, SPHX_E_INVALID,
+                                "periodic box axis %d holds %d cells of width 2*h_max; at least 3 are needed", k, d);
+                g.w[k] = ext[k] / d;
+                g.L[k] = ext[k];
+                g.halfL[k] = 0.5 * ext[k];
+            } else {
+                g.w[k] = cs;
+            }
+            g.lo[k] = lo[k];
+            g.inv_w[k] = 1.0 / g.w[k];
+            ncell *= (uint64_t)d;
+        }
+        if (ncell > (1ull << 27)) return fail(c, SPHX_E_CAPACITY, "grid of %llu cells exceeds 2^27", (unsigned long long)ncell);
+        g.periodic = periodic ? 1 : 0;
+        g.ncell = (uint32_t)ncell;
+        g.inv_nx = 1.0 / (double)g.dims[0];
     }
-    uint64_t ncell = 1;
-    for (int k = 0; k < 3; ++k) {
-        const int d = g.dims[k];
-        if (periodic) {
-            if (d < 3)
-                return fail(c, SPHX_E_INVALID,
-                            "periodic box axis %d holds %d cells of width 2*h_max; at least 3 are needed", k, d);
-            g.w[k] = ext[k] / d;
-            g.L[k] = ext[k];
-            g.halfL[k] = 0.5 * ext[k];
-        } else {
-            g.w[k] = cs;
-        }
-        g.lo[k] = lo[k];
-        g.inv_w[k] = 1.0 / g.w[k];
-        ncell *= (uint64_t)d;
-    }
-    if (ncell > (1ull << 27)) return fail(c, SPHX_E_CAPACITY, "grid of %llu cells exceeds 2^27", (unsigned long long)ncell);
-    g.periodic = periodic ? 1 : 0;
-    g.ncell = (uint32_t)ncell;
     if (g.ncell > c->S.cell_cap) {
         int rc;
         if ((rc = dalloc(c, &c->S.cell_count, g.ncell))) return rc;
@@ -1060,7 +1067,20 @@ int sphx_set_params(SphxCtx* c, const SphxParams* p) {
     return SPHX_OK;
 }
 
-int sphx_upload(SphxCtx* c, const SphxState* h) {
+}  // extern "C"
+
+namespace {
+
+// What a checkpoint load sets beside the state of sphx_upload: the grid of the
+// saved context, the upload slot of each particle and the previous
+// accelerations (all in the order of the given state).
+struct UploadExtras {
+    const Grid* grid = nullptr;
+    const uint32_t* orig = nullptr;
+    const double* ap[3] = {nullptr, nullptr, nullptr};
+};
+
+int do_upload(SphxCtx* c, const SphxState* h, const UploadExtras* ex) {
     if (!c || !h) return SPHX_E_INVALID;
     if (h->n > c->cap) return fail(c, SPHX_E_CAPACITY, "n = %u exceeds capacity %u", h->n, c->cap);
     if (h->n && (!h->x || !h->y || !h->z || !h->vx || !h->vy || !h->vz || !h->m || !h->h))
@@ -1089,7 +1109,7 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
     c->h0 = h->n ? h->h[0] : 0.0;
     c->has_mhd = mhd;
     make_kernel_params(c);
-    int rc = setup_grid(c, h);
+    int rc = setup_grid(c, h, ex ? ex->grid : nullptr);
     if (rc) return rc;
     c->has_charge = h->q != nullptr;
     c->has_u = h->u != nullptr;
@@ -1102,7 +1122,9 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
         for (int k = 0; k < 8; ++k) HIP_TRY(c, hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyHostToDevice, c->stream));
         std::vector<double> zero(h->n, 0.0);
         double* z[] = {P.apx, P.apy, P.apz};
-        for (double* d : z) HIP_TRY(c, hipMemcpyAsync(d, zero.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        for (int k = 0; k < 3; ++k)
+            HIP_TRY(c, hipMemcpyAsync(z[k], ex && ex->ap[k] ? ex->ap[k] : zero.data(), bytes, hipMemcpyHostToDevice,
+                                      c->stream));
         if (mhd) {
             HIP_TRY(c, hipMemcpyAsync(P.bx, h->bx, bytes, hipMemcpyHostToDevice, c->stream));
             HIP_TRY(c, hipMemcpyAsync(P.by, h->by, bytes, hipMemcpyHostToDevice, c->stream));
@@ -1116,13 +1138,23 @@ int sphx_upload(SphxCtx* c, const SphxState* h) {
         }
         if (h->id) HIP_TRY(c, hipMemcpyAsync(P.id, h->id, sizeof(uint32_t) * h->n, hipMemcpyHostToDevice, c->stream));
         else launch_iota_u32(P.id, h->n, c->stream);
-        launch_iota_u32(P.orig, h->n, c->stream);
+        if (ex && ex->orig)
+            HIP_TRY(c, hipMemcpyAsync(P.orig, ex->orig, sizeof(uint32_t) * h->n, hipMemcpyHostToDevice, c->stream));
+        else
+            launch_iota_u32(P.orig, h->n, c->stream);
         HIP_TRY(c, hipStreamSynchronize(c->stream));  // `zero` goes out of scope
     }
     launch_fill_u32(c->L.overflow, 0u, 4, c->stream);
     c->sorted = c->listed = c->dens = c->sp = c->acc = false;
+    c->ticks = 0;
     return launch_check(c);
 }
+
+}  // namespace
+
+extern "C" {
+
+int sphx_upload(SphxCtx* c, const SphxState* h) { return do_upload(c, h, nullptr); }
 
 int sphx_download(SphxCtx* c, SphxState* h, uint32_t fields) {
     if (!c || !h) return SPHX_E_INVALID;
@@ -1288,6 +1320,7 @@ int sphx_step(SphxCtx* c, uint32_t nsteps) {
         if (c->prm.xsph_epsilon != 0.0 && (rc = do_xsph(c))) return rc;
         if (c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_INDUCTION) && (rc = do_induction(c))) return rc;
         if (c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_CLEANING) && (rc = do_div_clean(c))) return rc;
+        ++c->ticks;
     }
     return check_overflow(c);
 }
@@ -1437,6 +1470,202 @@ int sphx_export_pairs(SphxCtx* c, uint32_t* pairs, uint64_t cap_pairs, uint64_t*
         pairs[2 * k] = (uint32_t)(packed[k] >> 32);
         pairs[2 * k + 1] = (uint32_t)packed[k];
     }
+    return SPHX_OK;
+}
+
+// ---- checkpoint / resume ---------------------------------------------------
+// Layout (native little-endian, 8-byte aligned): CkptHeader, then n-element
+// arrays in the context's sorted order: f64 x, y, z, the positions at the
+// last list build xr, yr, zr, vx, vy, vz, apx, apy, apz, m, h, [bx, by, bz,
+// psi], [q], [u], then u32 id and the upload slot.
+namespace {
+constexpr char kCkptMagic[8] = {'S', 'P', 'H', 'X', 'C', 'K', 'P', '1'};
+constexpr uint32_t kCkptVersion = 1;
+enum : uint32_t { kCkptMhd = 1, kCkptCharge = 2, kCkptU = 4 };
+struct CkptHeader {
+    char magic[8];
+    uint32_t version, header_bytes, params_bytes, n;
+    uint32_t flags, ks, list_age, last_life;
+    uint64_t ticks;
+    double lo[3], w[3], L[3];
+    int32_t dims[3], periodic;
+    SphxParams params;
+};
+uint32_t ckpt_flags(const SphxCtx* c) {
+    return (c->has_mhd ? kCkptMhd : 0u) | (c->has_charge && c->has_mhd ? kCkptCharge : 0u) | (c->has_u ? kCkptU : 0u);
+}
+uint64_t ckpt_arrays(uint32_t flags) {  // f64 arrays per particle
+    return 14u + ((flags & kCkptMhd) ? 4u : 0u) + ((flags & kCkptCharge) ? 1u : 0u) + ((flags & kCkptU) ? 1u : 0u);
+}
+uint64_t ckpt_bytes(uint32_t n, uint32_t flags) {
+    return sizeof(CkptHeader) + (uint64_t)n * (8u * ckpt_arrays(flags) + 8u);
+}
+}  // namespace
+
+int sphx_checkpoint_size(SphxCtx* c, uint64_t* bytes) {
+    if (!c || !bytes) return SPHX_E_INVALID;
+    if (!c->grid_set) return fail(c, SPHX_E_STATE, "upload particles first");
+    *bytes = ckpt_bytes(c->n, ckpt_flags(c));
+    return SPHX_OK;
+}
+
+int sphx_save(SphxCtx* c, void* buf, uint64_t cap, uint64_t* written) {
+    SPHX_ENTER(c);
+    const uint32_t n = c->n, flags = ckpt_flags(c);
+    const uint64_t need = ckpt_bytes(n, flags);
+    if (written) *written = need;
+    if (!buf || cap < need) return fail(c, SPHX_E_CAPACITY, "a checkpoint needs %llu bytes", (unsigned long long)need);
+    int rc;
+    // the list and the sorted order are part of what a resume reproduces
+    if ((rc = ensure_list(c))) return rc;
+    CkptHeader h;
+    std::memset(&h, 0, sizeof h);
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = kCkptVersion;
+    h.header_bytes = sizeof(CkptHeader);
+    h.params_bytes = sizeof(SphxParams);
+    h.n = n;
+    h.flags = flags;
+    h.ks = (uint32_t)c->L.ks;
+    h.list_age = c->list_age;
+    h.last_life = c->last_life;
+    h.ticks = c->ticks;
+    for (int k = 0; k < 3; ++k) {
+        h.lo[k] = c->grid.lo[k];
+        h.w[k] = c->grid.w[k];
+        h.L[k] = c->grid.L[k];
+        h.dims[k] = c->grid.dims[k];
+    }
+    h.periodic = c->grid.periodic;
+    h.params = c->prm;
+    unsigned char* out = static_cast<unsigned char*>(buf);
+    std::memcpy(out, &h, sizeof h);
+    double* f = reinterpret_cast<double*>(out + sizeof h);
+    const size_t N = n;
+    Particles& P = c->P();
+    std::vector<double4> xr(n);
+    if (n) HIP_TRY(c, hipMemcpyAsync(xr.data(), c->D.xref, sizeof(double4) * N, hipMemcpyDeviceToHost, c->stream));
+    std::vector<const double*> src = {P.x, P.y, P.z};
+    std::vector<const double*> rest = {P.vx, P.vy, P.vz, P.apx, P.apy, P.apz, P.m, P.h};
+    if (flags & kCkptMhd) rest.insert(rest.end(), {P.bx, P.by, P.bz, P.psi});
+    if (flags & kCkptCharge) rest.push_back(P.q);
+    if (flags & kCkptU) rest.push_back(P.u);
+    for (size_t k = 0; k < 3; ++k)
+        if (n) HIP_TRY(c, hipMemcpyAsync(f + k * N, src[k], sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+    for (size_t k = 0; k < rest.size(); ++k)
+        if (n) HIP_TRY(c, hipMemcpyAsync(f + (6 + k) * N, rest[k], sizeof(double) * N, hipMemcpyDeviceToHost,
+                                         c->stream));
+    uint32_t* ids = reinterpret_cast<uint32_t*>(f + ckpt_arrays(flags) * N);
+    if (n) {
+        HIP_TRY(c, hipMemcpyAsync(ids, P.id, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(ids + N, P.orig, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        f[3 * N + i] = xr[i].x;
+        f[4 * N + i] = xr[i].y;
+        f[5 * N + i] = xr[i].z;
+    }
+    return SPHX_OK;
+}
+
+int sphx_load(SphxCtx* c, const void* buf, uint64_t size) {
+    if (!c) return SPHX_E_INVALID;
+    if (!buf || size < sizeof(CkptHeader)) return fail(c, SPHX_E_INVALID, "not a checkpoint (too short)");
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof h);
+    if (std::memcmp(h.magic, kCkptMagic, 8) || h.version != kCkptVersion || h.header_bytes != sizeof(CkptHeader) ||
+        h.params_bytes != sizeof(SphxParams))
+        return fail(c, SPHX_E_INVALID, "not a checkpoint of this library version");
+    if (h.flags & ~(uint32_t)(kCkptMhd | kCkptCharge | kCkptU) || size != ckpt_bytes(h.n, h.flags))
+        return fail(c, SPHX_E_INVALID, "checkpoint size does not match its header");
+    if (h.n > c->cap) return fail(c, SPHX_E_CAPACITY, "checkpoint of %u particles exceeds capacity %u", h.n, c->cap);
+    if ((int)h.ks != c->L.ks)
+        return fail(c, SPHX_E_INVALID, "checkpoint lists hold %u neighbours, this context %d", h.ks, c->L.ks);
+    int rc = check_params(c, &h.params);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    const uint32_t n = h.n;
+    const size_t N = n;
+    const unsigned char* in = static_cast<const unsigned char*>(buf);
+    // the arrays may not be 8-byte aligned in the caller's buffer: copied out
+    std::vector<double> f(ckpt_arrays(h.flags) * N);
+    std::vector<uint32_t> ids(2 * N);
+    std::memcpy(f.data(), in + sizeof h, sizeof(double) * f.size());
+    std::memcpy(ids.data(), in + sizeof h + sizeof(double) * f.size(), sizeof(uint32_t) * ids.size());
+    const uint32_t* orig = ids.data() + N;
+    {
+        std::vector<uint8_t> seen(N, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (orig[i] >= n || seen[orig[i]]) return fail(c, SPHX_E_INVALID, "checkpoint slot map is not a permutation");
+            seen[orig[i]] = 1;
+        }
+    }
+    Grid g{};
+    for (int k = 0; k < 3; ++k) {
+        g.lo[k] = h.lo[k];
+        g.w[k] = h.w[k];
+        g.inv_w[k] = 1.0 / h.w[k];
+        g.L[k] = h.L[k];
+        g.halfL[k] = 0.5 * h.L[k];
+        g.dims[k] = h.dims[k];
+        if (!(h.w[k] > 0.0) || h.dims[k] < 1) return fail(c, SPHX_E_INVALID, "checkpoint grid is invalid");
+    }
+    g.periodic = h.periodic;
+    g.ncell = (uint32_t)((uint64_t)h.dims[0] * (uint64_t)h.dims[1] * (uint64_t)h.dims[2]);
+    g.inv_nx = 1.0 / (double)g.dims[0];
+    const SphxParams old = c->prm;
+    c->prm = h.params;
+    const double* a = f.data();
+    // the state at the positions of the last list build, in the saved sorted
+    // order: the sort reproduces that order and the build the saved list
+    SphxState st{};
+    st.n = n;
+    st.x = const_cast<double*>(a + 3 * N); st.y = const_cast<double*>(a + 4 * N); st.z = const_cast<double*>(a + 5 * N);
+    st.vx = const_cast<double*>(a + 6 * N); st.vy = const_cast<double*>(a + 7 * N); st.vz = const_cast<double*>(a + 8 * N);
+    st.m = const_cast<double*>(a + 12 * N);
+    st.h = const_cast<double*>(a + 13 * N);
+    size_t k = 14;
+    if (h.flags & kCkptMhd) {
+        st.bx = const_cast<double*>(a + k++ * N); st.by = const_cast<double*>(a + k++ * N);
+        st.bz = const_cast<double*>(a + k++ * N); st.psi = const_cast<double*>(a + k++ * N);
+    }
+    if (h.flags & kCkptCharge) st.q = const_cast<double*>(a + k++ * N);
+    if (h.flags & kCkptU) st.u = const_cast<double*>(a + k++ * N);
+    st.id = ids.data();
+    UploadExtras ex;
+    ex.grid = &g;
+    ex.orig = orig;
+    ex.ap[0] = a + 9 * N; ex.ap[1] = a + 10 * N; ex.ap[2] = a + 11 * N;
+    if ((rc = do_upload(c, &st, &ex))) {
+        c->prm = old;
+        make_kernel_params(c);
+        return rc;
+    }
+    if (!n) return SPHX_OK;
+    if ((rc = do_neighbours(c)) || (rc = ensure_list(c))) return rc;
+    // then the current positions (and velocities) through the upload slots,
+    // with the moved-skin/2 test against the build positions
+    std::vector<double> x3(3 * N), v3(3 * N);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t u = orig[i];
+        for (int d = 0; d < 3; ++d) {
+            x3[d * N + u] = a[d * N + i];
+            v3[d * N + u] = a[(6 + d) * N + i];
+        }
+    }
+    double* dx3 = nullptr;
+    if ((rc = dalloc(c, &dx3, 6 * N))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(dx3, x3.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dx3 + 3 * N, v3.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+    rc = adj_load(c, dx3, dx3 + 3 * N);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->allocs.pop_back();
+    (void)hipFree(dx3);
+    if (rc) return rc;
+    c->list_age = h.list_age;
+    c->last_life = h.last_life;
+    c->ticks = h.ticks;
     return SPHX_OK;
 }
 

@@ -259,6 +259,21 @@ class Context:
     def synchronize(self) -> None:
         self._check(self._lib.sphx_synchronize(self._h))
 
+    # -- checkpoint / resume
+    def save(self) -> bytes:
+        """sphx_save: the checkpoint of the current state as bytes."""
+        size = C.c_uint64(0)
+        self._check(self._lib.sphx_checkpoint_size(self._h, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        written = C.c_uint64(0)
+        self._check(self._lib.sphx_save(self._h, buf, size.value, C.byref(written)))
+        return buf.raw[: written.value]
+
+    def load(self, blob: bytes) -> None:
+        """sphx_load: parameters and state from a checkpoint; the context continues bit for bit."""
+        self._check(self._lib.sphx_load(self._h, blob, len(blob)))
+        self.n = int.from_bytes(blob[20:24], "little")  # the header's particle count
+
     def adjoint(self, nsteps: int, target: np.ndarray, checkpoint_every: int = 0,
                 with_grad_x: bool = False) -> Dict[str, object]:
         """Loss mean|x_N - target|^2 and its gradient in the initial velocities

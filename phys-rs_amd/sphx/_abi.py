@@ -107,6 +107,9 @@ SIGNATURES = {
     "sphx_step": (C.c_int, [C.c_void_p, C.c_uint32]),
     "sphx_adjoint": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32] + [C.POINTER(C.c_double)] * 11),
     "sphx_synchronize": (C.c_int, [C.c_void_p]),
+    "sphx_checkpoint_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "sphx_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "sphx_load": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "sphx_export_pairs": (C.c_int, [C.c_void_p, PU, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sphx_export_pairs_noalloc": (C.c_int, [C.c_void_p, PU, C.c_uint64, C.POINTER(C.c_uint64)]),
     "sphx_export_candidates": (C.c_int, [C.c_void_p, PD, PU, C.c_uint64, C.POINTER(C.c_uint64)]),
