@@ -311,9 +311,10 @@ int sphx_export_pairs_noalloc(SphxCtx* ctx, uint32_t* pairs, uint64_t cap_pairs,
  * BoundingRadius of particle i (upload order).  Writes every (id_a, id_b) with
  * id_a < id_b and |x_a - x_b|^2 < (4 R_a)^2, R_a the radius of the smaller id
  * (the reference's query radius 4 R around the querying entity), as pairs
- * sorted ascending; *count receives the total (even when > cap_pairs).  Needs
- * 4 max(R) + skin <= the cell width (SphxParams.cell_size), else
- * SPHX_E_INVALID.  Exact equality d^2 = (4 R)^2 is not recorded. */
+ * sorted ascending; *count receives the total (even when > cap_pairs).  Any
+ * radius: each particle searches the grid cells within 4 R + skin/2 of it.
+ * Exact equality d^2 = (4 R)^2 is not recorded.  SPHX_E_INVALID for a
+ * negative or non-finite radius. */
 int sphx_export_candidates(SphxCtx* ctx, const double* radius, uint32_t* pairs, uint64_t cap_pairs,
                            uint64_t* count);
 

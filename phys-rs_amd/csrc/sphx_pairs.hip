@@ -1350,45 +1350,54 @@ void launch_export_pairs(const Particles& P, uint32_t n, const Grid& g, const Nb
 // d^2 < (4 R_i)^2, R in sorted order.  The host checks 4 R_max + skin <= the
 // cell width, so the stencil holds every such pair of the binned particles.
 __global__ void __launch_bounds__(kBlock) k_export_candidates(Particles P, uint32_t n, Grid g,
-                                                               const uint32_t* __restrict__ skey,
                                                                const uint32_t* __restrict__ start,
-                                                               const double* __restrict__ R, uint32_t* __restrict__ out,
+                                                               const double* __restrict__ R, double half_skin,
+                                                               uint32_t* __restrict__ out,
                                                                unsigned long long* counter, uint64_t out_cap) {
     SPHX_PASS_PROLOGUE();
     const uint32_t idi = P.id[i];
-    const double xi = P.x[i], yi = P.y[i], zi = P.z[i];
+    const double xs[3] = {P.x[i], P.y[i], P.z[i]};
     const double r4 = 4.0 * R[i], r2max = r4 * r4;
-    const uint32_t key = skey[i];
-    const int nx = g.dims[0], ny = g.dims[1];
-    const int c[3] = {(int)(key % (uint32_t)nx), (int)((key / (uint32_t)nx) % (uint32_t)ny),
-                      (int)(key / ((uint32_t)nx * (uint32_t)ny))};
-    // distinct neighbour coordinates per axis
-    int nb[3][3], cnt[3];
+    // Cells hold the particles binned at the last list build, each within
+    // skin/2 of its position there: every j with |x_i - x_j| < 4 R_i is binned
+    // within 4 R_i + skin/2 of x_i.  Per axis, the cells overlapping that
+    // interval: clamped into the grid for a closed box (the binning clamps the
+    // same way), wrapped for a periodic one (each cell once).
+    const double reach = r4 + half_skin;
+    int lo[3], cnt[3];
+    bool all[3];
     for (int a = 0; a < 3; ++a) {
         const int d = g.dims[a];
-        cnt[a] = 0;
-        for (int o = -1; o <= 1; ++o) {
-            int v = c[a] + o;
-            if (g.periodic) {
-                v = (v % d + d) % d;
-            } else if (v < 0 || v >= d) {
-                continue;
-            }
-            bool seen = false;
-            for (int q = 0; q < cnt[a]; ++q) seen = seen || nb[a][q] == v;
-            if (!seen) nb[a][cnt[a]++] = v;
+        double ua = floor((xs[a] - reach - g.lo[a]) * g.inv_w[a]), ub = floor((xs[a] + reach - g.lo[a]) * g.inv_w[a]);
+        if (g.periodic) {
+            all[a] = ub - ua + 1.0 >= (double)d;
+            const int ia = (int)ua;
+            lo[a] = all[a] ? 0 : ((ia % d) + d) % d;
+            cnt[a] = all[a] ? d : (int)(ub - ua) + 1;
+        } else {
+            all[a] = false;
+            ua = fmin(fmax(ua, 0.0), (double)(d - 1));
+            ub = fmin(fmax(ub, 0.0), (double)(d - 1));
+            lo[a] = (int)ua;
+            cnt[a] = (int)ub - (int)ua + 1;
         }
     }
-    for (int iz = 0; iz < cnt[2]; ++iz)
-        for (int iy = 0; iy < cnt[1]; ++iy)
+    const uint32_t nx = (uint32_t)g.dims[0], ny = (uint32_t)g.dims[1];
+    for (int iz = 0; iz < cnt[2]; ++iz) {
+        int cz = lo[2] + iz;
+        if (cz >= g.dims[2]) cz -= g.dims[2];
+        for (int iy = 0; iy < cnt[1]; ++iy) {
+            int cy = lo[1] + iy;
+            if (cy >= g.dims[1]) cy -= g.dims[1];
             for (int ix = 0; ix < cnt[0]; ++ix) {
-                const uint32_t cell = ((uint32_t)nb[2][iz] * (uint32_t)ny + (uint32_t)nb[1][iy]) * (uint32_t)nx +
-                                      (uint32_t)nb[0][ix];
+                int cx = lo[0] + ix;
+                if (cx >= g.dims[0]) cx -= g.dims[0];
+                const uint32_t cell = ((uint32_t)cz * ny + (uint32_t)cy) * nx + (uint32_t)cx;
                 for (uint32_t j = start[cell], e = start[cell + 1]; j < e; ++j) {
                     const uint32_t idj = P.id[j];
                     if (!(idi < idj)) continue;
                     double dx, dy, dz;
-                    pair_vec(g, xi, yi, zi, P.x[j], P.y[j], P.z[j], dx, dy, dz);
+                    pair_vec(g, xs[0], xs[1], xs[2], P.x[j], P.y[j], P.z[j], dx, dy, dz);
                     const double d2 = (dx * dx + dy * dy) + dz * dz;
                     if (d2 < r2max) {
                         const unsigned long long pos = atomicAdd(counter, 1ull);
@@ -1399,13 +1408,15 @@ __global__ void __launch_bounds__(kBlock) k_export_candidates(Particles P, uint3
                     }
                 }
             }
+        }
+    }
 }
-void launch_export_candidates(const Particles& P, uint32_t n, const Grid& g, const uint32_t* skey,
-                              const uint32_t* cell_start, const double* R, uint32_t* out,
-                              unsigned long long* counter, uint64_t out_cap, hipStream_t s) {
+void launch_export_candidates(const Particles& P, uint32_t n, const Grid& g, const uint32_t* cell_start,
+                              const double* R, double half_skin, uint32_t* out, unsigned long long* counter,
+                              uint64_t out_cap, hipStream_t s) {
     if (n)
-        hipLaunchKernelGGL(k_export_candidates, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, g, skey, cell_start, R,
-                           out, counter, out_cap);
+        hipLaunchKernelGGL(k_export_candidates, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, g, cell_start, R,
+                           half_skin, out, counter, out_cap);
 }
 
 }  // namespace sphx

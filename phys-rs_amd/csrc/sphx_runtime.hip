@@ -1711,21 +1711,10 @@ int sphx_export_candidates(SphxCtx* c, const double* radius, uint32_t* pairs, ui
     if (!count) return fail(c, SPHX_E_INVALID, "count is NULL");
     const uint32_t n = c->n;
     if (n && !radius) return fail(c, SPHX_E_INVALID, "bounding radii are required");
-    double rmax = 0.0;
-    for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t i = 0; i < n; ++i)
         if (!std::isfinite(radius[i]) || radius[i] < 0.0)
             return fail(c, SPHX_E_INVALID, "bounding radius %u is negative or not finite", i);
-        rmax = std::max(rmax, radius[i]);
-    }
-    // the particles were binned at the last list build and have moved less than
-    // skin/2 since: pairs within 4 R lie in neighbouring cells when
-    // 4 R_max + skin fits in a cell
     const Grid& g = c->grid;
-    for (int a = 0; a < 3; ++a) {
-        if (4.0 * rmax + c->kp.skin > g.w[a])
-            return fail(c, SPHX_E_INVALID, "4 * bounding radius (%g) + skin exceeds the cell width %g; raise "
-                        "SphxParams.cell_size", 4.0 * rmax, g.w[a]);
-    }
     int rc;
     if ((rc = ensure_list(c))) return rc;
     *count = 0;
@@ -1747,8 +1736,8 @@ int sphx_export_candidates(SphxCtx* c, const double* radius, uint32_t* pairs, ui
     unsigned long long got = 0;
     for (int pass = 0; pass < 2; ++pass) {
         launch_fill_u32(reinterpret_cast<uint32_t*>(c->pair_counter), 0u, 2, c->stream);
-        launch_export_candidates(P, n, g, c->S.skey, c->S.cell_start, c->cand_r, c->pair_buf, c->pair_counter,
-                                 c->pair_buf_cap, c->stream);
+        launch_export_candidates(P, n, g, c->S.cell_start, c->cand_r, 0.5 * c->kp.skin, c->pair_buf,
+                                 c->pair_counter, c->pair_buf_cap, c->stream);
         if ((rc = launch_check(c))) return rc;
         HIP_TRY(c, hipMemcpyAsync(&got, c->pair_counter, sizeof got, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));

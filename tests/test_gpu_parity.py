@@ -675,13 +675,36 @@ def test_export_candidates_match_brute_force(require_gpu, periodic):
         ctx.step(5)
         cur = ctx.download(sphx.F_POS)
         got1 = ctx.export_candidates(radius)
-        # a radius beyond the cell is refused, not truncated
-        with pytest.raises(sphx.SphxError):
-            ctx.export_candidates(radius * 3.0)
+        # radii beyond the cell width: three cells per axis and more
+        got3 = ctx.export_candidates(radius * 3.0)
+        with pytest.raises(sphx.SphxError) as e:
+            ctx.export_candidates(-radius)
+        assert e.value.code == 4
     ref0 = brute_candidates(st, radius, periodic)
     moved = dict(st)
     moved.update(cur)
     ref1 = brute_candidates(moved, radius, periodic)
-    assert len(ref0) > 3000
+    ref3 = brute_candidates(moved, radius * 3.0, periodic)
+    assert len(ref0) > 3000 and len(ref3) > 20 * len(ref1)
     assert np.array_equal(got0, ref0)
     assert np.array_equal(got1, ref1)
+    assert np.array_equal(got3, ref3)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_export_candidates_radius_beyond_half_the_box(require_gpu, periodic):
+    # 4 R above half the box: in a periodic box every cell is searched once
+    # (minimum image), in an open one the search is clamped to the grid
+    rng = scenarios.SplitMix64(32)
+    n = 600
+    st = {k: rng.uniform(n) for k in ("x", "y", "z")}
+    st.update(vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n), m=np.full(n, 1.0 / n), h=np.full(n, 0.04),
+              id=np.argsort(rng.uniform(n)).astype(np.uint32))
+    radius = 0.15 * (0.5 + rng.uniform(n))
+    kw = dict(boundary_flags=sphx.BND_PERIODIC, box_lo=(0, 0, 0), box_hi=(1, 1, 1)) if periodic else {}
+    p = sphx.make_params(1e-5, tait=sphx.TaitEquationConfig(1.0, 5.0, 7.0), **kw)
+    with make_ctx(st, p) as ctx:
+        got = ctx.export_candidates(radius)
+    ref = brute_candidates(st, radius, periodic)
+    assert len(ref) > 0.4 * n * (n - 1) / 2
+    assert np.array_equal(got, ref)
