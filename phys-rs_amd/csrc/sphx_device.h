@@ -538,6 +538,38 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalF
         for_neighbour_groups(L, i, one, flush);
 }
 
+// body(e, record) for the entries kb .. ke - 1 of sorted particle i (kb a
+// multiple of 4), records from global memory, W records loaded before the W
+// bodies run (their gathers in flight together); flush() after each group of
+// four.  Entries past the list's end in the last group are replaced by
+// particle i itself in its own cell (r = 0), so the bodies must contribute
+// nothing at r = 0.
+template <int W, class Q, int NQ, int NQ2, class Body, class Flush = NoFlush>
+__device__ __forceinline__ void walk_ahead(const NbrList& L, uint32_t i, uint32_t kb, uint32_t ke,
+                                           const GlobalFetch<Q, NQ, NQ2>& F, Body&& body, Flush&& flush = Flush()) {
+    static_assert(4 % W == 0, "W divides the group of four");
+    constexpr uint32_t kOwnCell = 13u << kIdxBits;  // stage 4 (oy = oz = 0), x offset 0
+    const uint32_t* __restrict__ ep = nbr_base(L, i);
+    const uint32_t cnt = min((uint32_t)L.ntot[i], ke);
+    for (uint32_t k = kb; k < cnt; k += 4) {
+        const uint4 grp = nbr_group(ep, k);
+        const uint32_t ent[4] = {grp.x, grp.y, grp.z, grp.w};
+#pragma unroll
+        for (int u0 = 0; u0 < 4; u0 += W) {
+            uint32_t e[W];
+            Q q[W][NQ + NQ2];
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                e[w] = k + (uint32_t)(u0 + w) < cnt ? ent[u0 + w] : (i | kOwnCell);
+                F(e[w], q[w]);
+            }
+#pragma unroll
+            for (int w = 0; w < W; ++w) body(e[w], q[w]);
+        }
+        flush();
+    }
+}
+
 // =========================================================================
 // Brick-staged passes (the Bricks decomposition).
 // =========================================================================

@@ -1034,27 +1034,36 @@ void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g
 // 130-131), so particle i's term carries +1 when id_i < id_j, -1 otherwise.
 // Neighbours come from the f64 record (x, rho, v) and the B record (one
 // 6-quad record); the new B goes to tmp3 and, with psi, to recb_next (Jacobi).
-template <bool UNI, int PER, class Fetch>
-__device__ __forceinline__ void induction_particle(const Particles& P, const Derived& D, const Grid& g,
-                                                   const NbrList& L, const KernelParams& kp, uint32_t i,
-                                                   const Fetch& fetch) {
+// Two lanes per particle: lane h (threadIdx & 1) walks half h of the list,
+// two neighbour records loaded ahead of their bodies, and the halves are added
+// in order.  The pair body is branch-free (a factor 0 for r <= 1e-10, where
+// the reference skips the pair: mhd.rs:100).  RES: the resistive term (eta != 0).
+template <bool UNI, bool RES, int PER>
+__global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                       KernelParams kp) {
+    const uint32_t i = particle_index() >> 1;
+    if (i >= n) return;
+    const uint32_t h = threadIdx.x & 1u;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i];
     const uint32_t idi = P.id[i];
     const double inv_rhoi = rcp_nr(rhoi);
     double ux = 0, uy = 0, uz = 0;
-    walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[6]) {
+    const uint32_t cnt = L.ntot[i], half = ((cnt + 7u) >> 3) << 2;
+    const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
+                                       reinterpret_cast<const double2*>(D.recb)};
+    walk_ahead<4>(L, i, h ? half : 0u, h ? 0xffffffffu : half, F, [&](uint32_t e, const double2 (&q)[6]) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = q[0], zr = q[1], vxy = q[2], vzp = q[3], bxy = q[4], bzp = q[5];
         double dx, dy, dz;
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        if (!(r2 > 1e-20)) return;
+        const bool ok = r2 > 1e-20;
         double r, inv_r;
-        sqrt_rsqrt(r2, r, inv_r);
+        sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
-        const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;
+        const double dwr = ((ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r) * (ok ? 1.0 : 0.0);
         const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;  // r_hat dW/dr
         const double vx = vxi - vxy.x, vy = vyi - vxy.y, vz = vzi - vzp.x;
         const double rhoj = zr.y, mj = UNI ? kp.m0 : P.m[j];
@@ -1062,13 +1071,24 @@ __device__ __forceinline__ void induction_particle(const Particles& P, const Der
         const double b_dot_g = (bxi * gx + byi * gy) + bzi * gz;
         const double inv_rhoj = rcp_nr(rhoj);
         const double f = mj * (inv_rhoi * inv_rhoj);
-        const double r_dot_g = (dx * gx + dy * gy) + dz * gz;
-        const double res = (2.0 * kp.eta * (mj * inv_rhoj)) * r_dot_g * rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
         const double sg = (idi < P.id[j]) ? kp.dt : -kp.dt;
-        ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (bxy.x - bxi));
-        uy += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (bxy.y - byi));
-        uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (bzp.x - bzi));
+        if (RES) {
+            const double r_dot_g = (dx * gx + dy * gy) + dz * gz;
+            const double res =
+                (2.0 * kp.eta * (mj * inv_rhoj)) * r_dot_g * rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
+            ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (bxy.x - bxi));
+            uy += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (bxy.y - byi));
+            uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (bzp.x - bzi));
+        } else {
+            ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g));
+            uy += sg * (f * (byi * v_dot_g - vy * b_dot_g));
+            uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g));
+        }
     });
+    ux += __shfl_xor(ux, 1);  // lane 0: first half + second half
+    uy += __shfl_xor(uy, 1);
+    uz += __shfl_xor(uz, 1);
+    if (h) return;
     const double bx = bxi + ux, by = byi + uy, bz = bzi + uz;
     D.tmp3[0][i] = bx;
     D.tmp3[1][i] = by;
@@ -1077,15 +1097,6 @@ __device__ __forceinline__ void induction_particle(const Particles& P, const Der
     rb[0] = make_double2(bx, by);
     rb[1] = make_double2(bz, P.psi[i]);
 }
-template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                       KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
-    induction_particle<UNI, -1>(P, D, g, L, kp, i,
-                                GlobalFetch<double2, 4, 2>{reinterpret_cast<const double2*>(D.r64),
-                                                           reinterpret_cast<const double2*>(D.recb)});
-}
-
 
 // FP32-mixed MHD passes: f32 pair terms on the cell-relative f32 record (as
 // the mixed force pass), the neighbour's B converted to f32, each group of
@@ -1190,34 +1201,52 @@ void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Gr
                                n, g, L, kp);
         return;
     }
-    if (kp.uniform)
-        hipLaunchKernelGGL(k_induction<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L, kp);
-    else
-        hipLaunchKernelGGL(k_induction<false>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L,
-                           kp);
+    const dim3 grid2(nblocks(2 * n));  // two lanes per particle
+    const bool res = kp.eta != 0.0;
+#define SPHX_INDUCTION(U, R, PER)                                                                              \
+    if (kp.uniform == (U ? 1 : 0) && res == R && g.periodic == PER) {                                          \
+        hipLaunchKernelGGL((k_induction<U, R, PER>), grid2, dim3(kBlock), occupancy_lds(4), s, P, D, n, g, L, kp); \
+        return;                                                                                               \
+    }
+    SPHX_INDUCTION(true, false, 1)
+    SPHX_INDUCTION(true, false, 0)
+    SPHX_INDUCTION(true, true, 1)
+    SPHX_INDUCTION(true, true, 0)
+    SPHX_INDUCTION(false, false, 1)
+    SPHX_INDUCTION(false, false, 0)
+    SPHX_INDUCTION(false, true, 1)
+    SPHX_INDUCTION(false, true, 0)
+#undef SPHX_INDUCTION
 }
 
 // ---------------------------------------------- divergence cleaning (a12)
 // divergence_cleaning_system (mhd.rs:172-259): Jacobi over B and psi, then
 // psi *= exp(-sigma c_h dt / h) for every particle.
-template <bool UNI, int PER, class Fetch>
-__device__ __forceinline__ void div_clean_particle(const Particles& P, const Derived& D, const Grid& g,
-                                                   const NbrList& L, const KernelParams& kp, uint32_t i,
-                                                   const Fetch& fetch) {
+// divergence_cleaning_system (mhd.rs:172-259) in gather form, two lanes per
+// particle as the induction pass (halves added in order, branch-free body).
+template <bool UNI, int PER>
+__global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                       KernelParams kp) {
+    const uint32_t i = particle_index() >> 1;
+    if (i >= n) return;
+    const uint32_t h = threadIdx.x & 1u;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i], psii = P.psi[i];
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
-    walk(L, i, fetch, [&](uint32_t e, const double2 (&q)[6]) {
+    const uint32_t cnt = L.ntot[i], half = ((cnt + 7u) >> 3) << 2;
+    const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
+                                       reinterpret_cast<const double2*>(D.recb)};
+    walk_ahead<4>(L, i, h ? half : 0u, h ? 0xffffffffu : half, F, [&](uint32_t e, const double2 (&q)[6]) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = q[0], zr = q[1], bxy = q[4], bzp = q[5];
         double dx, dy, dz;
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        if (!(r2 > 1e-20)) return;
+        const bool ok = r2 > 1e-20;
         double r, inv_r;
-        sqrt_rsqrt(r2, r, inv_r);
+        sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
-        const double dwr = (ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r;
+        const double dwr = ((ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r) * (ok ? 1.0 : 0.0);
         const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;
         const double vol = (UNI ? kp.m0 : P.m[j]) * rcp_nr(zr.y);
         div += vol * (((bxy.x - bxi) * gx + (bxy.y - byi) * gy) + (bzp.x - bzi) * gz);
@@ -1226,6 +1255,11 @@ __device__ __forceinline__ void div_clean_particle(const Particles& P, const Der
         gpy = fma(dp, gy, gpy);
         gpz = fma(dp, gz, gpz);
     });
+    div += __shfl_xor(div, 1);
+    gpx += __shfl_xor(gpx, 1);
+    gpy += __shfl_xor(gpy, 1);
+    gpz += __shfl_xor(gpz, 1);
+    if (h) return;
     const double dt = kp.dt;
     const double bx = bxi - gpx * dt, by = byi - gpy * dt, bz = bzi - gpz * dt;
     D.tmp3[0][i] = bx;
@@ -1237,14 +1271,6 @@ __device__ __forceinline__ void div_clean_particle(const Particles& P, const Der
     double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
     rb[0] = make_double2(bx, by);
     rb[1] = make_double2(bz, psi);
-}
-template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                       KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
-    div_clean_particle<UNI, -1>(P, D, g, L, kp, i,
-                                GlobalFetch<double2, 4, 2>{reinterpret_cast<const double2*>(D.r64),
-                                                           reinterpret_cast<const double2*>(D.recb)});
 }
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
@@ -1306,11 +1332,17 @@ void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Gr
                                n, g, L, kp);
         return;
     }
-    if (kp.uniform)
-        hipLaunchKernelGGL(k_div_clean<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L, kp);
-    else
-        hipLaunchKernelGGL(k_div_clean<false>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, n, g, L,
-                           kp);
+    const dim3 grid2(nblocks(2 * n));  // two lanes per particle
+#define SPHX_CLEAN(U, PER)                                                                                    \
+    if (kp.uniform == (U ? 1 : 0) && g.periodic == PER) {                                                      \
+        hipLaunchKernelGGL((k_div_clean<U, PER>), grid2, dim3(kBlock), occupancy_lds(4), s, P, D, n, g, L, kp);  \
+        return;                                                                                               \
+    }
+    SPHX_CLEAN(true, 1)
+    SPHX_CLEAN(true, 0)
+    SPHX_CLEAN(false, 1)
+    SPHX_CLEAN(false, 0)
+#undef SPHX_CLEAN
 }
 
 // ---------------------------------------------------------- pair export
