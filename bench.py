@@ -54,7 +54,8 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed)
 # pair (add/sub/mul/div/sqrt = 1, fma = 2) from the reference formulas.
 ALG = {
     "sort":      {"bytes": 224, "flops_pair": 0.0},   # hash 28 + counting sort 12 + reorder 184
-    "neighbour": {"bytes": 32, "flops_pair": 0.0},    # x, y, z, h read; candidates are not work
+    # x, y, z, h read and one 4-byte list entry written per neighbour; candidates are not work
+    "neighbour": {"bytes": 32, "bytes_pair": 4, "flops_pair": 0.0},
     "density":   {"bytes": 56, "flops_pair": 26.0},   # x, m, h -> rho, P
     "force":     {"bytes": 128, "flops_pair": 58.0},  # x, v, m, h, rho, P -> F_p, a*
     "visc_only": {"bytes": 120, "flops_pair": 45.0},  # x, v, m, h, rho, F_p -> a_n
@@ -261,7 +262,7 @@ def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
             if v["launches"] == 0:
                 continue
             avg_s = v["ms"] / v["launches"] * 1e-3
-            gbs = ALG[k]["bytes"] * n / avg_s / 1e9
+            gbs = (ALG[k]["bytes"] * n + ALG[k].get("bytes_pair", 0) * pairs) / avg_s / 1e9
             tflops = ALG[k]["flops_pair"] * pairs / avg_s / 1e12
             # the mixed form's pair math is f32 (against the f32 vector peak)
             f32 = prec == "mixed" and k in ("density", "force", "visc_only", "mhd")
@@ -311,7 +312,7 @@ def roofline(n: int, times: dict, pairs: int, steps: int):
         if v["launches"] == 0:
             continue
         avg_s = v["ms"] / v["launches"] * 1e-3
-        b = ALG[k]["bytes"] * n
+        b = ALG[k]["bytes"] * n + ALG[k].get("bytes_pair", 0) * pairs
         f = ALG[k]["flops_pair"] * pairs
         kern[k] = {
             "avg_ms": round(v["ms"] / v["launches"], 5),
@@ -323,7 +324,7 @@ def roofline(n: int, times: dict, pairs: int, steps: int):
     dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
     v = times[dom]
     avg_s = v["ms"] / v["launches"] * 1e-3
-    b = ALG[dom]["bytes"] * n
+    b = ALG[dom]["bytes"] * n + ALG[dom].get("bytes_pair", 0) * pairs
     f = ALG[dom]["flops_pair"] * pairs
     ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
     if f > 0 and f / b > ridge:
@@ -503,11 +504,14 @@ def main() -> int:
         "kernels": kern,
     }
     if not args.no_extras and args.precision == "fp64":
-        n2, secs2, _, _ = run_gpu("mixed", args.steps, args.warmup)
+        n2, secs2, times2, _ = run_gpu("mixed", args.steps, args.warmup)
         mixed = {"ms_per_step": round(secs2 / args.steps * 1e3, 4),
                  "value": round(n2 * args.steps / secs2, 1),
                  "speedup_vs_fp64": round(secs / secs2, 3)}
         mixed.update(eval_errors(END_STATE["fp64"]))
+        mixed["kernels"] = {k: {"avg_ms": round(v["ms"] / v["launches"], 5), "launches": v["launches"],
+                                "ms_per_step": round(v["ms"] / args.steps, 5)}
+                            for k, v in times2.items() if v["launches"]}
         out["mixed"] = mixed
         # the same FP64 run with the neighbour list rebuilt at every tick
         n3, secs3, times3, _ = run_gpu("fp64", args.steps, args.warmup, skin=0.0)

@@ -32,6 +32,8 @@
 |                            |                           | jittered periodic box; uniform B on a lattice: a = 0       |
 | orc_xsph                   | sph.rs:349-405            | uniform v fixed point; two-particle closed form            |
 | orc_energy                 | new                       | conservation over an inviscid run                          |
+| orc_omp_eval / orc_omp_step| gather form of the force  | equals the pair form to round-off (Tait and adiabatic,     |
+|   (sph_omp.c)              | evaluation and the tick   | penalty, Lorentz, Maxwell stress, du/dt), and over ticks   |
 """
 import math
 
