@@ -225,7 +225,7 @@ def mhd_eval_errors(st, p):
                     "to the largest FP64 change"}
 
 
-def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
+def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0, skin_h: float = -1.0):
     """Per-kernel breakdown of the periodic MHD box in the FP64 and FP32-mixed forms
     (seeded random fill of n_fill particles, or the lattice^3 jittered lattice)."""
     import sphx
@@ -236,9 +236,11 @@ def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0):
     else:
         st, p = scenarios.mhd_periodic(random_fill=n_fill)
     n = len(st["x"])
+    if skin_h >= 0:
+        p.neighbour_skin = skin_h * float(st["h"].max())
     kind = f"lattice_{lattice}^3" if lattice else "random"
     out = {"report": "mhd", "workload": f"mhd_periodic_{kind}_{n}", "particles": n, "steps": steps,
-           "h": float(st["h"][0])}
+           "h": float(st["h"][0]), "skin_h": skin_h if skin_h >= 0 else 0.04}
     for prec in ("fp64", "mixed"):
         p.precision = sphx.PREC_FP64 if prec == "fp64" else sphx.PREC_FP32_MIXED
         with sphx.Context(n, p) as ctx:
@@ -441,6 +443,7 @@ def main() -> int:
     ap.add_argument("--mhd-report", action="store_true")
     ap.add_argument("--adjoint-report", action="store_true", help="only the 65,536-particle adjoint/forward run")
     ap.add_argument("--mhd-particles", type=int, default=4194304)
+    ap.add_argument("--mhd-skin", type=float, default=-1.0, help="list skin in units of h (default 0.04)")
     ap.add_argument("--mhd-lattice", type=int, default=0,
                     help="use the n^3 jittered lattice (64: the 262,144-particle box) instead of the random fill")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -469,7 +472,8 @@ def main() -> int:
         return 0
     if args.mhd_report:
         steps = args.steps if args.mhd_lattice else min(args.steps, 20)
-        print(json.dumps(mhd_report(args.mhd_particles, steps, min(args.warmup, 2), args.mhd_lattice)), flush=True)
+        print(json.dumps(mhd_report(args.mhd_particles, steps, min(args.warmup, 2), args.mhd_lattice,
+                                    args.mhd_skin)), flush=True)
         return 0
 
     n, secs, times, e = run_gpu(args.precision, args.steps, args.warmup)

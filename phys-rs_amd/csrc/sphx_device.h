@@ -63,17 +63,42 @@ __device__ __forceinline__ void min_image(double& d, double L, double halfL) {
     else if (d < -halfL) d += L;
 }
 
+// The Maxwell-stress record of a particle: b = B / rho and |b|^2 / 2, so that
+// the stress term of a pair, (B (B.g) - |B|^2 g / 2) / rho^2 for either end,
+// is b (b.g) - (|b|^2 / 2) g with no reciprocal per pair.  Every writer
+// (density and cleaning epilogues, k_stress_rec) forms it here, so the bits do
+// not depend on which one wrote it.
+__device__ __forceinline__ double4 stress_record(double bx, double by, double bz, double rho) {
+    const double ir = 1.0 / rho;
+    const double sx = bx * ir, sy = by * ir, sz = bz * ir;
+    return make_double4(sx, sy, sz, 0.5 * ((sx * sx + sy * sy) + sz * sz));
+}
+
+// The minimum image of a neighbour-list pair, branch-free: d - k L with
+// k = rint(d / L), one rounding for k = +-1 as in d -= L.  A periodic axis
+// holds at least 3 cells no narrower than a list pair's reach, so a list pair
+// has |d| < L / 3 or |d| > 2 L / 3 (positions stay in the box), where k is
+// exactly the shift min_image applies: the same bits.
+__device__ __forceinline__ void min_image_list(double& d, double L, double invL) {
+    d = fma(-rint(d * invL), L, d);
+}
+
 // x_j - x_i as core/coordinates.rs:51-65 forms it in one sector (0 + (b - a)),
 // with the minimum image in a periodic box.
-// PER: 1 periodic, 0 closed box, -1 read from the grid (a compile-time PER
-// keeps a pair body free of the branch).
+// PER: 1 periodic (neighbour-list pairs only: min_image_list), 0 closed box,
+// -1 read from the grid (a compile-time PER keeps a pair body free of the
+// branch).
 template <int PER = -1>
 __device__ __forceinline__ void pair_vec(const Grid& g, double xi, double yi, double zi, double xj,
                                          double yj, double zj, double& dx, double& dy, double& dz) {
     dx = xj - xi;
     dy = yj - yi;
     dz = zj - zi;
-    if (PER == 1 || (PER < 0 && g.periodic)) {
+    if (PER == 1) {
+        min_image_list(dx, g.L[0], g.invL[0]);
+        min_image_list(dy, g.L[1], g.invL[1]);
+        min_image_list(dz, g.L[2], g.invL[2]);
+    } else if (PER < 0 && g.periodic) {
         min_image(dx, g.L[0], g.halfL[0]);
         min_image(dy, g.L[1], g.halfL[1]);
         min_image(dz, g.L[2], g.halfL[2]);

@@ -26,6 +26,7 @@ struct Grid {
     int periodic;     // 1: periodic box, cells wrap, minimum image
     double L[3];      // periodic box length
     double halfL[3];
+    double invL[3];   // 1 / L (periodic), 0 otherwise
     uint32_t ncell;
     double inv_nx;    // 1 / dims[0] (cell_x_of)
 };
@@ -64,6 +65,7 @@ struct Derived {
     Rec32* r32;
     double4* recb;             // (bx, by, bz, psi) in sorted order, written by the reorder and the MHD passes
     double4* recb_next;        // the MHD passes' output (Jacobi), swapped with recb afterwards
+    double4* recs;             // Maxwell stress: (B / rho, |B|^2 / (2 rho^2)) in sorted order (stress_record)
     double4* xref;             // positions at the last neighbour build (sorted order)
     double *rho, *p, *pr2;     // density, pressure, P / rho^2
     double *ax, *ay, *az;      // acceleration
@@ -180,6 +182,7 @@ struct KernelParams {          // what the kernels need from SphxParams, by valu
     int uniform;
     double m0, h0, inv_h0;
     int rec64;                 // keep the f64 gather record current (the FP64 form)
+    int srec;                  // Maxwell stress on: density and cleaning passes write D.recs
     double skin;               // neighbour-list skin (list radius 2 h_avg + skin)
     double half_skin_sq;       // (skin / 2)^2: displacement since the last build that forces a rebuild
 };
@@ -195,6 +198,7 @@ void launch_reorder(const Particles& src, const Particles& dst, const Sort& S, u
                     bool mhd, bool charge, bool mixed, bool energy, const Mixed& X, const Grid& g, double4* recb,
                     double4* xref, hipStream_t s);
 void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
+void launch_stress_rec(const Particles& P, const Derived& D, uint32_t n, hipStream_t s);
 void launch_bricks(const Grid& g, const Sort& S, const Bricks& B, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,

@@ -411,6 +411,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
     rec[1] = make_double2(zi, rho);
     rec[2] = make_double2(P.vx[i], P.vy[i]);
     rec[3] = make_double2(P.vz[i], pr2);
+    if (kp.srec) D.recs[i] = stress_record(P.bx[i], P.by[i], P.bz[i], rho);
 }
 
 template <bool UNI>
@@ -480,6 +481,7 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
     rec[0] = make_float4(xi, yi, zi, (float)rho);
     rec[1] = make_float4((float)vxi, (float)vyi, (float)vzi, (float)pr2);
+    if (kp.srec) D.recs[i] = stress_record(P.bx[i], P.by[i], P.bz[i], rho);
 }
 
 template <bool UNI>
@@ -606,6 +608,16 @@ __device__ __forceinline__ void force_epilogue(const Particles& P, const Derived
     D.az[p] = F[2] / mi;
 }
 
+// The i term of the Maxwell-stress sum (see force_particle):
+// fm <- -(b_i (b_i . gs) - e_i gs + fm).
+__device__ __forceinline__ void stress_close(const double2* __restrict__ RS, uint32_t i, const double gs[3],
+                                             double fm[3]) {
+    const double2 sxy = RS[2 * i], sze = RS[2 * i + 1];
+    const double bg = fma(sze.x, gs[2], fma(sxy.y, gs[1], sxy.x * gs[0]));
+    const double b[3] = {sxy.x, sxy.y, sze.x};
+    for (int a = 0; a < 3; ++a) fm[a] = -(fma(b[a], bg, -sze.y * gs[a]) + fm[a]);
+}
+
 // LANES = 2: the calling lane and its neighbour (lane ^ 1) share particle i.
 // ENERGY (adiabatic EOS, full pass only): du_i/dt = sum_j m_j (P_i/rho_i^2 +
 // Pi_ij/2) v_ij . grad_i W_ij, from the same pair terms (v_ij . grad_i W =
@@ -617,12 +629,6 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
     const double mi = P.m[i], rhoi = D.rho[i], pr2i = D.pr2[i];
-    double bxi = 0, byi = 0, bzi = 0, irho2i = 0, hb2i = 0;
-    if (STRESS) {
-        bxi = P.bx[i]; byi = P.by[i]; bzi = P.bz[i];
-        irho2i = 1.0 / (rhoi * rhoi);
-        hb2i = 0.5 * ((bxi * bxi + byi * byi) + bzi * bzi);
-    }
     // Pi = mu (beta mu - alpha c) / rho_bar with mu = h s / den, den = r^2 + 0.01 h^2
     // (sph.rs:281-300), formed with one reciprocal:
     // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s.
@@ -635,7 +641,15 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     const double s_lo = ref_conv ? -INFINITY : 0.0, s_hi = ref_conv ? 0.0 : INFINITY;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     double de = 0.0;  // ENERGY: sum of (P_i/rho_i^2 + Pi/2) s dW/r
-    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
+    // STRESS (Maxwell stress M = B B - |B|^2 I / 2, SPMHD):
+    //   F_i = m_i m_j sum_j (M_i / rho_i^2 + M_j / rho_j^2) . grad_i W
+    // with b = B / rho and e = |b|^2 / 2 from the stress records and
+    // grad_i W = -(dW/r) d, d = x_j - x_i: the j terms are summed per pair,
+    //   fm = sum_j (dW/r) (b_j (b_j . d) - e_j d),
+    // and the i term leaves the sum through gs = sum_j (dW/r) d:
+    //   F_i = -(b_i (b_i . gs) - e_i gs + fm).
+    double gs[3] = {0, 0, 0};
+    const double2* __restrict__ RS = reinterpret_cast<const double2*>(D.recs);
     // The pair body is branch-free: a pair without a term adds +-0, which
     // leaves a sum unchanged (the sums start at +0 and never become -0).
     // With uniform m and h, m_i m_j sigma_g is folded out of the sums.
@@ -677,23 +691,21 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
             if (ENERGY) de = fma(fma(0.5, pi_ij, pr2i), sdot * dwr, de);
         }
         if (STRESS) {
-            const double gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;  // grad_i W (times m_i m_j)
-            double2 bxy, bzp;
+            double2 sxy, sze;  // (b_x, b_y), (b_z, e) of j
             if constexpr (Fetch::NR >= 6) {
-                bxy = q[4];
-                bzp = q[5];
+                sxy = q[4];
+                sze = q[5];
             } else {
-                bxy = RB[2 * j];
-                bzp = RB[2 * j + 1];
+                sxy = RS[2 * j];
+                sze = RS[2 * j + 1];
             }
-            const double bxj = bxy.x, byj = bxy.y, bzj = bzp.x;
-            const double irho2j = rcp_nr(rhoj * rhoj);
-            const double hb2j = 0.5 * ((bxj * bxj + byj * byj) + bzj * bzj);
-            const double big = (bxi * gx + byi * gy) + bzi * gz;
-            const double bjg = (bxj * gx + byj * gy) + bzj * gz;
-            fm[0] += (bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j;
-            fm[1] += (byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j;
-            fm[2] += (bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j;
+            gs[0] = fma(dwr, dx, gs[0]);
+            gs[1] = fma(dwr, dy, gs[1]);
+            gs[2] = fma(dwr, dz, gs[2]);
+            const double bd = fma(sze.x, dz, fma(sxy.y, dy, sxy.x * dx));
+            fm[0] = fma(dwr, fma(sxy.x, bd, -sze.y * dx), fm[0]);
+            fm[1] = fma(dwr, fma(sxy.y, bd, -sze.y * dy), fm[1]);
+            fm[2] = fma(dwr, fma(sze.x, bd, -sze.y * dz), fm[2]);
         }
     };
     // The list is summed in two halves, entries 0 .. half - 1 and half .. end
@@ -712,24 +724,30 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         for (int a = 0; a < 3; ++a) {  // lane h = 0: first half + second half
             fp[a] += __shfl_xor(fp[a], 1);
             fv[a] += __shfl_xor(fv[a], 1);
-            fm[a] += __shfl_xor(fm[a], 1);
+            if (STRESS) {
+                fm[a] += __shfl_xor(fm[a], 1);
+                gs[a] += __shfl_xor(gs[a], 1);
+            }
         }
         if (ENERGY) de += __shfl_xor(de, 1);
         if (h) return;
     } else {
         sum_range(0u, half);
         const double p0[3] = {fp[0], fp[1], fp[2]}, v0[3] = {fv[0], fv[1], fv[2]}, m0[3] = {fm[0], fm[1], fm[2]};
+        const double g0[3] = {gs[0], gs[1], gs[2]};
         const double e0 = de;
-        for (int a = 0; a < 3; ++a) fp[a] = fv[a] = fm[a] = 0.0;
+        for (int a = 0; a < 3; ++a) fp[a] = fv[a] = fm[a] = gs[a] = 0.0;
         de = 0.0;
         sum_range(half, 0xffffffffu);
         for (int a = 0; a < 3; ++a) {
             fp[a] = p0[a] + fp[a];
             fv[a] = v0[a] + fv[a];
             fm[a] = m0[a] + fm[a];
+            gs[a] = g0[a] + gs[a];
         }
         de = e0 + de;
     }
+    if (STRESS) stress_close(RS, i, gs, fm);
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
         const double k = (kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi);
@@ -777,7 +795,7 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
         [&](uint32_t i, const BrickFetch<double2, 4, W, PF, NQ2>& f) {
             force_particle<VISC_ONLY, ENERGY, STRESS, UNI, PER, 2>(P, D, g, L, kp, charge, i, f);
         },
-        reinterpret_cast<const double2*>(D.recb));
+        reinterpret_cast<const double2*>(D.recs));
 }
 
 // FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue (and the
@@ -787,17 +805,11 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
                                                      const Grid& g, const NbrList& L, const KernelParams& kp,
                                                      int charge, uint32_t i, const Fetch& fetch) {
     const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
-    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
+    const double2* __restrict__ RS = reinterpret_cast<const double2*>(D.recs);
     const float4 own0 = R[2 * i], own1 = R[2 * i + 1];
     const float xi = own0.x, yi = own0.y, zi = own0.z, rhoi = own0.w;
     const float vxi = own1.x, vyi = own1.y, vzi = own1.z, pr2i = own1.w;
     const float hi = X.p4[i].w, mi = UNI ? 0.0f : X.m[i];
-    float bxi = 0, byi = 0, bzi = 0, irho2i = 0, hb2i = 0;
-    if (STRESS) {
-        bxi = (float)P.bx[i]; byi = (float)P.by[i]; bzi = (float)P.bz[i];
-        irho2i = 1.0f / (rhoi * rhoi);
-        hb2i = 0.5f * ((bxi * bxi + byi * byi) + bzi * bzi);
-    }
     const float inv_h0 = (float)kp.inv_h0, h0 = (float)kp.h0;
     // viscosity with one reciprocal, as in the f64 pass:
     // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s
@@ -813,6 +825,10 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     float gp[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
     double de = 0.0;
     float ge = 0.0f;
+    // STRESS: the j terms and gs = sum (dW/r) d as in the f64 pass, f32 per
+    // group of four entries
+    double gs[3] = {0, 0, 0};
+    float gm[3] = {0, 0, 0}, gg[3] = {0, 0, 0};
     // branch-free pair body, as in the f64 pass (factors 1 or 0, not selects)
     walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[Fetch::NR]) {
         const uint32_t j = e & kIdxMask;
@@ -857,39 +873,46 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             if (ENERGY) ge = fmaf(fmaf(0.5f, pi_ij, pr2i), sdot * dwr, ge);
         }
         if (STRESS) {
-            const float gx = -dx * dwr, gy = -dy * dwr, gz = -dz * dwr;
-            // B_j: quads 2 and 3 of a staged record hold the bits of (bx, by)
-            // and (bz, psi); unstaged it is read from the B records
-            double2 bxy, bzp;
+            // the stress record of j: quads 2 and 3 of a staged record hold
+            // the bits of (b_x, b_y) and (b_z, e); unstaged it is read from
+            // the stress records
+            double2 sxy, sze;
             if constexpr (Fetch::NR >= 4) {
                 const float4 u = q[2], v = q[3];
-                bxy = make_double2(__hiloint2double(__float_as_int(u.y), __float_as_int(u.x)),
+                sxy = make_double2(__hiloint2double(__float_as_int(u.y), __float_as_int(u.x)),
                                    __hiloint2double(__float_as_int(u.w), __float_as_int(u.z)));
-                bzp = make_double2(__hiloint2double(__float_as_int(v.y), __float_as_int(v.x)), 0.0);
+                sze = make_double2(__hiloint2double(__float_as_int(v.y), __float_as_int(v.x)),
+                                   __hiloint2double(__float_as_int(v.w), __float_as_int(v.z)));
             } else {
-                bxy = RB[2 * j];
-                bzp = RB[2 * j + 1];
+                sxy = RS[2 * j];
+                sze = RS[2 * j + 1];
             }
-            const float bxj = (float)bxy.x, byj = (float)bxy.y, bzj = (float)bzp.x;
-            const float irho2j = __builtin_amdgcn_rcpf(rhoj * rhoj);
-            const float hb2j = 0.5f * ((bxj * bxj + byj * byj) + bzj * bzj);
-            const float big = (bxi * gx + byi * gy) + bzi * gz;
-            const float bjg = (bxj * gx + byj * gy) + bzj * gz;
-            fm[0] += (double)((bxi * big - hb2i * gx) * irho2i + (bxj * bjg - hb2j * gx) * irho2j);
-            fm[1] += (double)((byi * big - hb2i * gy) * irho2i + (byj * bjg - hb2j * gy) * irho2j);
-            fm[2] += (double)((bzi * big - hb2i * gz) * irho2i + (bzj * bjg - hb2j * gz) * irho2j);
+            const float bx = (float)sxy.x, by = (float)sxy.y, bz = (float)sze.x, ej = (float)sze.y;
+            gg[0] = fmaf(dwr, dx, gg[0]);
+            gg[1] = fmaf(dwr, dy, gg[1]);
+            gg[2] = fmaf(dwr, dz, gg[2]);
+            const float bd = fmaf(bz, dz, fmaf(by, dy, bx * dx));
+            gm[0] = fmaf(dwr, fmaf(bx, bd, -ej * dx), gm[0]);
+            gm[1] = fmaf(dwr, fmaf(by, bd, -ej * dy), gm[1]);
+            gm[2] = fmaf(dwr, fmaf(bz, bd, -ej * dz), gm[2]);
         }
     }, [&]() {
         for (int a = 0; a < 3; ++a) {
             if (!VISC_ONLY) fp[a] += (double)gp[a];
             fv[a] += (double)gv[a];
             gp[a] = gv[a] = 0.0f;
+            if (STRESS) {
+                fm[a] += (double)gm[a];
+                gs[a] += (double)gg[a];
+                gm[a] = gg[a] = 0.0f;
+            }
         }
         if (ENERGY) {
             de += (double)ge;
             ge = 0.0f;
         }
     });
+    if (STRESS) stress_close(RS, i, gs, fm);
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
         const double k = ((kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi)) * 0.75;
@@ -931,7 +954,7 @@ k_forces_mixed_staged(
         [&](uint32_t i, const BrickFetch<float4, 2, W, PF, NQ2>& f) {
             force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI>(P, D, X, g, L, kp, charge, i, f);
         },
-        reinterpret_cast<const float4*>(D.recb));
+        reinterpret_cast<const float4*>(D.recs));
 }
 
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
@@ -1271,6 +1294,7 @@ __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, ui
     double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
     rb[0] = make_double2(bx, by);
     rb[1] = make_double2(bz, psi);
+    if (kp.srec) D.recs[i] = stress_record(bx, by, bz, D.rho[i]);
 }
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
@@ -1318,6 +1342,7 @@ __global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived
     double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
     rb[0] = make_double2(bx, by);
     rb[1] = make_double2(bz, psi);
+    if (kp.srec) D.recs[i] = stress_record(bx, by, bz, D.rho[i]);
 }
 
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,

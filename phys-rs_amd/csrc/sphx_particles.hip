@@ -170,6 +170,18 @@ void launch_eos(const Particles& P, const Derived& D, uint32_t n, const KernelPa
                 hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_eos, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n, kp, (int)mixed);
 }
+
+// The Maxwell-stress records from the current B and density, for a force pass
+// that follows a change of B without a cleaning pass (which writes them).
+__global__ void k_stress_rec(Particles P, Derived D, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    D.recs[i] = stress_record(P.bx[i], P.by[i], P.bz[i], D.rho[i]);
+}
+void launch_stress_rec(const Particles& P, const Derived& D, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_stress_rec, dim3(nblocks(n)), dim3(kBlock), 0, s, P, D, n);
+}
+
 void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const KernelParams& kp, bool mixed,
                         const Mixed& X, hipStream_t s) {
     (void)X;

@@ -124,6 +124,9 @@ struct SphxCtx {
     bool has_mhd = false, has_charge = false;
     bool has_u = false;                    // an internal energy was uploaded
     bool pres = false;                     // P (adiabatic: from the current u) matches the density
+    // the Maxwell-stress records (D.recs) match B and the density; the density
+    // and cleaning passes write them, the induction pass makes them stale
+    bool srec = false;
     bool uniform = false;                  // all m equal and all h equal
     double m0 = 0.0, h0 = 0.0;
     double hmax = 0.0;                     // largest smoothing radius (h is fixed per particle)
@@ -243,6 +246,8 @@ void make_kernel_params(SphxCtx* c) {
     k.mhd = p.mhd_flags;
     k.uniform = c->uniform ? 1 : 0;
     k.rec64 = c->mixed() ? 0 : 1;
+    k.srec = (c->has_mhd && (p.mhd_flags & SPHX_MHD_MAXWELL_STRESS)) ? 1 : 0;
+    c->srec = false;
     k.skin = p.neighbour_skin < 0.0 ? 0.04 * c->hmax : p.neighbour_skin;
     k.half_skin_sq = 0.25 * k.skin * k.skin;
     k.m0 = c->m0;
@@ -407,6 +412,7 @@ int setup_grid(SphxCtx* c, const SphxState* host, const Grid* fixed = nullptr) {
                 g.w[k] = ext[k] / d;
                 g.L[k] = ext[k];
                 g.halfL[k] = 0.5 * ext[k];
+                g.invL[k] = 1.0 / ext[k];
             } else {
                 g.w[k] = cs;
             }
@@ -570,6 +576,7 @@ int do_density(SphxCtx* c) {
                        c->stream);
     }
     c->dens = c->pres = true;
+    c->srec = c->kp.srec != 0;
     return launch_check(c);
 }
 
@@ -587,6 +594,11 @@ int do_forces(SphxCtx* c, bool visc_only) {
         Timer t(c, kDensity);
         launch_eos(c->P(), c->D, c->n, c->kp, c->mixed(), c->stream);
         c->pres = true;
+    }
+    if (stress && !c->srec) {  // B changed since the density and cleaning passes
+        Timer t(c, kMhd);
+        launch_stress_rec(c->P(), c->D, c->n, c->stream);
+        c->srec = true;
     }
     {
         Timer t(c, visc_only ? kViscOnly : kForce);
@@ -708,6 +720,7 @@ int do_induction(SphxCtx* c) {
     swap3(P.bx, P.by, P.bz, c->D.tmp3);
     std::swap(c->D.recb, c->D.recb_next);
     c->acc = false;
+    c->srec = false;
     return launch_check(c);
 }
 
@@ -724,6 +737,7 @@ int do_div_clean(SphxCtx* c) {
     std::swap(P.psi, c->D.tmp1);
     std::swap(c->D.recb, c->D.recb_next);
     c->acc = false;
+    c->srec = c->kp.srec != 0;  // the pass wrote the records of the new B
     return launch_check(c);
 }
 
@@ -998,6 +1012,7 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     for (double** d : dd)
         if (dalloc(c, d, N)) return bail(SPHX_E_OOM);
     if (dalloc(c, &D.r64, N) || dalloc(c, &D.r32, N) || dalloc(c, &D.recb, N) || dalloc(c, &D.recb_next, N) ||
+        dalloc(c, &D.recs, N) ||
         dalloc(c, &D.xref, N))
         return bail(SPHX_E_OOM);
     {
@@ -1608,8 +1623,10 @@ int sphx_load(SphxCtx* c, const void* buf, uint64_t size) {
         g.inv_w[k] = 1.0 / h.w[k];
         g.L[k] = h.L[k];
         g.halfL[k] = 0.5 * h.L[k];
+        g.invL[k] = h.periodic ? 1.0 / h.L[k] : 0.0;
         g.dims[k] = h.dims[k];
-        if (!(h.w[k] > 0.0) || h.dims[k] < 1) return fail(c, SPHX_E_INVALID, "checkpoint grid is invalid");
+        if (!(h.w[k] > 0.0) || h.dims[k] < (h.periodic ? 3 : 1) || (h.periodic && !(h.L[k] > 0.0)))
+            return fail(c, SPHX_E_INVALID, "checkpoint grid is invalid");
     }
     g.periodic = h.periodic;
     g.ncell = (uint32_t)((uint64_t)h.dims[0] * (uint64_t)h.dims[1] * (uint64_t)h.dims[2]);
