@@ -197,6 +197,21 @@ __device__ __forceinline__ void sqrt_rsqrt(double x, double& sq, double& rs) {
     sq = fma(d, h, g);
     rs = 2.0 * h;
 }
+// One Goldschmidt step: sqrt(x) and 1/sqrt(x) to about 2^-44 relative, for
+// the force passes (their results are compared relative to the largest
+// acceleration, 1e-11).
+__device__ __forceinline__ void sqrt_rsqrt1(double x, double& sq, double& rs) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double e = fma(-h, g, 0.5);
+    sq = fma(g, e, g);
+    rs = 2.0 * fma(h, e, h);
+}
+// One Newton step on the hardware reciprocal: about 2^-46 relative.
+__device__ __forceinline__ double rcp_nr1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
 // Cubic-spline shape functions without the normalisation: W = sigma_w * w_shape(q)
 // with sigma_w = 1/(pi h^3); dW/dr = sigma_g * dw_shape(q) with sigma_g = 1/(pi h^4).
 __device__ __forceinline__ double w_shape(double q) {

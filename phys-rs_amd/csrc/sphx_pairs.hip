@@ -663,7 +663,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const bool ok = r2 > 1e-20;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
         double r, inv_r;
-        sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
+        sqrt_rsqrt1(ok ? r2 : 1.0, r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
         // (dW/dr) / r (without sigma_g m_i m_j when uniform); a factor 1 or 0
         // rather than a select, which the compiler would turn back into a
@@ -683,7 +683,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
             const double hs = (ssign * ph.h_avg) * fmin(fmax(sdot, s_lo), s_hi);
             const double den = fma(0.01 * ph.h_avg, ph.h_avg, r2);
             const double num = hs * fma(beta2, hs, ac2 * den);
-            const double pi_ij = num * rcp_nr((den * den) * (rhoi + rhoj));
+            const double pi_ij = num * rcp_nr1((den * den) * (rhoi + rhoj));
             const double sc = pi_ij * dwr;
             fv[0] = fma(dx, sc, fv[0]);
             fv[1] = fma(dy, sc, fv[1]);
