@@ -237,9 +237,11 @@ void launch_vv_velocity(const Particles& P, const Derived& D, uint32_t n, const 
 void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                  const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                      const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
+                      const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
+                      const uint32_t* cell_start, hipStream_t s);
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                      const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s);
+                      const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
+                      const uint32_t* cell_start, hipStream_t s);
 void launch_export_candidates(const Particles& P, uint32_t n, const Grid& g, const uint32_t* cell_start,
                               const double* R, double half_skin, uint32_t* out, unsigned long long* counter,
                               uint64_t out_cap, hipStream_t s);

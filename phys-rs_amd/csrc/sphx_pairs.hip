@@ -650,6 +650,12 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     //   F_i = -(b_i (b_i . gs) - e_i gs + fm).
     double gs[3] = {0, 0, 0};
     const double2* __restrict__ RS = reinterpret_cast<const double2*>(D.recs);
+    // MERGE (full pass with the adiabatic EOS and the stress): the pressure
+    // term's j part joins the stress sum, sum_j (dW/r) (b_j (b_j . d) -
+    // (e_j + P_j/rho_j^2) d), and its i part is P_i/rho_i^2 gs; the stored
+    // pressure sum then holds pressure and stress, which no pass reuses (the
+    // viscosity-only pass is not available with the adiabatic EOS)
+    constexpr bool MERGE = ENERGY && STRESS && !VISC_ONLY;
     // The pair body is branch-free: a pair without a term adds +-0, which
     // leaves a sum unchanged (the sums start at +0 and never become -0).
     // With uniform m and h, m_i m_j sigma_g is folded out of the sums.
@@ -670,7 +676,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         // branch around the arithmetic
         const double dwr0 = (dw_shape(r * ph.inv_h) * inv_r) * (ok ? 1.0 : 0.0);
         const double dwr = UNI ? dwr0 : (mi * P.m[j]) * (ph.sig_g * dwr0);
-        if (!VISC_ONLY) {  // sph.rs:208-221
+        if (!VISC_ONLY && !MERGE) {  // sph.rs:208-221
             const double sc = (pr2i + vzp.y) * dwr;
             fp[0] = fma(dx, sc, fp[0]);
             fp[1] = fma(dy, sc, fp[1]);
@@ -703,9 +709,10 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
             gs[1] = fma(dwr, dy, gs[1]);
             gs[2] = fma(dwr, dz, gs[2]);
             const double bd = fma(sze.x, dz, fma(sxy.y, dy, sxy.x * dx));
-            fm[0] = fma(dwr, fma(sxy.x, bd, -sze.y * dx), fm[0]);
-            fm[1] = fma(dwr, fma(sxy.y, bd, -sze.y * dy), fm[1]);
-            fm[2] = fma(dwr, fma(sze.x, bd, -sze.y * dz), fm[2]);
+            const double ej = MERGE ? sze.y + vzp.y : sze.y;
+            fm[0] = fma(dwr, fma(sxy.x, bd, -ej * dx), fm[0]);
+            fm[1] = fma(dwr, fma(sxy.y, bd, -ej * dy), fm[1]);
+            fm[2] = fma(dwr, fma(sze.x, bd, -ej * dz), fm[2]);
         }
     };
     // The list is summed in two halves, entries 0 .. half - 1 and half .. end
@@ -722,7 +729,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         const uint32_t h = threadIdx.x & 1u;
         sum_range(h ? half : 0u, h ? 0xffffffffu : half);
         for (int a = 0; a < 3; ++a) {  // lane h = 0: first half + second half
-            fp[a] += __shfl_xor(fp[a], 1);
+            if (!MERGE) fp[a] += __shfl_xor(fp[a], 1);
             fv[a] += __shfl_xor(fv[a], 1);
             if (STRESS) {
                 fm[a] += __shfl_xor(fm[a], 1);
@@ -748,6 +755,12 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         de = e0 + de;
     }
     if (STRESS) stress_close(RS, i, gs, fm);
+    if (MERGE) {
+        for (int a = 0; a < 3; ++a) {
+            fp[a] = fma(pr2i, gs[a], fm[a]);
+            fm[a] = 0.0;
+        }
+    }
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
         const double k = (kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi);
@@ -789,7 +802,9 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
     // staged halo skip the staged test: VISC_ONLY 0.322 against 0.315 ms)
     constexpr int W = 2;
     constexpr int NQ2 = STRESS ? 2 : 0;
-    constexpr bool PF = !STRESS;  // the stress terms leave no registers for the list prefetch
+    // the next two list entries loaded ahead (with the stress: 4M MHD box
+    // 1.69 -> 1.64 ms per launch despite a few spilled registers)
+    constexpr bool PF = true;
     brick_pass<double2, 4, W, PF, 2, NQ2>(
         B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<double2, 4, W, PF, NQ2>& f) {
@@ -1061,22 +1076,27 @@ void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g
 // two neighbour records loaded ahead of their bodies, and the halves are added
 // in order.  The pair body is branch-free (a factor 0 for r <= 1e-10, where
 // the reference skips the pair: mhd.rs:100).  RES: the resistive term (eta != 0).
-template <bool UNI, bool RES, int PER>
-__global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                       KernelParams kp) {
-    const uint32_t i = particle_index() >> 1;
-    if (i >= n) return;
+// range(kb, ke, body) walks list entries kb .. ke - 1 of particle i with
+// records of 6 quads (the f64 record, then the B record).
+template <bool UNI, bool RES, int PER, class Range>
+__device__ __forceinline__ void induction_particle(const Particles& P, const Derived& D, const Grid& g,
+                                                   const NbrList& L, const KernelParams& kp, uint32_t i,
+                                                   Range&& range) {
     const uint32_t h = threadIdx.x & 1u;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i];
     const uint32_t idi = P.id[i];
-    const double inv_rhoi = rcp_nr(rhoi);
+    // m_j / (rho_i rho_j) = mi_rho / rho_j (uniform m: mi_rho = m0 / rho_i)
+    const double mi_rho = (UNI ? kp.m0 : 1.0) * rcp_nr(rhoi);
     double ux = 0, uy = 0, uz = 0;
     const uint32_t cnt = L.ntot[i], half = ((cnt + 7u) >> 3) << 2;
-    const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
-                                       reinterpret_cast<const double2*>(D.recb)};
-    walk_ahead<4>(L, i, h ? half : 0u, h ? 0xffffffffu : half, F, [&](uint32_t e, const double2 (&q)[6]) {
+    // per pair, with g = (dW/r) d (d = x_j - x_i) and v = v_i - v_j:
+    //   dB_i += +-dt m_j / (rho_i rho_j) (B_i (v . g) - v (B_i . g))
+    // = f (B_i (v . d) - v (B_i . d)),  f = +-dt (m_j / (rho_i rho_j)) dW/r,
+    // one hardware-estimate refinement step for 1/r and 1/rho_j (the terms
+    // are compared relative to the largest change of B)
+    range(h ? half : 0u, h ? 0xffffffffu : half, [&](uint32_t e, const double2 (&q)[6]) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = q[0], zr = q[1], vxy = q[2], vzp = q[3], bxy = q[4], bzp = q[5];
         double dx, dy, dz;
@@ -1084,28 +1104,28 @@ __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, ui
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const bool ok = r2 > 1e-20;
         double r, inv_r;
-        sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
+        sqrt_rsqrt1(ok ? r2 : 1.0, r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
         const double dwr = ((ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r) * (ok ? 1.0 : 0.0);
-        const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;  // r_hat dW/dr
         const double vx = vxi - vxy.x, vy = vyi - vxy.y, vz = vzi - vzp.x;
-        const double rhoj = zr.y, mj = UNI ? kp.m0 : P.m[j];
-        const double v_dot_g = (vx * gx + vy * gy) + vz * gz;
-        const double b_dot_g = (bxi * gx + byi * gy) + bzi * gz;
-        const double inv_rhoj = rcp_nr(rhoj);
-        const double f = mj * (inv_rhoi * inv_rhoj);
-        const double sg = (idi < P.id[j]) ? kp.dt : -kp.dt;
+        const double rhoj = zr.y;
+        const double v_dot_d = fma(vz, dz, fma(vy, dy, vx * dx));
+        const double b_dot_d = fma(bzi, dz, fma(byi, dy, bxi * dx));
+        const double inv_rhoj = rcp_nr1(rhoj);
+        const double sdt = (idi < P.id[j]) ? kp.dt : -kp.dt;
+        const double f = (sdt * dwr) * ((UNI ? mi_rho : P.m[j] * mi_rho) * inv_rhoj);
         if (RES) {
-            const double r_dot_g = (dx * gx + dy * gy) + dz * gz;
-            const double res =
-                (2.0 * kp.eta * (mj * inv_rhoj)) * r_dot_g * rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2));
-            ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (bxy.x - bxi));
-            uy += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (bxy.y - byi));
-            uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (bzp.x - bzi));
+            const double mj = UNI ? kp.m0 : P.m[j];
+            const double r_dot_g = r2 * dwr;
+            const double res = sdt * ((2.0 * kp.eta * (mj * inv_rhoj)) * r_dot_g *
+                                      rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2)));
+            ux = fma(f, fma(bxi, v_dot_d, -vx * b_dot_d), fma(res, bxy.x - bxi, ux));
+            uy = fma(f, fma(byi, v_dot_d, -vy * b_dot_d), fma(res, bxy.y - byi, uy));
+            uz = fma(f, fma(bzi, v_dot_d, -vz * b_dot_d), fma(res, bzp.x - bzi, uz));
         } else {
-            ux += sg * (f * (bxi * v_dot_g - vx * b_dot_g));
-            uy += sg * (f * (byi * v_dot_g - vy * b_dot_g));
-            uz += sg * (f * (bzi * v_dot_g - vz * b_dot_g));
+            ux = fma(f, fma(bxi, v_dot_d, -vx * b_dot_d), ux);
+            uy = fma(f, fma(byi, v_dot_d, -vy * b_dot_d), uy);
+            uz = fma(f, fma(bzi, v_dot_d, -vz * b_dot_d), uz);
         }
     });
     ux += __shfl_xor(ux, 1);  // lane 0: first half + second half
@@ -1119,6 +1139,43 @@ __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, ui
     double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
     rb[0] = make_double2(bx, by);
     rb[1] = make_double2(bz, P.psi[i]);
+}
+
+template <bool UNI, bool RES, int PER>
+__global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                       KernelParams kp) {
+    const uint32_t i = particle_index() >> 1;
+    if (i >= n) return;
+    const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
+                                       reinterpret_cast<const double2*>(D.recb)};
+    induction_particle<UNI, RES, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
+        walk_ahead<4>(L, i, kb, ke, F, body);
+    });
+}
+
+// Brick-staged: the neighbours' f64 and B records (96 bytes) from LDS, the
+// same halves, order and arithmetic as k_induction (bit-identical); two
+// entries per step with the next two loaded ahead (4M MHD box: 1.48 ->
+// 1.37 ms per launch; unstaged 1.50 ms, bound by the gathers' address path).
+template <bool UNI, bool RES, int PER>
+__global__ void __launch_bounds__(2 * kStagedBlock) k_induction_staged(Particles P, Derived D, Grid g, NbrList L,
+                                                                       KernelParams kp, Bricks B,
+                                                                       const uint32_t* __restrict__ start,
+                                                                       uint32_t cap) {
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    using Fetch = BrickFetch<double2, 4, 2, true, 2>;
+    brick_pass<double2, 4, 2, true, 2, 2>(
+        B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
+        [&](uint32_t i, const Fetch& f) {
+            induction_particle<UNI, RES, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
+                Fetch ff = f;
+                ff.kbeg = kb;
+                ff.kend = ke;
+                walk(L, i, ff, body);
+            });
+        },
+        reinterpret_cast<const double2*>(D.recb));
 }
 
 // FP32-mixed MHD passes: f32 pair terms on the cell-relative f32 record (as
@@ -1213,7 +1270,8 @@ __global__ void __launch_bounds__(kBlock) k_induction_mixed(Particles P, Derived
 // 4.7 ms per step for the pair) than these gathers at three workgroups per CU,
 // and so did staged FP32-mixed forms (64-byte records: 2.87 against 2.57 ms).
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                      const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
+                      const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
+                      const uint32_t* cell_start, hipStream_t s) {
     if (!n) return;
     if (mixed) {
         if (kp.uniform)
@@ -1225,10 +1283,20 @@ void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Gr
         return;
     }
     const dim3 grid2(nblocks(2 * n));  // two lanes per particle
-    const bool res = kp.eta != 0.0;
+    const bool res = kp.eta != 0.0, staged = B.on && B.nactive;
+    StagedLaunch sl{};
+    if (staged) {
+        sl = staged_launch(B, 8, 96);
+        sl.block = dim3(2u * (uint32_t)B.threads);  // two lanes per particle
+    }
 #define SPHX_INDUCTION(U, R, PER)                                                                              \
     if (kp.uniform == (U ? 1 : 0) && res == R && g.periodic == PER) {                                          \
-        hipLaunchKernelGGL((k_induction<U, R, PER>), grid2, dim3(kBlock), occupancy_lds(4), s, P, D, n, g, L, kp); \
+        if (staged)                                                                                           \
+            hipLaunchKernelGGL((k_induction_staged<U, R, PER>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B, \
+                               cell_start, sl.cap);                                                           \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_induction<U, R, PER>), grid2, dim3(kBlock), occupancy_lds(4), s, P, D, n, g, L, \
+                               kp);                                                                           \
         return;                                                                                               \
     }
     SPHX_INDUCTION(true, false, 1)
@@ -1246,20 +1314,21 @@ void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Gr
 // divergence_cleaning_system (mhd.rs:172-259): Jacobi over B and psi, then
 // psi *= exp(-sigma c_h dt / h) for every particle.
 // divergence_cleaning_system (mhd.rs:172-259) in gather form, two lanes per
-// particle as the induction pass (halves added in order, branch-free body).
-template <bool UNI, int PER>
-__global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
-                                                       KernelParams kp) {
-    const uint32_t i = particle_index() >> 1;
-    if (i >= n) return;
+// particle as the induction pass (halves added in order, branch-free body);
+// range as for induction_particle.
+template <bool UNI, int PER, class Range>
+__device__ __forceinline__ void clean_particle(const Particles& P, const Derived& D, const Grid& g,
+                                               const NbrList& L, const KernelParams& kp, uint32_t i,
+                                               Range&& range) {
     const uint32_t h = threadIdx.x & 1u;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i], psii = P.psi[i];
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
     const uint32_t cnt = L.ntot[i], half = ((cnt + 7u) >> 3) << 2;
-    const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
-                                       reinterpret_cast<const double2*>(D.recb)};
-    walk_ahead<4>(L, i, h ? half : 0u, h ? 0xffffffffu : half, F, [&](uint32_t e, const double2 (&q)[6]) {
+    // per pair, with g = (dW/r) d (d = x_j - x_i) and w = (m_j / rho_j) dW/r:
+    //   div B_i += w (B_j - B_i) . d,   grad psi_i += w (psi_j - psi_i) d,
+    // one refinement step for 1/r and 1/rho_j (as the induction pass)
+    range(h ? half : 0u, h ? 0xffffffffu : half, [&](uint32_t e, const double2 (&q)[6]) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = q[0], zr = q[1], bxy = q[4], bzp = q[5];
         double dx, dy, dz;
@@ -1267,16 +1336,15 @@ __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, ui
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const bool ok = r2 > 1e-20;
         double r, inv_r;
-        sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
+        sqrt_rsqrt1(ok ? r2 : 1.0, r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
         const double dwr = ((ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r) * (ok ? 1.0 : 0.0);
-        const double gx = dx * dwr, gy = dy * dwr, gz = dz * dwr;
-        const double vol = (UNI ? kp.m0 : P.m[j]) * rcp_nr(zr.y);
-        div += vol * (((bxy.x - bxi) * gx + (bxy.y - byi) * gy) + (bzp.x - bzi) * gz);
-        const double dp = vol * (bzp.y - psii);
-        gpx = fma(dp, gx, gpx);
-        gpy = fma(dp, gy, gpy);
-        gpz = fma(dp, gz, gpz);
+        const double w = ((UNI ? kp.m0 : P.m[j]) * rcp_nr1(zr.y)) * dwr;
+        div = fma(w, fma(bzp.x - bzi, dz, fma(bxy.y - byi, dy, (bxy.x - bxi) * dx)), div);
+        const double dp = w * (bzp.y - psii);
+        gpx = fma(dp, dx, gpx);
+        gpy = fma(dp, dy, gpy);
+        gpz = fma(dp, dz, gpz);
     });
     div += __shfl_xor(div, 1);
     gpx += __shfl_xor(gpx, 1);
@@ -1296,6 +1364,41 @@ __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, ui
     rb[1] = make_double2(bz, psi);
     if (kp.srec) D.recs[i] = stress_record(bx, by, bz, D.rho[i]);
 }
+
+template <bool UNI, int PER>
+__global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
+                                                       KernelParams kp) {
+    const uint32_t i = particle_index() >> 1;
+    if (i >= n) return;
+    const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
+                                       reinterpret_cast<const double2*>(D.recb)};
+    clean_particle<UNI, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
+        walk_ahead<4>(L, i, kb, ke, F, body);
+    });
+}
+
+// Brick-staged, as k_induction_staged (bit-identical to k_div_clean).
+template <bool UNI, int PER>
+__global__ void __launch_bounds__(2 * kStagedBlock) k_div_clean_staged(Particles P, Derived D, Grid g, NbrList L,
+                                                                       KernelParams kp, Bricks B,
+                                                                       const uint32_t* __restrict__ start,
+                                                                       uint32_t cap) {
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    using Fetch = BrickFetch<double2, 4, 2, true, 2>;
+    brick_pass<double2, 4, 2, true, 2, 2>(
+        B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
+        [&](uint32_t i, const Fetch& f) {
+            clean_particle<UNI, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
+                Fetch ff = f;
+                ff.kbeg = kb;
+                ff.kend = ke;
+                walk(L, i, ff, body);
+            });
+        },
+        reinterpret_cast<const double2*>(D.recb));
+}
+
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
                                                              NbrList L, KernelParams kp) {
@@ -1346,7 +1449,8 @@ __global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived
 }
 
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
-                      const KernelParams& kp, bool mixed, const Mixed& X, hipStream_t s) {
+                      const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
+                      const uint32_t* cell_start, hipStream_t s) {
     if (!n) return;
     if (mixed) {
         if (kp.uniform)
@@ -1358,9 +1462,20 @@ void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Gr
         return;
     }
     const dim3 grid2(nblocks(2 * n));  // two lanes per particle
+    const bool staged = B.on && B.nactive;
+    StagedLaunch sl{};
+    if (staged) {
+        sl = staged_launch(B, 8, 96);
+        sl.block = dim3(2u * (uint32_t)B.threads);  // two lanes per particle
+    }
 #define SPHX_CLEAN(U, PER)                                                                                    \
     if (kp.uniform == (U ? 1 : 0) && g.periodic == PER) {                                                      \
-        hipLaunchKernelGGL((k_div_clean<U, PER>), grid2, dim3(kBlock), occupancy_lds(4), s, P, D, n, g, L, kp);  \
+        if (staged)                                                                                           \
+            hipLaunchKernelGGL((k_div_clean_staged<U, PER>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B,    \
+                               cell_start, sl.cap);                                                           \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_div_clean<U, PER>), grid2, dim3(kBlock), occupancy_lds(4), s, P, D, n, g, L,   \
+                               kp);                                                                           \
         return;                                                                                               \
     }
     SPHX_CLEAN(true, 1)
