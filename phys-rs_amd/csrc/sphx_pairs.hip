@@ -633,12 +633,10 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     // (sph.rs:281-300), formed with one reciprocal:
     // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s.
     const double ac2 = -2.0 * kp.alpha * kp.speed_of_sound, beta2 = 2.0 * kp.beta;
-    const bool ref_conv = kp.visc_conv == SPHX_VISC_REFERENCE;
-    // s = (v_i - v_j).(x_j - x_i) in the REFERENCE convention, its negative in MONAGHAN's
-    const double ssign = ref_conv ? 1.0 : -1.0;
-    // s clamped to [s_lo, s_hi] is 0 when the pair has no viscosity term
-    // (REFERENCE: active for s < 0; MONAGHAN: for s > 0), so hs = 0 and Pi = 0
-    const double s_lo = ref_conv ? -INFINITY : 0.0, s_hi = ref_conv ? 0.0 : INFINITY;
+    // s = (v_i - v_j).(x_j - x_i); the pair has a viscosity term for s < 0
+    // (REFERENCE) or s > 0 (MONAGHAN): hs = h min(sv s, 0) with sv = 1 or -1
+    // is then h s (REFERENCE), -h s (MONAGHAN) or 0, so Pi = 0 needs no select
+    const double sv = kp.visc_conv == SPHX_VISC_REFERENCE ? 1.0 : -1.0;
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     double de = 0.0;  // ENERGY: sum of (P_i/rho_i^2 + Pi/2) s dW/r
     // STRESS (Maxwell stress M = B B - |B|^2 I / 2, SPMHD):
@@ -668,14 +666,15 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const bool ok = r2 > 1e-20;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
-        double r, inv_r;
-        sqrt_rsqrt1(ok ? r2 : 1.0, r, inv_r);
+        double r, inv_2r;
+        sqrt_rhalf1(ok ? r2 : 1.0, r, inv_2r);
+        inv_2r = ok ? inv_2r : 0.0;
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
-        // (dW/dr) / r (without sigma_g m_i m_j when uniform); a factor 1 or 0
-        // rather than a select, which the compiler would turn back into a
-        // branch around the arithmetic
-        const double dwr0 = (dw_shape(r * ph.inv_h) * inv_r) * (ok ? 1.0 : 0.0);
-        const double dwr = UNI ? dwr0 : (mi * P.m[j]) * (ph.sig_g * dwr0);
+        // (dW/dr) / (2 r): every sum below is half its value, exactly, and
+        // the factor 2 is folded into the constants after the walk (uniform m
+        // and h: into m_i m_j sigma_g, otherwise into sigma_g here)
+        const double dwr0 = dw_shape(r * ph.inv_h) * inv_2r;
+        const double dwr = UNI ? dwr0 : (mi * P.m[j]) * ((2.0 * ph.sig_g) * dwr0);
         if (!VISC_ONLY && !MERGE) {  // sph.rs:208-221
             const double sc = (pr2i + vzp.y) * dwr;
             fp[0] = fma(dx, sc, fp[0]);
@@ -686,7 +685,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         const double sdot = fma(vzi - vzp.x, dz, fma(vyi - vxy.y, dy, (vxi - vxy.x) * dx));
         const double rhoj = zr.y;
         {
-            const double hs = (ssign * ph.h_avg) * fmin(fmax(sdot, s_lo), s_hi);
+            const double hs = ph.h_avg * fmin(sv * sdot, 0.0);
             const double den = fma(0.01 * ph.h_avg, ph.h_avg, r2);
             const double num = hs * fma(beta2, hs, ac2 * den);
             const double pi_ij = num * rcp_nr1((den * den) * (rhoi + rhoj));
@@ -763,13 +762,13 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     }
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
-        const double k = (kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi);
+        const double k = 2.0 * ((kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi));  // 2: the half dW/(2r)
         for (int a = 0; a < 3; ++a) {
             fp[a] *= k;
             fv[a] *= k;
             fm[a] *= k;
         }
-        if (ENERGY) D.du[i] = -de * (kp.m0 * ((ih2 * ih2) * kInvPi));
+        if (ENERGY) D.du[i] = -de * (2.0 * (kp.m0 * ((ih2 * ih2) * kInvPi)));
     } else if (ENERGY) {
         D.du[i] = -de / mi;
     }
