@@ -328,7 +328,18 @@ typedef struct SphxTimes {       /* device milliseconds, accumulated since last 
     double mhd_ms;               /* xsph + induction + cleaning                     */
     uint64_t launches[7];        /* launches counted per slot above                 */
 } SphxTimes;
-/* enable != 0: bracket each kernel with HIP events on the ctx stream. */
+/* Bracket kernels with HIP events on the ctx stream: enable = 0 off,
+ * SPHX_TIMING_ALL every kernel, otherwise the slots whose SPHX_TIMING_* bits
+ * are set (each event pair costs a few microseconds of the step, so a
+ * throughput measurement times only the kernel it needs). */
+#define SPHX_TIMING_ALL 1
+#define SPHX_TIMING_SORT (1 << 1)
+#define SPHX_TIMING_NEIGHBOUR (1 << 2)
+#define SPHX_TIMING_DENSITY (1 << 3)
+#define SPHX_TIMING_FORCE (1 << 4)
+#define SPHX_TIMING_VISC_ONLY (1 << 5)
+#define SPHX_TIMING_INTEGRATE (1 << 6)
+#define SPHX_TIMING_MHD (1 << 7)
 int sphx_enable_timing(SphxCtx* ctx, int enable);
 int sphx_kernel_times(SphxCtx* ctx, SphxTimes* out, int reset);
 

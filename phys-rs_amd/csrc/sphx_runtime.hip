@@ -137,8 +137,8 @@ struct SphxCtx {
     uint32_t list_age = 0, last_life = 0;
     uint64_t ticks = 0;                    // sphx_step ticks since the upload (kept by checkpoints)
     int spec_mode = 0;                     // SPHX_SPECULATE: 0 by list lifetime, 1 always, 2 never
-    // timing
-    bool timing = false;
+    // timing: the slots bracketed with events (bit slot + 1, SPHX_TIMING_*)
+    uint32_t timing = 0;
     std::vector<TimedRange> pending;
     std::vector<hipEvent_t> free_events;
     double ms[kNumSlots] = {0};
@@ -490,13 +490,13 @@ struct Timer {
     int slot;
     hipEvent_t a = nullptr;
     Timer(SphxCtx* ctx, int s) : c(ctx), slot(s) {
-        if (c->timing) {
+        if (c->timing & (2u << s)) {
             a = take_event(c);
             if (a) (void)hipEventRecord(a, c->stream);
         }
     }
     ~Timer() {
-        if (!c->timing || !a) return;
+        if (!a) return;
         hipEvent_t b = take_event(c);
         if (!b) return;
         (void)hipEventRecord(b, c->stream);
@@ -1791,7 +1791,9 @@ int sphx_export_candidates(SphxCtx* c, const double* radius, uint32_t* pairs, ui
 
 int sphx_enable_timing(SphxCtx* c, int enable) {
     if (!c) return SPHX_E_INVALID;
-    c->timing = enable != 0;
+    if (enable != 0 && enable != SPHX_TIMING_ALL && (enable & ~0xfe))
+        return fail(c, SPHX_E_INVALID, "unknown timing slots 0x%x", (unsigned)enable);
+    c->timing = enable == SPHX_TIMING_ALL ? 0xfeu : (uint32_t)enable;
     return SPHX_OK;
 }
 

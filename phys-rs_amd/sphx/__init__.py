@@ -21,7 +21,8 @@ from ._abi import (  # noqa: F401  (re-exported constants)
     BND_PENALTY_CUBE, BND_PERIODIC, BND_REFLECT, EOS_ADIABATIC, EOS_ISOTHERMAL, EOS_POLYTROPIC, EOS_TAIT,
     F_ACC, F_ALL, F_B, F_P, F_POS, F_PSI, F_RHO, F_U, F_VEL, FORCES_FULL, FORCES_VISC_ONLY,
     MHD_CLEANING, MHD_INDUCTION, MHD_LORENTZ_CHARGE, MHD_MAXWELL_STRESS, PREC_FP32_MIXED,
-    PREC_FP64, VISC_MONAGHAN, VISC_REFERENCE, SphxParams,
+    PREC_FP64, TIMING_ALL, TIMING_DENSITY, TIMING_FORCE, TIMING_INTEGRATE, TIMING_MHD, TIMING_NEIGHBOUR,
+    TIMING_SORT, TIMING_VISC_ONLY, VISC_MONAGHAN, VISC_REFERENCE, SphxParams,
 )
 
 
@@ -319,8 +320,10 @@ class Context:
                                                      C.byref(count)))
         return out[: count.value]
 
-    def enable_timing(self, on: bool = True) -> None:
-        self._check(self._lib.sphx_enable_timing(self._h, 1 if on else 0))
+    def enable_timing(self, on=True) -> None:
+        """True: every kernel; False: none; an int: the TIMING_* slots to time."""
+        mask = (TIMING_ALL if on else 0) if isinstance(on, bool) else int(on)
+        self._check(self._lib.sphx_enable_timing(self._h, mask))
 
     def kernel_times(self, reset: bool = False) -> Dict[str, object]:
         t = _abi.SphxTimes()

@@ -23,6 +23,9 @@ MHD_INDUCTION, MHD_CLEANING, MHD_MAXWELL_STRESS, MHD_LORENTZ_CHARGE = 1, 2, 4, 8
 PREC_FP64, PREC_FP32_MIXED = 0, 1
 F_POS, F_VEL, F_RHO, F_P, F_ACC, F_B, F_PSI, F_U, F_ALL = 1, 2, 4, 8, 16, 32, 64, 128, 255
 FORCES_FULL, FORCES_VISC_ONLY = 0, 1
+TIMING_ALL = 1
+(TIMING_SORT, TIMING_NEIGHBOUR, TIMING_DENSITY, TIMING_FORCE, TIMING_VISC_ONLY, TIMING_INTEGRATE,
+ TIMING_MHD) = (1 << 1, 1 << 2, 1 << 3, 1 << 4, 1 << 5, 1 << 6, 1 << 7)
 
 D3 = C.c_double * 3
 

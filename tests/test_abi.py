@@ -23,6 +23,19 @@ def test_header_and_binding_agree():
     assert declared_symbols() == sorted(_abi.SIGNATURES)
 
 
+def test_header_constants_and_binding_agree():
+    # every SPHX_<NAME> integer #define of the header is the binding's <NAME>
+    text = (ROOT / "include" / "sphx.h").read_text()
+    found = 0
+    for name, expr in re.findall(r"#define\s+SPHX_([A-Z0-9_]+)\s+(.+)", text):
+        expr = re.sub(r"/\*.*", "", expr).strip()
+        if not re.fullmatch(r"[0-9()<> ]+", expr):
+            continue
+        assert getattr(_abi, name, None) == eval(expr), name
+        found += 1
+    assert found >= 8
+
+
 def test_library_exports_every_declared_symbol():
     lib = C.CDLL(str(_abi.LIB_PATH))
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]

@@ -101,6 +101,23 @@ def test_pairs_dense_periodic_chunked(require_gpu):
     assert rel_err(rho, s["rho"]) < 1e-13
 
 
+def test_timing_selected_slots(require_gpu):
+    # only the chosen kernels are bracketed with events; unknown bits are an error
+    st, p = scenarios.dam_break(nx=16, ny=16, nz=8, v_sigma=0.1, seed=2)
+    with make_ctx(st, p) as ctx:
+        ctx.enable_timing(sphx.TIMING_FORCE | sphx.TIMING_VISC_ONLY)
+        ctx.step(3)
+        t = ctx.kernel_times(reset=True)
+        assert t["force"]["launches"] >= 3 and t["visc_only"]["launches"] >= 1
+        assert all(t[k]["launches"] == 0 for k in ("sort", "neighbour", "density", "integrate", "mhd"))
+        ctx.enable_timing(True)
+        ctx.step(1)
+        t = ctx.kernel_times(reset=True)
+        assert t["density"]["launches"] >= 1 and t["integrate"]["launches"] == 2
+        with pytest.raises(sphx.SphxError):
+            ctx.enable_timing(1 << 9)
+
+
 def test_skin_lists_keep_the_exact_neighbour_set(require_gpu):
     # a large skin keeps one list for many ticks of a moving fluid: at every
     # later state the exported set is still exactly {r < 2 h_avg}, and the
