@@ -223,9 +223,11 @@ __device__ __forceinline__ double rcp_nr1(double x) {
 }
 // Cubic-spline shape functions without the normalisation: W = sigma_w * w_shape(q)
 // with sigma_w = 1/(pi h^3); dW/dr = sigma_g * dw_shape(q) with sigma_g = 1/(pi h^4).
+// For q < 1, 1 - 1.5 q^2 + 0.75 q^3 = 1 - 0.75 q^2 (2 - q), formed from t = 2 - q
+// (no constant to load into a register per pair).
 __device__ __forceinline__ double w_shape(double q) {
     const double t = fmax(2.0 - q, 0.0);
-    const double a = fma(q * q, fma(0.75, q, -1.5), 1.0);
+    const double a = fma((q * q) * t, -0.75, 1.0);
     const double b = 0.25 * ((t * t) * t);
     return q < 1.0 ? a : b;
 }
@@ -233,7 +235,7 @@ __device__ __forceinline__ double w_shape(double q) {
 // these times 0.25 is bitwise the sum of w_shape
 __device__ __forceinline__ double w4_shape(double q) {
     const double t = fmax(2.0 - q, 0.0);
-    const double a = fma(q * q, fma(3.0, q, -6.0), 4.0);
+    const double a = fma((q * q) * t, -3.0, 4.0);
     return q < 1.0 ? a : (t * t) * t;
 }
 __device__ __forceinline__ double dw_shape(double q) {

@@ -371,9 +371,9 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         double dx, dy, dz;
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zh.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        // r2 = 0 (coincident particles) as 1e-300: q ~ 1e-148 gives W(0) exactly
         double r, inv_r;
-        sqrt_rsqrt(r2 > 0.0 ? r2 : 1.0, r, inv_r);
-        r = r2 > 0.0 ? r : 0.0;
+        sqrt_rsqrt(fmax(r2, 1e-300), r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : zh.y);
         if (UNI) {  // m, sigma_w and the 1/4 of w4_shape are folded out of the sum
             acc += w4_shape(r * ph.inv_h);
