@@ -122,6 +122,10 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
         // first slot of [a, e) (ascending x, whole stage staged) with
         // (x_slot - x_i) + off beyond the row's reach: below -rreach (left
         // cell) or at least rreach (right cell)
+        // the row's y and z offsets folded into the owner's coordinates once
+        // (x_j - (x_i - off) differs from (x_j - x_i) + off by a few ulp of
+        // the cell width, far inside the 1e-5 margin of the superset test)
+        const float yo = yf - st.oy, zo = zf - st.oz;
         auto cut = [&](uint32_t a, uint32_t e, float off, bool left) {
             while (a < e) {
                 const uint32_t mid = (a + e) >> 1;
@@ -153,8 +157,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                     for (int u = 0; u < 4; ++u) q[u] = row[k + (uint32_t)u];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        // d_ji = -d_ij exactly in f32, so f32 decisions are symmetric
-                        const float dx = (q[u].x - xf) + ox, dy = (q[u].y - yf) + st.oy, dz = (q[u].z - zf) + st.oz;
+                        const float dx = (q[u].x - xf) + ox, dy = q[u].y - yo, dz = q[u].z - zo;
                         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         float lim = lim_u;
                         if (!UNI) {
@@ -211,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const float dx = fmaf(q[u].w - own, wx, q[u].x - xf);
-                        const float dy = (q[u].y - yf) + st.oy, dz = (q[u].z - zf) + st.oz;
+                        const float dy = q[u].y - yo, dz = q[u].z - zo;
                         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         bool in = r2 < lim;
                         if constexpr (decltype(ex)::value) {
