@@ -1185,15 +1185,15 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_induction_staged(Particles
 // four list entries summed in f32 and added to f64 sums; B, psi and the
 // update stay f64.  dW/dr / r with its 0.75 folded into sigma_g.
 struct MixedPairG {
-    float dx, dy, dz, r2, gx, gy, gz, h_avg;
+    float dx, dy, dz, r2, dwr, h_avg;  // dwr = (dW/dr) / r
 };
-template <bool UNI>
-__device__ __forceinline__ MixedPairG mixed_pair_g(const Mixed& X, float hi, float inv_h0, float h0, float sig0,
-                                                   uint32_t e, const float4& a, float xi, float yi, float zi,
-                                                   float wx, float wy, float wz) {
+template <bool UNI, class Fetch>
+__device__ __forceinline__ MixedPairG mixed_pair_g(const Mixed& X, const Fetch& fetch, float hi, float inv_h0,
+                                                   float h0, float sig0, uint32_t e, const float4& a, float xi,
+                                                   float yi, float zi, float wx, float wy, float wz) {
     MixedPairG p;
     float ox, oy, oz;
-    entry_offset(e, wx, wy, wz, ox, oy, oz);
+    fetch.offset(e, wx, wy, wz, ox, oy, oz);
     p.dx = (a.x - xi) + ox;
     p.dy = (a.y - yi) + oy;
     p.dz = (a.z - zi) + oz;
@@ -1208,19 +1208,27 @@ __device__ __forceinline__ MixedPairG mixed_pair_g(const Mixed& X, float hi, flo
         const float ih2 = inv_h * inv_h;
         sig = (ih2 * ih2) * (float)(0.75 * kInvPi);
     }
-    const float dwr = (sig * dw43_shape_f((p.r2 * inv_r) * inv_h)) * inv_r;
-    p.gx = p.dx * dwr;
-    p.gy = p.dy * dwr;
-    p.gz = p.dz * dwr;
+    p.dwr = (sig * dw43_shape_f((p.r2 * inv_r) * inv_h)) * inv_r;
     return p;
 }
 
-template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_induction_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
-                                                             NbrList L, KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
+// (b_x, b_y, b_z, psi) of a neighbour from quads 2 and 3 of its record (the
+// bits of the f64 B record), in f32
+__device__ __forceinline__ float4 mixed_b_record(const float4& u, const float4& v) {
+    return make_float4((float)__hiloint2double(__float_as_int(u.y), __float_as_int(u.x)),
+                       (float)__hiloint2double(__float_as_int(u.w), __float_as_int(u.z)),
+                       (float)__hiloint2double(__float_as_int(v.y), __float_as_int(v.x)),
+                       (float)__hiloint2double(__float_as_int(v.w), __float_as_int(v.z)));
+}
+
+// FP32-mixed induction of particle i: walk(fetch) gives records of NR quads,
+// the f32 record (x, rho | v, P/rho^2) and, with RES, the B record.  Per pair
+// dB_i += f (B_i (v . d) - v (B_i . d)), f = +-dt m_j / (rho_i rho_j) (dW/r).
+template <bool UNI, bool RES, class Fetch>
+__device__ __forceinline__ void induction_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
+                                                         const Grid& g, const NbrList& L, const KernelParams& kp,
+                                                         uint32_t i, const Fetch& fetch) {
     const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
-    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     const float4 own0 = R[2 * i], own1 = R[2 * i + 1];
     const float xi = own0.x, yi = own0.y, zi = own0.z, rhoi = own0.w;
     const float vxi = own1.x, vyi = own1.y, vzi = own1.z;
@@ -1232,26 +1240,31 @@ __global__ void __launch_bounds__(kBlock) k_induction_mixed(Particles P, Derived
     const float sig0 = (float)((kp.inv_h0 * kp.inv_h0) * (kp.inv_h0 * kp.inv_h0) * (0.75 * kInvPi));
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     const float eta2 = (float)(2.0 * kp.eta), dt = (float)kp.dt;
+    const float mi_rho = (UNI ? m0 : 1.0f) * inv_rhoi;
     double u[3] = {0, 0, 0};
     float gu[3] = {0, 0, 0};
-    walk(L, i, GlobalFetch<float4, 2>{R}, [&](uint32_t e, const float4 (&q)[2]) {
+    walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[Fetch::NR]) {
         const uint32_t j = e & kIdxMask;
-        const MixedPairG p = mixed_pair_g<UNI>(X, hi, inv_h0, h0, sig0, e, q[0], xi, yi, zi, wx, wy, wz);
+        const MixedPairG p = mixed_pair_g<UNI>(X, fetch, hi, inv_h0, h0, sig0, e, q[0], xi, yi, zi, wx, wy, wz);
         const float vx = vxi - q[1].x, vy = vyi - q[1].y, vz = vzi - q[1].z;
-        const float rhoj = q[0].w, mj = UNI ? m0 : X.m[j];
-        const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
-        const float bxj = (float)bxy.x, byj = (float)bxy.y, bzj = (float)bzp.x;
-        const float v_dot_g = (vx * p.gx + vy * p.gy) + vz * p.gz;
-        const float b_dot_g = (bxi * p.gx + byi * p.gy) + bzi * p.gz;
-        const float inv_rhoj = __builtin_amdgcn_rcpf(rhoj);
-        const float f = mj * (inv_rhoi * inv_rhoj);
-        const float r_dot_g = (p.dx * p.gx + p.dy * p.gy) + p.dz * p.gz;
-        const float res = ((eta2 * (mj * inv_rhoj)) * r_dot_g) *
-                          __builtin_amdgcn_rcpf(fmaf(0.01f * p.h_avg, p.h_avg, p.r2));
-        const float sg = (idi < P.id[j]) ? dt : -dt;
-        gu[0] += sg * (f * (bxi * v_dot_g - vx * b_dot_g) + res * (bxj - bxi));
-        gu[1] += sg * (f * (byi * v_dot_g - vy * b_dot_g) + res * (byj - byi));
-        gu[2] += sg * (f * (bzi * v_dot_g - vz * b_dot_g) + res * (bzj - bzi));
+        const float inv_rhoj = __builtin_amdgcn_rcpf(q[0].w);
+        const float v_dot_d = fmaf(vz, p.dz, fmaf(vy, p.dy, vx * p.dx));
+        const float b_dot_d = fmaf(bzi, p.dz, fmaf(byi, p.dy, bxi * p.dx));
+        const float sdt = (idi < P.id[j]) ? dt : -dt;
+        const float f = (sdt * p.dwr) * ((UNI ? mi_rho : X.m[j] * mi_rho) * inv_rhoj);
+        if constexpr (RES) {
+            const float4 bj = mixed_b_record(q[2], q[3]);
+            const float mj = UNI ? m0 : X.m[j];
+            const float res = sdt * (((eta2 * (mj * inv_rhoj)) * (p.r2 * p.dwr)) *
+                                     __builtin_amdgcn_rcpf(fmaf(0.01f * p.h_avg, p.h_avg, p.r2)));
+            gu[0] = fmaf(f, fmaf(bxi, v_dot_d, -vx * b_dot_d), fmaf(res, bj.x - bxi, gu[0]));
+            gu[1] = fmaf(f, fmaf(byi, v_dot_d, -vy * b_dot_d), fmaf(res, bj.y - byi, gu[1]));
+            gu[2] = fmaf(f, fmaf(bzi, v_dot_d, -vz * b_dot_d), fmaf(res, bj.z - bzi, gu[2]));
+        } else {
+            gu[0] = fmaf(f, fmaf(bxi, v_dot_d, -vx * b_dot_d), gu[0]);
+            gu[1] = fmaf(f, fmaf(byi, v_dot_d, -vy * b_dot_d), gu[1]);
+            gu[2] = fmaf(f, fmaf(bzi, v_dot_d, -vz * b_dot_d), gu[2]);
+        }
     }, [&]() {
         for (int a = 0; a < 3; ++a) {
             u[a] += (double)gu[a];
@@ -1267,25 +1280,61 @@ __global__ void __launch_bounds__(kBlock) k_induction_mixed(Particles P, Derived
     rb[1] = make_double2(bz, P.psi[i]);
 }
 
-// Unstaged: brick-staged forms of these two passes (96-byte records, one
-// workgroup per CU) measured slower on the 4M periodic MHD box (5.1 against
-// 4.7 ms per step for the pair) than these gathers at three workgroups per CU,
-// and so did staged FP32-mixed forms (64-byte records: 2.87 against 2.57 ms).
+template <bool UNI, bool RES>
+__global__ void __launch_bounds__(kBlock) k_induction_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
+                                                             NbrList L, KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    induction_mixed_particle<UNI, RES>(
+        P, D, X, g, L, kp, i,
+        GlobalFetch<float4, 2, RES ? 2 : 0>{reinterpret_cast<const float4*>(D.r32),
+                                            reinterpret_cast<const float4*>(D.recb)});
+}
+
+// Brick-staged: the f32 records (and, with RES, the B records) from LDS, four
+// entries a step with the next four loaded ahead (bit-identical to the
+// gather form).
+template <bool UNI, bool RES>
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_induction_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, Bricks B,
+                         const uint32_t* __restrict__ start, uint32_t cap) {
+    constexpr int NQ2 = RES ? 2 : 0;
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    brick_pass<float4, 2, 4, true, 1, NQ2>(
+        B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+        [&](uint32_t i, const BrickFetch<float4, 2, 4, true, NQ2>& f) {
+            induction_mixed_particle<UNI, RES>(P, D, X, g, L, kp, i, f);
+        },
+        reinterpret_cast<const float4*>(D.recb));
+}
+
+// Brick-staged wherever the force passes are; the gather forms below that.
 void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                       const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
                       const uint32_t* cell_start, hipStream_t s) {
     if (!n) return;
+    const bool res = kp.eta != 0.0, staged = B.on && B.nactive;
     if (mixed) {
-        if (kp.uniform)
-            hipLaunchKernelGGL(k_induction_mixed<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, X,
-                               n, g, L, kp);
-        else
-            hipLaunchKernelGGL(k_induction_mixed<false>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, X,
-                               n, g, L, kp);
+#define SPHX_INDUCTION_MIXED(U, R)                                                                             \
+        if (kp.uniform == (U ? 1 : 0) && res == R) {                                                           \
+            if (staged) {                                                                                      \
+                const StagedLaunch sl = staged_launch(B, 16, R ? 64 : 32);                                     \
+                hipLaunchKernelGGL((k_induction_mixed_staged<U, R>), sl.grid, sl.block, sl.dyn, s, P, D, X, g, L, \
+                                   kp, B, cell_start, sl.cap);                                                 \
+            } else {                                                                                           \
+                hipLaunchKernelGGL((k_induction_mixed<U, R>), dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, \
+                                   P, D, X, n, g, L, kp);                                                      \
+            }                                                                                                  \
+            return;                                                                                            \
+        }
+        SPHX_INDUCTION_MIXED(true, false)
+        SPHX_INDUCTION_MIXED(true, true)
+        SPHX_INDUCTION_MIXED(false, false)
+        SPHX_INDUCTION_MIXED(false, true)
+#undef SPHX_INDUCTION_MIXED
         return;
     }
     const dim3 grid2(nblocks(2 * n));  // two lanes per particle
-    const bool res = kp.eta != 0.0, staged = B.on && B.nactive;
     StagedLaunch sl{};
     if (staged) {
         sl = staged_launch(B, 8, 96);
@@ -1401,12 +1450,15 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_div_clean_staged(Particles
         reinterpret_cast<const double2*>(D.recb));
 }
 
-template <bool UNI>
-__global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
-                                                             NbrList L, KernelParams kp) {
-    SPHX_PASS_PROLOGUE();
+// FP32-mixed cleaning of particle i: records of the f32 record (x, rho | v,
+// P/rho^2; the velocity quad is not read) and the B record.  Per pair, with
+// w = (m_j / rho_j) (dW/r): div B += w (B_j - B_i) . d, grad psi += w (psi_j -
+// psi_i) d, f32 per group of four entries, f64 sums.
+template <bool UNI, class Fetch>
+__device__ __forceinline__ void clean_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
+                                                     const Grid& g, const NbrList& L, const KernelParams& kp,
+                                                     uint32_t i, const Fetch& fetch) {
     const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
-    const double2* __restrict__ RB = reinterpret_cast<const double2*>(D.recb);
     const float4 own0 = R[2 * i];
     const float xi = own0.x, yi = own0.y, zi = own0.z;
     const double bxd = P.bx[i], byd = P.by[i], bzd = P.bz[i], psid = P.psi[i];
@@ -1417,17 +1469,16 @@ __global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double div = 0, gp[3] = {0, 0, 0};
     float gdiv = 0, ggp[3] = {0, 0, 0};
-    // the second quad of the record (velocity) is not used: its load is dropped
-    walk(L, i, GlobalFetch<float4, 2>{R}, [&](uint32_t e, const float4 (&q)[2]) {
+    walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[Fetch::NR]) {
         const uint32_t j = e & kIdxMask;
-        const MixedPairG p = mixed_pair_g<UNI>(X, hi, inv_h0, h0, sig0, e, q[0], xi, yi, zi, wx, wy, wz);
-        const float vol = (UNI ? m0 : X.m[j]) * __builtin_amdgcn_rcpf(q[0].w);
-        const double2 bxy = RB[2 * j], bzp = RB[2 * j + 1];
-        gdiv += vol * ((((float)bxy.x - bxi) * p.gx + ((float)bxy.y - byi) * p.gy) + ((float)bzp.x - bzi) * p.gz);
-        const float dp = vol * ((float)bzp.y - psii);
-        ggp[0] = fmaf(dp, p.gx, ggp[0]);
-        ggp[1] = fmaf(dp, p.gy, ggp[1]);
-        ggp[2] = fmaf(dp, p.gz, ggp[2]);
+        const MixedPairG p = mixed_pair_g<UNI>(X, fetch, hi, inv_h0, h0, sig0, e, q[0], xi, yi, zi, wx, wy, wz);
+        const float w = ((UNI ? m0 : X.m[j]) * __builtin_amdgcn_rcpf(q[0].w)) * p.dwr;
+        const float4 bj = mixed_b_record(q[2], q[3]);
+        gdiv = fmaf(w, fmaf(bj.z - bzi, p.dz, fmaf(bj.y - byi, p.dy, (bj.x - bxi) * p.dx)), gdiv);
+        const float dp = w * (bj.w - psii);
+        ggp[0] = fmaf(dp, p.dx, ggp[0]);
+        ggp[1] = fmaf(dp, p.dy, ggp[1]);
+        ggp[2] = fmaf(dp, p.dz, ggp[2]);
     }, [&]() {
         div += (double)gdiv;
         gdiv = 0.0f;
@@ -1450,12 +1501,44 @@ __global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived
     if (kp.srec) D.recs[i] = stress_record(bx, by, bz, D.rho[i]);
 }
 
+template <bool UNI>
+__global__ void __launch_bounds__(kBlock) k_div_clean_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
+                                                             NbrList L, KernelParams kp) {
+    SPHX_PASS_PROLOGUE();
+    clean_mixed_particle<UNI>(P, D, X, g, L, kp, i,
+                              GlobalFetch<float4, 2, 2>{reinterpret_cast<const float4*>(D.r32),
+                                                        reinterpret_cast<const float4*>(D.recb)});
+}
+
+// Brick-staged, as k_induction_mixed_staged.
+template <bool UNI>
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_div_clean_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, Bricks B,
+                         const uint32_t* __restrict__ start, uint32_t cap) {
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    brick_pass<float4, 2, 4, true, 1, 2>(
+        B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+        [&](uint32_t i, const BrickFetch<float4, 2, 4, true, 2>& f) {
+            clean_mixed_particle<UNI>(P, D, X, g, L, kp, i, f);
+        },
+        reinterpret_cast<const float4*>(D.recb));
+}
+
 void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                       const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
                       const uint32_t* cell_start, hipStream_t s) {
     if (!n) return;
+    const bool staged = B.on && B.nactive;
     if (mixed) {
-        if (kp.uniform)
+        const StagedLaunch sl = staged ? staged_launch(B, 16, 64) : StagedLaunch{};
+        if (staged && kp.uniform)
+            hipLaunchKernelGGL(k_div_clean_mixed_staged<true>, sl.grid, sl.block, sl.dyn, s, P, D, X, g, L, kp, B,
+                               cell_start, sl.cap);
+        else if (staged)
+            hipLaunchKernelGGL(k_div_clean_mixed_staged<false>, sl.grid, sl.block, sl.dyn, s, P, D, X, g, L, kp, B,
+                               cell_start, sl.cap);
+        else if (kp.uniform)
             hipLaunchKernelGGL(k_div_clean_mixed<true>, dim3(nblocks(n)), dim3(kBlock), occupancy_lds(3), s, P, D, X,
                                n, g, L, kp);
         else
@@ -1464,7 +1547,6 @@ void launch_div_clean(const Particles& P, const Derived& D, uint32_t n, const Gr
         return;
     }
     const dim3 grid2(nblocks(2 * n));  // two lanes per particle
-    const bool staged = B.on && B.nactive;
     StagedLaunch sl{};
     if (staged) {
         sl = staged_launch(B, 8, 96);
