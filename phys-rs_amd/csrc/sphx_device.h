@@ -244,6 +244,23 @@ __device__ __forceinline__ double dw_shape(double q) {
     const double b = -0.75 * (t * t);
     return q < 1.0 ? a : b;
 }
+// The same shapes from the two one-sided pieces s = max(1 - q, 0) and
+// t = max(1 - q/2, 0), formed from r and nih = -1/h, nih2 = -1/(2h) (no q, no
+// select): for q < 1, (2 - q)^3 / 4 - (1 - q)^3 = 1 - 1.5 q^2 + 0.75 q^3 and
+// 3 (1 - q)^2 - 0.75 (2 - q)^2 = -3 q + 2.25 q^2; for 1 <= q < 2 only the
+// t piece is left, (2 - q)^3 / 4 and -0.75 (2 - q)^2; both are 0 from q = 2 on.
+// w_shape(q) / 2 = t^3 - s^3 / 2
+__device__ __forceinline__ double wh_shape_r(double r, double nih, double nih2) {
+    const double s = fmax(fma(r, nih, 1.0), 0.0);
+    const double t = fmax(fma(r, nih2, 1.0), 0.0);
+    return fma((s * s) * s, -0.5, (t * t) * t);
+}
+// dw_shape(q) / 3 = s^2 - t^2 (0 at r = 0 exactly: s = t = 1)
+__device__ __forceinline__ double dw3_shape_r(double r, double nih, double nih2) {
+    const double s = fmax(fma(r, nih, 1.0), 0.0);
+    const double t = fmax(fma(r, nih2, 1.0), 0.0);
+    return fma(-t, t, s * s);
+}
 __device__ __forceinline__ float w_shape_f(float q) {
     const float t = fmaxf(2.0f - q, 0.0f);
     const float a = fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
@@ -877,6 +894,10 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
 #pragma unroll
                 for (int u = 0; u < W; ++u) c[u] = a[u];
                 load(min(k + W, (cnt - 1u) & ~(uint32_t)(W - 1)), a);
+                // two-entry steps: keep the load at the top of the iteration
+                // (the scheduler may otherwise sink it next to its wait at the
+                // bottom; the four-entry steps measured slower with the barrier)
+                if constexpr (W == 2) __builtin_amdgcn_sched_barrier(0);
             } else {
                 load(k, c);
             }

@@ -369,6 +369,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     double rho = P.m[i] * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
     double acc = 0.0;
+    const double nih0 = -kp.inv_h0, nih20 = -0.5 * kp.inv_h0;
     auto pair = [&](uint32_t e, const double2 (&q)[2]) {
         const double2 xy = q[0], zh = q[1];
         double dx, dy, dz;
@@ -378,10 +379,11 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         double r, inv_r;
         sqrt_rsqrt(fmax(r2, 1e-300), r, inv_r);
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : zh.y);
-        if (UNI) {  // m, sigma_w and the 1/4 of w4_shape are folded out of the sum
-            acc += w4_shape(r * ph.inv_h);
+        if (UNI) {  // m, sigma_w and the factor 2 of wh_shape are folded out of the sum
+            acc += wh_shape_r(r, nih0, nih20);
         } else {
-            acc = fma(P.m[e & kIdxMask], ph.sig_w * w_shape(r * ph.inv_h), acc);
+            const double nih = -ph.inv_h;
+            acc = fma(P.m[e & kIdxMask], (2.0 * ph.sig_w) * wh_shape_r(r, nih, 0.5 * nih), acc);
         }
     };
     const uint32_t half = (((uint32_t)L.ntot[i] + 7u) >> 3) << 2;
@@ -403,7 +405,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         sum_range(half, 0xffffffffu);
         acc = a0 + acc;
     }
-    rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 0.25), acc, rho) : rho + acc;
+    rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 2.0), acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho, kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
@@ -625,7 +627,9 @@ __device__ __forceinline__ void stress_close(const double2* __restrict__ RS, uin
 // ENERGY (adiabatic EOS, full pass only): du_i/dt = sum_j m_j (P_i/rho_i^2 +
 // Pi_ij/2) v_ij . grad_i W_ij, from the same pair terms (v_ij . grad_i W =
 // -s dW/r with s = (v_i - v_j).(x_j - x_i)), summed beside the force.
-template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, int PER, int LANES, class Fetch>
+// B0 (beta = 0): the viscosity term is Pi = cv hs' / (den (rho_i + rho_j))
+// with cv = -2 alpha c (and h, when uniform) folded out of the sum.
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, bool B0, int PER, int LANES, class Fetch>
 __device__ __forceinline__ void force_particle(const Particles& P, const Derived& D, const Grid& g,
                                                const NbrList& L, const KernelParams& kp, int charge, uint32_t i,
                                                const Fetch& fetch) {
@@ -634,8 +638,12 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     const double mi = P.m[i], rhoi = D.rho[i], pr2i = D.pr2[i];
     // Pi = mu (beta mu - alpha c) / rho_bar with mu = h s / den, den = r^2 + 0.01 h^2
     // (sph.rs:281-300), formed with one reciprocal:
-    // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s.
+    // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s,
+    // and for beta = 0, Pi = -2 alpha c hs / (den (rho_i + rho_j)).
     const double ac2 = -2.0 * kp.alpha * kp.speed_of_sound, beta2 = 2.0 * kp.beta;
+    // the factor left out of the viscosity sum (B0), and half of it for the energy sum
+    const double cv = B0 ? (UNI ? ac2 * kp.h0 : ac2) : 1.0, hcv = 0.5 * cv;
+    const double nih0 = -kp.inv_h0, nih20 = -0.5 * kp.inv_h0;
     // s = (v_i - v_j).(x_j - x_i); the pair has a viscosity term for s < 0
     // (REFERENCE) or s > 0 (MONAGHAN): hs = h min(sv s, 0) with sv = 1 or -1
     // is then h s (REFERENCE), -h s (MONAGHAN) or 0, so Pi = 0 needs no select
@@ -668,16 +676,17 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         double dx, dy, dz;
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        const bool ok = r2 > 1e-20;  // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force
+        // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force (the clamp keeps 1/r finite)
         double r, inv_2r;
-        sqrt_rhalf1(ok ? r2 : 1.0, r, inv_2r);
-        inv_2r = ok ? inv_2r : 0.0;
+        sqrt_rhalf1(fmax(r2, 1e-300), r, inv_2r);
+        inv_2r = r2 > 1e-20 ? inv_2r : 0.0;
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
-        // (dW/dr) / (2 r): every sum below is half its value, exactly, and
-        // the factor 2 is folded into the constants after the walk (uniform m
-        // and h: into m_i m_j sigma_g, otherwise into sigma_g here)
-        const double dwr0 = dw_shape(r * ph.inv_h) * inv_2r;
-        const double dwr = UNI ? dwr0 : (mi * P.m[j]) * ((2.0 * ph.sig_g) * dwr0);
+        const double nih = UNI ? nih0 : -ph.inv_h, nih2 = UNI ? nih20 : 0.5 * nih;
+        // (dW/dr) / (6 r): every sum below is a sixth of its value, and the
+        // factor 6 is folded into the constants after the walk (uniform m and
+        // h: into m_i m_j sigma_g, otherwise into sigma_g here)
+        const double dwr0 = dw3_shape_r(r, nih, nih2) * inv_2r;
+        const double dwr = UNI ? dwr0 : (mi * P.m[j]) * ((6.0 * ph.sig_g) * dwr0);
         if (!VISC_ONLY && !MERGE) {  // sph.rs:208-221
             const double sc = (pr2i + vzp.y) * dwr;
             fp[0] = fma(dx, sc, fp[0]);
@@ -688,15 +697,20 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         const double sdot = fma(vzi - vzp.x, dz, fma(vyi - vxy.y, dy, (vxi - vxy.x) * dx));
         const double rhoj = zr.y;
         {
-            const double hs = ph.h_avg * fmin(sv * sdot, 0.0);
+            const double ms = fmin(sv * sdot, 0.0);
             const double den = fma(0.01 * ph.h_avg, ph.h_avg, r2);
-            const double num = hs * fma(beta2, hs, ac2 * den);
-            const double pi_ij = num * rcp_nr1((den * den) * (rhoi + rhoj));
+            double pi_ij;  // Pi / cv
+            if (B0) {
+                pi_ij = (UNI ? ms : ph.h_avg * ms) * rcp_nr1(den * (rhoi + rhoj));
+            } else {
+                const double hs = ph.h_avg * ms;
+                pi_ij = (hs * fma(beta2, hs, ac2 * den)) * rcp_nr1((den * den) * (rhoi + rhoj));
+            }
             const double sc = pi_ij * dwr;
             fv[0] = fma(dx, sc, fv[0]);
             fv[1] = fma(dy, sc, fv[1]);
             fv[2] = fma(dz, sc, fv[2]);
-            if (ENERGY) de = fma(fma(0.5, pi_ij, pr2i), sdot * dwr, de);
+            if (ENERGY) de = fma(fma(hcv, pi_ij, pr2i), sdot * dwr, de);
         }
         if (STRESS) {
             double2 sxy, sze;  // (b_x, b_y), (b_z, e) of j
@@ -765,15 +779,18 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     }
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
-        const double k = 2.0 * ((kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi));  // 2: the half dW/(2r)
+        const double k = 6.0 * ((kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi));  // 6: the sixth dW/(6r)
+        const double kv = B0 ? k * cv : k;
         for (int a = 0; a < 3; ++a) {
             fp[a] *= k;
-            fv[a] *= k;
+            fv[a] *= kv;
             fm[a] *= k;
         }
-        if (ENERGY) D.du[i] = -de * (2.0 * (kp.m0 * ((ih2 * ih2) * kInvPi)));
-    } else if (ENERGY) {
-        D.du[i] = -de / mi;
+        if (ENERGY) D.du[i] = -de * (6.0 * (kp.m0 * ((ih2 * ih2) * kInvPi)));
+    } else {
+        if (B0)
+            for (int a = 0; a < 3; ++a) fv[a] *= cv;
+        if (ENERGY) D.du[i] = -de / mi;
     }
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
@@ -785,8 +802,11 @@ __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint3
                                                     KernelParams kp, int charge) {
     const uint32_t i = particle_index() >> 1;
     if (i >= n) return;
-    force_particle<VISC_ONLY, ENERGY, STRESS, UNI, -1, 2>(P, D, g, L, kp, charge, i,
-                                           GlobalFetch<double2, 4>{reinterpret_cast<const double2*>(D.r64)});
+    const GlobalFetch<double2, 4> f{reinterpret_cast<const double2*>(D.r64)};
+    if (kp.beta == 0.0)
+        force_particle<VISC_ONLY, ENERGY, STRESS, UNI, true, -1, 2>(P, D, g, L, kp, charge, i, f);
+    else
+        force_particle<VISC_ONLY, ENERGY, STRESS, UNI, false, -1, 2>(P, D, g, L, kp, charge, i, f);
 }
 
 // Two lanes per particle (each walks half of its list): twice the waves per CU
@@ -810,14 +830,17 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
     brick_pass<double2, 4, W, PF, 2, NQ2>(
         B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<double2, 4, W, PF, NQ2>& f) {
-            force_particle<VISC_ONLY, ENERGY, STRESS, UNI, PER, 2>(P, D, g, L, kp, charge, i, f);
+            if (kp.beta == 0.0)
+                force_particle<VISC_ONLY, ENERGY, STRESS, UNI, true, PER, 2>(P, D, g, L, kp, charge, i, f);
+            else
+                force_particle<VISC_ONLY, ENERGY, STRESS, UNI, false, PER, 2>(P, D, g, L, kp, charge, i, f);
         },
         reinterpret_cast<const double2*>(D.recs));
 }
 
 // FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue (and the
 // ENERGY sum as in the f64 pass, f32 per group of four entries).
-template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, class Fetch>
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, bool B0, class Fetch>
 __device__ __forceinline__ void force_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
                                                      const Grid& g, const NbrList& L, const KernelParams& kp,
                                                      int charge, uint32_t i, const Fetch& fetch) {
@@ -835,6 +858,12 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     const float ssign = ref_conv ? 1.0f : -1.0f;
     // s clamped to [s_lo, s_hi] (one med3) is 0 for a pair without a viscosity term
     const float s_lo = ref_conv ? -INFINITY : 0.0f, s_hi = ref_conv ? 0.0f : INFINITY;
+    // B0 (beta = 0): Pi = cv s' / (den (rho_i + rho_j)) with cv = -2 alpha c
+    // sv (and h, when uniform) folded out of the sum, as in the f64 pass
+    const double cv = B0 ? (UNI ? (-2.0 * kp.alpha * kp.speed_of_sound) * (double)ssign * kp.h0
+                                : (-2.0 * kp.alpha * kp.speed_of_sound) * (double)ssign)
+                         : 1.0;
+    const float hcv = (float)(0.5 * cv);
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double fp[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fm[3] = {0, 0, 0};
     // f32 sums over each group of four list entries, added to the f64 sums
@@ -879,15 +908,20 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
         const float sdot = fmaf(vzi - b.z, dz, fmaf(vyi - b.y, dy, (vxi - b.x) * dx));
         const float rhoj = a.w;
         {
-            const float hs = (ssign * h_avg) * __builtin_amdgcn_fmed3f(sdot, s_lo, s_hi);
+            const float ms = __builtin_amdgcn_fmed3f(sdot, s_lo, s_hi);
             const float den = fmaf(0.01f * h_avg, h_avg, r2);
-            const float num = hs * fmaf(beta2, hs, ac2 * den);
-            const float pi_ij = num * __builtin_amdgcn_rcpf((den * den) * (rhoi + rhoj));
+            float pi_ij;  // Pi / cv
+            if (B0) {
+                pi_ij = (UNI ? ms : h_avg * ms) * __builtin_amdgcn_rcpf(den * (rhoi + rhoj));
+            } else {
+                const float hs = (ssign * h_avg) * ms;
+                pi_ij = (hs * fmaf(beta2, hs, ac2 * den)) * __builtin_amdgcn_rcpf((den * den) * (rhoi + rhoj));
+            }
             const float sc = pi_ij * dwr;
             gv[0] = fmaf(dx, sc, gv[0]);
             gv[1] = fmaf(dy, sc, gv[1]);
             gv[2] = fmaf(dz, sc, gv[2]);
-            if (ENERGY) ge = fmaf(fmaf(0.5f, pi_ij, pr2i), sdot * dwr, ge);
+            if (ENERGY) ge = fmaf(fmaf(hcv, pi_ij, pr2i), sdot * dwr, ge);
         }
         if (STRESS) {
             // the stress record of j: quads 2 and 3 of a staged record hold
@@ -933,14 +967,17 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
         const double k = ((kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi)) * 0.75;
+        const double kv = B0 ? k * cv : k;
         for (int a = 0; a < 3; ++a) {
             fp[a] *= k;
-            fv[a] *= k;
+            fv[a] *= kv;
             fm[a] *= k;
         }
         if (ENERGY) D.du[i] = -de * ((kp.m0 * ((ih2 * ih2) * kInvPi)) * 0.75);
-    } else if (ENERGY) {
-        D.du[i] = -de / P.m[i];
+    } else {
+        if (B0)
+            for (int a = 0; a < 3; ++a) fv[a] *= cv;
+        if (ENERGY) D.du[i] = -de / P.m[i];
     }
     force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
 }
@@ -949,8 +986,11 @@ template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D, Mixed X, uint32_t n, Grid g,
                                                           NbrList L, KernelParams kp, int charge) {
     SPHX_PASS_PROLOGUE();
-    force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI>(P, D, X, g, L, kp, charge, i,
-                                                 GlobalFetch<float4, 2>{reinterpret_cast<const float4*>(D.r32)});
+    const GlobalFetch<float4, 2> f{reinterpret_cast<const float4*>(D.r32)};
+    if (kp.beta == 0.0)
+        force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, true>(P, D, X, g, L, kp, charge, i, f);
+    else
+        force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, false>(P, D, X, g, L, kp, charge, i, f);
 }
 
 // With STRESS the B records (bx, by, bz, psi) in f64 are staged beside the
@@ -969,7 +1009,10 @@ k_forces_mixed_staged(
     brick_pass<float4, 2, W, PF, 1, NQ2>(
         B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<float4, 2, W, PF, NQ2>& f) {
-            force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI>(P, D, X, g, L, kp, charge, i, f);
+            if (kp.beta == 0.0)
+                force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, true>(P, D, X, g, L, kp, charge, i, f);
+            else
+                force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, false>(P, D, X, g, L, kp, charge, i, f);
         },
         reinterpret_cast<const float4*>(D.recs));
 }
