@@ -565,6 +565,16 @@ __device__ __forceinline__ void entry_offset(uint32_t e, float wx, float wy, flo
 // Sorted particle index of the calling thread in a one-thread-per-particle pass.
 __device__ __forceinline__ uint32_t particle_index() { return xcd_block() * kBlock + threadIdx.x; }
 
+// Two lanes per particle: lanes l and l + 32 of a wave share one particle,
+// lane l < 32 walking the first half of its list and lane l + 32 the second,
+// so that each half-wave walks 32 consecutive particles (their list groups
+// are contiguous).  pair_slot(t): the particle slot of thread t (t / 2 of
+// 32 slots per wave); list_half(): 0 or 1; other_half(v): the value of the
+// lane that shares the particle.
+__device__ __forceinline__ uint32_t pair_slot(uint32_t t) { return ((t >> 6) << 5) | (t & 31u); }
+__device__ __forceinline__ uint32_t list_half() { return (threadIdx.x >> 5) & 1u; }
+__device__ __forceinline__ double other_half(double v) { return __shfl_xor(v, 32); }
+
 // Gather record of list entry e: NQ 16-byte quads of type Q at record j of
 // glob, then (NQ2 > 0) NQ2 quads at record j of glob2.
 template <class Q, int NQ, int NQ2 = 0>
@@ -759,29 +769,30 @@ static constexpr int kRowCodes = 27;
 // k of staged record r at dst[k * cap + r], so that one quad of 64 random
 // records spreads over all LDS banks (record-major, a 64-byte record stride
 // would put quad k of every record in the same 16 of the 64 banks).  Wave w
-// copies halo rows w, w + waves, ..., each one contiguous range.  The
-// threads also fill the row-code table rc of the `no` brick rows.  With NQ2
-// > 0 a second array of NQ2 quads per record (src2) is staged as quads NQ ..
-// NQ + NQ2 - 1 of the same records.
+// copies halo rows w, w + waves, ..., each one contiguous range, eight quads
+// per lane in flight: the eight loads (past the row's end: its last quad
+// again, stored again to its own slot) and then the eight stores, with no
+// branch between them (a branch would let the compiler sink each load to
+// its store).
 template <int NQ>
 __device__ __forceinline__ void brick_copy(const BrickHdr& H, int nh, const uint4* __restrict__ src,
                                            uint4* __restrict__ dst, uint32_t cap) {
+    const SPHX_LDS BrickHdr* HL = (const SPHX_LDS BrickHdr*)&H;
     const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
     for (int r = w; r < nh; r += nw) {
-        const uint4 h = H.halo[r];
-        const uint32_t nq = h.y * NQ;
-        const uint4* __restrict__ s = src + (size_t)h.x * NQ;
-        uint4* __restrict__ d = dst + h.z;
-        uint32_t q = lane;
-        for (; q + 192u < nq; q += 256u) {
-            const uint4 a = s[q], b = s[q + 64u], c = s[q + 128u], e = s[q + 192u];
-            d[(q % NQ) * cap + q / NQ] = a;
-            d[((q + 64u) % NQ) * cap + (q + 64u) / NQ] = b;
-            d[((q + 128u) % NQ) * cap + (q + 128u) / NQ] = c;
-            d[((q + 192u) % NQ) * cap + (q + 192u) / NQ] = e;
+        const uint32_t gs = HL->halo[r].x, nq = HL->halo[r].y * (uint32_t)NQ, base = HL->halo[r].z;
+        const uint4* __restrict__ s = src + (size_t)gs * NQ;
+        for (uint32_t q0 = lane; q0 < nq; q0 += 512u) {
+            uint32_t qq[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) qq[u] = min(q0 + 64u * (uint32_t)u, nq - 1u);
+            uint4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = s[qq[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) dst[(qq[u] % NQ) * cap + base + qq[u] / NQ] = v[u];
         }
-        for (; q < nq; q += 64u) d[(q % NQ) * cap + q / NQ] = s[q];
     }
 }
 template <int NQ, int NQ2 = 0>
@@ -968,7 +979,7 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
                                                    (uint32_t)__builtin_amdgcn_readfirstlane(HL->whole)};
     const uint32_t nown = HL->own_cum[no];
     uint32_t k = 0;
-    for (uint32_t u = threadIdx.x / LANES; u < nown; u += blockDim.x / LANES) {
+    for (uint32_t u = LANES == 2 ? pair_slot(threadIdx.x) : threadIdx.x; u < nown; u += blockDim.x / LANES) {
         while (u >= HL->own_cum[k + 1]) ++k;
         BrickFetch<Q, NQ, W, PF, NQ2> f = base_fetch;
         f.rc += k * (uint32_t)kRowCodes;

@@ -394,9 +394,9 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         walk(L, i, f, pair);
     };
     if (LANES == 2) {
-        const uint32_t h = threadIdx.x & 1u;
+        const uint32_t h = list_half();
         sum_range(h ? half : 0u, h ? 0xffffffffu : half);
-        acc += __shfl_xor(acc, 1);  // lane 0: first half + second half
+        acc += other_half(acc);  // lane 0: first half + second half
         if (h) return;
     } else {
         sum_range(0u, half);
@@ -422,7 +422,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                      KernelParams kp) {
-    const uint32_t i = particle_index() >> 1;  // two lanes per particle
+    const uint32_t i = pair_slot(particle_index());  // two lanes per particle
     if (i >= n) return;
     density_particle<UNI, -1, 2>(P, D, g, L, kp, i,
                                  GlobalFetch<double2, 2>{reinterpret_cast<const double2*>(P.pos)});
@@ -742,17 +742,17 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         walk(L, i, f, pair);
     };
     if (LANES == 2) {
-        const uint32_t h = threadIdx.x & 1u;
+        const uint32_t h = list_half();
         sum_range(h ? half : 0u, h ? 0xffffffffu : half);
         for (int a = 0; a < 3; ++a) {  // lane h = 0: first half + second half
-            if (!MERGE) fp[a] += __shfl_xor(fp[a], 1);
-            fv[a] += __shfl_xor(fv[a], 1);
+            if (!MERGE) fp[a] += other_half(fp[a]);
+            fv[a] += other_half(fv[a]);
             if (STRESS) {
-                fm[a] += __shfl_xor(fm[a], 1);
-                gs[a] += __shfl_xor(gs[a], 1);
+                fm[a] += other_half(fm[a]);
+                gs[a] += other_half(gs[a]);
             }
         }
-        if (ENERGY) de += __shfl_xor(de, 1);
+        if (ENERGY) de += other_half(de);
         if (h) return;
     } else {
         sum_range(0u, half);
@@ -800,7 +800,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
 template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI>
 __global__ void __launch_bounds__(kBlock) k_forces(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                     KernelParams kp, int charge) {
-    const uint32_t i = particle_index() >> 1;
+    const uint32_t i = pair_slot(particle_index());
     if (i >= n) return;
     const GlobalFetch<double2, 4> f{reinterpret_cast<const double2*>(D.r64)};
     if (kp.beta == 0.0)
@@ -1127,7 +1127,7 @@ template <bool UNI, bool RES, int PER, class Range>
 __device__ __forceinline__ void induction_particle(const Particles& P, const Derived& D, const Grid& g,
                                                    const NbrList& L, const KernelParams& kp, uint32_t i,
                                                    Range&& range) {
-    const uint32_t h = threadIdx.x & 1u;
+    const uint32_t h = list_half();
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i];
@@ -1173,9 +1173,9 @@ __device__ __forceinline__ void induction_particle(const Particles& P, const Der
             uz = fma(f, fma(bzi, v_dot_d, -vz * b_dot_d), uz);
         }
     });
-    ux += __shfl_xor(ux, 1);  // lane 0: first half + second half
-    uy += __shfl_xor(uy, 1);
-    uz += __shfl_xor(uz, 1);
+    ux += other_half(ux);  // lane 0: first half + second half
+    uy += other_half(uy);
+    uz += other_half(uz);
     if (h) return;
     const double bx = bxi + ux, by = byi + uy, bz = bzi + uz;
     D.tmp3[0][i] = bx;
@@ -1189,7 +1189,7 @@ __device__ __forceinline__ void induction_particle(const Particles& P, const Der
 template <bool UNI, bool RES, int PER>
 __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                        KernelParams kp) {
-    const uint32_t i = particle_index() >> 1;
+    const uint32_t i = pair_slot(particle_index());
     if (i >= n) return;
     const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
                                        reinterpret_cast<const double2*>(D.recb)};
@@ -1414,7 +1414,7 @@ template <bool UNI, int PER, class Range>
 __device__ __forceinline__ void clean_particle(const Particles& P, const Derived& D, const Grid& g,
                                                const NbrList& L, const KernelParams& kp, uint32_t i,
                                                Range&& range) {
-    const uint32_t h = threadIdx.x & 1u;
+    const uint32_t h = list_half();
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i], psii = P.psi[i];
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
@@ -1440,10 +1440,10 @@ __device__ __forceinline__ void clean_particle(const Particles& P, const Derived
         gpy = fma(dp, dy, gpy);
         gpz = fma(dp, dz, gpz);
     });
-    div += __shfl_xor(div, 1);
-    gpx += __shfl_xor(gpx, 1);
-    gpy += __shfl_xor(gpy, 1);
-    gpz += __shfl_xor(gpz, 1);
+    div += other_half(div);
+    gpx += other_half(gpx);
+    gpy += other_half(gpy);
+    gpz += other_half(gpz);
     if (h) return;
     const double dt = kp.dt;
     const double bx = bxi - gpx * dt, by = byi - gpy * dt, bz = bzi - gpz * dt;
@@ -1462,7 +1462,7 @@ __device__ __forceinline__ void clean_particle(const Particles& P, const Derived
 template <bool UNI, int PER>
 __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, uint32_t n, Grid g, NbrList L,
                                                        KernelParams kp) {
-    const uint32_t i = particle_index() >> 1;
+    const uint32_t i = pair_slot(particle_index());
     if (i >= n) return;
     const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
                                        reinterpret_cast<const double2*>(D.recb)};
