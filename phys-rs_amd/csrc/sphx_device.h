@@ -575,6 +575,9 @@ __device__ __forceinline__ uint32_t pair_slot(uint32_t t) { return ((t >> 6) << 
 __device__ __forceinline__ uint32_t list_half() { return (threadIdx.x >> 5) & 1u; }
 __device__ __forceinline__ double other_half(double v) { return __shfl_xor(v, 32); }
 
+// The entry code of a particle's own cell: stage 4 (oy = oz = 0), x offset 0.
+static constexpr uint32_t kOwnCode = 13u << kIdxBits;
+
 // Gather record of list entry e: NQ 16-byte quads of type Q at record j of
 // glob, then (NQ2 > 0) NQ2 quads at record j of glob2.
 template <class Q, int NQ, int NQ2 = 0>
@@ -598,6 +601,8 @@ struct GlobalFetch {
             for (int k = 0; k < NQ2; ++k) q[NQ + k] = p2[k];
         }
     }
+    // the record of the walking particle i itself
+    __device__ __forceinline__ void own(uint32_t i, Q (&q)[NR]) const { (*this)(i | kOwnCode, q); }
 };
 
 // body(e, record) for every list entry of sorted particle i, records from
@@ -626,7 +631,7 @@ template <int W, class Q, int NQ, int NQ2, class Body, class Flush = NoFlush>
 __device__ __forceinline__ void walk_ahead(const NbrList& L, uint32_t i, uint32_t kb, uint32_t ke,
                                            const GlobalFetch<Q, NQ, NQ2>& F, Body&& body, Flush&& flush = Flush()) {
     static_assert(4 % W == 0, "W divides the group of four");
-    constexpr uint32_t kOwnCell = 13u << kIdxBits;  // stage 4 (oy = oz = 0), x offset 0
+    constexpr uint32_t kOwnCell = kOwnCode;
     const uint32_t* __restrict__ ep = nbr_base(L, i);
     const uint32_t cnt = min((uint32_t)L.ntot[i], ke);
     for (uint32_t k = kb; k < cnt; k += 4) {
@@ -846,6 +851,9 @@ struct BrickFetch {
         for (int k = 0; k < NR; ++k)
             q[k] = lds_quad<Q>((const SPHX_LDS lds_quad_t*)(size_t)(addr + (uint32_t)k * qstride));
     }
+    // the record of the walking particle i itself (from its own cell's halo
+    // row: LDS when staged; the same bits as the global record)
+    __device__ __forceinline__ void own(uint32_t i, Q (&q)[NR]) const { (*this)(i | kOwnCode, q); }
     __device__ __forceinline__ void operator()(uint32_t e, Q (&q)[NR]) const {
         uint32_t addr;
         if (locate(e, addr)) {

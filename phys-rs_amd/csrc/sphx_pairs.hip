@@ -366,7 +366,9 @@ template <bool UNI, int PER, int LANES, class Fetch>
 __device__ __forceinline__ void density_particle(const Particles& P, const Derived& D, const Grid& g,
                                                  const NbrList& L, const KernelParams& kp, uint32_t i,
                                                  const Fetch& fetch) {
-    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
+    double2 own[2];  // the particle's own position record (x, y | z, h)
+    fetch.own(i, own);
+    const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = own[1].y;
     double rho = P.m[i] * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
     double acc = 0.0;
     const double nih0 = -kp.inv_h0, nih20 = -0.5 * kp.inv_h0;
@@ -450,7 +452,9 @@ template <bool UNI, class Fetch>
 __device__ __forceinline__ void density_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
                                                        const Grid& g, const NbrList& L, const KernelParams& kp,
                                                        uint32_t i, const Fetch& fetch) {
-    const float4 own = X.p4[i];
+    float4 ownr[1];  // the particle's own position record
+    fetch.own(i, ownr);
+    const float4 own = ownr[0];
     const float xi = own.x, yi = own.y, zi = own.z, hi = own.w;
     const double hd = P.h[i];
     double rho = P.m[i] * (1.0 / (kPi * ((hd * hd) * hd)));
@@ -633,9 +637,12 @@ template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, bool B0, int PER, 
 __device__ __forceinline__ void force_particle(const Particles& P, const Derived& D, const Grid& g,
                                                const NbrList& L, const KernelParams& kp, int charge, uint32_t i,
                                                const Fetch& fetch) {
-    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
-    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
-    const double mi = P.m[i], rhoi = D.rho[i], pr2i = D.pr2[i];
+    // the particle's own gather record (x, y | z, rho | vx, vy | vz, P/rho^2)
+    double2 own[Fetch::NR];
+    fetch.own(i, own);
+    const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = UNI ? kp.h0 : P.h[i];
+    const double vxi = own[2].x, vyi = own[2].y, vzi = own[3].x;
+    const double mi = UNI ? kp.m0 : P.m[i], rhoi = own[1].y, pr2i = own[3].y;
     // Pi = mu (beta mu - alpha c) / rho_bar with mu = h s / den, den = r^2 + 0.01 h^2
     // (sph.rs:281-300), formed with one reciprocal:
     // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s,
@@ -844,9 +851,10 @@ template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, bool B0, class Fet
 __device__ __forceinline__ void force_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
                                                      const Grid& g, const NbrList& L, const KernelParams& kp,
                                                      int charge, uint32_t i, const Fetch& fetch) {
-    const float4* __restrict__ R = reinterpret_cast<const float4*>(D.r32);
     const double2* __restrict__ RS = reinterpret_cast<const double2*>(D.recs);
-    const float4 own0 = R[2 * i], own1 = R[2 * i + 1];
+    float4 own[Fetch::NR];  // the particle's own gather record
+    fetch.own(i, own);
+    const float4 own0 = own[0], own1 = own[1];
     const float xi = own0.x, yi = own0.y, zi = own0.z, rhoi = own0.w;
     const float vxi = own1.x, vyi = own1.y, vzi = own1.z, pr2i = own1.w;
     const float hi = X.p4[i].w, mi = UNI ? 0.0f : X.m[i];
