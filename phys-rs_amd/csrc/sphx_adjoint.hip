@@ -135,7 +135,8 @@ void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const V
     if (n) hipLaunchKernelGGL(k_adj_loss, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, target, xb, part);
 }
 
-// Two lanes per particle (the VJP passes): lane h of the pair sums list
+// Two lanes per particle (the VJP passes: lanes l and l + 32 of a wave, as
+// in the force passes): lane h of the pair sums list
 // entries [0, half) (h = 0) or [half, end) (h = 1), half a multiple of 4 near
 // the middle, and lane 0 adds the second half to the first.  At the sizes the
 // adjoint runs (tens of thousands of particles, one thread per particle is a
@@ -143,7 +144,7 @@ void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const V
 __device__ __forceinline__ uint32_t list_half(const NbrList& L, uint32_t i) {
     return (((uint32_t)L.ntot[i] + 7u) >> 3) << 2;
 }
-__device__ __forceinline__ double pair_total(double v) { return v + __shfl_xor(v, 1); }
+__device__ __forceinline__ double pair_total(double v) { return v + other_half(v); }
 
 uint32_t vjp_blocks(uint32_t n) { return nblocks(2 * n); }
 
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
                                                         Vec3 vb, double* __restrict__ rhob,
                                                         double* __restrict__ alpha_acc, int overwrite) {
     __shared__ double lds[kBlock];
-    const uint32_t lane = particle_index(), i = lane >> 1, h = lane & 1u;
+    const uint32_t i = pair_slot(particle_index()), h = lane_half();
     double da = 0.0;
     if (i < n) {
         const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i];
@@ -262,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n, Grid g, NbrList L,
                                                          KernelParams kp, const double* __restrict__ rhob, Vec3 xb) {
-    const uint32_t lane = particle_index(), i = lane >> 1, h = lane & 1u;
+    const uint32_t i = pair_slot(particle_index()), h = lane_half();
     if (i >= n) return;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i], rbi = rhob[i];
     const double2* __restrict__ pos = reinterpret_cast<const double2*>(P.pos);

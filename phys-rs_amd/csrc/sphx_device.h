@@ -569,10 +569,10 @@ __device__ __forceinline__ uint32_t particle_index() { return xcd_block() * kBlo
 // lane l < 32 walking the first half of its list and lane l + 32 the second,
 // so that each half-wave walks 32 consecutive particles (their list groups
 // are contiguous).  pair_slot(t): the particle slot of thread t (t / 2 of
-// 32 slots per wave); list_half(): 0 or 1; other_half(v): the value of the
+// 32 slots per wave); lane_half(): 0 or 1; other_half(v): the value of the
 // lane that shares the particle.
 __device__ __forceinline__ uint32_t pair_slot(uint32_t t) { return ((t >> 6) << 5) | (t & 31u); }
-__device__ __forceinline__ uint32_t list_half() { return (threadIdx.x >> 5) & 1u; }
+__device__ __forceinline__ uint32_t lane_half() { return (threadIdx.x >> 5) & 1u; }
 __device__ __forceinline__ double other_half(double v) { return __shfl_xor(v, 32); }
 
 // The entry code of a particle's own cell: stage 4 (oy = oz = 0), x offset 0.

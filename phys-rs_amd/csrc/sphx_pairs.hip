@@ -396,7 +396,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         walk(L, i, f, pair);
     };
     if (LANES == 2) {
-        const uint32_t h = list_half();
+        const uint32_t h = lane_half();
         sum_range(h ? half : 0u, h ? 0xffffffffu : half);
         acc += other_half(acc);  // lane 0: first half + second half
         if (h) return;
@@ -683,10 +683,12 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         double dx, dy, dz;
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force (the clamp keeps 1/r finite)
+        // r <= 1e-10: dW/dr = 0 (sph.rs:52), no force.  The clamp keeps 1/r
+        // finite; below the cut r keeps only its low word, a denormal below
+        // 2^-1042, so that s = t = 1 exactly and the shape is exactly 0.
         double r, inv_2r;
         sqrt_rhalf1(fmax(r2, 1e-300), r, inv_2r);
-        inv_2r = r2 > 1e-20 ? inv_2r : 0.0;
+        r = __hiloint2double(r2 > 1e-20 ? __double2hiint(r) : 0, __double2loint(r));
         const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
         const double nih = UNI ? nih0 : -ph.inv_h, nih2 = UNI ? nih20 : 0.5 * nih;
         // (dW/dr) / (6 r): every sum below is a sixth of its value, and the
@@ -749,7 +751,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
         walk(L, i, f, pair);
     };
     if (LANES == 2) {
-        const uint32_t h = list_half();
+        const uint32_t h = lane_half();
         sum_range(h ? half : 0u, h ? 0xffffffffu : half);
         for (int a = 0; a < 3; ++a) {  // lane h = 0: first half + second half
             if (!MERGE) fp[a] += other_half(fp[a]);
@@ -1135,7 +1137,7 @@ template <bool UNI, bool RES, int PER, class Range>
 __device__ __forceinline__ void induction_particle(const Particles& P, const Derived& D, const Grid& g,
                                                    const NbrList& L, const KernelParams& kp, uint32_t i,
                                                    Range&& range) {
-    const uint32_t h = list_half();
+    const uint32_t h = lane_half();
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i];
@@ -1422,7 +1424,7 @@ template <bool UNI, int PER, class Range>
 __device__ __forceinline__ void clean_particle(const Particles& P, const Derived& D, const Grid& g,
                                                const NbrList& L, const KernelParams& kp, uint32_t i,
                                                Range&& range) {
-    const uint32_t h = list_half();
+    const uint32_t h = lane_half();
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
     const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i], psii = P.psi[i];
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
