@@ -509,11 +509,13 @@ __device__ __forceinline__ void for_neighbours(const NbrList& L, uint32_t i, F&&
 // f(e) for every entry, and flush() after each group of four consecutive
 // entries (0-3, 4-7, ...) and after the last one: the FP32-mixed passes sum a
 // group in f32 and add it to their f64 sums at the flush.
+// Entries kbeg .. kend - 1 only (kbeg a multiple of 4).
 template <typename F, typename G>
-__device__ __forceinline__ void for_neighbour_groups(const NbrList& L, uint32_t i, F&& f, G&& flush) {
+__device__ __forceinline__ void for_neighbour_groups(const NbrList& L, uint32_t i, F&& f, G&& flush,
+                                                     uint32_t kbeg = 0, uint32_t kend = 0xffffffffu) {
     const uint32_t* __restrict__ e = nbr_base(L, i);
-    const uint32_t cnt = L.ntot[i];
-    uint32_t k = 0;
+    const uint32_t cnt = min((uint32_t)L.ntot[i], kend);
+    uint32_t k = kbeg;
     for (; k + 3 < cnt; k += 4) {
         const uint4 q = nbr_group(e, k);
         f(q.x);
@@ -618,7 +620,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const GlobalF
     if (std::is_same<typename std::decay<Flush>::type, NoFlush>::value)
         for_neighbours(L, i, one, F.kbeg, F.kend);
     else
-        for_neighbour_groups(L, i, one, flush);
+        for_neighbour_groups(L, i, one, flush, F.kbeg, F.kend);
 }
 
 // body(e, record) for the entries kb .. ke - 1 of sorted particle i (kb a

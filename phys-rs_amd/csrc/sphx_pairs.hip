@@ -849,7 +849,10 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
 
 // FP32-mixed force pass: f32 pair terms, f64 sums, same epilogue (and the
 // ENERGY sum as in the f64 pass, f32 per group of four entries).
-template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, bool B0, class Fetch>
+// With STRESS the list is summed in two halves, entries 0 .. half - 1 and
+// half .. end (half a multiple of 4 near the middle), added in that order:
+// with LANES = 2 the lanes l and l + 32 of a wave take one half each.
+template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, bool B0, int LANES, class Fetch>
 __device__ __forceinline__ void force_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
                                                      const Grid& g, const NbrList& L, const KernelParams& kp,
                                                      int charge, uint32_t i, const Fetch& fetch) {
@@ -886,7 +889,7 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     double gs[3] = {0, 0, 0};
     float gm[3] = {0, 0, 0}, gg[3] = {0, 0, 0};
     // branch-free pair body, as in the f64 pass (factors 1 or 0, not selects)
-    walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[Fetch::NR]) {
+    auto pair = [&](uint32_t e, const float4 (&q)[Fetch::NR]) {
         const uint32_t j = e & kIdxMask;
         const float4 a = q[0], b = q[1];
         float ox, oy, oz;
@@ -957,7 +960,8 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             gm[1] = fmaf(dwr, fmaf(by, bd, -ej * dy), gm[1]);
             gm[2] = fmaf(dwr, fmaf(bz, bd, -ej * dz), gm[2]);
         }
-    }, [&]() {
+    };
+    auto flush = [&]() {
         for (int a = 0; a < 3; ++a) {
             if (!VISC_ONLY) fp[a] += (double)gp[a];
             fv[a] += (double)gv[a];
@@ -972,7 +976,45 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             de += (double)ge;
             ge = 0.0f;
         }
-    });
+    };
+    auto sum_range = [&](uint32_t kb, uint32_t ke) {
+        Fetch f = fetch;
+        f.kbeg = kb;
+        f.kend = ke;
+        walk(L, i, f, pair, flush);
+    };
+    if (!STRESS) {
+        walk(L, i, fetch, pair, flush);
+    } else {
+        const uint32_t cnt = L.ntot[i], half = ((cnt + 7u) >> 3) << 2;
+        if (LANES == 2) {
+            const uint32_t h = lane_half();
+            sum_range(h ? half : 0u, h ? 0xffffffffu : half);
+            for (int a = 0; a < 3; ++a) {  // lane half 0: first half + second half
+                if (!VISC_ONLY) fp[a] += other_half(fp[a]);
+                fv[a] += other_half(fv[a]);
+                fm[a] += other_half(fm[a]);
+                gs[a] += other_half(gs[a]);
+            }
+            if (ENERGY) de += other_half(de);
+            if (h) return;
+        } else {
+            sum_range(0u, half);
+            const double p0[3] = {fp[0], fp[1], fp[2]}, v0[3] = {fv[0], fv[1], fv[2]};
+            const double m0[3] = {fm[0], fm[1], fm[2]}, g0[3] = {gs[0], gs[1], gs[2]};
+            const double e0 = de;
+            for (int a = 0; a < 3; ++a) fp[a] = fv[a] = fm[a] = gs[a] = 0.0;
+            de = 0.0;
+            sum_range(half, 0xffffffffu);
+            for (int a = 0; a < 3; ++a) {
+                fp[a] = p0[a] + fp[a];
+                fv[a] = v0[a] + fv[a];
+                fm[a] = m0[a] + fm[a];
+                gs[a] = g0[a] + gs[a];
+            }
+            de = e0 + de;
+        }
+    }
     if (STRESS) stress_close(RS, i, gs, fm);
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
@@ -998,31 +1040,47 @@ __global__ void __launch_bounds__(kBlock) k_forces_mixed(Particles P, Derived D,
     SPHX_PASS_PROLOGUE();
     const GlobalFetch<float4, 2> f{reinterpret_cast<const float4*>(D.r32)};
     if (kp.beta == 0.0)
-        force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, true>(P, D, X, g, L, kp, charge, i, f);
+        force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, true, 1>(P, D, X, g, L, kp, charge, i, f);
     else
-        force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, false>(P, D, X, g, L, kp, charge, i, f);
+        force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, false, 1>(P, D, X, g, L, kp, charge, i, f);
 }
 
-// With STRESS the B records (bx, by, bz, psi) in f64 are staged beside the
-// f32 records as quads 2 and 3 (64 bytes a record); its halo fills one CU's
-// LDS, so the registers are sized for one workgroup per CU (two waves per
-// SIMD), which keeps the sixteen record quads of a four-entry step out of
-// scratch memory.
-template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI, int W, bool PF>
-__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(STRESS ? 2 : 4)))
+// Staged FP32-mixed force pass without the Maxwell stress: one lane per
+// particle, 32-byte records.
+template <bool VISC_ONLY, bool ENERGY, bool UNI, int W, bool PF>
+__global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_forces_mixed_staged(
     Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, int charge, Bricks B,
     const uint32_t* __restrict__ start, uint32_t cap) {
-    constexpr int NQ2 = STRESS ? 2 : 0;
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    brick_pass<float4, 2, W, PF, 1, NQ2>(
+    brick_pass<float4, 2, W, PF, 1, 0>(
         B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
-        [&](uint32_t i, const BrickFetch<float4, 2, W, PF, NQ2>& f) {
+        [&](uint32_t i, const BrickFetch<float4, 2, W, PF, 0>& f) {
             if (kp.beta == 0.0)
-                force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, true>(P, D, X, g, L, kp, charge, i, f);
+                force_mixed_particle<VISC_ONLY, ENERGY, false, UNI, true, 1>(P, D, X, g, L, kp, charge, i, f);
             else
-                force_mixed_particle<VISC_ONLY, ENERGY, STRESS, UNI, false>(P, D, X, g, L, kp, charge, i, f);
+                force_mixed_particle<VISC_ONLY, ENERGY, false, UNI, false, 1>(P, D, X, g, L, kp, charge, i, f);
+        });
+}
+
+// With the Maxwell stress the B records (bx, by, bz, psi) in f64 are staged
+// beside the f32 records as quads 2 and 3 (64 bytes a record); its halo fills
+// one CU's LDS, so two lanes per particle (lanes l and l + 32, each walking
+// half of the list) give the CU four waves per SIMD.
+template <bool VISC_ONLY, bool ENERGY, bool UNI>
+__global__ void __launch_bounds__(2 * kStagedBlock) k_forces_mixed_stress_staged(
+    Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, int charge, Bricks B,
+    const uint32_t* __restrict__ start, uint32_t cap) {
+    __shared__ BrickHdr H;
+    extern __shared__ uint4 sh_dyn[];
+    brick_pass<float4, 2, 4, false, 2, 2>(
+        B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+        [&](uint32_t i, const BrickFetch<float4, 2, 4, false, 2>& f) {
+            if (kp.beta == 0.0)
+                force_mixed_particle<VISC_ONLY, ENERGY, true, UNI, true, 2>(P, D, X, g, L, kp, charge, i, f);
+            else
+                force_mixed_particle<VISC_ONLY, ENERGY, true, UNI, false, 2>(P, D, X, g, L, kp, charge, i, f);
         },
         reinterpret_cast<const float4*>(D.recs));
 }
@@ -1038,10 +1096,15 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
     if (B.on && B.nactive) {
 #define SPHX_FORCES_STAGED(V, E, M, U)                                                                      \
         if (visc_only == V && energy == E && stress == M && uni == U) {                                     \
-            if (mixed) {                                                                                    \
-                const StagedLaunch sl = staged_launch(B, 16, M ? 64 : 32);                                  \
-                hipLaunchKernelGGL((k_forces_mixed_staged<V, E, M, U, 4, true>), sl.grid, sl.block, sl.dyn, s, P, \
-                                   D, X, g, L, kp, ch, B, cell_start, sl.cap);                              \
+            if (mixed && M) {                                                                               \
+                StagedLaunch sl = staged_launch(B, 8, 64);                                                  \
+                sl.block = dim3(2u * (uint32_t)B.threads); /* two lanes per particle */                      \
+                hipLaunchKernelGGL((k_forces_mixed_stress_staged<V, E, U>), sl.grid, sl.block, sl.dyn, s, P, D, X, \
+                                   g, L, kp, ch, B, cell_start, sl.cap);                                    \
+            } else if (mixed) {                                                                             \
+                const StagedLaunch sl = staged_launch(B, 16, 32);                                           \
+                hipLaunchKernelGGL((k_forces_mixed_staged<V, E, U, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, \
+                                   X, g, L, kp, ch, B, cell_start, sl.cap);                                 \
             } else {                                                                                        \
                 StagedLaunch sl = staged_launch(B, 8, M ? 96 : 64);                                         \
                 sl.block = dim3(2u * (uint32_t)B.threads); /* two lanes per particle */                      \
