@@ -70,9 +70,9 @@ STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
 # 256 MiB copy) of the FP64 kernels on this workload, from the rocprofv3
 # counter runs summarised in profiles/dam_break_1M_fp64_mixed_kernels.md.
 MEASURED_TRAFFIC = {
-    "force": (557.5 + 50.9) * 1e6,
-    "visc_only": (574.3 + 25.4) * 1e6,
-    "density": (472.4 + 93.0) * 1e6,
+    "force": (486.0 + 50.8) * 1e6,
+    "visc_only": (511.2 + 25.4) * 1e6,
+    "density": (438.5 + 93.0) * 1e6,
 }
 STEP_FLOPS_PER_PAIR = 26.0 + 58.0 + 45.0
 
@@ -237,9 +237,22 @@ def mhd_eval_errors(st, p):
                     "to the largest FP64 change"}
 
 
+# List skin (in units of h) of each form on the random-fill MHD box, the
+# optimum of the measured sweep (bench.py --mhd-report --mhd-skin S on the
+# 4,194,304-particle box, where a 0.04 h list lives two ticks): the FP32-mixed
+# passes cost less per list entry, so the longer lists of a larger skin cost
+# them less than the builds it saves.  The lattice box keeps 0.04 h (its lists
+# live about twelve ticks; 0.12 h measured slower there in both forms).
+MHD_SKIN_H = {"fp64": 0.04, "mixed": 0.12}
+
+
 def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0, skin_h: float = -1.0):
     """Per-kernel breakdown of the periodic MHD box in the FP64 and FP32-mixed forms
-    (seeded random fill of n_fill particles, or the lattice^3 jittered lattice)."""
+    (seeded random fill of n_fill particles, or the lattice^3 jittered lattice).
+    `steps` ticks are timed with only the sort and build bracketed (their count);
+    the per-kernel breakdown comes from the next `steps` ticks, every kernel
+    bracketed.  skin_h < 0: each form's skin from MHD_SKIN_H (random fill) or
+    0.04 h (lattice)."""
     import sphx
     from sphx import scenarios
 
@@ -248,23 +261,27 @@ def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0, skin_h: f
     else:
         st, p = scenarios.mhd_periodic(random_fill=n_fill)
     n = len(st["x"])
-    if skin_h >= 0:
-        p.neighbour_skin = skin_h * float(st["h"].max())
     kind = f"lattice_{lattice}^3" if lattice else "random"
     out = {"report": "mhd", "workload": f"mhd_periodic_{kind}_{n}", "particles": n, "steps": steps,
-           "h": float(st["h"][0]), "skin_h": skin_h if skin_h >= 0 else 0.04}
+           "h": float(st["h"][0])}
     for prec in ("fp64", "mixed"):
         p.precision = sphx.PREC_FP64 if prec == "fp64" else sphx.PREC_FP32_MIXED
+        sk = skin_h if skin_h >= 0 else (0.04 if lattice else MHD_SKIN_H[prec])
+        p.neighbour_skin = sk * float(st["h"].max())
         with sphx.Context(n, p) as ctx:
             ctx.upload(st)
             ctx.step(warmup)
             sync_all(ctx)
-            ctx.enable_timing(True)
+            ctx.enable_timing(sphx.TIMING_SORT | sphx.TIMING_NEIGHBOUR)
             ctx.kernel_times(reset=True)
             t0 = time.perf_counter()
             ctx.step(steps)
             sync_all(ctx)
             secs = time.perf_counter() - t0
+            builds = ctx.kernel_times(reset=True)["neighbour"]["launches"]
+            ctx.enable_timing(True)
+            ctx.step(steps)
+            sync_all(ctx)
             times = ctx.kernel_times(reset=True)
             e = ctx.energy()
             if prec == "fp64":
@@ -286,7 +303,9 @@ def mhd_report(n_fill: int, steps: int, warmup: int, lattice: int = 0, skin_h: f
                        "flop_frac": round(tflops / (FP32_PEAK_TFLOPS if f32 else FP64_PEAK_TFLOPS), 4),
                        "flop_peak": "fp32 157.3" if f32 else "fp64 78.6"}
         out[prec] = {"ms_per_step": round(secs / steps * 1e3, 4), "particle_steps_per_s": round(n * steps / secs, 1),
-                     "directed_neighbour_pairs": pairs, "max_neighbours": e["max_neighbours"], "kernels": kern}
+                     "skin_h": sk, "neighbour_builds": builds,
+                     "directed_neighbour_pairs": pairs, "max_neighbours": e["max_neighbours"], "kernels": kern,
+                     "kernels_note": "HIP events around every kernel over the next `steps` ticks"}
     out["mixed_speedup"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
     out["mixed_errors"] = mhd_eval_errors(end, p)
     return out
