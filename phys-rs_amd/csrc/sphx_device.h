@@ -605,6 +605,8 @@ struct GlobalFetch {
     }
     // the record of the walking particle i itself
     __device__ __forceinline__ void own(uint32_t i, Q (&q)[NR]) const { (*this)(i | kOwnCode, q); }
+    // the length of particle i's list
+    __device__ __forceinline__ uint32_t count(const NbrList& L, uint32_t i) const { return L.ntot[i]; }
 };
 
 // body(e, record) for every list entry of sorted particle i, records from
@@ -820,11 +822,15 @@ __device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uin
 
 // Record of list entry e of a particle in brick row k: the row-code table of
 // row k (rc) gives its LDS address when it is staged.
-template <class Q, int NQ, int W_ = 2, bool PF_ = false, int NQ2 = 0>
+// PIN_: hold the walk's list prefetch at the top of each step with a full
+// scheduling barrier (always done in two-entry walks; in four-entry walks
+// only where it measured faster).
+template <class Q, int NQ, int W_ = 2, bool PF_ = false, int NQ2 = 0, bool PIN_ = false>
 struct BrickFetch {
     static_assert(sizeof(Q) == 16, "records are made of 16-byte quads");
     static constexpr int W = W_;
     static constexpr bool PF = PF_;
+    static constexpr bool PIN = PIN_ || W_ == 2;
     static constexpr int NR = NQ + NQ2;  // quads per record: NQ from glob, then NQ2 from glob2
     const SPHX_LDS lds_quad_t* rc;    // the row-code table of the particle's brick row
     const SPHX_LDS lds_quad_t* code;  // per entry code: the cell offset
@@ -833,6 +839,8 @@ struct BrickFetch {
     uint32_t qstride;                 // bytes from quad k to quad k + 1 of a record (16 cap)
     uint32_t whole;                   // every list entry of the brick's particles is staged (uniform)
     uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries (kbeg a multiple of 4)
+    uint32_t n_list = 0;              // the length of the walking particle's list (loaded by brick_pass)
+    __device__ __forceinline__ uint32_t count(const NbrList&, uint32_t) const { return n_list; }
     // LDS byte address of entry e's record; false if it is not staged
     __device__ __forceinline__ bool locate(uint32_t e, uint32_t& addr) const {
         const lds_quad_t h = rc[e >> kIdxBits];
@@ -881,15 +889,15 @@ struct BrickFetch {
 // bodies, written branch-free, form one basic block the compiler can
 // interleave.  In a brick whose halo rows are all staged whole (F.whole)
 // every entry is staged and the test is skipped.  flush() as in for_neighbour_groups (it needs W = 4).
-template <class Q, int NQ, int W, bool PF, int NQ2, class Body, class Flush = NoFlush>
-__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF, NQ2>& F,
+template <class Q, int NQ, int W, bool PF, int NQ2, bool PIN, class Body, class Flush = NoFlush>
+__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF, NQ2, PIN>& F,
                                      Body&& body, Flush&& flush = Flush()) {
     constexpr int NR = NQ + NQ2;
     static_assert(W == 2 || W == 4, "two or four entries per iteration");
     static_assert(W == 4 || std::is_same<typename std::decay<Flush>::type, NoFlush>::value,
                   "groups of four entries need four entries per iteration");
     const uint32_t* __restrict__ e = nbr_base(L, i);
-    const uint32_t cnt = min((uint32_t)L.ntot[i], F.kend);
+    const uint32_t cnt = min(F.count(L, i), F.kend);
     // W entries of the list from position k (k a multiple of W): one 16- or
     // 8-byte load
     auto load = [&](uint32_t k, uint32_t (&c)[W]) {
@@ -915,10 +923,9 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
 #pragma unroll
                 for (int u = 0; u < W; ++u) c[u] = a[u];
                 load(min(k + W, (cnt - 1u) & ~(uint32_t)(W - 1)), a);
-                // two-entry steps: keep the load at the top of the iteration
-                // (the scheduler may otherwise sink it next to its wait at the
-                // bottom; the four-entry steps measured slower with the barrier)
-                if constexpr (W == 2) __builtin_amdgcn_sched_barrier(0);
+                // keep the load at the top of the iteration (the scheduler
+                // may otherwise sink it next to its wait)
+                if constexpr (BrickFetch<Q, NQ, W, PF, NQ2, PIN>::PIN) __builtin_amdgcn_sched_barrier(0);
             } else {
                 load(k, c);
             }
@@ -971,9 +978,9 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
 // quads each) and, with NQ2 > 0, of `src2` (NQ2 quads each, read as quads NQ
 // .. NQ + NQ2 - 1), then body(i, fetch) for each of the brick's particles.
 // LANES = 2: two neighbouring lanes share each particle (body sees both).
-template <class Q, int NQ, int W, bool PF, int LANES = 1, int NQ2 = 0, class Body>
+template <class Q, int NQ, int W, bool PF, int LANES = 1, int NQ2 = 0, bool PIN = false, class Body>
 __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const uint32_t* __restrict__ start,
-                                           uint32_t cap, const Q* __restrict__ src, BrickHdr& H,
+                                           uint32_t cap, const NbrList& L, const Q* __restrict__ src, BrickHdr& H,
                                            uint4* __restrict__ sh, Body&& body, const Q* __restrict__ src2 = nullptr) {
     const uint32_t hy = (uint32_t)B.b[1] + 2u, nh = hy * ((uint32_t)B.b[2] + 2u);
     const uint32_t BY = (uint32_t)B.b[1], no = BY * (uint32_t)B.b[2];
@@ -982,18 +989,26 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
     uint4* rc = sh;
     uint4* rec = sh + brick_rc_quads(B);
     brick_setup(B.active[xcd_block()], B, g, start, cap, H);
+    // the list length of the thread's first particle, loaded while the halo
+    // is staged (its latency then overlaps the staging's)
+    const uint32_t nown = HL->own_cum[no], u0 = LANES == 2 ? pair_slot(threadIdx.x) : threadIdx.x;
+    uint32_t k = 0, n0 = 0;
+    if (u0 < nown) {
+        while (u0 >= HL->own_cum[k + 1]) ++k;
+        n0 = L.ntot[HL->own_start[k] + (u0 - HL->own_cum[k])];
+    }
     brick_stage<NQ, NQ2>(H, (int)nh, reinterpret_cast<const uint4*>(src), reinterpret_cast<const uint4*>(src2), rec,
                          cap, rc, no, BY, hy);
-    const BrickFetch<Q, NQ, W, PF, NQ2> base_fetch{(const SPHX_LDS lds_quad_t*)rc,
+    const BrickFetch<Q, NQ, W, PF, NQ2, PIN> base_fetch{(const SPHX_LDS lds_quad_t*)rc,
                                                    (const SPHX_LDS lds_quad_t*)HL->code, src, src2, cap * 16u,
                                                    (uint32_t)__builtin_amdgcn_readfirstlane(HL->whole)};
-    const uint32_t nown = HL->own_cum[no];
-    uint32_t k = 0;
-    for (uint32_t u = LANES == 2 ? pair_slot(threadIdx.x) : threadIdx.x; u < nown; u += blockDim.x / LANES) {
+    for (uint32_t u = u0; u < nown; u += blockDim.x / LANES) {
         while (u >= HL->own_cum[k + 1]) ++k;
-        BrickFetch<Q, NQ, W, PF, NQ2> f = base_fetch;
+        const uint32_t i = HL->own_start[k] + (u - HL->own_cum[k]);
+        BrickFetch<Q, NQ, W, PF, NQ2, PIN> f = base_fetch;
         f.rc += k * (uint32_t)kRowCodes;
-        body(HL->own_start[k] + (u - HL->own_cum[k]), f);
+        f.n_list = u == u0 ? n0 : (uint32_t)L.ntot[i];
+        body(i, f);
     }
 }
 

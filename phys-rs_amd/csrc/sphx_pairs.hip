@@ -388,7 +388,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
             acc = fma(P.m[e & kIdxMask], (2.0 * ph.sig_w) * wh_shape_r(r, nih, 0.5 * nih), acc);
         }
     };
-    const uint32_t half = (((uint32_t)L.ntot[i] + 7u) >> 3) << 2;
+    const uint32_t half = ((fetch.count(L, i) + 7u) >> 3) << 2;
     auto sum_range = [&](uint32_t kb, uint32_t ke) {
         Fetch f = fetch;
         f.kbeg = kb;
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_
     uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    brick_pass<double2, 2, W, PF>(B, g, start, cap, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
+    brick_pass<double2, 2, W, PF>(B, g, start, cap, L, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
                                   [&](uint32_t i, const BrickFetch<double2, 2, W, PF>& f) {
                                       density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f);
                                   });
@@ -506,7 +506,7 @@ k_density_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, Kerne
                        const uint32_t* __restrict__ start, uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    brick_pass<float4, 1, W, PF>(B, g, start, cap, X.p4, H, sh_dyn,
+    brick_pass<float4, 1, W, PF>(B, g, start, cap, L, X.p4, H, sh_dyn,
                                  [&](uint32_t i, const BrickFetch<float4, 1, W, PF>& f) {
                                      density_mixed_particle<UNI>(P, D, X, g, L, kp, i, f);
                                  });
@@ -581,14 +581,16 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
 // reloaded by VISC_ONLY, so the reuse is bit-identical.  Then Lorentz
 // q(E + v x B) (lorentz.rs:27-37), viscosity and stress sums, penalty walls
 // (sph.rs:451-483) and a = F / m (integrators.rs:115).
+// VISC_ONLY: sp holds the stored sum of particle p, loaded by the caller
+// before its walk (so that the load's latency is spent in the walk).
 template <bool VISC_ONLY>
 __device__ __forceinline__ void force_epilogue(const Particles& P, const Derived& D, const KernelParams& kp,
                                                bool charge, uint32_t p, const double fp[3], const double fv[3],
-                                               const double fm[3]) {
-    const double mi = P.m[p];
+                                               const double fm[3], const double sp[3]) {
+    const double mi = kp.uniform ? kp.m0 : P.m[p];
     double S[3];
     if (VISC_ONLY) {
-        S[0] = D.spx[p]; S[1] = D.spy[p]; S[2] = D.spz[p];
+        S[0] = sp[0]; S[1] = sp[1]; S[2] = sp[2];
     } else {
         S[0] = mi * kp.g[0] + fp[0];
         S[1] = mi * kp.g[1] + fp[1];
@@ -643,6 +645,10 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = UNI ? kp.h0 : P.h[i];
     const double vxi = own[2].x, vyi = own[2].y, vzi = own[3].x;
     const double mi = UNI ? kp.m0 : P.m[i], rhoi = own[1].y, pr2i = own[3].y;
+    double sp[3] = {0, 0, 0};  // VISC_ONLY: the stored sum, loaded ahead of the walk
+    if (VISC_ONLY) {
+        sp[0] = D.spx[i]; sp[1] = D.spy[i]; sp[2] = D.spz[i];
+    }
     // Pi = mu (beta mu - alpha c) / rho_bar with mu = h s / den, den = r^2 + 0.01 h^2
     // (sph.rs:281-300), formed with one reciprocal:
     // Pi = hs (2 beta hs - 2 alpha c den) / (den^2 (rho_i + rho_j)), hs = h s,
@@ -743,7 +749,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
     // The list is summed in two halves, entries 0 .. half - 1 and half .. end
     // (half a multiple of 4 near the middle), and the halves are added in
     // that order; the staged pass gives each half a lane of its own.
-    const uint32_t cnt = L.ntot[i], half = ((cnt + 7u) >> 3) << 2;
+    const uint32_t cnt = fetch.count(L, i), half = ((cnt + 7u) >> 3) << 2;
     auto sum_range = [&](uint32_t kb, uint32_t ke) {
         Fetch f = fetch;
         f.kbeg = kb;
@@ -801,7 +807,7 @@ __device__ __forceinline__ void force_particle(const Particles& P, const Derived
             for (int a = 0; a < 3; ++a) fv[a] *= cv;
         if (ENERGY) D.du[i] = -de / mi;
     }
-    force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
+    force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm, sp);
 }
 
 // Two lanes per particle, as the staged pass (same halves, same sums): at
@@ -837,7 +843,7 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
     // 1.69 -> 1.64 ms per launch despite a few spilled registers)
     constexpr bool PF = true;
     brick_pass<double2, 4, W, PF, 2, NQ2>(
-        B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
+        B, g, start, cap, L, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<double2, 4, W, PF, NQ2>& f) {
             if (kp.beta == 0.0)
                 force_particle<VISC_ONLY, ENERGY, STRESS, UNI, true, PER, 2>(P, D, g, L, kp, charge, i, f);
@@ -860,6 +866,10 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     float4 own[Fetch::NR];  // the particle's own gather record
     fetch.own(i, own);
     const float4 own0 = own[0], own1 = own[1];
+    double sp[3] = {0, 0, 0};  // VISC_ONLY: the stored sum, loaded ahead of the walk
+    if (VISC_ONLY) {
+        sp[0] = D.spx[i]; sp[1] = D.spy[i]; sp[2] = D.spz[i];
+    }
     const float xi = own0.x, yi = own0.y, zi = own0.z, rhoi = own0.w;
     const float vxi = own1.x, vyi = own1.y, vzi = own1.z, pr2i = own1.w;
     const float hi = X.p4[i].w, mi = UNI ? 0.0f : X.m[i];
@@ -986,7 +996,7 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     if (!STRESS) {
         walk(L, i, fetch, pair, flush);
     } else {
-        const uint32_t cnt = L.ntot[i], half = ((cnt + 7u) >> 3) << 2;
+        const uint32_t cnt = fetch.count(L, i), half = ((cnt + 7u) >> 3) << 2;
         if (LANES == 2) {
             const uint32_t h = lane_half();
             sum_range(h ? half : 0u, h ? 0xffffffffu : half);
@@ -1031,7 +1041,7 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             for (int a = 0; a < 3; ++a) fv[a] *= cv;
         if (ENERGY) D.du[i] = -de / P.m[i];
     }
-    force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm);
+    force_epilogue<VISC_ONLY>(P, D, kp, charge != 0, i, fp, fv, fm, sp);
 }
 
 template <bool VISC_ONLY, bool ENERGY, bool STRESS, bool UNI>
@@ -1054,9 +1064,11 @@ k_forces_mixed_staged(
     const uint32_t* __restrict__ start, uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    brick_pass<float4, 2, W, PF, 1, 0>(
-        B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
-        [&](uint32_t i, const BrickFetch<float4, 2, W, PF, 0>& f) {
+    // the full pass holds its list prefetch at the top of each step (faster
+    // there; slower in the viscosity-only pass)
+    brick_pass<float4, 2, W, PF, 1, 0, !VISC_ONLY>(
+        B, g, start, cap, L, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+        [&](uint32_t i, const BrickFetch<float4, 2, W, PF, 0, !VISC_ONLY>& f) {
             if (kp.beta == 0.0)
                 force_mixed_particle<VISC_ONLY, ENERGY, false, UNI, true, 1>(P, D, X, g, L, kp, charge, i, f);
             else
@@ -1075,7 +1087,7 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_mixed_stress_staged
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
     brick_pass<float4, 2, 4, false, 2, 2>(
-        B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+        B, g, start, cap, L, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<float4, 2, 4, false, 2>& f) {
             if (kp.beta == 0.0)
                 force_mixed_particle<VISC_ONLY, ENERGY, true, UNI, true, 2>(P, D, X, g, L, kp, charge, i, f);
@@ -1284,7 +1296,7 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_induction_staged(Particles
     extern __shared__ uint4 sh_dyn[];
     using Fetch = BrickFetch<double2, 4, 2, true, 2>;
     brick_pass<double2, 4, 2, true, 2, 2>(
-        B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
+        B, g, start, cap, L, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const Fetch& f) {
             induction_particle<UNI, RES, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
                 Fetch ff = f;
@@ -1417,7 +1429,7 @@ k_induction_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, Ker
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
     brick_pass<float4, 2, 4, true, 1, NQ2>(
-        B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+        B, g, start, cap, L, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<float4, 2, 4, true, NQ2>& f) {
             induction_mixed_particle<UNI, RES>(P, D, X, g, L, kp, i, f);
         },
@@ -1554,7 +1566,7 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_div_clean_staged(Particles
     extern __shared__ uint4 sh_dyn[];
     using Fetch = BrickFetch<double2, 4, 2, true, 2>;
     brick_pass<double2, 4, 2, true, 2, 2>(
-        B, g, start, cap, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
+        B, g, start, cap, L, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const Fetch& f) {
             clean_particle<UNI, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
                 Fetch ff = f;
@@ -1634,7 +1646,7 @@ k_div_clean_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, Ker
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
     brick_pass<float4, 2, 4, true, 1, 2>(
-        B, g, start, cap, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
+        B, g, start, cap, L, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
         [&](uint32_t i, const BrickFetch<float4, 2, 4, true, 2>& f) {
             clean_mixed_particle<UNI>(P, D, X, g, L, kp, i, f);
         },
