@@ -190,20 +190,21 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
             const bool ok = r2 > 1e-20;  // r <= 1e-10: no pressure force (sph.rs:52)
             double r, inv_r;
-            sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
+            sqrt_rsqrt1(ok ? r2 : 1.0, r, inv_r);
             const double h = UNI ? kp.h0 : 0.5 * (hi + P.h[j]);
-            const double inv_h = UNI ? kp.inv_h0 : rcp_nr(h);
-            const double ih2 = inv_h * inv_h, sig_g = (ih2 * ih2) * kInvPi;
-            const double qk = r * inv_h;
-            const double tk = fmax(2.0 - qk, 0.0);
-            // dW/dr and d2W/dr2 of the cubic spline, zero from q = 2 on
-            const double dw = sig_g * (qk < 1.0 ? qk * fma(2.25, qk, -3.0) : -0.75 * (tk * tk));
-            const double d2w = (sig_g * inv_h) * (qk < 1.0 ? fma(4.5, qk, -3.0) : 1.5 * tk);
+            const double inv_h = UNI ? kp.inv_h0 : rcp_nr1(h);
+            const double ih2 = inv_h * inv_h, sig3 = 3.0 * ((ih2 * ih2) * kInvPi);
+            // dW/dr and d2W/dr2 of the cubic spline from its one-sided pieces
+            // s = (1 - q)+ and t = (1 - q/2)+: 3 sigma_g (s^2 - t^2) and
+            // 3 (sigma_g / h) (t - 2 s), zero from q = 2 on
+            const double sk = fmax(fma(r, -inv_h, 1.0), 0.0), tk = fmax(fma(r, -0.5 * inv_h, 1.0), 0.0);
+            const double dw = sig3 * fma(-tk, tk, sk * sk);
+            const double d2w = (sig3 * inv_h) * fma(-2.0, sk, tk);
             const double okf = ok ? 1.0 : 0.0;
             const double G = (dw * inv_r) * okf;
             const double Gp = ((d2w - dw * inv_r) * inv_r) * okf;  // dG/dr
             const double mj = UNI ? kp.m0 : P.m[j], mm = mi * mj;
-            const double inv_mj = UNI ? inv_mi : rcp_nr(mj);
+            const double inv_mj = UNI ? inv_mi : rcp_nr1(mj);
             const double ux = uxi - a01.x * inv_mj, uy = uyi - a01.y * inv_mj, uz = uzi - a2.x * inv_mj;
             const double c = fma(uz, dz, fma(uy, dy, ux * dx));
             const double rhoj = zr.y;
@@ -212,8 +213,8 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
             const double sdot = fma(vz, dz, fma(vy, dy, vx * dx));
             const double actf = (ref_conv ? (sdot < 0.0) : (sdot > 0.0)) ? 1.0 : 0.0;
             const double Dn = fma(0.01 * h, h, r2);
-            const double inv_Dn = rcp_nr(Dn);
-            const double inv_rb = 2.0 * rcp_nr(rhoi + rhoj);  // 1 / rho_bar
+            const double inv_Dn = rcp_nr1(Dn);
+            const double inv_rb = 2.0 * rcp_nr1(rhoi + rhoj);  // 1 / rho_bar
             const double hsc = h * sc;
             const double mu = ((hsc * sdot) * inv_Dn) * actf;
             const double Pi = fma(beta * mu, mu, -alpha * cs * mu) * inv_rb;
@@ -277,10 +278,11 @@ __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n,
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const bool ok = r2 > 0.0;
         double r, inv_r;
-        sqrt_rsqrt(ok ? r2 : 1.0, r, inv_r);
-        const double inv_h = UNI ? kp.inv_h0 : rcp_nr(0.5 * (hi + zh.y));
+        sqrt_rsqrt1(ok ? r2 : 1.0, r, inv_r);
+        const double inv_h = UNI ? kp.inv_h0 : rcp_nr1(0.5 * (hi + zh.y));
         const double ih2 = inv_h * inv_h;
-        const double dwr = ((((ih2 * ih2) * kInvPi) * dw_shape(r * inv_h)) * inv_r) * (ok ? 1.0 : 0.0);
+        const double dwr = (((3.0 * ((ih2 * ih2) * kInvPi)) * dw3_shape_r(r, -inv_h, -0.5 * inv_h)) * inv_r) *
+                           (ok ? 1.0 : 0.0);
         const double coef = UNI ? ((rbi + rhob[j]) * mi) * dwr : fma(rbi, P.m[j], rhob[j] * mi) * dwr;
         gx = fma(-coef, dx, gx);
         gy = fma(-coef, dy, gy);
