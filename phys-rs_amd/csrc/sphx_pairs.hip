@@ -369,7 +369,10 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
     double2 own[2];  // the particle's own position record (x, y | z, h)
     fetch.own(i, own);
     const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = own[1].y;
-    double rho = P.m[i] * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
+    double rho = (UNI ? kp.m0 : P.m[i]) * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
+    // the epilogue's per-particle loads, issued ahead of the walk
+    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
+    const double ui = kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0;
     double acc = 0.0;
     const double nih0 = -kp.inv_h0, nih20 = -0.5 * kp.inv_h0;
     auto pair = [&](uint32_t e, const double2 (&q)[2]) {
@@ -408,7 +411,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         acc = a0 + acc;
     }
     rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 2.0), acc, rho) : rho + acc;
-    const double pr = eos_pressure(kp, rho, kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0);
+    const double pr = eos_pressure(kp, rho, ui);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
     D.p[i] = pr;
@@ -416,8 +419,8 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
     double2* rec = reinterpret_cast<double2*>(D.r64 + i);
     rec[0] = make_double2(xi, yi);
     rec[1] = make_double2(zi, rho);
-    rec[2] = make_double2(P.vx[i], P.vy[i]);
-    rec[3] = make_double2(P.vz[i], pr2);
+    rec[2] = make_double2(vxi, vyi);
+    rec[3] = make_double2(vzi, pr2);
     if (kp.srec) D.recs[i] = stress_record(P.bx[i], P.by[i], P.bz[i], rho);
 }
 
@@ -456,8 +459,11 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     fetch.own(i, ownr);
     const float4 own = ownr[0];
     const float xi = own.x, yi = own.y, zi = own.z, hi = own.w;
-    const double hd = P.h[i];
-    double rho = P.m[i] * (1.0 / (kPi * ((hd * hd) * hd)));
+    const double hd = UNI ? kp.h0 : P.h[i];
+    double rho = (UNI ? kp.m0 : P.m[i]) * (1.0 / (kPi * ((hd * hd) * hd)));
+    // the epilogue's per-particle loads, issued ahead of the walk
+    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
+    const double ui = kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0;
     const float inv_h0 = (float)kp.inv_h0;
     const float wx = (float)g.w[0], wy = (float)g.w[1], wz = (float)g.w[2];
     double acc = 0.0;
@@ -481,13 +487,12 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
         gacc = 0.0f;
     });
     rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 0.25), acc, rho) : rho + acc;
-    const double pr = eos_pressure(kp, rho, kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0);
+    const double pr = eos_pressure(kp, rho, ui);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
     D.p[i] = pr;
     D.pr2[i] = pr2;
     float4* rec = reinterpret_cast<float4*>(D.r32 + i);
-    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
     rec[0] = make_float4(xi, yi, zi, (float)rho);
     rec[1] = make_float4((float)vxi, (float)vyi, (float)vzi, (float)pr2);
     if (kp.srec) D.recs[i] = stress_record(P.bx[i], P.by[i], P.bz[i], rho);
