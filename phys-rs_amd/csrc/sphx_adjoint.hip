@@ -135,18 +135,26 @@ void launch_adj_loss(const Particles& P, uint32_t n, const Vec3& target, const V
     if (n) hipLaunchKernelGGL(k_adj_loss, dim3(nblocks(n)), dim3(kBlock), 0, s, P, n, target, xb, part);
 }
 
-// Two lanes per particle (the VJP passes: lanes l and l + 32 of a wave, as
-// in the force passes): lane h of the pair sums list
-// entries [0, half) (h = 0) or [half, end) (h = 1), half a multiple of 4 near
-// the middle, and lane 0 adds the second half to the first.  At the sizes the
-// adjoint runs (tens of thousands of particles, one thread per particle is a
-// few waves per CU) this doubles the waves that hide the gather latency.
-__device__ __forceinline__ uint32_t list_half(const NbrList& L, uint32_t i) {
-    return (((uint32_t)L.ntot[i] + 7u) >> 3) << 2;
+// Four lanes per particle (the VJP passes): lanes l, l + 16, l + 32 and
+// l + 48 of a wave share particle (wave * 16 + l % 16); lane quarter q =
+// lane / 16 sums list entries [q qs, (q + 1) qs) (the last quarter to the
+// end), qs a multiple of 4 near a quarter of the list, and the quarter sums
+// are added as (q0 + q1) + (q2 + q3) into quarter 0.  At the sizes the
+// adjoint runs (tens of thousands of particles) the passes are bound by the
+// latency of their gathers, and four lanes give four times the waves of one.
+__device__ __forceinline__ uint32_t vjp_slot(uint32_t t) { return ((t >> 6) << 4) | (t & 15u); }
+__device__ __forceinline__ uint32_t vjp_quarter() { return (threadIdx.x >> 4) & 3u; }
+__device__ __forceinline__ void vjp_range(const NbrList& L, uint32_t i, uint32_t q, uint32_t& kb, uint32_t& ke) {
+    const uint32_t qs = (((uint32_t)L.ntot[i] + 15u) >> 4) << 2;
+    kb = q * qs;
+    ke = q == 3u ? 0xffffffffu : (q + 1u) * qs;
 }
-__device__ __forceinline__ double pair_total(double v) { return v + other_half(v); }
+__device__ __forceinline__ double quarter_total(double v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
 
-uint32_t vjp_blocks(uint32_t n) { return nblocks(2 * n); }
+uint32_t vjp_blocks(uint32_t n) { return nblocks(4 * n); }
 
 // VJP of the pressure + viscosity acceleration at the current (sorted)
 // state.  ab4: abar records in sorted order; neighbours' x, rho, v, P/rho^2
@@ -160,7 +168,7 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
                                                         Vec3 vb, double* __restrict__ rhob,
                                                         double* __restrict__ alpha_acc, int overwrite) {
     __shared__ double lds[kBlock];
-    const uint32_t i = pair_slot(particle_index()), h = lane_half();
+    const uint32_t i = vjp_slot(particle_index()), h = vjp_quarter();
     double da = 0.0;
     if (i < n) {
         const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i];
@@ -180,7 +188,8 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
         // Branch-free pair body with two refined reciprocals (1/Dn, 1/rho_bar)
         // in place of divisions; a pair outside the kernel support, at r <= 1e-10
         // or (for the viscosity terms) not approaching/separating contributes 0.
-        const uint32_t half = list_half(L, i);
+        uint32_t kb, ke;
+        vjp_range(L, i, h, kb, ke);
         for_neighbours(L, i, [&](uint32_t e) {
             const uint32_t j = e & kIdxMask;
             const double2 xy = R[4 * j], zr = R[4 * j + 1], vxy = R[4 * j + 2], vzp = R[4 * j + 3];
@@ -236,10 +245,10 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
             gvz = fma(wh, dz, gvz);
             grho = fma(cg, fma(-0.5 * Pi, inv_rb, dTi), grho);
             da = fma(0.5 * cg, (-cs * mu) * inv_rb, da);  // each pair is visited from both ends
-        }, h ? half : 0u, h ? 0xffffffffu : half);
-        gx = pair_total(gx); gy = pair_total(gy); gz = pair_total(gz);
-        gvx = pair_total(gvx); gvy = pair_total(gvy); gvz = pair_total(gvz);
-        grho = pair_total(grho);
+        }, kb, ke);
+        gx = quarter_total(gx); gy = quarter_total(gy); gz = quarter_total(gz);
+        gvx = quarter_total(gvx); gvy = quarter_total(gvy); gvz = quarter_total(gvz);
+        grho = quarter_total(grho);
         if (!h) {
             const uint32_t u = P.orig[i];
             if (overwrite) {
@@ -264,12 +273,13 @@ __global__ void __launch_bounds__(kBlock) k_vjp_forces(Particles P, Derived D, u
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n, Grid g, NbrList L,
                                                          KernelParams kp, const double* __restrict__ rhob, Vec3 xb) {
-    const uint32_t i = pair_slot(particle_index()), h = lane_half();
+    const uint32_t i = vjp_slot(particle_index()), h = vjp_quarter();
     if (i >= n) return;
     const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i], mi = P.m[i], rbi = rhob[i];
     const double2* __restrict__ pos = reinterpret_cast<const double2*>(P.pos);
     double gx = 0, gy = 0, gz = 0;
-    const uint32_t half = list_half(L, i);
+    uint32_t kb, ke;
+    vjp_range(L, i, h, kb, ke);
     for_neighbours(L, i, [&](uint32_t e) {
         const uint32_t j = e & kIdxMask;
         const double2 xy = pos[2 * j], zh = pos[2 * j + 1];
@@ -287,8 +297,8 @@ __global__ void __launch_bounds__(kBlock) k_vjp_density(Particles P, uint32_t n,
         gx = fma(-coef, dx, gx);
         gy = fma(-coef, dy, gy);
         gz = fma(-coef, dz, gz);
-    }, h ? half : 0u, h ? 0xffffffffu : half);
-    gx = pair_total(gx); gy = pair_total(gy); gz = pair_total(gz);
+    }, kb, ke);
+    gx = quarter_total(gx); gy = quarter_total(gy); gz = quarter_total(gz);
     if (h) return;
     const uint32_t u = P.orig[i];
     xb.x[u] += gx; xb.y[u] += gy; xb.z[u] += gz;
@@ -298,7 +308,7 @@ void launch_vjp(const Particles& P, const Derived& D, uint32_t n, const Grid& g,
                 const KernelParams& kp, const double4* ab4, const Vec3& xb, const Vec3& vb, double* rhob,
                 double* alpha_acc, hipStream_t s, bool overwrite) {
     if (!n) return;
-    const dim3 grid(vjp_blocks(n)), block(kBlock);  // two lanes per particle
+    const dim3 grid(vjp_blocks(n)), block(kBlock);  // four lanes per particle
     const int ow = overwrite ? 1 : 0;
     if (kp.uniform) {
         hipLaunchKernelGGL(k_vjp_forces<true>, grid, block, occupancy_lds(3), s, P, D, n, g, L, kp, ab4, xb, vb, rhob,
