@@ -1207,20 +1207,37 @@ void launch_xsph(const Particles& P, const Derived& D, uint32_t n, const Grid& g
 // 130-131), so particle i's term carries +1 when id_i < id_j, -1 otherwise.
 // Neighbours come from the f64 record (x, rho, v) and the B record (one
 // 6-quad record); the new B goes to tmp3 and, with psi, to recb_next (Jacobi).
-// Two lanes per particle: lane h (threadIdx & 1) walks half h of the list,
+// (dW/dr) / r of a pair of the MHD passes from r^2, 0 for r <= 1e-10
+// (mhd.rs:100, where the reference skips the pair): the clamp keeps 1/r
+// finite, and below the cut r keeps only its low word (a denormal) so that
+// the spline's one-sided pieces give exactly 0.  One refinement step on the
+// hardware rsqrt (the terms are compared relative to the largest change).
+template <bool UNI>
+__device__ __forceinline__ double mhd_dwr(const KernelParams& kp, double r2, double hi, double hj) {
+    double r, inv_r;
+    sqrt_rsqrt1(fmax(r2, 1e-300), r, inv_r);
+    r = __hiloint2double(r2 > 1e-20 ? __double2hiint(r) : 0, __double2loint(r));
+    const PairH ph = pair_h<UNI>(kp, hi, hj);
+    const double nih = -ph.inv_h;
+    return ((3.0 * ph.sig_g) * dw3_shape_r(r, nih, 0.5 * nih)) * inv_r;
+}
+
+// Two lanes per particle: lane h (lanes l and l + 32 of a wave) walks half h of the list,
 // two neighbour records loaded ahead of their bodies, and the halves are added
 // in order.  The pair body is branch-free (a factor 0 for r <= 1e-10, where
 // the reference skips the pair: mhd.rs:100).  RES: the resistive term (eta != 0).
 // range(kb, ke, body) walks list entries kb .. ke - 1 of particle i with
 // records of 6 quads (the f64 record, then the B record).
-template <bool UNI, bool RES, int PER, class Range>
+template <bool UNI, bool RES, int PER, class Own, class Range>
 __device__ __forceinline__ void induction_particle(const Particles& P, const Derived& D, const Grid& g,
                                                    const NbrList& L, const KernelParams& kp, uint32_t i,
-                                                   Range&& range) {
+                                                   Own&& own_record, Range&& range) {
     const uint32_t h = lane_half();
-    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
-    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i], rhoi = D.rho[i];
-    const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i];
+    double2 own[6];  // the particle's own records: (x, y | z, rho | vx, vy | vz, P/rho^2 | bx, by | bz, psi)
+    own_record(own);
+    const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = UNI ? kp.h0 : P.h[i];
+    const double vxi = own[2].x, vyi = own[2].y, vzi = own[3].x, rhoi = own[1].y;
+    const double bxi = own[4].x, byi = own[4].y, bzi = own[5].x;
     const uint32_t idi = P.id[i];
     // m_j / (rho_i rho_j) = mi_rho / rho_j (uniform m: mi_rho = m0 / rho_i)
     const double mi_rho = (UNI ? kp.m0 : 1.0) * rcp_nr(rhoi);
@@ -1237,11 +1254,7 @@ __device__ __forceinline__ void induction_particle(const Particles& P, const Der
         double dx, dy, dz;
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        const bool ok = r2 > 1e-20;
-        double r, inv_r;
-        sqrt_rsqrt1(ok ? r2 : 1.0, r, inv_r);
-        const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
-        const double dwr = ((ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r) * (ok ? 1.0 : 0.0);
+        const double dwr = mhd_dwr<UNI>(kp, r2, hi, UNI ? hi : P.h[j]);
         const double vx = vxi - vxy.x, vy = vyi - vxy.y, vz = vzi - vzp.x;
         const double rhoj = zr.y;
         const double v_dot_d = fma(vz, dz, fma(vy, dy, vx * dx));
@@ -1252,8 +1265,9 @@ __device__ __forceinline__ void induction_particle(const Particles& P, const Der
         if (RES) {
             const double mj = UNI ? kp.m0 : P.m[j];
             const double r_dot_g = r2 * dwr;
+            const double h_avg = UNI ? kp.h0 : 0.5 * (hi + P.h[j]);
             const double res = sdt * ((2.0 * kp.eta * (mj * inv_rhoj)) * r_dot_g *
-                                      rcp_nr(fma(0.01 * ph.h_avg, ph.h_avg, r2)));
+                                      rcp_nr(fma(0.01 * h_avg, h_avg, r2)));
             ux = fma(f, fma(bxi, v_dot_d, -vx * b_dot_d), fma(res, bxy.x - bxi, ux));
             uy = fma(f, fma(byi, v_dot_d, -vy * b_dot_d), fma(res, bxy.y - byi, uy));
             uz = fma(f, fma(bzi, v_dot_d, -vz * b_dot_d), fma(res, bzp.x - bzi, uz));
@@ -1273,7 +1287,7 @@ __device__ __forceinline__ void induction_particle(const Particles& P, const Der
     D.tmp3[2][i] = bz;
     double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
     rb[0] = make_double2(bx, by);
-    rb[1] = make_double2(bz, P.psi[i]);
+    rb[1] = make_double2(bz, own[5].y);
 }
 
 template <bool UNI, bool RES, int PER>
@@ -1283,9 +1297,10 @@ __global__ void __launch_bounds__(kBlock) k_induction(Particles P, Derived D, ui
     if (i >= n) return;
     const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
                                        reinterpret_cast<const double2*>(D.recb)};
-    induction_particle<UNI, RES, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
-        walk_ahead<4>(L, i, kb, ke, F, body);
-    });
+    induction_particle<UNI, RES, PER>(P, D, g, L, kp, i, [&](double2 (&q)[6]) { F.own(i, q); },
+                                      [&](uint32_t kb, uint32_t ke, auto&& body) {
+                                          walk_ahead<4>(L, i, kb, ke, F, body);
+                                      });
 }
 
 // Brick-staged: the neighbours' f64 and B records (96 bytes) from LDS, the
@@ -1303,7 +1318,8 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_induction_staged(Particles
     brick_pass<double2, 4, 2, true, 2, 2>(
         B, g, start, cap, L, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const Fetch& f) {
-            induction_particle<UNI, RES, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
+            induction_particle<UNI, RES, PER>(P, D, g, L, kp, i, [&](double2 (&q)[6]) { f.own(i, q); },
+                                              [&](uint32_t kb, uint32_t ke, auto&& body) {
                 Fetch ff = f;
                 ff.kbeg = kb;
                 ff.kend = ke;
@@ -1500,13 +1516,15 @@ void launch_induction(const Particles& P, const Derived& D, uint32_t n, const Gr
 // divergence_cleaning_system (mhd.rs:172-259) in gather form, two lanes per
 // particle as the induction pass (halves added in order, branch-free body);
 // range as for induction_particle.
-template <bool UNI, int PER, class Range>
+template <bool UNI, int PER, class Own, class Range>
 __device__ __forceinline__ void clean_particle(const Particles& P, const Derived& D, const Grid& g,
                                                const NbrList& L, const KernelParams& kp, uint32_t i,
-                                               Range&& range) {
+                                               Own&& own_record, Range&& range) {
     const uint32_t h = lane_half();
-    const double xi = P.x[i], yi = P.y[i], zi = P.z[i], hi = P.h[i];
-    const double bxi = P.bx[i], byi = P.by[i], bzi = P.bz[i], psii = P.psi[i];
+    double2 own[6];  // the particle's own records, as in the induction pass
+    own_record(own);
+    const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = UNI ? kp.h0 : P.h[i];
+    const double bxi = own[4].x, byi = own[4].y, bzi = own[5].x, psii = own[5].y;
     double div = 0, gpx = 0, gpy = 0, gpz = 0;
     const uint32_t cnt = L.ntot[i], half = ((cnt + 7u) >> 3) << 2;
     // per pair, with g = (dW/r) d (d = x_j - x_i) and w = (m_j / rho_j) dW/r:
@@ -1518,11 +1536,7 @@ __device__ __forceinline__ void clean_particle(const Particles& P, const Derived
         double dx, dy, dz;
         pair_vec<PER>(g, xi, yi, zi, xy.x, xy.y, zr.x, dx, dy, dz);
         const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        const bool ok = r2 > 1e-20;
-        double r, inv_r;
-        sqrt_rsqrt1(ok ? r2 : 1.0, r, inv_r);
-        const PairH ph = pair_h<UNI>(kp, hi, UNI ? hi : P.h[j]);
-        const double dwr = ((ph.sig_g * dw_shape(r * ph.inv_h)) * inv_r) * (ok ? 1.0 : 0.0);
+        const double dwr = mhd_dwr<UNI>(kp, r2, hi, UNI ? hi : P.h[j]);
         const double w = ((UNI ? kp.m0 : P.m[j]) * rcp_nr1(zr.y)) * dwr;
         div = fma(w, fma(bzp.x - bzi, dz, fma(bxy.y - byi, dy, (bxy.x - bxi) * dx)), div);
         const double dp = w * (bzp.y - psii);
@@ -1546,7 +1560,7 @@ __device__ __forceinline__ void clean_particle(const Particles& P, const Derived
     double2* rb = reinterpret_cast<double2*>(D.recb_next + i);
     rb[0] = make_double2(bx, by);
     rb[1] = make_double2(bz, psi);
-    if (kp.srec) D.recs[i] = stress_record(bx, by, bz, D.rho[i]);
+    if (kp.srec) D.recs[i] = stress_record(bx, by, bz, own[1].y);
 }
 
 template <bool UNI, int PER>
@@ -1556,9 +1570,10 @@ __global__ void __launch_bounds__(kBlock) k_div_clean(Particles P, Derived D, ui
     if (i >= n) return;
     const GlobalFetch<double2, 4, 2> F{reinterpret_cast<const double2*>(D.r64),
                                        reinterpret_cast<const double2*>(D.recb)};
-    clean_particle<UNI, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
-        walk_ahead<4>(L, i, kb, ke, F, body);
-    });
+    clean_particle<UNI, PER>(P, D, g, L, kp, i, [&](double2 (&q)[6]) { F.own(i, q); },
+                             [&](uint32_t kb, uint32_t ke, auto&& body) {
+                                 walk_ahead<4>(L, i, kb, ke, F, body);
+                             });
 }
 
 // Brick-staged, as k_induction_staged (bit-identical to k_div_clean).
@@ -1573,7 +1588,8 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_div_clean_staged(Particles
     brick_pass<double2, 4, 2, true, 2, 2>(
         B, g, start, cap, L, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const Fetch& f) {
-            clean_particle<UNI, PER>(P, D, g, L, kp, i, [&](uint32_t kb, uint32_t ke, auto&& body) {
+            clean_particle<UNI, PER>(P, D, g, L, kp, i, [&](double2 (&q)[6]) { f.own(i, q); },
+                                     [&](uint32_t kb, uint32_t ke, auto&& body) {
                 Fetch ff = f;
                 ff.kbeg = kb;
                 ff.kend = ke;
