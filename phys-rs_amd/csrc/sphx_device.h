@@ -825,12 +825,16 @@ __device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uin
 // PIN_: hold the walk's list prefetch at the top of each step with a full
 // scheduling barrier (always done in two-entry walks; in four-entry walks
 // only where it measured faster).
-template <class Q, int NQ, int W_ = 2, bool PF_ = false, int NQ2 = 0, bool PIN_ = false>
+// T64_: the pass uses only the first 8 bytes of a record's last quad, read
+// from LDS as 8 bytes (its second half is left 0): fewer bytes per random
+// gather, the LDS cost that bounds the staged passes after their VALU work.
+template <class Q, int NQ, int W_ = 2, bool PF_ = false, int NQ2 = 0, bool PIN_ = false, bool T64_ = false>
 struct BrickFetch {
     static_assert(sizeof(Q) == 16, "records are made of 16-byte quads");
     static constexpr int W = W_;
     static constexpr bool PF = PF_;
     static constexpr bool PIN = PIN_ || W_ == 2;
+    static constexpr bool T64 = T64_;
     static constexpr int NR = NQ + NQ2;  // quads per record: NQ from glob, then NQ2 from glob2
     const SPHX_LDS lds_quad_t* rc;    // the row-code table of the particle's brick row
     const SPHX_LDS lds_quad_t* code;  // per entry code: the cell offset
@@ -858,8 +862,17 @@ struct BrickFetch {
     }
     __device__ __forceinline__ void from_lds(uint32_t addr, Q (&q)[NR]) const {
 #pragma unroll
-        for (int k = 0; k < NR; ++k)
-            q[k] = lds_quad<Q>((const SPHX_LDS lds_quad_t*)(size_t)(addr + (uint32_t)k * qstride));
+        for (int k = 0; k < NR; ++k) {
+            const uint32_t a = addr + (uint32_t)k * qstride;
+            if (T64 && k == NR - 1) {
+                typedef uint32_t lds_pair_t __attribute__((ext_vector_type(2)));
+                const lds_pair_t v = *(const SPHX_LDS lds_pair_t*)(size_t)a;
+                const lds_quad_t w = {v.x, v.y, 0u, 0u};
+                __builtin_memcpy(&q[k], &w, sizeof w);
+            } else {
+                q[k] = lds_quad<Q>((const SPHX_LDS lds_quad_t*)(size_t)a);
+            }
+        }
     }
     // the record of the walking particle i itself (from its own cell's halo
     // row: LDS when staged; the same bits as the global record)
@@ -889,8 +902,8 @@ struct BrickFetch {
 // bodies, written branch-free, form one basic block the compiler can
 // interleave.  In a brick whose halo rows are all staged whole (F.whole)
 // every entry is staged and the test is skipped.  flush() as in for_neighbour_groups (it needs W = 4).
-template <class Q, int NQ, int W, bool PF, int NQ2, bool PIN, class Body, class Flush = NoFlush>
-__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF, NQ2, PIN>& F,
+template <class Q, int NQ, int W, bool PF, int NQ2, bool PIN, bool T64, class Body, class Flush = NoFlush>
+__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64>& F,
                                      Body&& body, Flush&& flush = Flush()) {
     constexpr int NR = NQ + NQ2;
     static_assert(W == 2 || W == 4, "two or four entries per iteration");
@@ -925,7 +938,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
                 load(min(k + W, (cnt - 1u) & ~(uint32_t)(W - 1)), a);
                 // keep the load at the top of the iteration (the scheduler
                 // may otherwise sink it next to its wait)
-                if constexpr (BrickFetch<Q, NQ, W, PF, NQ2, PIN>::PIN) __builtin_amdgcn_sched_barrier(0);
+                if constexpr (BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64>::PIN) __builtin_amdgcn_sched_barrier(0);
             } else {
                 load(k, c);
             }
@@ -978,7 +991,8 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
 // quads each) and, with NQ2 > 0, of `src2` (NQ2 quads each, read as quads NQ
 // .. NQ + NQ2 - 1), then body(i, fetch) for each of the brick's particles.
 // LANES = 2: two neighbouring lanes share each particle (body sees both).
-template <class Q, int NQ, int W, bool PF, int LANES = 1, int NQ2 = 0, bool PIN = false, class Body>
+template <class Q, int NQ, int W, bool PF, int LANES = 1, int NQ2 = 0, bool PIN = false, bool T64 = false,
+          class Body>
 __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const uint32_t* __restrict__ start,
                                            uint32_t cap, const NbrList& L, const Q* __restrict__ src, BrickHdr& H,
                                            uint4* __restrict__ sh, Body&& body, const Q* __restrict__ src2 = nullptr) {
@@ -999,13 +1013,13 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
     }
     brick_stage<NQ, NQ2>(H, (int)nh, reinterpret_cast<const uint4*>(src), reinterpret_cast<const uint4*>(src2), rec,
                          cap, rc, no, BY, hy);
-    const BrickFetch<Q, NQ, W, PF, NQ2, PIN> base_fetch{(const SPHX_LDS lds_quad_t*)rc,
+    const BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64> base_fetch{(const SPHX_LDS lds_quad_t*)rc,
                                                    (const SPHX_LDS lds_quad_t*)HL->code, src, src2, cap * 16u,
                                                    (uint32_t)__builtin_amdgcn_readfirstlane(HL->whole)};
     for (uint32_t u = u0; u < nown; u += blockDim.x / LANES) {
         while (u >= HL->own_cum[k + 1]) ++k;
         const uint32_t i = HL->own_start[k] + (u - HL->own_cum[k]);
-        BrickFetch<Q, NQ, W, PF, NQ2, PIN> f = base_fetch;
+        BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64> f = base_fetch;
         f.rc += k * (uint32_t)kRowCodes;
         f.n_list = u == u0 ? n0 : (uint32_t)L.ntot[i];
         body(i, f);

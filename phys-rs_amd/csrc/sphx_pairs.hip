@@ -368,7 +368,7 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
                                                  const Fetch& fetch) {
     double2 own[2];  // the particle's own position record (x, y | z, h)
     fetch.own(i, own);
-    const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = own[1].y;
+    const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = UNI ? kp.h0 : own[1].y;
     double rho = (UNI ? kp.m0 : P.m[i]) * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
     // the epilogue's per-particle loads, issued ahead of the walk
     const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
@@ -444,8 +444,9 @@ __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_
     uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    brick_pass<double2, 2, W, PF>(B, g, start, cap, L, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
-                                  [&](uint32_t i, const BrickFetch<double2, 2, W, PF>& f) {
+    // uniform h: the neighbour's h (the last 8 bytes of its record) is not read
+    brick_pass<double2, 2, W, PF, 1, 0, false, UNI>(B, g, start, cap, L, reinterpret_cast<const double2*>(P.pos), H,
+                                                    sh_dyn, [&](uint32_t i, const auto& f) {
                                       density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f);
                                   });
 }
@@ -847,9 +848,11 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
     // the next two list entries loaded ahead (with the stress: 4M MHD box
     // 1.69 -> 1.64 ms per launch despite a few spilled registers)
     constexpr bool PF = true;
-    brick_pass<double2, 4, W, PF, 2, NQ2>(
+    // VISC_ONLY without the stress: the neighbour's P/rho^2 (the last 8 bytes
+    // of its record) is not read
+    brick_pass<double2, 4, W, PF, 2, NQ2, false, VISC_ONLY && !STRESS>(
         B, g, start, cap, L, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
-        [&](uint32_t i, const BrickFetch<double2, 4, W, PF, NQ2>& f) {
+        [&](uint32_t i, const auto& f) {
             if (kp.beta == 0.0)
                 force_particle<VISC_ONLY, ENERGY, STRESS, UNI, true, PER, 2>(P, D, g, L, kp, charge, i, f);
             else
