@@ -342,9 +342,8 @@ __device__ __forceinline__ void set_record_velocity(const Derived& D, const Kern
         D.r32[i].vy = (float)vy;
         D.r32[i].vz = (float)vz;
     }
-    if (kp.rec64) {
-        D.r64[i].vx = vx;
-        D.r64[i].vy = vy;
+    if (kp.rec64) {  // (vx, vy) as one 16-byte store (offset 32 of the 64-byte record)
+        *reinterpret_cast<double2*>(&D.r64[i].vx) = make_double2(vx, vy);
         D.r64[i].vz = vz;
     }
 }
