@@ -472,9 +472,10 @@ __device__ __forceinline__ void cell_slots(const Grid& g, const uint32_t* __rest
 // =========================================================================
 // Entry k of sorted particle i (layout in NbrList): group k / 4 of the
 // particle's list at nbr_base + (k / 4) * 256, entry k % 4 within it.
-__device__ __forceinline__ uint32_t* nbr_base(const NbrList& L, uint32_t i) {
-    return L.idx + ((((size_t)(i >> 6) * (uint32_t)(L.ks >> 2)) << 6) + (i & 63u)) * 4u;
+__device__ __forceinline__ size_t nbr_index(const NbrList& L, uint32_t i) {
+    return ((((size_t)(i >> 6) * (uint32_t)(L.ks >> 2)) << 6) + (i & 63u)) * 4u;
 }
+__device__ __forceinline__ uint32_t* nbr_base(const NbrList& L, uint32_t i) { return L.idx + nbr_index(L, i); }
 __device__ __forceinline__ size_t nbr_off(uint32_t k) { return ((size_t)(k >> 2) << 8) + (k & 3u); }
 // the four entries of the group holding entry k (k a multiple of 4)
 __device__ __forceinline__ uint4 nbr_group(const uint32_t* __restrict__ e, uint32_t k) {
@@ -584,6 +585,7 @@ static constexpr uint32_t kOwnCode = 13u << kIdxBits;
 template <class Q, int NQ, int NQ2 = 0>
 struct GlobalFetch {
     static constexpr int NR = NQ + NQ2;  // quads per record: NQ from glob, then NQ2 from glob2
+    static constexpr bool BREL = false;  // positions cell-relative: pair vectors add the entry's cell offset
     const Q* __restrict__ glob;
     const Q* __restrict__ glob2 = nullptr;
     uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries
@@ -667,13 +669,64 @@ struct BrickHdr {
     uint32_t whole;                     // 1: every halo row is staged whole (no list entry is read from global)
 };
 
-// Brick `id`'s halo rows (cells x0 - 1 .. x0 + b[0] of rows y0 - 1 .. y0 + b[1],
-// z0 - 1 .. z0 + b[2], wrapped in a periodic box, clamped in a closed one;
-// the first `cap` records are staged), its own rows and the table of entry
-// codes.  A periodic row that wraps in x stages only its unwrapped part.
-// Needs blockDim.x >= 155.
+// Halo row `lane` (< (b[1] + 2) (b[2] + 2)) of brick `id`: cells x0 - 1 ..
+// x0 + b[0] of row y0 - 1 + lane % (b[1] + 2), z0 - 1 + lane / (b[1] + 2),
+// wrapped in a periodic box, clamped in a closed one (a row outside it is
+// empty).  gs: its first sorted particle, gl: its particles; cut: a periodic
+// row that wraps in x (only its unwrapped part is staged).  Called by one
+// wave per brick, lane = row; the slot base of a row is the exclusive prefix
+// of gl over the rows (brick_row_base).
+__device__ __forceinline__ void brick_row(uint32_t id, const Bricks& B, const Grid& g,
+                                          const uint32_t* __restrict__ start, int lane, uint32_t& gs,
+                                          uint32_t& gl, bool& cut, bool* whole_x = nullptr) {
+    const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+    const int BX = B.b[0], BY = B.b[1], BZ = B.b[2];
+    const uint32_t nbx = (uint32_t)B.nb[0], nby = (uint32_t)B.nb[1];
+    const int x0 = (int)(id % nbx) * BX, y0 = (int)((id / nbx) % nby) * BY, z0 = (int)(id / (nbx * nby)) * BZ;
+    const int hy = BY + 2, nh = hy * (BZ + 2);
+    gs = 0;
+    gl = 0;
+    cut = false;
+    if (lane >= nh) return;
+    int y = y0 - 1 + lane % hy, z = z0 - 1 + lane / hy;
+    bool ok = true;
+    if (g.periodic) {
+        y = (y % ny + ny) % ny;
+        z = (z % nz + nz) % nz;
+    } else {
+        ok = y >= 0 && y < ny && z >= 0 && z < nz;
+    }
+    if (!ok) return;
+    int xa = x0 - 1, xe = x0 + BX;
+    const bool wx = g.periodic && xe - xa + 1 >= nx;  // the row holds the whole period in x
+    if (wx) { xa = 0; xe = nx - 1; }
+    cut = g.periodic && (xa < 0 || xe > nx - 1);
+    xa = max(xa, 0);
+    xe = min(xe, nx - 1);
+    const uint32_t row = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+    gs = start[row + (uint32_t)xa];
+    gl = start[row + (uint32_t)xe + 1u] - gs;
+    if (whole_x) *whole_x = wx;
+}
+// exclusive prefix of gl over the lanes of the wave: the LDS slot of the
+// row's first record
+__device__ __forceinline__ uint32_t brick_row_base(uint32_t gl) {
+    const int lane = (int)(threadIdx.x & 63u);
+    uint32_t incl = gl;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    return incl - gl;
+}
+
+// Brick `id`'s halo rows (brick_row; the first `cap` records are staged),
+// its own rows and the table of entry codes.  Needs blockDim.x >= 155.
+// BREL (records staged brick-relative): a row that holds the whole period in
+// x is not staged, since its cells would need two images.
 __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const Grid& g,
-                                            const uint32_t* __restrict__ start, uint32_t cap, BrickHdr& H) {
+                                            const uint32_t* __restrict__ start, uint32_t cap, BrickHdr& H,
+                                            bool brel = false) {
     const int nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
     const int BX = B.b[0], BY = B.b[1], BZ = B.b[2];
     const uint32_t nbx = (uint32_t)B.nb[0], nby = (uint32_t)B.nb[1];
@@ -681,37 +734,12 @@ __device__ __forceinline__ void brick_setup(uint32_t id, const Bricks& B, const 
     const uint32_t t = threadIdx.x;
     const int lane = (int)(t & 63u);
     if (t < 64) {
-        const int hy = BY + 2, nh = hy * (BZ + 2);
-        uint32_t gs = 0, gl = 0;
-        bool cut = false;
-        if (lane < nh) {
-            int y = y0 - 1 + lane % hy, z = z0 - 1 + lane / hy;
-            bool ok = true;
-            if (g.periodic) {
-                y = (y % ny + ny) % ny;
-                z = (z % nz + nz) % nz;
-            } else {
-                ok = y >= 0 && y < ny && z >= 0 && z < nz;
-            }
-            if (ok) {
-                int xa = x0 - 1, xe = x0 + BX;
-                if (g.periodic && xe - xa + 1 >= nx) { xa = 0; xe = nx - 1; }
-                // a periodic row cut at x = 0 or nx - 1 leaves its wrapped cells unstaged
-                cut = g.periodic && (xa < 0 || xe > nx - 1);
-                xa = max(xa, 0);
-                xe = min(xe, nx - 1);
-                const uint32_t row = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
-                gs = start[row + (uint32_t)xa];
-                gl = start[row + (uint32_t)xe + 1u] - gs;
-            }
-        }
-        uint32_t incl = gl;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
-        }
-        const uint32_t base = incl - gl;
-        const uint32_t staged = base >= cap ? 0u : min(gl, cap - base);
+        const int nh = (BY + 2) * (BZ + 2);
+        uint32_t gs, gl;
+        bool cut, wx;
+        brick_row(id, B, g, start, lane, gs, gl, cut, &wx);
+        const uint32_t base = brick_row_base(gl);
+        const uint32_t staged = base >= cap || (brel && wx) ? 0u : min(gl, cap - base);
         if (lane < nh) H.halo[lane] = make_uint4(gs, staged, base, 0u);
         const bool whole = __all(staged == gl && !cut) != 0;
         if (lane == 0) H.whole = whole ? 1u : 0u;
@@ -773,6 +801,24 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 // (adj = rec + (lb - gs) * 16 - (c << 31), all modulo 2^32).
 static constexpr int kRowCodes = 27;
 
+// A cell-relative f32 position moved to the brick origin: x + k_x w_x etc.
+__device__ __forceinline__ void brel_shift(float4& q, int kx, int ky, int kz, float wx, float wy, float wz) {
+    q.x = q.x + (float)kx * wx;
+    q.y = q.y + (float)ky * wy;
+    q.z = q.z + (float)kz * wz;
+}
+
+// What the staging needs to store FP32-mixed positions brick-relative
+// (BrickFetch BREL): halo row r holds cells y = ky0 + r % hy, z = kz0 + r / hy
+// from the origin; a record's x cell comes from its sorted cell key.
+struct BrelStage {
+    const uint32_t* __restrict__ skey;
+    int ox0, ky0, kz0, hy;             // origin cell x; cell offsets of halo row 0
+    uint32_t nx;
+    double inv_nx;
+    float wx, wy, wz;
+};
+
 // Copy the staged records (NQ 16-byte quads each) into LDS, quad-major: quad
 // k of staged record r at dst[k * cap + r], so that one quad of 64 random
 // records spreads over all LDS banks (record-major, a 64-byte record stride
@@ -781,10 +827,11 @@ static constexpr int kRowCodes = 27;
 // per lane in flight: the eight loads (past the row's end: its last quad
 // again, stored again to its own slot) and then the eight stores, with no
 // branch between them (a branch would let the compiler sink each load to
-// its store).
-template <int NQ>
+// its store).  BREL: quad 0 of each record (a cell-relative float4
+// position) is stored brick-relative, its record's cell key loaded beside it.
+template <int NQ, bool BREL = false>
 __device__ __forceinline__ void brick_copy(const BrickHdr& H, int nh, const uint4* __restrict__ src,
-                                           uint4* __restrict__ dst, uint32_t cap) {
+                                           uint4* __restrict__ dst, uint32_t cap, const BrelStage& bs) {
     const SPHX_LDS BrickHdr* HL = (const SPHX_LDS BrickHdr*)&H;
     const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
@@ -796,17 +843,35 @@ __device__ __forceinline__ void brick_copy(const BrickHdr& H, int nh, const uint
 #pragma unroll
             for (int u = 0; u < 8; ++u) qq[u] = min(q0 + 64u * (uint32_t)u, nq - 1u);
             uint4 v[8];
+            uint32_t key[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = s[qq[u]];
+            for (int u = 0; u < 8; ++u) {
+                v[u] = s[qq[u]];
+                if constexpr (BREL) key[u] = bs.skey[gs + qq[u] / NQ];
+            }
+            if constexpr (BREL) {
+                const int ky = bs.ky0 + r % bs.hy, kz = bs.kz0 + r / bs.hy;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    float4 f;
+                    __builtin_memcpy(&f, &v[u], sizeof f);
+                    const int kx = (int)cell_x_of(key[u], bs.nx, bs.inv_nx) - bs.ox0;
+                    brel_shift(f, kx, ky, kz, bs.wx, bs.wy, bs.wz);
+                    uint4 t;
+                    __builtin_memcpy(&t, &f, sizeof t);
+                    if (qq[u] % NQ == 0) v[u] = t;  // quad 0 only
+                }
+            }
 #pragma unroll
             for (int u = 0; u < 8; ++u) dst[(qq[u] % NQ) * cap + base + qq[u] / NQ] = v[u];
         }
     }
 }
-template <int NQ, int NQ2 = 0>
+template <int NQ, int NQ2 = 0, bool BREL = false>
 __device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uint4* __restrict__ src,
                                             const uint4* __restrict__ src2, uint4* __restrict__ dst, uint32_t cap,
-                                            uint4* __restrict__ rc, uint32_t no, uint32_t by, uint32_t hy) {
+                                            uint4* __restrict__ rc, uint32_t no, uint32_t by, uint32_t hy,
+                                            const BrelStage& bs) {
     for (uint32_t t = threadIdx.x; t < no * (uint32_t)kRowCodes; t += blockDim.x) {
         const uint32_t k = t / (uint32_t)kRowCodes, c = t - k * (uint32_t)kRowCodes;
         const uint32_t st = c / 3u, sy = st % 3u, sz = st / 3u;
@@ -814,8 +879,8 @@ __device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uin
         const uint32_t adj = lds_addr(dst) + (h.z - h.x) * 16u - (c << 31);
         rc[t] = make_uint4(h.x | (c << kIdxBits), h.y, adj, 0u);
     }
-    brick_copy<NQ>(H, nh, src, dst, cap);
-    if constexpr (NQ2 > 0) brick_copy<NQ2>(H, nh, src2, dst + (size_t)NQ * cap, cap);
+    brick_copy<NQ, BREL>(H, nh, src, dst, cap, bs);
+    if constexpr (NQ2 > 0) brick_copy<NQ2>(H, nh, src2, dst + (size_t)NQ * cap, cap, bs);
     __syncthreads();
 }
 
@@ -827,13 +892,28 @@ __device__ __forceinline__ void brick_stage(const BrickHdr& H, int nh, const uin
 // T64_: the pass uses only the first 8 bytes of a record's last quad, read
 // from LDS as 8 bytes (its second half is left 0): fewer bytes per random
 // gather, the LDS cost that bounds the staged passes after their VALU work.
-template <class Q, int NQ, int W_ = 2, bool PF_ = false, int NQ2 = 0, bool PIN_ = false, bool T64_ = false>
+// S16_: in a brick whose halo is staged whole the walk reads the particle's
+// slot list (NbrList::s16) instead of its entries, and the body receives the
+// slot in place of the entry: only for bodies that use neither the
+// neighbour's sorted index nor its cell code.
+// BREL_ (FP32-mixed records, float4 quad 0 = cell-relative x, y, z): every
+// record the fetch returns holds its position relative to the brick's origin
+// cell (x0 + b[0]/2, y0 + b[1]/2, z0 + b[2]/2): x + k_x w_x (f32) with k the
+// cell's offset from the origin, converted once when staged (brick_copy) and
+// on the fly when read from global memory (from the entry's cell code), the
+// same f32 operations either way.  A pair vector is then x_j - x_i, with no
+// cell offset.
+template <class Q, int NQ, int W_ = 2, bool PF_ = false, int NQ2 = 0, bool PIN_ = false, bool T64_ = false,
+          bool S16_ = false, bool BREL_ = false>
 struct BrickFetch {
     static_assert(sizeof(Q) == 16, "records are made of 16-byte quads");
+    static_assert(!BREL_ || std::is_same<Q, float4>::value, "brick-relative records are FP32-mixed records");
     static constexpr int W = W_;
     static constexpr bool PF = PF_;
     static constexpr bool PIN = PIN_ || W_ == 2;
     static constexpr bool T64 = T64_;
+    static constexpr bool S16 = S16_;
+    static constexpr bool BREL = BREL_;
     static constexpr int NR = NQ + NQ2;  // quads per record: NQ from glob, then NQ2 from glob2
     const SPHX_LDS lds_quad_t* rc;    // the row-code table of the particle's brick row
     const SPHX_LDS lds_quad_t* code;  // per entry code: the cell offset
@@ -841,6 +921,10 @@ struct BrickFetch {
     const Q* __restrict__ glob2;
     uint32_t qstride;                 // bytes from quad k to quad k + 1 of a record (16 cap)
     uint32_t whole;                   // every list entry of the brick's particles is staged (uniform)
+    uint32_t rec0 = 0;                // S16: LDS byte address of slot 0's quad 0
+    const uint16_t* __restrict__ s16 = nullptr;  // S16: the particle's slot list (null: walk the entries)
+    int kx = 0, ky = 0, kz = 0;       // BREL: the walking particle's cell offset from the brick origin
+    float wx = 0, wy = 0, wz = 0;     // BREL: cell widths (f32)
     uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries (kbeg a multiple of 4)
     uint32_t n_list = 0;              // the length of the walking particle's list (loaded by brick_pass)
     __device__ __forceinline__ uint32_t count(const NbrList&, uint32_t) const { return n_list; }
@@ -852,12 +936,17 @@ struct BrickFetch {
     }
     // LDS byte address of a staged entry's record
     __device__ __forceinline__ uint32_t address(uint32_t e) const { return e * 16u + rc[e >> kIdxBits].z; }
-    // cell offset of entry e's neighbour times the cell widths (as entry_offset)
+    // cell offset of entry e's neighbour times the cell widths (as
+    // entry_offset; BREL: none, the records are brick-relative)
     __device__ __forceinline__ void offset(uint32_t e, float, float, float, float& ox, float& oy, float& oz) const {
-        const lds_quad_t t = code[e >> kIdxBits];
-        ox = __uint_as_float(t.x);
-        oy = __uint_as_float(t.y);
-        oz = __uint_as_float(t.z);
+        if constexpr (BREL) {
+            ox = oy = oz = 0.0f;
+        } else {
+            const lds_quad_t t = code[e >> kIdxBits];
+            ox = __uint_as_float(t.x);
+            oy = __uint_as_float(t.y);
+            oz = __uint_as_float(t.z);
+        }
     }
     __device__ __forceinline__ void from_lds(uint32_t addr, Q (&q)[NR]) const {
 #pragma unroll
@@ -890,6 +979,11 @@ struct BrickFetch {
 #pragma unroll
                 for (int k = 0; k < NQ2; ++k) q[NQ + k] = p2[k];
             }
+            if constexpr (BREL) {  // the cell of entry e: code c = stage * 3 + x offset + 1
+                const uint32_t c = e >> kIdxBits, st = (c * 11u) >> 5, sz = (st * 11u) >> 5;
+                const int ox = (int)(c - 3u * st) - 1, oy = (int)(st - 3u * sz) - 1, oz = (int)sz - 1;
+                brel_shift(q[0], kx + ox, ky + oy, kz + oz, wx, wy, wz);
+            }
         }
     }
 };
@@ -901,15 +995,60 @@ struct BrickFetch {
 // bodies, written branch-free, form one basic block the compiler can
 // interleave.  In a brick whose halo rows are all staged whole (F.whole)
 // every entry is staged and the test is skipped.  flush() as in for_neighbour_groups (it needs W = 4).
-template <class Q, int NQ, int W, bool PF, int NQ2, bool PIN, bool T64, class Body, class Flush = NoFlush>
-__device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64>& F,
-                                     Body&& body, Flush&& flush = Flush()) {
+template <class Q, int NQ, int W, bool PF, int NQ2, bool PIN, bool T64, bool S16, bool BREL, class Body,
+          class Flush = NoFlush>
+__device__ __forceinline__ void walk(const NbrList& L, uint32_t i,
+                                     const BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64, S16, BREL>& F, Body&& body,
+                                     Flush&& flush = Flush()) {
     constexpr int NR = NQ + NQ2;
     static_assert(W == 2 || W == 4, "two or four entries per iteration");
     static_assert(W == 4 || std::is_same<typename std::decay<Flush>::type, NoFlush>::value,
                   "groups of four entries need four entries per iteration");
-    const uint32_t* __restrict__ e = nbr_base(L, i);
     const uint32_t cnt = min(F.count(L, i), F.kend);
+    if constexpr (S16) {
+        if (F.whole && F.s16) {
+            // the slot list: a group of four 16-bit slots per 8-byte load,
+            // the next group loaded ahead (the last group again past the
+            // end), walked W entries at a time; record address rec0 + 16 slot
+            const uint16_t* __restrict__ s = F.s16;
+            uint32_t k = F.kbeg;
+            if (k >= cnt) return;
+            const uint32_t last = (cnt - 1u) & ~3u;
+            auto load = [&](uint32_t kk) { return *reinterpret_cast<const uint2*>(s + ((size_t)(kk >> 2) << 8)); };
+            uint2 a = load(k);
+            for (; k + 3u < cnt; k += 4u) {
+                const uint2 c = a;
+                a = load(min(k + 4u, last));
+                if constexpr (PIN) __builtin_amdgcn_sched_barrier(0);
+                const uint32_t sl[4] = {c.x & 0xffffu, c.x >> 16, c.y & 0xffffu, c.y >> 16};
+#pragma unroll
+                for (int u0 = 0; u0 < 4; u0 += W) {
+                    Q q[W][NR];
+#pragma unroll
+                    for (int w = 0; w < W; ++w) F.from_lds(F.rec0 + sl[u0 + w] * 16u, q[w]);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) body(sl[u0 + w], q[w]);
+                    // two-entry walks: the second pair's reads stay after the first pair
+                    if constexpr (W == 2) if (u0 == 0) __builtin_amdgcn_sched_barrier(0);
+                }
+                flush();
+            }
+            if (k < cnt) {
+                const uint32_t sl[3] = {a.x & 0xffffu, a.x >> 16, a.y & 0xffffu};
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    if (k + (uint32_t)u < cnt) {
+                        Q q[NR];
+                        F.from_lds(F.rec0 + sl[u] * 16u, q);
+                        body(sl[u], q);
+                    }
+                }
+                flush();
+            }
+            return;
+        }
+    }
+    const uint32_t* __restrict__ e = nbr_base(L, i);
     // W entries of the list from position k (k a multiple of W): one 16- or
     // 8-byte load
     auto load = [&](uint32_t k, uint32_t (&c)[W]) {
@@ -937,7 +1076,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
                 load(min(k + W, (cnt - 1u) & ~(uint32_t)(W - 1)), a);
                 // keep the load at the top of the iteration (the scheduler
                 // may otherwise sink it next to its wait)
-                if constexpr (BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64>::PIN) __builtin_amdgcn_sched_barrier(0);
+                if constexpr (PIN || W == 2) __builtin_amdgcn_sched_barrier(0);
             } else {
                 load(k, c);
             }
@@ -991,7 +1130,7 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i, const BrickFe
 // .. NQ + NQ2 - 1), then body(i, fetch) for each of the brick's particles.
 // LANES = 2: two neighbouring lanes share each particle (body sees both).
 template <class Q, int NQ, int W, bool PF, int LANES = 1, int NQ2 = 0, bool PIN = false, bool T64 = false,
-          class Body>
+          bool S16 = false, bool BREL = false, class Body>
 __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const uint32_t* __restrict__ start,
                                            uint32_t cap, const NbrList& L, const Q* __restrict__ src, BrickHdr& H,
                                            uint4* __restrict__ sh, Body&& body, const Q* __restrict__ src2 = nullptr) {
@@ -1001,7 +1140,8 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
     // dynamic LDS: the row-code table (brick_rc_quads), then the records
     uint4* rc = sh;
     uint4* rec = sh + brick_rc_quads(B);
-    brick_setup(B.active[xcd_block()], B, g, start, cap, H);
+    const uint32_t id = B.active[xcd_block()];
+    brick_setup(id, B, g, start, cap, H, BREL);
     // the list length of the thread's first particle, loaded while the halo
     // is staged (its latency then overlaps the staging's)
     const uint32_t nown = HL->own_cum[no], u0 = LANES == 2 ? pair_slot(threadIdx.x) : threadIdx.x;
@@ -1010,17 +1150,46 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
         while (u0 >= HL->own_cum[k + 1]) ++k;
         n0 = L.ntot[HL->own_start[k] + (u0 - HL->own_cum[k])];
     }
-    brick_stage<NQ, NQ2>(H, (int)nh, reinterpret_cast<const uint4*>(src), reinterpret_cast<const uint4*>(src2), rec,
-                         cap, rc, no, BY, hy);
-    const BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64> base_fetch{(const SPHX_LDS lds_quad_t*)rc,
-                                                   (const SPHX_LDS lds_quad_t*)HL->code, src, src2, cap * 16u,
-                                                   (uint32_t)__builtin_amdgcn_readfirstlane(HL->whole)};
+    BrelStage bs{};
+    if constexpr (BREL) {
+        bs.skey = B.skey;
+        bs.ox0 = (int)(id % (uint32_t)B.nb[0]) * B.b[0] + B.b[0] / 2;
+        bs.ky0 = -1 - (int)BY / 2;
+        bs.kz0 = -1 - B.b[2] / 2;
+        bs.hy = (int)hy;
+        bs.nx = (uint32_t)g.dims[0];
+        bs.inv_nx = g.inv_nx;
+        bs.wx = (float)g.w[0];
+        bs.wy = (float)g.w[1];
+        bs.wz = (float)g.w[2];
+    }
+    brick_stage<NQ, NQ2, BREL>(H, (int)nh, reinterpret_cast<const uint4*>(src), reinterpret_cast<const uint4*>(src2),
+                               rec, cap, rc, no, BY, hy, bs);
+    using Fetch = BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64, S16, BREL>;
+    Fetch base_fetch{(const SPHX_LDS lds_quad_t*)rc, (const SPHX_LDS lds_quad_t*)HL->code, src, src2, cap * 16u,
+                     (uint32_t)__builtin_amdgcn_readfirstlane(HL->whole)};
+    base_fetch.rec0 = lds_addr(rec);
+    if constexpr (BREL) {
+        base_fetch.wx = (float)g.w[0];
+        base_fetch.wy = (float)g.w[1];
+        base_fetch.wz = (float)g.w[2];
+    }
+    // the slot lists are read only in a brick whose halo is staged whole
+    const bool slots = S16 && L.s16 != nullptr && base_fetch.whole;
     for (uint32_t u = u0; u < nown; u += blockDim.x / LANES) {
         while (u >= HL->own_cum[k + 1]) ++k;
         const uint32_t i = HL->own_start[k] + (u - HL->own_cum[k]);
-        BrickFetch<Q, NQ, W, PF, NQ2, PIN, T64> f = base_fetch;
+        Fetch f = base_fetch;
         f.rc += k * (uint32_t)kRowCodes;
         f.n_list = u == u0 ? n0 : (uint32_t)L.ntot[i];
+        if (slots) f.s16 = L.s16 + nbr_index(L, i);
+        if constexpr (BREL) {
+            // the particle's cell offset from the brick origin (its x only
+            // where some record comes from global memory)
+            f.ky = (int)(k % BY) - (int)BY / 2;
+            f.kz = (int)(k / BY) - B.b[2] / 2;
+            if (!base_fetch.whole) f.kx = (int)cell_x_of(B.skey[i], bs.nx, bs.inv_nx) - bs.ox0;
+        }
         body(i, f);
     }
 }

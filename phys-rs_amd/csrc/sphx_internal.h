@@ -115,6 +115,11 @@ static constexpr uint32_t kMaxParticles = 1u << kIdxBits;
 
 struct NbrList {
     uint32_t* idx;
+    // the same entries as LDS record slots of the particle's brick halo (16
+    // bits each, same index as idx), written by the build when the staged
+    // passes are on; a staged pass reads them in a brick whose halo rows are
+    // all staged whole (null: not kept)
+    uint16_t* s16;
     uint16_t* ntot;            // entries per particle (<= ks)
     uint32_t* overflow;        // [0] largest list seen, [1] some list overflowed, [2] a particle moved
                                // more than skin/2 since the build
@@ -140,6 +145,9 @@ struct Bricks {
     uint32_t* pos;             // exclusive scan of flag [cap + 1]; pos[count] = active bricks
     uint32_t* sums;            // scan scratch
     uint32_t* active;          // ids of the bricks that hold particles, in brick order [cap]
+    uint32_t* adj;             // per brick and halo row (kMaxHaloRows a brick): the LDS slot of a record in
+                               // the row minus its sorted index (mod 2^32), for the slot lists [cap * kMaxHaloRows]
+    const uint32_t* skey;      // cell key of each sorted particle (the sort's)
     int cus;                   // compute units: the staged passes run cus * k workgroups
     int on;                    // staged passes enabled
     int threads;               // workgroup size of the staged passes (256 or 512)
@@ -201,8 +209,10 @@ void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
 void launch_stress_rec(const Particles& P, const Derived& D, uint32_t n, hipStream_t s);
 void launch_bricks(const Grid& g, const Sort& S, const Bricks& B, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
+// B: with the staged passes on (B.on) the build also writes the slot lists (L.s16)
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, bool uniform, const Mixed& X, double h_max, double skin, hipStream_t s);
+                      const NbrList& L, const Bricks& B, bool uniform, const Mixed& X, double h_max, double skin,
+                      hipStream_t s);
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                     const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
                     const uint32_t* cell_start, hipStream_t s);

@@ -64,10 +64,13 @@ namespace sphx {
 // 1e-4 margin, so no neighbour is cut; owners outside that bound do not cut.
 // Rows whose three cells are not consecutive (a periodic wrap) walk the
 // cells one at a time.
+// With the staged passes on, each entry's LDS slot in the particle's brick
+// halo (sorted index + the brick row's slot offset, see k_brick_adj) is
+// written beside it into the slot list, four 16-bit slots a group.
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T, Grid g,
                                                    const uint32_t* __restrict__ skey,
-                                                   const uint32_t* __restrict__ start, NbrList L,
+                                                   const uint32_t* __restrict__ start, NbrList L, Bricks B,
                                                    float reach, double skin) {
     // staged candidates: cell-relative x, y, z and h; three spare records so
     // that a four-slot step may read past the chunk (those slots are masked)
@@ -94,9 +97,26 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
     // the last four accepted entries (b3 newest): the list is written a group
     // of four (16 bytes) at a time
     uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+    // slot lists: the particle's brick (its halo rows' slot offsets at
+    // B.adj[brow + row]) and the last four slots (newest in the top 16 bits)
+    const bool slots = B.on && L.s16 != nullptr;  // uniform
+    uint16_t* __restrict__ out16 = slots ? L.s16 + nbr_index(L, p) : nullptr;
+    uint64_t sb = 0;
+    uint32_t brow = 0;
+    int ky = 0, kz = 0, hyb = 0;
+    if (slots) {
+        const int BX = B.b[0], BY = B.b[1], BZ = B.b[2];
+        const uint32_t bx = (uint32_t)(cx / BX), by = (uint32_t)(tc.cy / BY), bz = (uint32_t)(tc.cz / BZ);
+        brow = ((bz * (uint32_t)B.nb[1] + by) * (uint32_t)B.nb[0] + bx) * (uint32_t)kMaxHaloRows;
+        ky = tc.cy - (int)by * BY;
+        kz = tc.cz - (int)bz * BZ;
+        hyb = BY + 2;
+    }
     for (int s = 0; s < kStages; ++s) {
         Stage st;
         stage_segments(g, start, tc, s, st);
+        // this stage's halo row of the particle's brick (stage = (oz + 1) * 3 + oy + 1)
+        const uint32_t adj = slots && valid ? B.adj[brow + (uint32_t)((ky + s % 3) + hyb * (kz + s / 3))] : 0u;
         uint32_t ra[3], rb[3];
         for (int o = 0; o < 3; ++o) cell_slots(g, start, st, cx, o - 1, ra[o], rb[o]);
         uint32_t self = 0xffffffffu;
@@ -183,12 +203,18 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                     const uint32_t k = (uint32_t)__builtin_ctz(mask);
                     mask &= mask - 1u;
                     if (total < (uint32_t)L.ks) {
+                        const uint32_t j = slot_particle(st, base + k);
                         b0 = b1;
                         b1 = b2;
                         b2 = b3;
-                        b3 = slot_particle(st, base + k) | code;
-                        if ((total & 3u) == 3u)
+                        b3 = j | code;
+                        sb = (sb >> 16) | ((uint64_t)((j + adj) & 0xffffu) << 48);
+                        if ((total & 3u) == 3u) {
                             *reinterpret_cast<uint4*>(out + nbr_off(total - 3u)) = make_uint4(b0, b1, b2, b3);
+                            if (slots)
+                                *reinterpret_cast<uint2*>(out16 + nbr_off(total - 3u)) =
+                                    make_uint2((uint32_t)sb, (uint32_t)(sb >> 32));
+                        }
                     }
                     ++total;
                 }
@@ -237,12 +263,18 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                     if (total < (uint32_t)L.ks) {
                         const uint32_t slot = base + k;
                         const uint32_t o = (slot >= ra[1] ? 1u : 0u) + (slot >= rb[1] ? 1u : 0u);
+                        const uint32_t j = slot_particle(st, slot);
                         b0 = b1;
                         b1 = b2;
                         b2 = b3;
-                        b3 = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
-                        if ((total & 3u) == 3u)
+                        b3 = j | (((uint32_t)s * 3u + o) << kIdxBits);
+                        sb = (sb >> 16) | ((uint64_t)((j + adj) & 0xffffu) << 48);
+                        if ((total & 3u) == 3u) {
                             *reinterpret_cast<uint4*>(out + nbr_off(total - 3u)) = make_uint4(b0, b1, b2, b3);
+                            if (slots)
+                                *reinterpret_cast<uint2*>(out16 + nbr_off(total - 3u)) =
+                                    make_uint2((uint32_t)sb, (uint32_t)(sb >> 32));
+                        }
                     }
                     ++total;
                 }
@@ -312,6 +344,10 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             const uint4 v = r == 1u ? make_uint4(b3, 0u, 0u, 0u)
                                     : (r == 2u ? make_uint4(b2, b3, 0u, 0u) : make_uint4(b1, b2, b3, 0u));
             *reinterpret_cast<uint4*>(out + nbr_off(kept - r)) = v;
+            if (slots) {
+                const uint64_t w = sb >> (16u * (4u - r));
+                *reinterpret_cast<uint2*>(out16 + nbr_off(kept - r)) = make_uint2((uint32_t)w, (uint32_t)(w >> 32));
+            }
         }
         L.ntot[p] = (uint16_t)kept;
     }
@@ -323,14 +359,15 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
 }
 
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, bool uniform, const Mixed& X, double h_max, double skin, hipStream_t s) {
+                      const NbrList& L, const Bricks& B, bool uniform, const Mixed& X, double h_max, double skin,
+                      hipStream_t s) {
     if (!n) return;
     const dim3 grid(tile_grid(n, T)), block(kBlock);
     const float reach = (float)(2.0 * h_max * (1.0 + 1e-4) + skin);
     if (uniform)
-        hipLaunchKernelGGL(k_build<true>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach, skin);
+        hipLaunchKernelGGL(k_build<true>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, B, reach, skin);
     else
-        hipLaunchKernelGGL(k_build<false>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach, skin);
+        hipLaunchKernelGGL(k_build<false>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, B, reach, skin);
 }
 
 // ------------------------------------------------------- density + EOS (a4, a5)
@@ -445,10 +482,9 @@ __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
     // uniform h: the neighbour's h (the last 8 bytes of its record) is not read
-    brick_pass<double2, 2, W, PF, 1, 0, false, UNI>(B, g, start, cap, L, reinterpret_cast<const double2*>(P.pos), H,
-                                                    sh_dyn, [&](uint32_t i, const auto& f) {
-                                      density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f);
-                                  });
+    brick_pass<double2, 2, W, PF, 1, 0, false, UNI>(
+        B, g, start, cap, L, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
+        [&](uint32_t i, const auto& f) { density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f); });
 }
 
 // FP32-mixed density: f32 pair terms on cell-relative coordinates, f64 sum.
@@ -456,10 +492,12 @@ template <bool UNI, class Fetch>
 __device__ __forceinline__ void density_mixed_particle(const Particles& P, const Derived& D, const Mixed& X,
                                                        const Grid& g, const NbrList& L, const KernelParams& kp,
                                                        uint32_t i, const Fetch& fetch) {
-    float4 ownr[1];  // the particle's own position record
+    float4 ownr[1];  // the particle's own position record (brick-relative with Fetch::BREL)
     fetch.own(i, ownr);
     const float4 own = ownr[0];
     const float xi = own.x, yi = own.y, zi = own.z, hi = own.w;
+    // the cell-relative position for the force record
+    const float4 p4i = Fetch::BREL ? X.p4[i] : own;
     const double hd = UNI ? kp.h0 : P.h[i];
     double rho = (UNI ? kp.m0 : P.m[i]) * (1.0 / (kPi * ((hd * hd) * hd)));
     // the epilogue's per-particle loads, issued ahead of the walk
@@ -470,11 +508,18 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     double acc = 0.0;
     float gacc = 0.0f;  // f32 sum over each group of four list entries
     walk(L, i, fetch, [&](uint32_t e, const float4 (&q)[1]) {
-        float ox, oy, oz;
-        fetch.offset(e, wx, wy, wz, ox, oy, oz);
-        const float dx = (q[0].x - xi) + ox;
-        const float dy = (q[0].y - yi) + oy;
-        const float dz = (q[0].z - zi) + oz;
+        float dx, dy, dz;
+        if constexpr (Fetch::BREL) {  // brick-relative records: no cell offset
+            dx = q[0].x - xi;
+            dy = q[0].y - yi;
+            dz = q[0].z - zi;
+        } else {
+            float ox, oy, oz;
+            fetch.offset(e, wx, wy, wz, ox, oy, oz);
+            dx = (q[0].x - xi) + ox;
+            dy = (q[0].y - yi) + oy;
+            dz = (q[0].z - zi) + oz;
+        }
         const float r = __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
         if (UNI) {  // m, sigma_w and the 1/4 of w4_shape are folded out of the sum
             gacc += w4_shape_f(r * inv_h0);
@@ -494,7 +539,7 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     D.p[i] = pr;
     D.pr2[i] = pr2;
     float4* rec = reinterpret_cast<float4*>(D.r32 + i);
-    rec[0] = make_float4(xi, yi, zi, (float)rho);
+    rec[0] = make_float4(p4i.x, p4i.y, p4i.z, (float)rho);
     rec[1] = make_float4((float)vxi, (float)vyi, (float)vzi, (float)pr2);
     if (kp.srec) D.recs[i] = stress_record(P.bx[i], P.by[i], P.bz[i], rho);
 }
@@ -512,10 +557,11 @@ k_density_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, Kerne
                        const uint32_t* __restrict__ start, uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    brick_pass<float4, 1, W, PF>(B, g, start, cap, L, X.p4, H, sh_dyn,
-                                 [&](uint32_t i, const BrickFetch<float4, 1, W, PF>& f) {
-                                     density_mixed_particle<UNI>(P, D, X, g, L, kp, i, f);
-                                 });
+    // brick-relative records; uniform m and h: the pair body needs only the
+    // record (the slot lists)
+    brick_pass<float4, 1, W, PF, 1, 0, false, false, UNI, true>(
+        B, g, start, cap, L, X.p4, H, sh_dyn,
+        [&](uint32_t i, const auto& f) { density_mixed_particle<UNI>(P, D, X, g, L, kp, i, f); });
 }
 
 // Staged launch of a pass with records of rec_bytes that runs at most
@@ -850,7 +896,8 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
     constexpr bool PF = true;
     // VISC_ONLY without the stress: the neighbour's P/rho^2 (the last 8 bytes
     // of its record) is not read
-    brick_pass<double2, 4, W, PF, 2, NQ2, false, VISC_ONLY && !STRESS>(
+    // uniform m and h: the pair body needs only the record (the slot lists)
+    brick_pass<double2, 4, W, PF, 2, NQ2, false, VISC_ONLY && !STRESS, UNI>(
         B, g, start, cap, L, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const auto& f) {
             if (kp.beta == 0.0)
@@ -910,11 +957,18 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     auto pair = [&](uint32_t e, const float4 (&q)[Fetch::NR]) {
         const uint32_t j = e & kIdxMask;
         const float4 a = q[0], b = q[1];
-        float ox, oy, oz;
-        fetch.offset(e, wx, wy, wz, ox, oy, oz);
-        const float dx = (a.x - xi) + ox;
-        const float dy = (a.y - yi) + oy;
-        const float dz = (a.z - zi) + oz;
+        float dx, dy, dz;
+        if constexpr (Fetch::BREL) {  // brick-relative records: no cell offset
+            dx = a.x - xi;
+            dy = a.y - yi;
+            dz = a.z - zi;
+        } else {
+            float ox, oy, oz;
+            fetch.offset(e, wx, wy, wz, ox, oy, oz);
+            dx = (a.x - xi) + ox;
+            dy = (a.y - yi) + oy;
+            dz = (a.z - zi) + oz;
+        }
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         // r <= 1e-10: no force; 1/r = 0 gives r = 0, q = 0 and dW/dq(0) = 0
         const float inv_r = r2 > 1e-20f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
@@ -1074,9 +1128,11 @@ k_forces_mixed_staged(
     extern __shared__ uint4 sh_dyn[];
     // the full pass holds its list prefetch at the top of each step (faster
     // there; slower in the viscosity-only pass)
-    brick_pass<float4, 2, W, PF, 1, 0, !VISC_ONLY>(
+    // brick-relative records; uniform m and h: the pair body needs only the
+    // record (the slot lists)
+    brick_pass<float4, 2, W, PF, 1, 0, !VISC_ONLY, false, UNI, true>(
         B, g, start, cap, L, reinterpret_cast<const float4*>(D.r32), H, sh_dyn,
-        [&](uint32_t i, const BrickFetch<float4, 2, W, PF, 0, !VISC_ONLY>& f) {
+        [&](uint32_t i, const auto& f) {
             if (kp.beta == 0.0)
                 force_mixed_particle<VISC_ONLY, ENERGY, false, UNI, true, 1>(P, D, X, g, L, kp, charge, i, f);
             else

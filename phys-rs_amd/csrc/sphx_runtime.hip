@@ -109,6 +109,7 @@ struct SphxCtx {
     uint32_t* bricks_host = nullptr;       // pinned: active brick count of the last sort
     hipEvent_t bricks_ready = nullptr;     // recorded after that count's copy
     bool bricks_pending = false;           // count not read yet
+    bool slots = false;                    // the list's slot lists are written (staged passes that read them)
     hipEvent_t moved_ready = nullptr;      // recorded after the kernel that raises the moved flag
     uint32_t* moved_host = nullptr;        // pinned, fine-grained: the moved flag as the kernels leave it
     uint32_t epoch = 1;                    // epoch of the moved flag (see MovedFlag); the words start at 0
@@ -463,7 +464,8 @@ int setup_grid(SphxCtx* c, const SphxState* host, const Grid* fixed = nullptr) {
         if (B.on && B.count > B.cap) {
             int rc;
             if ((rc = dalloc(c, &B.flag, B.count)) || (rc = dalloc(c, &B.pos, (size_t)B.count + 1)) ||
-                (rc = dalloc(c, &B.sums, ((size_t)B.count + 1023) / 1024 + 1)) || (rc = dalloc(c, &B.active, B.count)))
+                (rc = dalloc(c, &B.sums, ((size_t)B.count + 1023) / 1024 + 1)) || (rc = dalloc(c, &B.active, B.count)) ||
+                (rc = dalloc(c, &B.adj, (size_t)B.count * kMaxHaloRows)))
                 return rc;
             B.cap = B.count;
         }
@@ -525,6 +527,14 @@ int launch_check(SphxCtx* c) {
 }
 
 // ---- passes ------------------------------------------------------------
+// The list as the pair passes see it: the slot lists only when the build
+// wrote them.
+NbrList pass_list(const SphxCtx* c) {
+    NbrList L = c->L;
+    if (!c->slots) L.s16 = nullptr;
+    return L;
+}
+
 int do_neighbours(SphxCtx* c) {
     const uint32_t n = c->n;
     {
@@ -552,8 +562,11 @@ int do_neighbours(SphxCtx* c) {
     {
         Timer t(c, kNbr);
         ++c->epoch;  // displacements count from this build
-        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->kp.uniform != 0, c->X, c->hmax, c->kp.skin,
-                         c->stream);
+        // the slot lists are read by the FP64 force passes and the FP32-mixed
+        // density and force passes with uniform m and h
+        c->slots = c->B.on && c->kp.uniform;
+        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, pass_list(c), c->B, c->kp.uniform != 0, c->X, c->hmax,
+                         c->kp.skin, c->stream);
     }
     c->sorted = c->listed = true;
     c->list_age = 0;
@@ -577,7 +590,7 @@ int do_density(SphxCtx* c) {
     if ((rc = ensure_list(c))) return rc;
     {
         Timer t(c, kDensity);
-        launch_density(c->P(), c->D, c->n, c->grid, c->L, c->kp, c->mixed(), c->X, c->B, c->S.cell_start,
+        launch_density(c->P(), c->D, c->n, c->grid, pass_list(c), c->kp, c->mixed(), c->X, c->B, c->S.cell_start,
                        c->stream);
     }
     c->dens = c->pres = true;
@@ -607,7 +620,7 @@ int do_forces(SphxCtx* c, bool visc_only) {
     }
     {
         Timer t(c, visc_only ? kViscOnly : kForce);
-        launch_forces(c->P(), c->D, c->n, c->grid, c->L, c->kp, visc_only, c->energy(), stress, charge,
+        launch_forces(c->P(), c->D, c->n, c->grid, pass_list(c), c->kp, visc_only, c->energy(), stress, charge,
                       c->mixed(), c->X, c->B, c->S.cell_start, c->stream);
     }
     c->sp = true;
@@ -1029,8 +1042,10 @@ int sphx_create(int device, uint32_t capacity, const SphxParams* params, SphxCtx
     if (dalloc(c, &c->S.key, N) || dalloc(c, &c->S.rank, N) || dalloc(c, &c->S.perm, N) ||
         dalloc(c, &c->S.skey, N) || dalloc(c, &c->S.tmp, N))
         return bail(SPHX_E_OOM);
+    c->B.skey = c->S.skey;
     if (dalloc(c, &c->L.ntot, N) || dalloc(c, &c->L.overflow, 4) ||
-        dalloc(c, &c->L.idx, ((N + 63) / 64) * 64 * (size_t)c->L.ks))
+        dalloc(c, &c->L.idx, ((N + 63) / 64) * 64 * (size_t)c->L.ks) ||
+        dalloc(c, &c->L.s16, ((N + 63) / 64) * 64 * (size_t)c->L.ks))
         return bail(SPHX_E_OOM);
     if (dalloc(c, &c->energy_partial, (size_t)energy_blocks(c->cap) * 9)) return bail(SPHX_E_OOM);
     if (dalloc(c, &c->pair_counter, 1)) return bail(SPHX_E_OOM);

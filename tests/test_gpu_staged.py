@@ -3,10 +3,15 @@
 The staged passes copy the neighbours' gather records of a brick of cells
 into LDS and walk the same neighbour lists against LDS, reading from global
 memory whatever is not staged (a periodic row that wraps in x, a halo larger
-than LDS).  The records, the order of every sum and the arithmetic are the
-same, so densities, accelerations and whole trajectories must be
-bit-identical to the unstaged passes.  SPHX_BRICKS is read when a context is
-created: "off" runs the unstaged passes, "bx,by,bz" sets the brick shape.
+than LDS).  In the FP64 form the records, the order of every sum and the
+arithmetic are the same, so densities, accelerations and whole trajectories
+must be bit-identical to the unstaged passes.  The staged FP32-mixed density
+and force passes hold positions relative to the brick's origin cell (the
+unstaged ones add each pair's cell offset to cell-relative positions), so
+their f32 pair vectors round differently: there the two must agree to well
+inside the mixed form's own error against FP64.  SPHX_BRICKS is read when a
+context is created: "off" runs the unstaged passes, "bx,by,bz" sets the
+brick shape.
 """
 import os
 
@@ -45,6 +50,26 @@ def assert_identical(a, b):
         assert np.array_equal(a[k], b[k]), k
 
 
+# staged against unstaged FP32-mixed: relative density (each form within
+# 1e-6 of FP64), acceleration relative to the largest (the bar the mixed form
+# is held to against FP64), and the state after the ticks relative to its
+# largest component
+MIXED_TOL = {"rho": 2e-6, "ax": 1e-4, "ay": 1e-4, "az": 1e-4}
+
+
+def assert_matches(got, ref, precision):
+    if precision == sphx.PREC_FP64:
+        assert_identical(got, ref)
+        return
+    for k in ref:
+        a, b = got[k], ref[k]
+        if k == "rho":
+            err = np.abs(a / b - 1.0).max()
+        else:
+            err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        assert err <= MIXED_TOL.get(k, 1e-6), (k, err)
+
+
 def random_box(n, seed, periodic, vsig=0.3):
     rng = scenarios.SplitMix64(seed)
     st = {k: rng.uniform(n) for k in ("x", "y", "z")}
@@ -69,7 +94,7 @@ def test_staged_dam_break_bit_identical(require_gpu, shape):
 
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
-def test_staged_random_variable_h_bit_identical(require_gpu, periodic, precision):
+def test_staged_random_variable_h_matches_unstaged(require_gpu, periodic, precision):
     # per-particle h and m; in the periodic box the first and last bricks of
     # each row wrap in x, whose wrapped cell is read from global memory
     st, p = random_box(4000, 11, periodic)
@@ -77,44 +102,44 @@ def test_staged_random_variable_h_bit_identical(require_gpu, periodic, precision
     ref = run(st, p, "off", 6)
     for shape in ("3,3,3", "2,4,1"):
         got = run(st, p, shape, 6)
-        assert_identical(got[0], ref[0])
-        assert_identical(got[1], ref[1])
+        assert_matches(got[0], ref[0], precision)
+        assert_matches(got[1], ref[1], precision)
 
 
-def test_staged_mixed_dam_break_bit_identical(require_gpu):
+def test_staged_mixed_dam_break_matches_unstaged(require_gpu):
     st, p = scenarios.dam_break(v_sigma=0.1, seed=5)
     p.precision = sphx.PREC_FP32_MIXED
     ref = run(st, p, "off", 12)
     got = run(st, p, "3,3,3", 12)
-    assert_identical(got[0], ref[0])
-    assert_identical(got[1], ref[1])
+    assert_matches(got[0], ref[0], p.precision)
+    assert_matches(got[1], ref[1], p.precision)
 
 
 @pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
 @pytest.mark.parametrize("variable", [False, True])
-def test_staged_adiabatic_gas_bit_identical(require_gpu, precision, variable):
+def test_staged_adiabatic_gas_matches_unstaged(require_gpu, precision, variable):
     # the energy equation's sums in the staged force passes (u compared too)
     st, p = scenarios.ideal_gas_box(16, variable=variable)
     p.precision = precision
     ref = run(st, p, "off", 6)
     for shape in ("3,3,3", "2,2,2"):
         got = run(st, p, shape, 6)
-        assert_identical(got[0], ref[0])
-        assert_identical(got[1], ref[1])
+        assert_matches(got[0], ref[0], precision)
+        assert_matches(got[1], ref[1], precision)
 
 
 @pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
-def test_staged_mhd_periodic_bit_identical(require_gpu, precision):
+def test_staged_mhd_periodic_matches_unstaged(require_gpu, precision):
     st, p = scenarios.mhd_periodic(16)
     p.precision = precision
     ref = run(st, p, "off", 4)
     got = run(st, p, "3,3,3", 4)
-    assert_identical(got[0], ref[0])
-    assert_identical(got[1], ref[1])
+    assert_matches(got[0], ref[0], precision)
+    assert_matches(got[1], ref[1], precision)
 
 
 @pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
-def test_staged_mhd_variable_h_bit_identical(require_gpu, precision):
+def test_staged_mhd_variable_h_matches_unstaged(require_gpu, precision):
     # per-particle h and m: the non-uniform Maxwell-stress force passes, whose
     # staged records carry the B records beside the gather records
     st, p = scenarios.mhd_periodic(16)
@@ -126,8 +151,8 @@ def test_staged_mhd_variable_h_bit_identical(require_gpu, precision):
     ref = run(st, p, "off", 4)
     for shape in ("3,3,3", "2,2,2"):
         got = run(st, p, shape, 4)
-        assert_identical(got[0], ref[0])
-        assert_identical(got[1], ref[1])
+        assert_matches(got[0], ref[0], precision)
+        assert_matches(got[1], ref[1], precision)
 
 
 def pass_sequence(state, params, bricks, nsteps, step):
