@@ -923,7 +923,9 @@ struct BrickFetch {
     uint32_t whole;                   // every list entry of the brick's particles is staged (uniform)
     uint32_t rec0 = 0;                // S16: LDS byte address of slot 0's quad 0
     const uint16_t* __restrict__ s16 = nullptr;  // S16: the particle's slot list (null: walk the entries)
-    int kx = 0, ky = 0, kz = 0;       // BREL: the walking particle's cell offset from the brick origin
+    // BREL: the walking particle's cell offset from the brick origin,
+    // (kx + 128) | (ky + 128) << 8 | (kz + 128) << 16 (one register)
+    uint32_t kxyz = 0;
     float wx = 0, wy = 0, wz = 0;     // BREL: cell widths (f32)
     uint32_t kbeg = 0, kend = 0xffffffffu;  // the walk's range of list entries (kbeg a multiple of 4)
     uint32_t n_list = 0;              // the length of the walking particle's list (loaded by brick_pass)
@@ -982,6 +984,8 @@ struct BrickFetch {
             if constexpr (BREL) {  // the cell of entry e: code c = stage * 3 + x offset + 1
                 const uint32_t c = e >> kIdxBits, st = (c * 11u) >> 5, sz = (st * 11u) >> 5;
                 const int ox = (int)(c - 3u * st) - 1, oy = (int)(st - 3u * sz) - 1, oz = (int)sz - 1;
+                const int kx = (int)(kxyz & 255u) - 128, ky = (int)((kxyz >> 8) & 255u) - 128,
+                          kz = (int)(kxyz >> 16) - 128;
                 brel_shift(q[0], kx + ox, ky + oy, kz + oz, wx, wy, wz);
             }
         }
@@ -1186,9 +1190,11 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
         if constexpr (BREL) {
             // the particle's cell offset from the brick origin (its x only
             // where some record comes from global memory)
-            f.ky = (int)(k % BY) - (int)BY / 2;
-            f.kz = (int)(k / BY) - B.b[2] / 2;
-            if (!base_fetch.whole) f.kx = (int)cell_x_of(B.skey[i], bs.nx, bs.inv_nx) - bs.ox0;
+            if (!base_fetch.whole) {
+                const int kx = (int)cell_x_of(B.skey[i], bs.nx, bs.inv_nx) - bs.ox0;
+                const int ky = (int)(k % BY) - (int)BY / 2, kz = (int)(k / BY) - B.b[2] / 2;
+                f.kxyz = (uint32_t)(kx + 128) | ((uint32_t)(ky + 128) << 8) | ((uint32_t)(kz + 128) << 16);
+            }
         }
         body(i, f);
     }

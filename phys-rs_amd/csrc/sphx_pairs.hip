@@ -496,8 +496,6 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     fetch.own(i, ownr);
     const float4 own = ownr[0];
     const float xi = own.x, yi = own.y, zi = own.z, hi = own.w;
-    // the cell-relative position for the force record
-    const float4 p4i = Fetch::BREL ? X.p4[i] : own;
     const double hd = UNI ? kp.h0 : P.h[i];
     double rho = (UNI ? kp.m0 : P.m[i]) * (1.0 / (kPi * ((hd * hd) * hd)));
     // the epilogue's per-particle loads, issued ahead of the walk
@@ -539,6 +537,8 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
     D.p[i] = pr;
     D.pr2[i] = pr2;
     float4* rec = reinterpret_cast<float4*>(D.r32 + i);
+    // the force record keeps the cell-relative position
+    const float4 p4i = Fetch::BREL ? X.p4[i] : own;
     rec[0] = make_float4(p4i.x, p4i.y, p4i.z, (float)rho);
     rec[1] = make_float4((float)vxi, (float)vyi, (float)vzi, (float)pr2);
     if (kp.srec) D.recs[i] = stress_record(P.bx[i], P.by[i], P.bz[i], rho);
