@@ -563,8 +563,11 @@ int do_neighbours(SphxCtx* c) {
         Timer t(c, kNbr);
         ++c->epoch;  // displacements count from this build
         // the slot lists are read by the FP64 force passes and the FP32-mixed
-        // density and force passes with uniform m and h
-        c->slots = c->B.on && c->kp.uniform;
+        // density and force passes with uniform m and h; not with the Maxwell
+        // stress, whose 96-byte FP64 records leave few bricks staged whole
+        // (4M MHD box: the slots cost the build more than the passes gain)
+        const bool stress = c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_MAXWELL_STRESS);
+        c->slots = c->B.on && c->kp.uniform && !stress;
         launch_build_nbr(c->P(), n, c->grid, c->S, c->T, pass_list(c), c->B, c->kp.uniform != 0, c->X, c->hmax,
                          c->kp.skin, c->stream);
     }
