@@ -407,9 +407,6 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
     fetch.own(i, own);
     const double xi = own[0].x, yi = own[0].y, zi = own[1].x, hi = UNI ? kp.h0 : own[1].y;
     double rho = (UNI ? kp.m0 : P.m[i]) * (1.0 / (kPi * ((hi * hi) * hi)));  // m_i W(0, h_i)  (sph.rs:89-91)
-    // the epilogue's per-particle loads, issued ahead of the walk
-    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
-    const double ui = kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0;
     double acc = 0.0;
     const double nih0 = -kp.inv_h0, nih20 = -0.5 * kp.inv_h0;
     auto pair = [&](uint32_t e, const double2 (&q)[2]) {
@@ -448,6 +445,8 @@ __device__ __forceinline__ void density_particle(const Particles& P, const Deriv
         acc = a0 + acc;
     }
     rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 2.0), acc, rho) : rho + acc;
+    const double vxi = P.vx[i], vyi = P.vy[i], vzi = P.vz[i];
+    const double ui = kp.eos == SPHX_EOS_ADIABATIC ? P.u[i] : 0.0;
     const double pr = eos_pressure(kp, rho, ui);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
@@ -481,8 +480,11 @@ __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_
     uint32_t cap) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
-    // uniform h: the neighbour's h (the last 8 bytes of its record) is not read
-    brick_pass<double2, 2, W, PF, 1, 0, false, UNI>(
+    // uniform h: the neighbour's h (the last 8 bytes of its record) is not
+    // read, and the pair body needs only the record (the slot lists); the
+    // epilogue's velocity loads wait for the walk (ahead of it they held
+    // registers that then spilled: 1M density 0.173 -> 0.168 ms)
+    brick_pass<double2, 2, W, PF, 1, 0, false, UNI, UNI>(
         B, g, start, cap, L, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
         [&](uint32_t i, const auto& f) { density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f); });
 }
@@ -519,18 +521,22 @@ __device__ __forceinline__ void density_mixed_particle(const Particles& P, const
             dz = (q[0].z - zi) + oz;
         }
         const float r = __builtin_amdgcn_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-        if (UNI) {  // m, sigma_w and the 1/4 of w4_shape are folded out of the sum
-            gacc += w4_shape_f(r * inv_h0);
+        if (UNI) {  // m, sigma_w and the factor 2 of the one-sided shape are folded out of the sum
+            // W / (2 sigma) = t^3 - s^3 / 2 from the one-sided pieces (as
+            // the f64 pass: 1M mixed density 0.128 -> 0.125 ms)
+            const float ps = fmaxf(fmaf(r, -inv_h0, 1.0f), 0.0f), pt = fmaxf(fmaf(r, -0.5f * inv_h0, 1.0f), 0.0f);
+            gacc += fmaf((ps * ps) * ps, -0.5f, (pt * pt) * pt);
         } else {
             const float inv_h = __builtin_amdgcn_rcpf(0.5f * (hi + q[0].w));
-            const float w = ((inv_h * inv_h) * inv_h * (float)(0.25 * kInvPi)) * w4_shape_f(r * inv_h);
+            const float ps = fmaxf(fmaf(r, -inv_h, 1.0f), 0.0f), pt = fmaxf(fmaf(r, -0.5f * inv_h, 1.0f), 0.0f);
+            const float w = ((inv_h * inv_h) * inv_h * (float)(2.0 * kInvPi)) * fmaf((ps * ps) * ps, -0.5f, (pt * pt) * pt);
             gacc += X.m[e & kIdxMask] * w;
         }
     }, [&]() {
         acc += (double)gacc;
         gacc = 0.0f;
     });
-    rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 0.25), acc, rho) : rho + acc;
+    rho = UNI ? fma(kp.m0 * ((((kp.inv_h0 * kp.inv_h0) * kp.inv_h0) * kInvPi) * 2.0), acc, rho) : rho + acc;
     const double pr = eos_pressure(kp, rho, ui);
     const double pr2 = pr / (rho * rho);
     D.rho[i] = rho;
@@ -978,11 +984,15 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
             h_avg = 0.5f * (hi + X.p4[j].w);
             inv_h = __builtin_amdgcn_rcpf(h_avg);
         }
-        const float dwr0 = dw43_shape_f(r * inv_h) * inv_r;  // its 0.75 is folded in below
+        // dW/dq / 3 = s^2 - t^2 from the one-sided pieces s = (1 - q)+, t =
+        // (1 - q/2)+ formed from r (no q, no select: 1M mixed force 0.155 ->
+        // 0.152 ms); r = 0 gives s = t = 1, so 0
+        const float ps = fmaxf(fmaf(r, -inv_h, 1.0f), 0.0f), pt = fmaxf(fmaf(r, -0.5f * inv_h, 1.0f), 0.0f);
+        const float dwr0 = fmaf(-pt, pt, ps * ps) * inv_r;  // its 3 is folded in below
         float dwr = dwr0;
         if (!UNI) {
             const float ih2 = inv_h * inv_h;
-            dwr = (mi * X.m[j]) * (((ih2 * ih2) * (float)(0.75 * kInvPi)) * dwr0);
+            dwr = (mi * X.m[j]) * (((ih2 * ih2) * (float)(3.0 * kInvPi)) * dwr0);
         }
         if (!VISC_ONLY) {
             const float sc = (pr2i + b.w) * dwr;
@@ -1090,14 +1100,14 @@ __device__ __forceinline__ void force_mixed_particle(const Particles& P, const D
     if (STRESS) stress_close(RS, i, gs, fm);
     if (UNI) {
         const double ih2 = kp.inv_h0 * kp.inv_h0;
-        const double k = ((kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi)) * 0.75;
+        const double k = ((kp.m0 * kp.m0) * ((ih2 * ih2) * kInvPi)) * 3.0;
         const double kv = B0 ? k * cv : k;
         for (int a = 0; a < 3; ++a) {
             fp[a] *= k;
             fv[a] *= kv;
             fm[a] *= k;
         }
-        if (ENERGY) D.du[i] = -de * ((kp.m0 * ((ih2 * ih2) * kInvPi)) * 0.75);
+        if (ENERGY) D.du[i] = -de * ((kp.m0 * ((ih2 * ih2) * kInvPi)) * 3.0);
     } else {
         if (B0)
             for (int a = 0; a < 3; ++a) fv[a] *= cv;
