@@ -129,6 +129,25 @@ def test_staged_adiabatic_gas_matches_unstaged(require_gpu, precision, variable)
 
 
 @pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
+def test_staged_box_of_three_cells_matches_unstaged(require_gpu, precision):
+    # a periodic box three cells wide (uniform h and m, so the slot lists are
+    # walked): every halo row of a 3x3x3 or 2x2x2 brick holds the whole period
+    # in x, and rows that wrap in y or z appear twice in the halo.  The FP64
+    # passes stage such rows (absolute positions, minimum image per pair); the
+    # brick-relative FP32-mixed records cannot (a cell would need two images),
+    # so those entries come from global memory, converted by their cell codes.
+    st, p = scenarios.ideal_gas_box(10)
+    p.precision = precision
+    # cells at least 2 h + skin wide (skin 0.04 h): three along each axis
+    assert int(1.0 / (2.04 * st["h"][0])) == 3
+    ref = run(st, p, "off", 4)
+    for shape in ("3,3,3", "2,2,2"):
+        got = run(st, p, shape, 4)
+        assert_matches(got[0], ref[0], precision)
+        assert_matches(got[1], ref[1], precision)
+
+
+@pytest.mark.parametrize("precision", [sphx.PREC_FP64, sphx.PREC_FP32_MIXED])
 def test_staged_mhd_periodic_matches_unstaged(require_gpu, precision):
     st, p = scenarios.mhd_periodic(16)
     p.precision = precision
