@@ -72,7 +72,7 @@ STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
 MEASURED_TRAFFIC = {
     "force": (302.7 + 50.7) * 1e6,
     "visc_only": (325.9 + 25.3) * 1e6,
-    "density": (429.3 + 101.3) * 1e6,
+    "density": (266.8 + 92.7) * 1e6,
 }
 STEP_FLOPS_PER_PAIR = 26.0 + 58.0 + 45.0
 
