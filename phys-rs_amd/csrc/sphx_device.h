@@ -167,10 +167,6 @@ __device__ __forceinline__ double kernel_d2w(double r, double inv_h) {
     return 0.0;
 }
 
-__device__ __forceinline__ double inv_h_pair(double hi, double hj, double inv_hi) {
-    return (hi == hj) ? inv_hi : 1.0 / (0.5 * (hi + hj));
-}
-
 // ---- f64 arithmetic for the pair passes ----------------------------------
 // The passes match the reference to a relative tolerance (the neighbour
 // predicate, which must round exactly, does not use these).  1/x: hardware
@@ -261,54 +257,11 @@ __device__ __forceinline__ double dw3_shape_r(double r, double nih, double nih2)
     const double t = fmax(fma(r, nih2, 1.0), 0.0);
     return fma(-t, t, s * s);
 }
-__device__ __forceinline__ float w_shape_f(float q) {
-    const float t = fmaxf(2.0f - q, 0.0f);
-    const float a = fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
-    const float b = 0.25f * ((t * t) * t);
-    return q < 1.0f ? a : b;
-}
-__device__ __forceinline__ float w4_shape_f(float q) {
-    const float t = fmaxf(2.0f - q, 0.0f);
-    const float a = fmaf(q * q, fmaf(3.0f, q, -6.0f), 4.0f);
-    return q < 1.0f ? a : (t * t) * t;
-}
 // dw_shape(q) / 0.75 (the negation of t^2 is free)
 __device__ __forceinline__ float dw43_shape_f(float q) {
     const float t = fmaxf(2.0f - q, 0.0f);
     const float a = q * fmaf(3.0f, q, -4.0f);
     return q < 1.0f ? a : -(t * t);
-}
-__device__ __forceinline__ float dw_shape_f(float q) {
-    const float t = fmaxf(2.0f - q, 0.0f);
-    const float a = q * fmaf(2.25f, q, -3.0f);
-    const float b = -0.75f * (t * t);
-    return q < 1.0f ? a : b;
-}
-
-// ---- f32 forms (FP32-mixed) ---------------------------------------------
-__device__ __forceinline__ float kernel_w_f(float r, float inv_h) {
-    const float q = r * inv_h;
-    const float sigma = (inv_h * inv_h) * inv_h * (float)kInvPi;
-    if (q < 1.0f) return sigma * fmaf(q * q, fmaf(0.75f, q, -1.5f), 1.0f);
-    if (q < 2.0f) {
-        const float t = 2.0f - q;
-        return (0.25f * sigma) * ((t * t) * t);
-    }
-    return 0.0f;
-}
-__device__ __forceinline__ float kernel_dw_f(float r, float inv_h) {
-    const float q = r * inv_h;
-    const float ih2 = inv_h * inv_h;
-    const float sigma = (ih2 * ih2) * (float)kInvPi;
-    if (q < 1.0f) return sigma * (q * fmaf(2.25f, q, -3.0f));
-    if (q < 2.0f) {
-        const float t = 2.0f - q;
-        return sigma * (-0.75f * (t * t));
-    }
-    return 0.0f;
-}
-__device__ __forceinline__ float inv_h_pair_f(float hi, float hj, float inv_hi) {
-    return (hi == hj) ? inv_hi : __builtin_amdgcn_rcpf(0.5f * (hi + hj));
 }
 
 // ---- equations of state (sph.rs:157-167; thermodynamics.rs:232-238) -----
