@@ -626,9 +626,10 @@ struct BrickHdr {
 // x0 + b[0] of row y0 - 1 + lane % (b[1] + 2), z0 - 1 + lane / (b[1] + 2),
 // wrapped in a periodic box, clamped in a closed one (a row outside it is
 // empty).  gs: its first sorted particle, gl: its particles; cut: a periodic
-// row that wraps in x (only its unwrapped part is staged).  Called by one
-// wave per brick, lane = row; the slot base of a row is the exclusive prefix
-// of gl over the rows (brick_row_base).
+// row that wraps in x (only its unwrapped part is staged); whole_x: the row
+// holds the whole period in x (a periodic box at most b[0] + 2 cells wide).
+// Called by one wave per brick, lane = row; the slot base of a row is the
+// exclusive prefix of gl over the rows (brick_row_base).
 __device__ __forceinline__ void brick_row(uint32_t id, const Bricks& B, const Grid& g,
                                           const uint32_t* __restrict__ start, int lane, uint32_t& gs,
                                           uint32_t& gl, bool& cut, bool* whole_x = nullptr) {
