@@ -114,7 +114,8 @@ typedef struct SphxParams {
     double grid_hi[3];         /* uploaded positions (padded), or from box_lo/hi when periodic   */
     int32_t precision;         /* enum SphxPrecision                                             */
     int32_t max_neighbours;    /* per-particle neighbour-list capacity, rounded up to a multiple
-                                  of 4; <= 0: 192                                                */
+                                  of 4; <= 0: 192.  Device memory: 6 bytes per particle and unit
+                                  of capacity (4-byte entries, 2-byte staging slots)           */
     double neighbour_skin;     /* list radius 2*h_avg + skin, rebuilt only once some particle has
                                   moved skin/2 since the last build; < 0: 0.04 * max(h) (default);
                                   0: rebuilt every tick.  Every pass still uses exactly the pairs
