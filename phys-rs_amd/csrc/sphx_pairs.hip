@@ -902,8 +902,10 @@ __global__ void __launch_bounds__(2 * kStagedBlock) k_forces_staged(Particles P,
     constexpr bool PF = true;
     // VISC_ONLY without the stress: the neighbour's P/rho^2 (the last 8 bytes
     // of its record) is not read
-    // uniform m and h: the pair body needs only the record (the slot lists)
-    brick_pass<double2, 4, W, PF, 2, NQ2, false, VISC_ONLY && !STRESS, UNI>(
+    // uniform m and h: the pair body needs only the record (the slot lists;
+    // not written with the Maxwell stress, whose 96-byte records leave few
+    // bricks staged whole)
+    brick_pass<double2, 4, W, PF, 2, NQ2, false, VISC_ONLY && !STRESS, UNI && !STRESS>(
         B, g, start, cap, L, reinterpret_cast<const double2*>(D.r64), H, sh_dyn,
         [&](uint32_t i, const auto& f) {
             if (kp.beta == 0.0)
