@@ -474,7 +474,9 @@ __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint
 // One lane per particle summing the two halves in turn (two lanes per
 // particle measured slower here: one workgroup per CU instead of two, 0.21 ->
 // 0.25 ms on the 1M dam break).
-template <bool UNI, int PER, int W, bool PF>
+// S16: the slot lists are written (the launch picks the form without the
+// slot walk when they are not, which saves its registers).
+template <bool UNI, int PER, int W, bool PF, bool S16>
 __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4))) k_density_staged(
     Particles P, Derived D, Grid g, NbrList L, KernelParams kp, Bricks B, const uint32_t* __restrict__ start,
     uint32_t cap) {
@@ -484,7 +486,7 @@ __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_
     // read, and the pair body needs only the record (the slot lists); the
     // epilogue's velocity loads wait for the walk (ahead of it they held
     // registers that then spilled: 1M density 0.173 -> 0.168 ms)
-    brick_pass<double2, 2, W, PF, 1, 0, false, UNI, UNI>(
+    brick_pass<double2, 2, W, PF, 1, 0, false, UNI, UNI && S16>(
         B, g, start, cap, L, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
         [&](uint32_t i, const auto& f) { density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f); });
 }
@@ -557,7 +559,7 @@ __global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D
     density_mixed_particle<UNI>(P, D, X, g, L, kp, i, GlobalFetch<float4, 1>{X.p4});
 }
 
-template <bool UNI, int W, bool PF>
+template <bool UNI, int W, bool PF, bool S16>
 __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_density_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, Bricks B,
                        const uint32_t* __restrict__ start, uint32_t cap) {
@@ -565,7 +567,7 @@ k_density_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, Kerne
     extern __shared__ uint4 sh_dyn[];
     // brick-relative records; uniform m and h: the pair body needs only the
     // record (the slot lists)
-    brick_pass<float4, 1, W, PF, 1, 0, false, false, UNI, true>(
+    brick_pass<float4, 1, W, PF, 1, 0, false, false, UNI && S16, true>(
         B, g, start, cap, L, X.p4, H, sh_dyn,
         [&](uint32_t i, const auto& f) { density_mixed_particle<UNI>(P, D, X, g, L, kp, i, f); });
 }
@@ -604,24 +606,30 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
         const StagedLaunch sl = staged_launch(B, 16, 32);
         // walk widths measured on the 1M dam break: four entries with the next
         // four loaded ahead for the uniform density (0.27 -> 0.24 ms)
-#define SPHX_DENS_STAGED(U, PER, W, PF)                                                                  \
-        hipLaunchKernelGGL((k_density_staged<U, PER, W, PF>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B, \
+#define SPHX_DENS_STAGED(U, PER, W, PF, S)                                                                  \
+        hipLaunchKernelGGL((k_density_staged<U, PER, W, PF, S>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B, \
                            cell_start, sl.cap)
-        if (uni && g.periodic) SPHX_DENS_STAGED(true, 1, 4, true);
-        else if (uni) SPHX_DENS_STAGED(true, 0, 4, true);
-        else if (g.periodic) SPHX_DENS_STAGED(false, 1, 2, false);
-        else SPHX_DENS_STAGED(false, 0, 2, false);
+        const bool slots = L.s16 != nullptr;
+        if (uni && g.periodic && slots) SPHX_DENS_STAGED(true, 1, 4, true, true);
+        else if (uni && g.periodic) SPHX_DENS_STAGED(true, 1, 4, true, false);
+        else if (uni && slots) SPHX_DENS_STAGED(true, 0, 4, true, true);
+        else if (uni) SPHX_DENS_STAGED(true, 0, 4, true, false);
+        else if (g.periodic) SPHX_DENS_STAGED(false, 1, 2, false, false);
+        else SPHX_DENS_STAGED(false, 0, 2, false, false);
 #undef SPHX_DENS_STAGED
         return;
     }
     if (mixed && B.on && B.nactive) {
         const StagedLaunch sl = staged_launch(B, 16, 16);
-        if (uni)
-            hipLaunchKernelGGL((k_density_mixed_staged<true, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, X, g, L,
-                               kp, B, cell_start, sl.cap);
+        if (uni && L.s16)
+            hipLaunchKernelGGL((k_density_mixed_staged<true, 4, true, true>), sl.grid, sl.block, sl.dyn, s, P, D, X,
+                               g, L, kp, B, cell_start, sl.cap);
+        else if (uni)
+            hipLaunchKernelGGL((k_density_mixed_staged<true, 4, true, false>), sl.grid, sl.block, sl.dyn, s, P, D, X,
+                               g, L, kp, B, cell_start, sl.cap);
         else
-            hipLaunchKernelGGL((k_density_mixed_staged<false, 4, false>), sl.grid, sl.block, sl.dyn, s, P, D, X, g,
-                               L, kp, B, cell_start, sl.cap);
+            hipLaunchKernelGGL((k_density_mixed_staged<false, 4, false, false>), sl.grid, sl.block, sl.dyn, s, P, D,
+                               X, g, L, kp, B, cell_start, sl.cap);
         return;
     }
     if (mixed && uni)
