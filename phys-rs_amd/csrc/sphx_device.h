@@ -877,6 +877,9 @@ struct BrickFetch {
     uint32_t whole;                   // every list entry of the brick's particles is staged (uniform)
     uint32_t rec0 = 0;                // S16: LDS byte address of slot 0's quad 0
     const uint16_t* __restrict__ s16 = nullptr;  // S16: the particle's slot list (null: walk the entries)
+    // S16, four-entry walks of a brick staged whole: the particle's slot list
+    // to fill while the entries are walked (the first pass after a build)
+    uint16_t* __restrict__ w16 = nullptr;
     // BREL: the walking particle's cell offset from the brick origin,
     // (kx + 128) | (ky + 128) << 8 | (kz + 128) << 16 (one register)
     uint32_t kxyz = 0;
@@ -1040,8 +1043,20 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i,
             }
             Q q[W][NR];
             if constexpr (decltype(whole)::value) {
+                uint32_t ad[W];
 #pragma unroll
-                for (int u = 0; u < W; ++u) F.from_lds(F.address(c[u]), q[u]);
+                for (int u = 0; u < W; ++u) {
+                    ad[u] = F.address(c[u]);
+                    F.from_lds(ad[u], q[u]);
+                }
+                if constexpr (S16 && W == 4) {
+                    if (F.w16) {  // the group's four slots, one 8-byte store
+                        const uint32_t s0 = (ad[0] - F.rec0) >> 4, s1 = (ad[1] - F.rec0) >> 4;
+                        const uint32_t s2 = (ad[2] - F.rec0) >> 4, s3 = (ad[3] - F.rec0) >> 4;
+                        *reinterpret_cast<uint2*>(F.w16 + ((size_t)(k >> 2) << 8)) =
+                            make_uint2(s0 | (s1 << 16), s2 | (s3 << 16));
+                    }
+                }
             } else {
                 uint32_t slot[W];
                 bool staged = true;
@@ -1078,6 +1093,14 @@ __device__ __forceinline__ void walk(const NbrList& L, uint32_t i,
                 body(c[u], q);
             }
         }
+        if constexpr (S16 && W == 4) {
+            if (F.w16) {  // the last, partial group (its entries are staged: the brick is whole)
+                uint32_t sl[3];
+#pragma unroll
+                for (int u = 0; u < 3; ++u) sl[u] = k + (uint32_t)u < cnt ? (F.address(c[u]) - F.rec0) >> 4 : 0u;
+                *reinterpret_cast<uint2*>(F.w16 + ((size_t)(k >> 2) << 8)) = make_uint2(sl[0] | (sl[1] << 16), sl[2]);
+            }
+        }
         flush();
     }
 }
@@ -1091,7 +1114,8 @@ template <class Q, int NQ, int W, bool PF, int LANES = 1, int NQ2 = 0, bool PIN 
           bool S16 = false, bool BREL = false, class Body>
 __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const uint32_t* __restrict__ start,
                                            uint32_t cap, const NbrList& L, const Q* __restrict__ src, BrickHdr& H,
-                                           uint4* __restrict__ sh, Body&& body, const Q* __restrict__ src2 = nullptr) {
+                                           uint4* __restrict__ sh, Body&& body, const Q* __restrict__ src2 = nullptr,
+                                           bool write16 = false) {
     const uint32_t hy = (uint32_t)B.b[1] + 2u, nh = hy * ((uint32_t)B.b[2] + 2u);
     const uint32_t BY = (uint32_t)B.b[1], no = BY * (uint32_t)B.b[2];
     const SPHX_LDS BrickHdr* HL = (const SPHX_LDS BrickHdr*)&H;
@@ -1132,7 +1156,8 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
         base_fetch.wy = (float)g.w[1];
         base_fetch.wz = (float)g.w[2];
     }
-    // the slot lists are read only in a brick whose halo is staged whole
+    // the slot lists are read (or, write16, filled) only in a brick whose
+    // halo is staged whole
     const bool slots = S16 && L.s16 != nullptr && base_fetch.whole;
     for (uint32_t u = u0; u < nown; u += blockDim.x / LANES) {
         while (u >= HL->own_cum[k + 1]) ++k;
@@ -1140,7 +1165,8 @@ __device__ __forceinline__ void brick_pass(const Bricks& B, const Grid& g, const
         Fetch f = base_fetch;
         f.rc += k * (uint32_t)kRowCodes;
         f.n_list = u == u0 ? n0 : (uint32_t)L.ntot[i];
-        if (slots) f.s16 = L.s16 + nbr_index(L, i);
+        if (slots && write16) f.w16 = L.s16 + nbr_index(L, i);
+        else if (slots) f.s16 = L.s16 + nbr_index(L, i);
         if constexpr (BREL) {
             // the particle's cell offset from the brick origin (its x only
             // where some record comes from global memory)

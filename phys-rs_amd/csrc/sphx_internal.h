@@ -116,9 +116,9 @@ static constexpr uint32_t kMaxParticles = 1u << kIdxBits;
 struct NbrList {
     uint32_t* idx;
     // the same entries as LDS record slots of the particle's brick halo (16
-    // bits each, same index as idx), written by the build when the staged
-    // passes are on; a staged pass reads them in a brick whose halo rows are
-    // all staged whole (null: not kept)
+    // bits each, same index as idx), written by the first density pass after
+    // each build in the bricks it stages whole; later staged passes read them
+    // in a brick whose halo rows are all staged whole (null: not written)
     uint16_t* s16;
     uint16_t* ntot;            // entries per particle (<= ks)
     uint32_t* overflow;        // [0] largest list seen, [1] some list overflowed, [2] a particle moved
@@ -145,8 +145,6 @@ struct Bricks {
     uint32_t* pos;             // exclusive scan of flag [cap + 1]; pos[count] = active bricks
     uint32_t* sums;            // scan scratch
     uint32_t* active;          // ids of the bricks that hold particles, in brick order [cap]
-    uint32_t* adj;             // per brick and halo row (kMaxHaloRows a brick): the LDS slot of a record in
-                               // the row minus its sorted index (mod 2^32), for the slot lists [cap * kMaxHaloRows]
     const uint32_t* skey;      // cell key of each sorted particle (the sort's)
     int cus;                   // compute units: the staged passes run cus * k workgroups
     int on;                    // staged passes enabled
@@ -209,13 +207,12 @@ void launch_tiles(const Grid& g, const Sort& S, const Tiles& T, hipStream_t s);
 void launch_stress_rec(const Particles& P, const Derived& D, uint32_t n, hipStream_t s);
 void launch_bricks(const Grid& g, const Sort& S, const Bricks& B, hipStream_t s);
 uint32_t tile_grid(uint32_t n, const Tiles& T);
-// B: with the staged passes on (B.on) the build also writes the slot lists (L.s16)
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, const Bricks& B, bool uniform, const Mixed& X, double h_max, double skin,
-                      hipStream_t s);
+                      const NbrList& L, bool uniform, const Mixed& X, double h_max, double skin, hipStream_t s);
+// write_slots: fill L.s16 (the first density pass after a build; L.s16 set)
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                     const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
-                    const uint32_t* cell_start, hipStream_t s);
+                    const uint32_t* cell_start, bool write_slots, hipStream_t s);
 // energy: also du/dt of the adiabatic EOS (full passes only)
 void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                    const KernelParams& kp, bool visc_only, bool energy, bool mhd_stress, bool charge, bool mixed,

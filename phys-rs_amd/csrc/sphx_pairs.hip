@@ -64,13 +64,10 @@ namespace sphx {
 // 1e-4 margin, so no neighbour is cut; owners outside that bound do not cut.
 // Rows whose three cells are not consecutive (a periodic wrap) walk the
 // cells one at a time.
-// With the staged passes on, each entry's LDS slot in the particle's brick
-// halo (sorted index + the brick row's slot offset, see k_brick_adj) is
-// written beside it into the slot list, four 16-bit slots a group.
 template <bool UNI>
 __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T, Grid g,
                                                    const uint32_t* __restrict__ skey,
-                                                   const uint32_t* __restrict__ start, NbrList L, Bricks B,
+                                                   const uint32_t* __restrict__ start, NbrList L,
                                                    float reach, double skin) {
     // staged candidates: cell-relative x, y, z and h; three spare records so
     // that a four-slot step may read past the chunk (those slots are masked)
@@ -97,26 +94,9 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
     // the last four accepted entries (b3 newest): the list is written a group
     // of four (16 bytes) at a time
     uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-    // slot lists: the particle's brick (its halo rows' slot offsets at
-    // B.adj[brow + row]) and the last four slots (newest in the top 16 bits)
-    const bool slots = B.on && L.s16 != nullptr;  // uniform
-    uint16_t* __restrict__ out16 = slots ? L.s16 + nbr_index(L, p) : nullptr;
-    uint64_t sb = 0;
-    uint32_t brow = 0;
-    int ky = 0, kz = 0, hyb = 0;
-    if (slots) {
-        const int BX = B.b[0], BY = B.b[1], BZ = B.b[2];
-        const uint32_t bx = (uint32_t)(cx / BX), by = (uint32_t)(tc.cy / BY), bz = (uint32_t)(tc.cz / BZ);
-        brow = ((bz * (uint32_t)B.nb[1] + by) * (uint32_t)B.nb[0] + bx) * (uint32_t)kMaxHaloRows;
-        ky = tc.cy - (int)by * BY;
-        kz = tc.cz - (int)bz * BZ;
-        hyb = BY + 2;
-    }
     for (int s = 0; s < kStages; ++s) {
         Stage st;
         stage_segments(g, start, tc, s, st);
-        // this stage's halo row of the particle's brick (stage = (oz + 1) * 3 + oy + 1)
-        const uint32_t adj = slots && valid ? B.adj[brow + (uint32_t)((ky + s % 3) + hyb * (kz + s / 3))] : 0u;
         uint32_t ra[3], rb[3];
         for (int o = 0; o < 3; ++o) cell_slots(g, start, st, cx, o - 1, ra[o], rb[o]);
         uint32_t self = 0xffffffffu;
@@ -203,18 +183,12 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                     const uint32_t k = (uint32_t)__builtin_ctz(mask);
                     mask &= mask - 1u;
                     if (total < (uint32_t)L.ks) {
-                        const uint32_t j = slot_particle(st, base + k);
                         b0 = b1;
                         b1 = b2;
                         b2 = b3;
-                        b3 = j | code;
-                        sb = (sb >> 16) | ((uint64_t)((j + adj) & 0xffffu) << 48);
-                        if ((total & 3u) == 3u) {
+                        b3 = slot_particle(st, base + k) | code;
+                        if ((total & 3u) == 3u)
                             *reinterpret_cast<uint4*>(out + nbr_off(total - 3u)) = make_uint4(b0, b1, b2, b3);
-                            if (slots)
-                                *reinterpret_cast<uint2*>(out16 + nbr_off(total - 3u)) =
-                                    make_uint2((uint32_t)sb, (uint32_t)(sb >> 32));
-                        }
                     }
                     ++total;
                 }
@@ -263,18 +237,12 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
                     if (total < (uint32_t)L.ks) {
                         const uint32_t slot = base + k;
                         const uint32_t o = (slot >= ra[1] ? 1u : 0u) + (slot >= rb[1] ? 1u : 0u);
-                        const uint32_t j = slot_particle(st, slot);
                         b0 = b1;
                         b1 = b2;
                         b2 = b3;
-                        b3 = j | (((uint32_t)s * 3u + o) << kIdxBits);
-                        sb = (sb >> 16) | ((uint64_t)((j + adj) & 0xffffu) << 48);
-                        if ((total & 3u) == 3u) {
+                        b3 = slot_particle(st, slot) | (((uint32_t)s * 3u + o) << kIdxBits);
+                        if ((total & 3u) == 3u)
                             *reinterpret_cast<uint4*>(out + nbr_off(total - 3u)) = make_uint4(b0, b1, b2, b3);
-                            if (slots)
-                                *reinterpret_cast<uint2*>(out16 + nbr_off(total - 3u)) =
-                                    make_uint2((uint32_t)sb, (uint32_t)(sb >> 32));
-                        }
                     }
                     ++total;
                 }
@@ -344,10 +312,6 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
             const uint4 v = r == 1u ? make_uint4(b3, 0u, 0u, 0u)
                                     : (r == 2u ? make_uint4(b2, b3, 0u, 0u) : make_uint4(b1, b2, b3, 0u));
             *reinterpret_cast<uint4*>(out + nbr_off(kept - r)) = v;
-            if (slots) {
-                const uint64_t w = sb >> (16u * (4u - r));
-                *reinterpret_cast<uint2*>(out16 + nbr_off(kept - r)) = make_uint2((uint32_t)w, (uint32_t)(w >> 32));
-            }
         }
         L.ntot[p] = (uint16_t)kept;
     }
@@ -359,15 +323,14 @@ __global__ void __launch_bounds__(kBlock) k_build(Particles P, Mixed X, Tiles T,
 }
 
 void launch_build_nbr(const Particles& P, uint32_t n, const Grid& g, const Sort& S, const Tiles& T,
-                      const NbrList& L, const Bricks& B, bool uniform, const Mixed& X, double h_max, double skin,
-                      hipStream_t s) {
+                      const NbrList& L, bool uniform, const Mixed& X, double h_max, double skin, hipStream_t s) {
     if (!n) return;
     const dim3 grid(tile_grid(n, T)), block(kBlock);
     const float reach = (float)(2.0 * h_max * (1.0 + 1e-4) + skin);
     if (uniform)
-        hipLaunchKernelGGL(k_build<true>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, B, reach, skin);
+        hipLaunchKernelGGL(k_build<true>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach, skin);
     else
-        hipLaunchKernelGGL(k_build<false>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, B, reach, skin);
+        hipLaunchKernelGGL(k_build<false>, grid, block, 0, s, P, X, T, g, S.skey, S.cell_start, L, reach, skin);
 }
 
 // ------------------------------------------------------- density + EOS (a4, a5)
@@ -476,10 +439,12 @@ __global__ void __launch_bounds__(kBlock) k_density(Particles P, Derived D, uint
 // 0.25 ms on the 1M dam break).
 // S16: the slot lists are written (the launch picks the form without the
 // slot walk when they are not, which saves its registers).
+// write16: the first density pass after a build fills the slot lists as it
+// walks the entries.
 template <bool UNI, int PER, int W, bool PF, bool S16>
 __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4))) k_density_staged(
     Particles P, Derived D, Grid g, NbrList L, KernelParams kp, Bricks B, const uint32_t* __restrict__ start,
-    uint32_t cap) {
+    uint32_t cap, int write16) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
     // uniform h: the neighbour's h (the last 8 bytes of its record) is not
@@ -488,7 +453,8 @@ __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_
     // registers that then spilled: 1M density 0.173 -> 0.168 ms)
     brick_pass<double2, 2, W, PF, 1, 0, false, UNI, UNI && S16>(
         B, g, start, cap, L, reinterpret_cast<const double2*>(P.pos), H, sh_dyn,
-        [&](uint32_t i, const auto& f) { density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f); });
+        [&](uint32_t i, const auto& f) { density_particle<UNI, PER, 1>(P, D, g, L, kp, i, f); }, nullptr,
+        write16 != 0);
 }
 
 // FP32-mixed density: f32 pair terms on cell-relative coordinates, f64 sum.
@@ -562,14 +528,15 @@ __global__ void __launch_bounds__(kBlock) k_density_mixed(Particles P, Derived D
 template <bool UNI, int W, bool PF, bool S16>
 __global__ void __launch_bounds__(kStagedBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_density_mixed_staged(Particles P, Derived D, Mixed X, Grid g, NbrList L, KernelParams kp, Bricks B,
-                       const uint32_t* __restrict__ start, uint32_t cap) {
+                       const uint32_t* __restrict__ start, uint32_t cap, int write16) {
     __shared__ BrickHdr H;
     extern __shared__ uint4 sh_dyn[];
     // brick-relative records; uniform m and h: the pair body needs only the
     // record (the slot lists)
     brick_pass<float4, 1, W, PF, 1, 0, false, false, UNI && S16, true>(
         B, g, start, cap, L, X.p4, H, sh_dyn,
-        [&](uint32_t i, const auto& f) { density_mixed_particle<UNI>(P, D, X, g, L, kp, i, f); });
+        [&](uint32_t i, const auto& f) { density_mixed_particle<UNI>(P, D, X, g, L, kp, i, f); }, nullptr,
+        write16 != 0);
 }
 
 // Staged launch of a pass with records of rec_bytes that runs at most
@@ -595,10 +562,17 @@ static StagedLaunch staged_launch(const Bricks& B, int max_waves, size_t rec_byt
     l.block = dim3((uint32_t)B.threads);
     return l;
 }
+// The record capacity of the density pass, which writes the slot lists in
+// the bricks it stages whole: a pass that reads them stages at most this
+// many records, so that every brick it stages whole has them written (the
+// FP64 force pass alone could stage a few more: its fixed LDS share is
+// split over half as many workgroups).
+static uint32_t slot_cap(const Bricks& B, bool mixed) { return staged_launch(B, 16, mixed ? 16 : 32).cap; }
 
 void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid& g, const NbrList& L,
                     const KernelParams& kp, bool mixed, const Mixed& X, const Bricks& B,
-                    const uint32_t* cell_start, hipStream_t s) {
+                    const uint32_t* cell_start, bool write_slots, hipStream_t s) {
+    const int w16 = write_slots ? 1 : 0;
     if (!n) return;
     const dim3 grid(nblocks(n)), block(kBlock);
     const bool uni = kp.uniform != 0;
@@ -608,7 +582,7 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
         // four loaded ahead for the uniform density (0.27 -> 0.24 ms)
 #define SPHX_DENS_STAGED(U, PER, W, PF, S)                                                                  \
         hipLaunchKernelGGL((k_density_staged<U, PER, W, PF, S>), sl.grid, sl.block, sl.dyn, s, P, D, g, L, kp, B, \
-                           cell_start, sl.cap)
+                           cell_start, sl.cap, w16)
         const bool slots = L.s16 != nullptr;
         if (uni && g.periodic && slots) SPHX_DENS_STAGED(true, 1, 4, true, true);
         else if (uni && g.periodic) SPHX_DENS_STAGED(true, 1, 4, true, false);
@@ -623,13 +597,13 @@ void launch_density(const Particles& P, const Derived& D, uint32_t n, const Grid
         const StagedLaunch sl = staged_launch(B, 16, 16);
         if (uni && L.s16)
             hipLaunchKernelGGL((k_density_mixed_staged<true, 4, true, true>), sl.grid, sl.block, sl.dyn, s, P, D, X,
-                               g, L, kp, B, cell_start, sl.cap);
+                               g, L, kp, B, cell_start, sl.cap, w16);
         else if (uni)
             hipLaunchKernelGGL((k_density_mixed_staged<true, 4, true, false>), sl.grid, sl.block, sl.dyn, s, P, D, X,
-                               g, L, kp, B, cell_start, sl.cap);
+                               g, L, kp, B, cell_start, sl.cap, 0);
         else
             hipLaunchKernelGGL((k_density_mixed_staged<false, 4, false, false>), sl.grid, sl.block, sl.dyn, s, P, D,
-                               X, g, L, kp, B, cell_start, sl.cap);
+                               X, g, L, kp, B, cell_start, sl.cap, 0);
         return;
     }
     if (mixed && uni)
@@ -1198,12 +1172,14 @@ void launch_forces(const Particles& P, const Derived& D, uint32_t n, const Grid&
                 hipLaunchKernelGGL((k_forces_mixed_stress_staged<V, E, U>), sl.grid, sl.block, sl.dyn, s, P, D, X, \
                                    g, L, kp, ch, B, cell_start, sl.cap);                                    \
             } else if (mixed) {                                                                             \
-                const StagedLaunch sl = staged_launch(B, 16, 32);                                           \
+                StagedLaunch sl = staged_launch(B, 16, 32);                                                 \
+                if (L.s16) sl.cap = std::min(sl.cap, slot_cap(B, true));                                    \
                 hipLaunchKernelGGL((k_forces_mixed_staged<V, E, U, 4, true>), sl.grid, sl.block, sl.dyn, s, P, D, \
                                    X, g, L, kp, ch, B, cell_start, sl.cap);                                 \
             } else {                                                                                        \
                 StagedLaunch sl = staged_launch(B, 8, M ? 96 : 64);                                         \
                 sl.block = dim3(2u * (uint32_t)B.threads); /* two lanes per particle */                      \
+                if (L.s16) sl.cap = std::min(sl.cap, slot_cap(B, false));                                   \
                 if (g.periodic)                                                                             \
                     hipLaunchKernelGGL((k_forces_staged<V, E, M, U, 1>), sl.grid, sl.block, sl.dyn, s, P, D, g, \
                                        L, kp, ch, B, cell_start, sl.cap);                                   \

@@ -109,7 +109,8 @@ struct SphxCtx {
     uint32_t* bricks_host = nullptr;       // pinned: active brick count of the last sort
     hipEvent_t bricks_ready = nullptr;     // recorded after that count's copy
     bool bricks_pending = false;           // count not read yet
-    bool slots = false;                    // the list's slot lists are written (staged passes that read them)
+    bool slots = false;                    // the list gets slot lists (staged passes that read them)
+    bool slots_fresh = false;              // the first density pass after the build has written them
     hipEvent_t moved_ready = nullptr;      // recorded after the kernel that raises the moved flag
     uint32_t* moved_host = nullptr;        // pinned, fine-grained: the moved flag as the kernels leave it
     uint32_t epoch = 1;                    // epoch of the moved flag (see MovedFlag); the words start at 0
@@ -464,8 +465,7 @@ int setup_grid(SphxCtx* c, const SphxState* host, const Grid* fixed = nullptr) {
         if (B.on && B.count > B.cap) {
             int rc;
             if ((rc = dalloc(c, &B.flag, B.count)) || (rc = dalloc(c, &B.pos, (size_t)B.count + 1)) ||
-                (rc = dalloc(c, &B.sums, ((size_t)B.count + 1023) / 1024 + 1)) || (rc = dalloc(c, &B.active, B.count)) ||
-                (rc = dalloc(c, &B.adj, (size_t)B.count * kMaxHaloRows)))
+                (rc = dalloc(c, &B.sums, ((size_t)B.count + 1023) / 1024 + 1)) || (rc = dalloc(c, &B.active, B.count)))
                 return rc;
             B.cap = B.count;
         }
@@ -527,11 +527,11 @@ int launch_check(SphxCtx* c) {
 }
 
 // ---- passes ------------------------------------------------------------
-// The list as the pair passes see it: the slot lists only when the build
-// wrote them.
-NbrList pass_list(const SphxCtx* c) {
+// The list as the pair passes see it: the slot lists only once the first
+// density pass after the build has written them (write: that pass itself).
+NbrList pass_list(const SphxCtx* c, bool write = false) {
     NbrList L = c->L;
-    if (!c->slots) L.s16 = nullptr;
+    if (!c->slots || (!c->slots_fresh && !write)) L.s16 = nullptr;
     return L;
 }
 
@@ -562,14 +562,15 @@ int do_neighbours(SphxCtx* c) {
     {
         Timer t(c, kNbr);
         ++c->epoch;  // displacements count from this build
-        // the slot lists are read by the FP64 force passes and the FP32-mixed
-        // density and force passes with uniform m and h; not with the Maxwell
+        // the slot lists are read by the staged density and force passes
+        // (FP64 and FP32-mixed) with uniform m and h; not with the Maxwell
         // stress, whose 96-byte FP64 records leave few bricks staged whole
-        // (4M MHD box: the slots cost the build more than the passes gain)
+        // (4M MHD box: no gain in its passes)
         const bool stress = c->has_mhd && (c->prm.mhd_flags & SPHX_MHD_MAXWELL_STRESS);
         c->slots = c->B.on && c->kp.uniform && !stress;
-        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, pass_list(c), c->B, c->kp.uniform != 0, c->X, c->hmax,
-                         c->kp.skin, c->stream);
+        c->slots_fresh = false;
+        launch_build_nbr(c->P(), n, c->grid, c->S, c->T, c->L, c->kp.uniform != 0, c->X, c->hmax, c->kp.skin,
+                         c->stream);
     }
     c->sorted = c->listed = true;
     c->list_age = 0;
@@ -593,8 +594,10 @@ int do_density(SphxCtx* c) {
     if ((rc = ensure_list(c))) return rc;
     {
         Timer t(c, kDensity);
-        launch_density(c->P(), c->D, c->n, c->grid, pass_list(c), c->kp, c->mixed(), c->X, c->B, c->S.cell_start,
-                       c->stream);
+        const bool write = c->slots && !c->slots_fresh;  // the first density pass on this list
+        launch_density(c->P(), c->D, c->n, c->grid, pass_list(c, write), c->kp, c->mixed(), c->X, c->B,
+                       c->S.cell_start, write, c->stream);
+        c->slots_fresh = c->slots;
     }
     c->dens = c->pres = true;
     c->srec = c->kp.srec != 0;

@@ -292,28 +292,11 @@ __global__ void k_brick_compact(Bricks B) {
     if (b < B.count && B.flag[b]) B.active[B.pos[b]] = b;
 }
 
-// The slot offsets of every brick's halo rows: one wave per brick, lane =
-// halo row, with the row bases of brick_setup (the exclusive prefix of the
-// rows' full counts, whatever a pass's record capacity): a record of sorted
-// particle j in row r sits at LDS slot j + adj[r] in every pass that stages
-// the row whole.
-__global__ void __launch_bounds__(256) k_brick_adj(const uint32_t* __restrict__ start, Grid g, Bricks B) {
-    const uint32_t b = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (b >= B.count) return;  // wave-uniform
-    const int lane = (int)(threadIdx.x & 63u);
-    uint32_t gs, gl;
-    bool cut;
-    brick_row(b, B, g, start, lane, gs, gl, cut);
-    const uint32_t base = brick_row_base(gl);
-    if (lane < kMaxHaloRows) B.adj[(size_t)b * kMaxHaloRows + (uint32_t)lane] = base - gs;
-}
-
 void launch_bricks(const Grid& g, const Sort& S, const Bricks& B, hipStream_t s) {
     if (!B.on) return;
     hipLaunchKernelGGL(k_brick_flag, dim3(nblocks(B.count)), dim3(kBlock), 0, s, S.cell_start, g, B);
     launch_scan(B.flag, B.pos, B.count, B.sums, s);
     hipLaunchKernelGGL(k_brick_compact, dim3(nblocks(B.count)), dim3(kBlock), 0, s, B);
-    hipLaunchKernelGGL(k_brick_adj, dim3((B.count + 3) / 4), dim3(256), 0, s, S.cell_start, g, B);
 }
 
 // Upper bound of the tile count: sum over rows of ceil(n_r / kTile).

@@ -27,8 +27,10 @@ FIELDS = sphx.F_POS | sphx.F_VEL | sphx.F_ACC | sphx.F_RHO
 
 
 def run(state, params, bricks, nsteps):
-    old = os.environ.get("SPHX_BRICKS")
-    os.environ["SPHX_BRICKS"] = bricks
+    # bricks None: the library's own choice of brick shape
+    old = os.environ.pop("SPHX_BRICKS", None)
+    if bricks is not None:
+        os.environ["SPHX_BRICKS"] = bricks
     try:
         with sphx.Context(len(state["x"]), params) as ctx:
             ctx.upload(state)
@@ -38,9 +40,8 @@ def run(state, params, bricks, nsteps):
             ctx.forces()
             last = ctx.download(FIELDS | (sphx.F_U if "u" in state else 0))
     finally:
-        if old is None:
-            del os.environ["SPHX_BRICKS"]
-        else:
+        os.environ.pop("SPHX_BRICKS", None)
+        if old is not None:
             os.environ["SPHX_BRICKS"] = old
     return first, last
 
@@ -88,6 +89,17 @@ def test_staged_dam_break_bit_identical(require_gpu, shape):
     st, p = scenarios.dam_break(v_sigma=0.1, seed=3)
     ref = run(st, p, "off", 12)
     got = run(st, p, shape, 12)
+    assert_identical(got[0], ref[0])
+    assert_identical(got[1], ref[1])
+
+
+def test_staged_dam_break_1M_bit_identical(require_gpu):
+    # the 1,048,576-particle dam break on the library's own brick shape: the
+    # force pass reads the slot lists the density pass wrote, also in the
+    # bricks whose halo comes close to either pass's record capacity
+    st, p = scenarios.dam_break(nx=128, ny=128, nz=64, spacing=0.01, box=(3.2, 1.6, 1.6), v_sigma=0.1, seed=7)
+    ref = run(st, p, "off", 8)
+    got = run(st, p, None, 8)
     assert_identical(got[0], ref[0])
     assert_identical(got[1], ref[1])
 
