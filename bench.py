@@ -70,9 +70,9 @@ STEP_BYTES_PER_PARTICLE = 720.0          # sum over one tick (no MHD)
 # 256 MiB copy) of the FP64 kernels on this workload, from the rocprofv3
 # counter runs summarised in profiles/dam_break_1M_fp64_mixed_kernels.md.
 MEASURED_TRAFFIC = {
-    "force": (302.7 + 50.7) * 1e6,
-    "visc_only": (325.9 + 25.3) * 1e6,
-    "density": (266.8 + 92.7) * 1e6,
+    "force": (301.6 + 50.7) * 1e6,
+    "visc_only": (325.3 + 25.4) * 1e6,
+    "density": (287.8 + 92.6) * 1e6,
 }
 STEP_FLOPS_PER_PAIR = 26.0 + 58.0 + 45.0
 
